@@ -1,0 +1,178 @@
+"""ctypes view of include/tg.h and include/tg_host.h.
+
+The library is the in-tree discrete_adjoint_amd/libtg.so built by
+discrete_adjoint_amd/build.py.  Loading fails loudly when it is missing:
+there is no CPU fallback behind this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtg.so")
+
+TG_OK = 0
+TG_ERR_ARG = -1
+TG_ERR_HIP = -2
+TG_ERR_STATE = -3
+TG_ERR_NOMEM = -4
+TG_ERR_NONFINITE = -5
+TG_ERR_EIGEN = -6
+TG_ERR_UNSUPPORTED = -7
+TG_ERR_NODEVICE = -8
+
+TG_FP64 = 0
+TG_MIXED = 1
+TG_MESH_UNIFORM = 0
+TG_MESH_LEAVES = 1
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+
+
+class tg_config(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int),
+        ("checkpoint_every", C.c_int),
+        ("max_steps", C.c_int),
+        ("device", C.c_int),
+        ("tiny", C.c_double),
+        ("gamma", C.c_double),
+        ("cfl_inner", C.c_double),
+        ("cfl_outer", C.c_double),
+        ("frict_tiny_early", C.c_double),
+        ("frict_tiny_late", C.c_double),
+        ("frict_tiny_switch", C.c_int),
+        ("tile_x", C.c_int),
+        ("tile_y", C.c_int),
+    ]
+
+
+class tg_mesh(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("dx", C.c_double),
+        ("dy", C.c_double),
+        ("x0", C.c_double),
+        ("y0", C.c_double),
+        ("grav", _dp),
+        ("dgrav", _dp),
+        ("curv", _dp),
+    ]
+
+
+class tg_params(C.Structure):
+    _fields_ = [
+        ("sin_int", C.c_double),
+        ("tan_bed", C.c_double),
+        ("kact_c1", C.c_double),
+        ("kact_root", C.c_double),
+        ("dkroot_dphi", C.c_double),
+        ("epsilon", C.c_double),
+        ("gravity_scale", C.c_double),
+        ("maxtime", C.c_double),
+        ("nextoutput", C.c_double),
+        ("nextsave", C.c_double),
+    ]
+
+
+class tg_stats(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double),
+        ("time", C.c_double),
+        ("mass", C.c_double),
+        ("hmax", C.c_double),
+        ("step", C.c_int),
+    ]
+
+
+class tg_functional(C.Structure):
+    _fields_ = [("terminal_weight", _dp)]
+
+
+class tg_dem(C.Structure):
+    _fields_ = [
+        ("nrows", C.c_int),
+        ("ncols", C.c_int),
+        ("xmin", C.c_double),
+        ("ymax", C.c_double),
+        ("res", C.c_double),
+        ("elev", _fp),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/*.h
+SIGNATURES = {
+    "tg_config_default": (None, [C.POINTER(tg_config)]),
+    "tg_params_from_angles": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.POINTER(tg_params)]),
+    "tg_create": (C.c_int, [C.POINTER(tg_config), C.POINTER(C.c_void_p)]),
+    "tg_destroy": (None, [C.c_void_p]),
+    "tg_last_error": (C.c_char_p, [C.c_void_p]),
+    "tg_set_mesh": (C.c_int, [C.c_void_p, C.POINTER(tg_mesh)]),
+    "tg_set_state": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "tg_get_state": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "tg_forward_step": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.POINTER(tg_stats)]),
+    "tg_forward_run": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int,
+                                 C.POINTER(tg_stats)]),
+    "tg_forward_replay": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, _dp,
+                                    C.POINTER(tg_stats)]),
+    "tg_get_dt_record": (C.c_int, [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)]),
+    "tg_adjoint_run": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.POINTER(tg_functional),
+                                 _dp, _dp]),
+    "tg_adjoint_step": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_double,
+                                  _dp, _dp]),
+    "tg_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "tg_time_kernel": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_int, _dp]),
+    "tg_num_cells": (C.c_int64, [C.c_void_p]),
+    # tg_host.h
+    "tg_host_dem_plane_runout": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double,
+                                           C.c_double, C.c_double, C.c_double, C.c_double,
+                                           _fp]),
+    "tg_host_dem_diamond_square": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64,
+                                             _fp]),
+    "tg_host_static_coefs": (C.c_int, [C.POINTER(tg_dem), C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, _dp, _dp, _dp]),
+    "tg_host_piles": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                _dp, _dp]),
+    "tg_host_box_weights": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, _dp]),
+}
+
+_LIB = None
+
+
+def load():
+    """Returns the loaded library; raises if libtg.so is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m discrete_adjoint_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 numpy array."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "need contiguous float64"
+    return a.ctypes.data_as(_dp)
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], "need contiguous float32"
+    return a.ctypes.data_as(_fp)

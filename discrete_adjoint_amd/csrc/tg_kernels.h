@@ -1,0 +1,98 @@
+// Device-side data structures shared by the kernels and the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace tg {
+
+// One uniform block of cells (one ensemble member).  Cells of all instances
+// are concatenated in the global SoA arrays; instance b owns
+// [off, off + nx*ny).
+struct InstDev {
+  int nx, ny;
+  long long off;
+  double dx, dy;
+  double sin_int, tan_bed, kc1, kroot, dkroot, eps, gscale;
+  double maxtime, nextoutput, nextsave;
+};
+
+// A launch tile: TILE_X x TILE_Y cells of instance `inst` starting at
+// (x0, y0).  One workgroup per tile.
+struct TileDev {
+  int inst, x0, y0, pad;
+};
+
+// Per-instance CFL reduction cells: bit patterns of non-negative doubles
+// (IEEE order of non-negative doubles equals unsigned-integer order, so
+// integer atomic min/max is an exact FP64 min/max).
+struct RedDev {
+  unsigned long long min_ev;  // min over wet cells of min(dx,dy)/evalue
+  unsigned long long hmax;    // max over wet cells of h
+};
+
+constexpr int TILE_X = 64;
+constexpr int TILE_Y = 4;
+constexpr int BLOCK = TILE_X * TILE_Y;
+
+struct Fields {
+  const double* h;
+  const double* hu;
+  const double* hv;
+};
+struct Statics {
+  const double* gx;
+  const double* gy;
+  const double* gz;
+  const double* dgx;
+  const double* dgy;
+  const double* kx;
+  const double* ky;
+};
+
+struct StepArgs {
+  const InstDev* inst;
+  const TileDev* tiles;
+  int ntiles;
+  Fields U;
+  Statics S;
+  double* kact;          // [n] forward kact of U
+  RedDev* red;           // [ninst]
+  double* dt_rec;        // [ninst * max_steps]
+  double* time;          // [ninst]
+  int max_steps;
+  int step;              // step index s (U = U^s)
+  double ft;             // frict_tiny of step s
+  double tiny, gamma, cfl_inner, cfl_outer;
+  double* h_out;
+  double* hu_out;
+  double* hv_out;
+  int* err;              // sticky error flag (non-finite state)
+};
+
+struct AdjArgs {
+  const InstDev* inst;
+  const TileDev* tiles;
+  int ntiles;
+  Fields U;              // U^s
+  Statics S;
+  Fields lam_next;       // lambda^{s+1}
+  const double* dt_rec;
+  int max_steps;
+  int step;              // s
+  double ft;
+  double tiny, gamma;
+  double* lam_h;         // lambda^s out
+  double* lam_hu;
+  double* lam_hv;
+  double* dphi_part;     // [ntiles] running sum of lambda^{s+1}.dU/dphi, or null
+};
+
+void launch_coef(const StepArgs& a, bool reduce, hipStream_t s);
+void launch_dt(const StepArgs& a, int ninst, hipStream_t s);
+void launch_update(const StepArgs& a, hipStream_t s);
+void launch_adjoint(const AdjArgs& a, hipStream_t s);
+void launch_reset_red(RedDev* red, int ninst, hipStream_t s);
+void launch_stats(const InstDev* inst, const TileDev* tiles, int ntiles, Fields U,
+                  double* part_mass, double* part_hmax, double tiny, hipStream_t s);
+
+}  // namespace tg

@@ -1,0 +1,549 @@
+// FP64 kernels of the forward step and the per-step discrete adjoint on
+// uniform blocks of cells.  One workgroup = one 64x4 tile of one instance
+// (tile table), so an ensemble of uneven blocks runs in one launch.
+//
+// Forward step s (replaces step.C:38-232, order 1):
+//   k_coef   : forward slopes -> kact (getcoef.f) + per-instance CFL min /
+//              hmax reduction (get_coef_and_eigen.C:29-180)
+//   k_dt     : dt = 0.5*min(0.5*min(dx/ev), 0.9*sqrt(hmax eps Gs/9.8)), incrtime
+//   k_update : slopes, HLL edge fluxes with outflow copies, update_states
+// Reverse step s (replaces calc_jacobian + calc_adjoint, dual_solver.C:200-206):
+//   k_adjoint: gathers lambda^s_c = g_c - sum_j J_j[slot->c]^T lambda^{s+1}_j,
+//              recomputing every Jacobian block from U^s in registers.
+#include "tg_kernels.h"
+#include "tg_math.h"
+
+namespace tg {
+
+namespace {
+
+__device__ __forceinline__ unsigned long long dbits(double v) {
+  return (unsigned long long)__double_as_longlong(v);
+}
+__device__ __forceinline__ double bitsd(unsigned long long v) {
+  return __longlong_as_double((long long)v);
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_xor(v, o, 64);
+    v = (w < v) ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_xor(v, o, 64);
+    v = (w > v) ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+struct Cell {
+  int i, j;
+  long long c;
+  bool act;
+  bool hasE, hasW, hasN, hasS;
+};
+
+__device__ __forceinline__ Cell cell_of(const TileDev& t, const InstDev& I) {
+  Cell q;
+  q.i = t.x0 + (int)(threadIdx.x % TILE_X);
+  q.j = t.y0 + (int)(threadIdx.x / TILE_X);
+  q.act = q.i < I.nx && q.j < I.ny;
+  q.c = I.off + (long long)q.j * I.nx + q.i;
+  q.hasE = q.i + 1 < I.nx;
+  q.hasW = q.i > 0;
+  q.hasN = q.j + 1 < I.ny;
+  q.hasS = q.j > 0;
+  return q;
+}
+
+// ---------------------------------------------------------------- forward --
+
+__global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
+  const TileDev t = a.tiles[blockIdx.x];
+  const InstDev I = a.inst[t.inst];
+  const Cell q = cell_of(t, I);
+  double minev = HUGE_VAL, hmax = 0.;
+  if (q.act) {
+    const long long c = q.c, nx = I.nx;
+    const double h = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
+    double hx = 0., hux = 0., hy = 0., hvy = 0.;
+    if (q.hasE && q.hasW && q.hasN && q.hasS) {  // boundary cells: zero slopes
+      hx = slope_fwd(a.U.h[c + 1], h, a.U.h[c - 1], I.dx, a.gamma, a.tiny);
+      hux = slope_fwd(a.U.hu[c + 1], hu, a.U.hu[c - 1], I.dx, a.gamma, a.tiny);
+      hy = slope_fwd(a.U.h[c + nx], h, a.U.h[c - nx], I.dy, a.gamma, a.tiny);
+      hvy = slope_fwd(a.U.hv[c + nx], hv, a.U.hv[c - nx], I.dy, a.gamma, a.tiny);
+    }
+    double k;
+    if (h > a.tiny) {
+      k = getcoef(h, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.eps, a.tiny);
+      if (reduce) {
+        double ev = eigen(h, hu, hv, k, a.S.gz[c], a.tiny);
+        if (ev > 1000000000.) atomicOr(a.err, 2);
+        double mdx = (I.dx > I.dy) ? I.dy : I.dx;
+        minev = mdx / ev;
+        hmax = h;
+      }
+    } else {
+      k = I.eps;
+    }
+    a.kact[c] = k;
+  }
+  if (reduce) {
+    __shared__ double smin[BLOCK / 64], smax[BLOCK / 64];
+    minev = wave_min(minev);
+    hmax = wave_max(hmax);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) { smin[w] = minev; smax[w] = hmax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < BLOCK / 64; k++) {
+        minev = (smin[k] < minev) ? smin[k] : minev;
+        hmax = (smax[k] > hmax) ? smax[k] : hmax;
+      }
+      if (minev < HUGE_VAL) atomicMin(&a.red[t.inst].min_ev, dbits(minev));
+      if (hmax > 0.) atomicMax(&a.red[t.inst].hmax, dbits(hmax));
+    }
+  }
+}
+
+__global__ void k_reset_red(RedDev* red, int ninst) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < ninst) {
+    red[b].min_ev = dbits(10000000.);  // get_coef_and_eigen.C:35
+    red[b].hmax = dbits(0.);
+  }
+}
+
+__global__ void k_dt(StepArgs a, int ninst) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ninst) return;
+  const InstDev I = a.inst[b];
+  double min_ev = bitsd(a.red[b].min_ev), hmax = bitsd(a.red[b].hmax);
+  double dt0 = a.cfl_inner * min_ev;
+  double dt1 = -0.9 * sqrt(hmax * I.eps * I.gscale / 9.8);
+  double m = (dt0 > -dt1) ? -dt1 : dt0;
+  double dt = a.cfl_outer * m;
+  if (dt == 0.0) dt = 0.5 * HUGE_VAL;
+  // TimeProps::incrtime, properties.h:397-410
+  double time = a.time[b];
+  if (time + dt > I.nextoutput) dt = I.nextoutput - time;
+  if (time + dt > I.nextsave) dt = I.nextsave - time;
+  if (time + dt > I.maxtime) dt = I.maxtime - time;
+  a.time[b] = time + dt;
+  a.dt_rec[(long long)b * a.max_steps + a.step] = dt;
+  a.red[b].min_ev = dbits(10000000.);
+  a.red[b].hmax = dbits(0.);
+}
+
+__device__ __forceinline__ Hfv hfv_at(const StepArgs& a, int dir, long long c) {
+  return hfv_fwd(dir, a.U.h[c], a.U.hu[c], a.U.hv[c], a.kact[c], a.S.gz[c], a.tiny);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_update(StepArgs a) {
+  const TileDev t = a.tiles[blockIdx.x];
+  const InstDev I = a.inst[t.inst];
+  const Cell q = cell_of(t, I);
+  if (!q.act) return;
+  const long long c = q.c, nx = I.nx;
+  const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
+  const double prev[3] = {a.U.h[c], a.U.hu[c], a.U.hv[c]};
+  // Element::get_slopes, element2.C:830-939 (only the components the
+  // corrector reads: hx, hvx, hy, huy)
+  double hx = 0., hvx = 0., hy = 0., huy = 0.;
+  if (q.hasE && q.hasW && q.hasN && q.hasS) {
+    hx = slope_fwd(a.U.h[c + 1], prev[0], a.U.h[c - 1], I.dx, a.gamma, a.tiny);
+    hvx = slope_fwd(a.U.hv[c + 1], prev[2], a.U.hv[c - 1], I.dx, a.gamma, a.tiny);
+    hy = slope_fwd(a.U.h[c + nx], prev[0], a.U.h[c - nx], I.dy, a.gamma, a.tiny);
+    huy = slope_fwd(a.U.hu[c + nx], prev[1], a.U.hu[c - nx], I.dy, a.gamma, a.tiny);
+  }
+  // calc_edge_states + boundary_flux (element2.C:2744-3072, 3374-3411): a
+  // face on the domain boundary takes the flux of the opposite face.
+  double fE[3] = {0., 0., 0.}, fW[3] = {0., 0., 0.}, fN[3] = {0., 0., 0.}, fS[3] = {0., 0., 0.};
+  {
+    const Hfv self = hfv_at(a, 0, c);
+    if (q.hasE) riemann(self, hfv_at(a, 0, c + 1), fE);
+    if (q.hasW) riemann(hfv_at(a, 0, c - 1), self, fW);
+    if (!q.hasE) { fE[0] = fW[0]; fE[1] = fW[1]; fE[2] = fW[2]; }
+    if (!q.hasW) { fW[0] = fE[0]; fW[1] = fE[1]; fW[2] = fE[2]; }
+  }
+  {
+    const Hfv self = hfv_at(a, 1, c);
+    if (q.hasN) riemann(self, hfv_at(a, 1, c + nx), fN);
+    if (q.hasS) riemann(hfv_at(a, 1, c - nx), self, fS);
+    if (!q.hasN) { fN[0] = fS[0]; fN[1] = fS[1]; fN[2] = fS[2]; }
+    if (!q.hasS) { fS[0] = fN[0]; fS[1] = fN[1]; fS[2] = fN[2]; }
+  }
+  // correct() -> update_states (corrector.C:36-140, residual.C:162-275)
+  const double kc = (prev[0] > a.tiny) ? a.kact[c] : I.eps;
+  double out[3];
+  SrcInfo si;
+  update_states(prev, out, fE, fN, fW, fS, dt / I.dx, dt / I.dy, dt, hx, hvx, hy, huy,
+                a.S.kx[c], a.S.ky[c], I.sin_int, I.tan_bed, a.S.gx[c], a.S.gy[c], a.S.gz[c],
+                a.S.dgx[c], a.S.dgy[c], kc, a.ft, a.tiny, &si);
+  if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
+  a.h_out[c] = out[0];
+  a.hu_out[c] = out[1];
+  a.hv_out[c] = out[2];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_stats(const InstDev* inst, const TileDev* tiles,
+                                                 Fields U, double* part_mass,
+                                                 double* part_hmax, double tiny) {
+  const TileDev t = tiles[blockIdx.x];
+  const InstDev I = inst[t.inst];
+  const Cell q = cell_of(t, I);
+  double m = 0., hm = 0.;
+  if (q.act) {
+    double h = U.h[q.c];
+    m = I.dx * I.dy * h;
+    if (h > tiny) hm = h;
+  }
+  __shared__ double sm[BLOCK / 64], sh[BLOCK / 64];
+  m = wave_sum(m);
+  hm = wave_max(hm);
+  if (threadIdx.x % 64 == 0) { sm[threadIdx.x / 64] = m; sh[threadIdx.x / 64] = hm; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < BLOCK / 64; k++) { m += sm[k]; hm = (sh[k] > hm) ? sh[k] : hm; }
+    part_mass[blockIdx.x] = m;
+    part_hmax[blockIdx.x] = hm;
+  }
+}
+
+// ---------------------------------------------------------------- adjoint --
+
+// Everything the Jacobian blocks need about one cell at U^s
+// (DualElem after update_dual_grid + update_states in calc_jacobian).
+struct DualInfo {
+  double h, hu, hv, g2;
+  double k;              // kact from the dual slopes (dual_grid_routines.C:68-84)
+  double dk;             // d kact / d phi_bed
+  double d[6];           // dual slopes hx, hux, hvx, hy, huy, hvy
+  double hsx[3], hsy[3]; // hslope_sens(dir, self/minus/plus)
+  SrcInfo si;
+};
+
+__device__ __forceinline__ DualInfo dual_info(const AdjArgs& a, const InstDev& I, int i, int j,
+                                              bool want_dk) {
+  DualInfo r;
+  const long long nx = I.nx, c = I.off + (long long)j * nx + i;
+  r.h = a.U.h[c]; r.hu = a.U.hu[c]; r.hv = a.U.hv[c]; r.g2 = a.S.gz[c];
+#pragma unroll
+  for (int k = 0; k < 6; k++) r.d[k] = 0.;
+#pragma unroll
+  for (int k = 0; k < 3; k++) r.hsx[k] = r.hsy[k] = 0.;
+  if (i + 1 < I.nx && i > 0 && j + 1 < I.ny && j > 0) {
+    const double ix = 1 / I.dx, isx = 1 / (I.dx + I.dx);
+    const double iy = 1 / I.dy, isy = 1 / (I.dy + I.dy);
+    DualSlope s = slope_dual(a.U.h[c + 1], r.h, a.U.h[c - 1], I.dx, ix, isx, a.gamma, a.tiny, true);
+    r.d[0] = s.d; r.hsx[0] = s.s0; r.hsx[1] = s.s1; r.hsx[2] = s.s2;
+    r.d[1] = slope_dual(a.U.hu[c + 1], r.hu, a.U.hu[c - 1], I.dx, ix, isx, a.gamma, a.tiny, false).d;
+    r.d[2] = slope_dual(a.U.hv[c + 1], r.hv, a.U.hv[c - 1], I.dx, ix, isx, a.gamma, a.tiny, false).d;
+    s = slope_dual(a.U.h[c + nx], r.h, a.U.h[c - nx], I.dy, iy, isy, a.gamma, a.tiny, true);
+    r.d[3] = s.d; r.hsy[0] = s.s0; r.hsy[1] = s.s1; r.hsy[2] = s.s2;
+    r.d[4] = slope_dual(a.U.hu[c + nx], r.hu, a.U.hu[c - nx], I.dy, iy, isy, a.gamma, a.tiny, false).d;
+    r.d[5] = slope_dual(a.U.hv[c + nx], r.hv, a.U.hv[c - nx], I.dy, iy, isy, a.gamma, a.tiny, false).d;
+  }
+  r.k = getcoef(r.h, r.hu, r.hv, r.d[0], r.d[1], r.d[3], r.d[5], I.kc1, I.kroot, I.eps, a.tiny);
+  r.dk = want_dk ? dgetcoef_dphi(r.h, r.hu, r.hv, r.d[0], r.d[1], r.d[3], r.d[5], I.kc1, I.kroot,
+                                 I.dkroot, I.eps, a.tiny)
+                 : 0.;
+  r.si = src_switches(r.h, r.hu, r.hv, r.d[0], r.d[2], r.d[3], r.d[4], a.S.kx[c], a.S.ky[c],
+                      I.tan_bed, r.g2, a.ft, a.tiny);
+  return r;
+}
+
+// J = dtdx*(px - nx) + dtdy*(py - ny), calc_jacobian_elem
+// (compute_jacobian_elem.C:21-24); null operands are ZERO_MATRIX.
+__device__ __forceinline__ M3 flux_block(const M3* px, const M3* nxm, const M3* py, const M3* nym,
+                                         double dtdx, double dtdy) {
+  M3 J;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double a = px ? px->a[i][j] : 0., b = nxm ? nxm->a[i][j] : 0.;
+      double c = py ? py->a[i][j] : 0., d = nym ? nym->a[i][j] : 0.;
+      J.a[i][j] = +dtdx * (a - b) + dtdy * (c - d);
+    }
+  return J;
+}
+
+// Source-term part of a neighbour block (compute_jacobian_elem.C:44-51).
+__device__ __forceinline__ void nbr_source(M3& J, const DualInfo& n, double sin_int, double dt,
+                                           double dh0, double dh1, double tiny) {
+  if (n.h > tiny) {
+    double alpha = sin_int * n.si.sgn[0] * n.k;
+    double betta = sin_int * n.si.sgn[1] * n.k;
+    if (!n.si.stop[0]) J.a[1][0] -= -dt * n.h * alpha * n.g2 * dh1;
+    if (!n.si.stop[1]) J.a[2][0] -= -dt * n.h * betta * n.g2 * dh0;
+  }
+}
+
+__device__ __forceinline__ void accum_T(double adj[3], const M3& J, const double lam[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) adj[k] += lam[l] * J.a[l][k];
+}
+
+// d(flux)/dk for the forward physical flux of one side
+__device__ __forceinline__ void dhfv_dk(int dir, double h, double k, double g3, double tiny,
+                                        double dF[3], double dW[3]) {
+  dF[0] = dF[1] = dF[2] = dW[0] = dW[1] = dW[2] = 0.;
+  if (h < tiny) return;
+  double a = sqrt(k * h * g3);
+  double da = (a > 0.) ? (h * g3) / (2. * a) : 0.;
+  if (dir == 0) dF[1] = 0.5 * h * g3 * h;
+  else dF[2] = 0.5 * h * g3 * h;
+  dW[0] = -da;
+  dW[2] = da;
+}
+
+__device__ __forceinline__ void driemann(const Hfv& l, const Hfv& r, const double dFl[3],
+                                         const double dWl[3], const double dFr[3],
+                                         const double dWr[3], double out[3]) {
+  out[0] = out[1] = out[2] = 0.;
+  if (l.u[0] == 0. && r.u[0] == 0.) return;
+  double sl, sr, dsl, dsr;
+  if (l.u[0] == 0.) {
+    double p = 2.0 * r.w[0] - r.w[1], q = 2.0 * r.w[2] - r.w[1];
+    sl = umin(0, p); dsl = (0 > p) ? 2.0 * dWr[0] - dWr[1] : 0.;
+    sr = umax(0, q); dsr = (0 > q) ? 0. : 2.0 * dWr[2] - dWr[1];
+    if (!(sr > 0)) dsr = 0.;
+  } else if (r.u[0] == 0.) {
+    double p = 2.0 * l.w[0] - l.w[1], q = 2.0 * l.w[2] - l.w[1];
+    sl = umin(0, p); dsl = (0 > p) ? 2.0 * dWl[0] - dWl[1] : 0.;
+    sr = umax(0, q); dsr = (0 > q) ? 0. : 2.0 * dWl[2] - dWl[1];
+    if (!(sr > 0)) dsr = 0.;
+  } else {
+    double mi = umin(l.w[0], r.w[0]);
+    double dmi = (l.w[0] > r.w[0]) ? dWr[0] : dWl[0];
+    sl = umin(0, mi); dsl = (0 > mi) ? dmi : 0.;
+    double ma = umax(l.w[2], r.w[2]);
+    double dma = (l.w[2] > r.w[2]) ? dWl[2] : dWr[2];
+    sr = umax(0, ma); dsr = (0 > ma) ? 0. : dma;
+    if (!(sr > 0)) dsr = 0.;
+  }
+  if (sl >= 0.0) {
+    for (int v = 0; v < 3; v++) out[v] = dFl[v];
+  } else if (sr <= 0.0) {
+    for (int v = 0; v < 3; v++) out[v] = dFr[v];
+  } else {
+    double inv = 1. / (sr - sl);
+    for (int v = 0; v < 3; v++) {
+      double vd = r.u[v] - l.u[v];
+      double f = (sr * l.f[v] - sl * r.f[v] + sl * sr * vd) * inv;
+      double dfdsr = (l.f[v] + sl * vd - f) * inv;
+      double dfdsl = (-r.f[v] + sr * vd + f) * inv;
+      out[v] = dfdsr * dsr + dfdsl * dsl + sr * inv * dFl[v] - sl * inv * dFr[v];
+    }
+  }
+}
+
+// lambda^{s+1}_c . d U^{s+1}_c / d phi_bed at frozen dt (bed-friction
+// gradient; no reference counterpart).  ci[0..4] = self, E, N, W, S.
+__device__ double dphi_term(const AdjArgs& a, const InstDev& I, const Cell& q,
+                            const DualInfo ci[5], const bool has[5], double dt,
+                            const double lam[3]) {
+  // faces E, N, W, S: (left, right) cell slots, with outflow copies
+  double flux[4][3], dflux[4][3];
+  for (int f = 0; f < 4; f++) {
+    const int side = f & 1;
+    int l, r;
+    bool ok = true;
+    if (f == 0) { if (has[1]) { l = 0; r = 1; } else if (has[3]) { l = 3; r = 0; } else ok = false; }
+    else if (f == 1) { if (has[2]) { l = 0; r = 2; } else if (has[4]) { l = 4; r = 0; } else ok = false; }
+    else if (f == 2) { if (has[3]) { l = 3; r = 0; } else if (has[1]) { l = 0; r = 1; } else ok = false; }
+    else { if (has[4]) { l = 4; r = 0; } else if (has[2]) { l = 0; r = 2; } else ok = false; }
+    if (!ok) {
+      for (int v = 0; v < 3; v++) flux[f][v] = dflux[f][v] = 0.;
+      continue;
+    }
+    const DualInfo& L = ci[l];
+    const DualInfo& R = ci[r];
+    Hfv hl = hfv_fwd(side, L.h, L.hu, L.hv, L.k, L.g2, a.tiny);
+    Hfv hr = hfv_fwd(side, R.h, R.hu, R.hv, R.k, R.g2, a.tiny);
+    riemann(hl, hr, flux[f]);
+    double dFl[3], dWl[3], dFr[3], dWr[3];
+    dhfv_dk(side, L.h, L.k, L.g2, a.tiny, dFl, dWl);
+    dhfv_dk(side, R.h, R.k, R.g2, a.tiny, dFr, dWr);
+    for (int v = 0; v < 3; v++) {
+      dFl[v] *= L.dk; dWl[v] *= L.dk; dFr[v] *= R.dk; dWr[v] *= R.dk;
+    }
+    driemann(hl, hr, dFl, dWl, dFr, dWr, dflux[f]);
+  }
+  const double dtdx = dt / I.dx, dtdy = dt / I.dy;
+  double out[3];
+  for (int v = 0; v < 3; v++)
+    out[v] = -dtdx * (dflux[0][v] - dflux[2][v]) - dtdy * (dflux[1][v] - dflux[3][v]);
+  const DualInfo& C = ci[0];
+  if (C.h - dtdx * (flux[0][0] - flux[2][0]) - dtdy * (flux[1][0] - flux[3][0]) < 0.) out[0] = 0.;
+  if (C.h > a.tiny) {
+    const long long c = q.c;
+    const double sec2 = 1. + I.tan_bed * I.tan_bed;
+    const double vx = C.hu / C.h, vy = C.hv / C.h;
+    const double g2 = C.g2, dg0 = a.S.dgx[c], dg1 = a.S.dgy[c], k0 = a.S.kx[c], k1 = a.S.ky[c];
+    double ds2x = C.si.sgn[0] * C.h * C.dk * (g2 * C.d[3] + dg1 * C.h) * I.sin_int;
+    double ds3x = C.si.sgn[2] * umax(g2 * C.h + vx * C.hu * k0, 0.0) * sec2;
+    double ds2y = C.si.sgn[1] * C.h * C.dk * (g2 * C.d[0] + dg0 * C.h) * I.sin_int;
+    double ds3y = C.si.sgn[3] * umax(g2 * C.h + vy * C.hv * k1, 0.0) * sec2;
+    out[1] += dt * (-ds2x - ds3x);
+    out[2] += dt * (-ds2y - ds3y);
+  }
+  return lam[0] * out[0] + lam[1] * out[1] + lam[2] * out[2];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
+  const TileDev t = a.tiles[blockIdx.x];
+  const InstDev I = a.inst[t.inst];
+  const Cell q = cell_of(t, I);
+  double dphi = 0.;
+  const bool want_dphi = a.dphi_part != nullptr;
+  if (q.act) {
+    const long long c = q.c, nx = I.nx;
+    const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
+    const double dtdx = dt / I.dx, dtdy = dt / I.dy;
+    const double hc = a.U.h[c];
+    double adj[3] = {0., 0., 0.};
+    // slots: 0 self, 1 E, 2 N, 3 W, 4 S
+    const bool has[5] = {true, q.hasE, q.hasN, q.hasW, q.hasS};
+    const long long nb[5] = {c, c + 1, c + nx, c - 1, c - nx};
+    const int ni[5] = {q.i, q.i + 1, q.i, q.i - 1, q.i};
+    const int nj[5] = {q.j, q.j, q.j + 1, q.j, q.j - 1};
+    DualInfo ci[5];
+    for (int s = 0; s < 5; s++)
+      if (has[s]) ci[s] = dual_info(a, I, ni[s], nj[s], want_dphi);
+    if (hc != 0.) {  // a void cell receives nothing (calc_jacobain.C:117-122)
+      // c-side Jacobians of the four faces (dual_riemannflux)
+      const DualHfv cx = dual_hfv(0, ci[0].h, ci[0].hu, ci[0].hv, ci[0].k, ci[0].g2, a.tiny);
+      const DualHfv cy = dual_hfv(1, ci[0].h, ci[0].hu, ci[0].hv, ci[0].k, ci[0].g2, a.tiny);
+      M3 JE = zero3(), JN = zero3(), JW = zero3(), JS = zero3();
+      if (has[1]) JE = dual_riemann_side(cx, dual_hfv(0, ci[1].h, ci[1].hu, ci[1].hv, ci[1].k, ci[1].g2, a.tiny), true);
+      if (has[2]) JN = dual_riemann_side(cy, dual_hfv(1, ci[2].h, ci[2].hu, ci[2].hv, ci[2].k, ci[2].g2, a.tiny), true);
+      if (has[3]) JW = dual_riemann_side(dual_hfv(0, ci[3].h, ci[3].hu, ci[3].hv, ci[3].k, ci[3].g2, a.tiny), cx, false);
+      if (has[4]) JS = dual_riemann_side(dual_hfv(1, ci[4].h, ci[4].hu, ci[4].hv, ci[4].k, ci[4].g2, a.tiny), cy, false);
+
+      // ---- self block (effelement 0): fjc(.,.,0) with boundary copies
+      {
+        const M3* pE = has[1] ? &JE : (has[3] ? &JW : nullptr);
+        const M3* nW = has[3] ? &JW : (has[1] ? &JE : nullptr);
+        const M3* pN = has[2] ? &JN : (has[4] ? &JS : nullptr);
+        const M3* nS = has[4] ? &JS : (has[2] ? &JN : nullptr);
+        M3 J = flux_block(pE, nW, pN, nS, dtdx, dtdy);
+        for (int i = 0; i < 3; i++) J.a[i][i] -= 1.;
+        const DualInfo& C = ci[0];
+        if (C.h > a.tiny) {
+          const double alpha = I.sin_int * C.si.sgn[0] * C.k;
+          const double betta = I.sin_int * C.si.sgn[1] * C.k;
+          const double vx = C.hu / C.h, vy = C.hv / C.h;
+          const double vx_sq = vx * vx, vy_sq = vy * vy, tb = I.tan_bed;
+          const double g0 = a.S.gx[c], g1 = a.S.gy[c], g2 = C.g2;
+          const double dg0 = a.S.dgx[c], dg1 = a.S.dgy[c], k0 = a.S.kx[c], k1 = a.S.ky[c];
+          const double dh0 = C.hsx[0], dh1 = C.hsy[0];
+          J.a[1][0] -= dt * (g0 - alpha * C.h * (2 * dg1 + g2 * dh1) - alpha * g2 * C.d[3]);
+          if (C.si.sgn[2] != 0. && !C.si.stop[0]) {
+            J.a[1][0] -= -dt * C.si.sgn[2] * tb * (g2 - vx_sq * k0);
+            J.a[1][1] -= -dt * C.si.sgn[2] * tb * (2 * vx * k0);
+            J.a[1][2] -= 0.;
+          }
+          J.a[2][0] -= dt * (g1 - betta * C.h * (2 * dg0 + g2 * dh0) - betta * g2 * C.d[0]);
+          if (C.si.sgn[3] != 0. && !C.si.stop[1]) {
+            J.a[2][0] -= -dt * C.si.sgn[3] * tb * (g2 - vy_sq * k1);
+            J.a[2][1] -= 0.;
+            J.a[2][2] -= -dt * C.si.sgn[3] * tb * (2 * vy * k1);
+          }
+        }
+        const double lam[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
+        accum_T(adj, J, lam);
+      }
+      // ---- neighbour blocks in the reference's visiting order S, E, N, W
+      // (neighbor[0..3] with positive_x_side = 1).
+      const int order[4] = {4, 1, 2, 3};
+      for (int o = 0; o < 4; o++) {
+        const int s = order[o];
+        if (!has[s]) continue;
+        const DualInfo& N = ci[s];
+        const long long n = nb[s];
+        M3 J;
+        if (s == 1) {         // c is the xm neighbour of n
+          const bool bnd = !(q.i + 2 < I.nx);
+          J = flux_block(bnd ? &JE : nullptr, &JE, nullptr, nullptr, dtdx, dtdy);
+          nbr_source(J, N, I.sin_int, dt, N.hsx[1], 0., a.tiny);
+        } else if (s == 3) {  // c is the xp neighbour of n
+          const bool bnd = !(q.i - 2 >= 0);
+          J = flux_block(&JW, bnd ? &JW : nullptr, nullptr, nullptr, dtdx, dtdy);
+          nbr_source(J, N, I.sin_int, dt, N.hsx[2], 0., a.tiny);
+        } else if (s == 2) {  // c is the ym neighbour of n
+          const bool bnd = !(q.j + 2 < I.ny);
+          J = flux_block(nullptr, nullptr, bnd ? &JN : nullptr, &JN, dtdx, dtdy);
+          nbr_source(J, N, I.sin_int, dt, 0., N.hsy[1], a.tiny);
+        } else {              // c is the yp neighbour of n
+          const bool bnd = !(q.j - 2 >= 0);
+          J = flux_block(nullptr, nullptr, &JS, bnd ? &JS : nullptr, dtdx, dtdy);
+          nbr_source(J, N, I.sin_int, dt, 0., N.hsy[2], a.tiny);
+        }
+        const double lam[3] = {a.lam_next.h[n], a.lam_next.hu[n], a.lam_next.hv[n]};
+        accum_T(adj, J, lam);
+      }
+    }
+    double out[3];
+    for (int j = 0; j < 3; j++) {
+      double v = 0. - adj[j];
+      if (fabs(v) < 1e-16) v = 0.;  // calc_adjoint.C:150-152
+      out[j] = v;
+    }
+    a.lam_h[c] = out[0];
+    a.lam_hu[c] = out[1];
+    a.lam_hv[c] = out[2];
+    if (want_dphi) {
+      const double lam[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
+      dphi = dphi_term(a, I, q, ci, has, dt, lam);
+    }
+  }
+  if (want_dphi) {
+    __shared__ double sp[BLOCK / 64];
+    dphi = wave_sum(dphi);
+    if (threadIdx.x % 64 == 0) sp[threadIdx.x / 64] = dphi;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < BLOCK / 64; k++) dphi += sp[k];
+      a.dphi_part[blockIdx.x] += dphi;
+    }
+  }
+}
+
+}  // namespace
+
+void launch_coef(const StepArgs& a, bool reduce, hipStream_t s) {
+  hipLaunchKernelGGL(k_coef, dim3(a.ntiles), dim3(BLOCK), 0, s, a, reduce ? 1 : 0);
+}
+void launch_dt(const StepArgs& a, int ninst, hipStream_t s) {
+  hipLaunchKernelGGL(k_dt, dim3((ninst + 63) / 64), dim3(64), 0, s, a, ninst);
+}
+void launch_update(const StepArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_update, dim3(a.ntiles), dim3(BLOCK), 0, s, a);
+}
+void launch_adjoint(const AdjArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_adjoint, dim3(a.ntiles), dim3(BLOCK), 0, s, a);
+}
+void launch_reset_red(RedDev* red, int ninst, hipStream_t s) {
+  hipLaunchKernelGGL(k_reset_red, dim3((ninst + 63) / 64), dim3(64), 0, s, red, ninst);
+}
+void launch_stats(const InstDev* inst, const TileDev* tiles, int ntiles, Fields U,
+                  double* part_mass, double* part_hmax, double tiny, hipStream_t s) {
+  hipLaunchKernelGGL(k_stats, dim3(ntiles), dim3(BLOCK), 0, s, inst, tiles, U, part_mass,
+                     part_hmax, tiny);
+}
+
+}  // namespace tg

@@ -1,0 +1,171 @@
+"""Synthetic problem set-up (the host side of the reference's driver: the
+GIS raster, setup_geoflow's per-cell coefficients and init_piles), built
+through the C host helpers in libtg.so.
+
+A Problem carries everything tg_set_mesh / tg_set_state / tg_params need:
+uniform block geometry, FP64 static coefficients (SoA), the initial state and
+the material angles.  Units are unscaled (LENGTH = HEIGHT = GRAVITY scale 1,
+epsilon = 1), as in the benchmark configurations.
+"""
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+
+@dataclass
+class Problem:
+    name: str
+    nx: int
+    ny: int
+    dx: float
+    dy: float
+    x0: float
+    y0: float
+    grav: np.ndarray          # (3, n)
+    dgrav: np.ndarray         # (2, n)
+    curv: np.ndarray          # (2, n)
+    h: np.ndarray
+    hu: np.ndarray
+    hv: np.ndarray
+    phi_int_deg: float
+    phi_bed_deg: float
+    nsteps: int
+    checkpoint_every: int
+    box: tuple = None         # functional box (xmin, xmax, ymin, ymax)
+    epsilon: float = 1.0
+    gravity_scale: float = 1.0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return self.nx * self.ny
+
+    def box_weights(self, box=None):
+        box = box or self.box
+        lib = _lib.load()
+        w = np.zeros(3 * self.n)
+        rc = lib.tg_host_box_weights(self.nx, self.ny, self.dx, self.dy, self.x0, self.y0,
+                                     box[0], box[1], box[2], box[3], _lib.dptr(w))
+        assert rc == 0
+        return w
+
+
+def _dem(nrows, ncols, xmin, ymax, res, elev):
+    d = _lib.tg_dem()
+    d.nrows, d.ncols, d.xmin, d.ymax, d.res = nrows, ncols, xmin, ymax, res
+    d.elev = _lib.fptr(elev)
+    return d
+
+
+def static_coefs(dem_args, nx, ny, dx, dy, x0, y0):
+    lib = _lib.load()
+    n = nx * ny
+    grav = np.zeros(3 * n)
+    dgrav = np.zeros(2 * n)
+    curv = np.zeros(2 * n)
+    dem = _dem(*dem_args)
+    rc = lib.tg_host_static_coefs(C.byref(dem), nx, ny, dx, dy, x0, y0, 1.0, 1.0,
+                                  _lib.dptr(grav), _lib.dptr(dgrav), _lib.dptr(curv))
+    if rc != 0:
+        raise ValueError(f"tg_host_static_coefs failed ({rc})")
+    return grav.reshape(3, n), dgrav.reshape(2, n), curv.reshape(2, n)
+
+
+def piles(nx, ny, dx, dy, x0, y0, specs):
+    """specs: list of (xc, yc, major, minor, rot_deg, height)."""
+    lib = _lib.load()
+    n = nx * ny
+    a = np.array(specs, dtype=np.float64).reshape(-1, 6)
+    cols = [np.ascontiguousarray(a[:, k]) for k in range(6)]
+    h, hu, hv = np.zeros(n), np.zeros(n), np.zeros(n)
+    rc = lib.tg_host_piles(nx, ny, dx, dy, x0, y0, len(specs), *[_lib.dptr(c) for c in cols],
+                           _lib.dptr(h), _lib.dptr(hu), _lib.dptr(hv))
+    assert rc == 0
+    return h, hu, hv
+
+
+def paraboloid_radius_for_volume(volume, aspect=0.1):
+    """Circular paraboloid with height = aspect * radius: V = pi/2 r^2 H."""
+    return (2.0 * volume / (math.pi * aspect)) ** (1.0 / 3.0)
+
+
+def fractal_dem(n=1025, relief=800.0, H=0.8, seed=1):
+    lib = _lib.load()
+    elev = np.zeros(n * n, dtype=np.float32)
+    rc = lib.tg_host_dem_diamond_square(n, relief, H, seed, _lib.fptr(elev))
+    assert rc == 0
+    return elev
+
+
+def plane_runout_problem(n=128, dx=10.0, nsteps=200):
+    """Uniform n x n grid of dx cells on an analytic DEM: a 35-degree plane
+    over 60% of the x extent with a C1 blend (100 m) into a flat runout; one
+    paraboloid pile 150 x 100 m, 30 m high, centred on the slope; phi_int 30,
+    phi_bed 20 degrees; functional box 200 x 200 m on the runout."""
+    L = n * dx
+    res = dx / 2.0
+    margin = 4 * res
+    xmin, ymax = -margin, L + margin
+    ncols = nrows = int(round((L + 2 * margin) / res)) + 1
+    elev = np.zeros(nrows * ncols, dtype=np.float32)
+    lib = _lib.load()
+    rc = lib.tg_host_dem_plane_runout(nrows, ncols, xmin, ymax, res, 35.0, 0.6 * L, 100.0,
+                                      _lib.fptr(elev))
+    assert rc == 0
+    x0 = y0 = 0.5 * dx
+    grav, dgrav, curv = static_coefs((nrows, ncols, xmin, ymax, res, elev), n, n, dx, dx, x0, y0)
+    h, hu, hv = piles(n, n, dx, dx, x0, y0, [(0.3 * L, 0.5 * L, 150.0, 100.0, 0.0, 30.0)])
+    xb = 0.6 * L + 100.0
+    box = (xb, xb + 200.0, 0.5 * L - 100.0, 0.5 * L + 100.0)
+    return Problem("plane_runout_%d" % n, n, n, dx, dx, x0, y0, grav, dgrav, curv, h, hu, hv,
+                   30.0, 20.0, nsteps, 50, box=box)
+
+
+def fractal_problem(n=512, dx=5.0, nsteps=1000, npiles=1, seed=3, dem_seed=1,
+                    volumes=None, phi_int=32.0, phi_bed=22.0, checkpoint_every=50):
+    """Uniform n x n grid on a window of a seeded 1025^2 diamond-square DEM
+    (5 m spacing, 800 m relief, H = 0.8); piles at seeded positions."""
+    dem_n, dem_res = 1025, 5.0
+    elev = fractal_dem(dem_n, 800.0, 0.8, dem_seed)
+    span = (dem_n - 1) * dem_res            # 5120 m
+    L = n * dx
+    if L > span:
+        raise ValueError("domain larger than the DEM")
+    off = 0.5 * (span - L)
+    x0 = y0 = off + 0.5 * dx
+    grav, dgrav, curv = static_coefs((dem_n, dem_n, 0.0, span, dem_res, elev), n, n, dx, dx,
+                                     x0, y0)
+    rng = np.random.default_rng(seed)
+    if volumes is None:
+        volumes = list(np.exp(rng.uniform(math.log(1e5), math.log(1e6), npiles)))
+    specs = []
+    for v in volumes:
+        r = paraboloid_radius_for_volume(v)
+        cx = off + rng.uniform(0.3, 0.7) * L
+        cy = off + rng.uniform(0.3, 0.7) * L
+        specs.append((cx, cy, r, r, 0.0, 0.1 * r))
+    h, hu, hv = piles(n, n, dx, dx, x0, y0, specs)
+    p = Problem("fractal_%d" % n, n, n, dx, dx, x0, y0, grav, dgrav, curv, h, hu, hv,
+                phi_int, phi_bed, nsteps, checkpoint_every)
+    p.meta["piles"] = specs
+    p.meta["seed"] = seed
+    return p
+
+
+def target_box(problem, h_final, cells=20, seed=5):
+    """Seeded cells x cells target region placed on the final deposit: the
+    window centre is a seeded draw among cells with h above 10% of max."""
+    rng = np.random.default_rng(seed)
+    hh = h_final.reshape(problem.ny, problem.nx)
+    cand = np.argwhere(hh > 0.1 * hh.max())
+    j, i = cand[rng.integers(len(cand))]
+    half = cells // 2
+    i0 = int(min(max(i - half, 0), problem.nx - cells))
+    j0 = int(min(max(j - half, 0), problem.ny - cells))
+    xa = problem.x0 + (i0 - 0.5) * problem.dx
+    ya = problem.y0 + (j0 - 0.5) * problem.dy
+    return (xa, xa + cells * problem.dx, ya, ya + cells * problem.dy)

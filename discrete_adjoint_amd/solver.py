@@ -1,0 +1,166 @@
+"""Host-side interface mirroring the reference's entry points for the hot
+path, over the C ABI (include/tg.h):
+
+  Simulation.step()        <- step()            src/geoflow/step.C:29
+  Simulation.run(n)        <- hpfem.C:222-300 time loop (no adaptation)
+  Simulation.dual_solver() <- dual_solver()     src/adjoint/dual_solver.C:80
+  Simulation.adjoint_step  <- calc_jacobian + calc_adjoint of one reverse step
+
+Errors: the reference prints and exits; here every non-zero tg_status raises
+TgError carrying the library's message.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+
+
+class TgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tg error {code}: {msg}")
+        self.code = code
+
+
+def default_config(**kw):
+    lib = _lib.load()
+    cfg = _lib.tg_config()
+    lib.tg_config_default(C.byref(cfg))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def params_from_angles(phi_int_deg, phi_bed_deg, epsilon=1.0, gravity_scale=1.0):
+    lib = _lib.load()
+    p = _lib.tg_params()
+    rc = lib.tg_params_from_angles(math.radians(phi_int_deg), math.radians(phi_bed_deg),
+                                   epsilon, gravity_scale, C.byref(p))
+    if rc:
+        raise TgError(rc, "tg_params_from_angles")
+    return p
+
+
+class Simulation:
+    """One uniform-block problem on one GPU."""
+
+    def __init__(self, problem, precision=_lib.TG_FP64, checkpoint_every=None, max_steps=None,
+                 device=0, **cfg_overrides):
+        self.lib = _lib.load()
+        self.problem = problem
+        ck = checkpoint_every or problem.checkpoint_every
+        ms = max_steps or max(problem.nsteps, 1)
+        self.cfg = default_config(precision=precision, checkpoint_every=ck, max_steps=ms,
+                                  device=device, **cfg_overrides)
+        self.ctx = C.c_void_p()
+        self._check(self.lib.tg_create(C.byref(self.cfg), C.byref(self.ctx)), nocontext=True)
+        self._arrays = []
+        m = _lib.tg_mesh()
+        m.kind = _lib.TG_MESH_UNIFORM
+        m.nx, m.ny, m.dx, m.dy, m.x0, m.y0 = (problem.nx, problem.ny, problem.dx, problem.dy,
+                                              problem.x0, problem.y0)
+        g = np.ascontiguousarray(problem.grav.reshape(-1))
+        dg = np.ascontiguousarray(problem.dgrav.reshape(-1))
+        cv = np.ascontiguousarray(problem.curv.reshape(-1))
+        self._arrays += [g, dg, cv]
+        m.grav, m.dgrav, m.curv = _lib.dptr(g), _lib.dptr(dg), _lib.dptr(cv)
+        self._check(self.lib.tg_set_mesh(self.ctx, C.byref(m)))
+        self.set_params(problem.phi_int_deg, problem.phi_bed_deg)
+        self.set_state(problem.h, problem.hu, problem.hv)
+
+    # -- plumbing -------------------------------------------------------------
+    def _check(self, rc, nocontext=False):
+        if rc != _lib.TG_OK:
+            msg = "" if nocontext else self.lib.tg_last_error(self.ctx).decode()
+            raise TgError(rc, msg)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.tg_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state ------------------------------------------------------------------
+    def set_params(self, phi_int_deg, phi_bed_deg):
+        p = self.problem
+        self.params = params_from_angles(phi_int_deg, phi_bed_deg, p.epsilon, p.gravity_scale)
+
+    def set_state(self, h, hu, hv):
+        h, hu, hv = (np.ascontiguousarray(a, dtype=np.float64) for a in (h, hu, hv))
+        self._check(self.lib.tg_set_state(self.ctx, _lib.dptr(h), _lib.dptr(hu), _lib.dptr(hv)))
+
+    def get_state(self):
+        n = self.problem.n
+        h, hu, hv = np.empty(n), np.empty(n), np.empty(n)
+        self._check(self.lib.tg_get_state(self.ctx, _lib.dptr(h), _lib.dptr(hu), _lib.dptr(hv)))
+        return h, hu, hv
+
+    # -- forward ----------------------------------------------------------------
+    def step(self):
+        st = _lib.tg_stats()
+        self._check(self.lib.tg_forward_step(self.ctx, C.byref(self.params), C.byref(st)))
+        return st
+
+    def run(self, nsteps=None):
+        st = _lib.tg_stats()
+        n = self.problem.nsteps if nsteps is None else nsteps
+        self._check(self.lib.tg_forward_run(self.ctx, C.byref(self.params), n, C.byref(st)))
+        return st
+
+    def replay(self, dt_seq):
+        dt = np.ascontiguousarray(dt_seq, dtype=np.float64)
+        st = _lib.tg_stats()
+        self._check(self.lib.tg_forward_replay(self.ctx, C.byref(self.params), len(dt),
+                                               _lib.dptr(dt), C.byref(st)))
+        return st
+
+    def dt_record(self):
+        cnt = C.c_int()
+        self._check(self.lib.tg_get_dt_record(self.ctx, None, 0, C.byref(cnt)))
+        out = np.empty(max(cnt.value, 1))
+        self._check(self.lib.tg_get_dt_record(self.ctx, _lib.dptr(out), cnt.value, C.byref(cnt)))
+        return out[: cnt.value]
+
+    # -- adjoint ----------------------------------------------------------------
+    def dual_solver(self, terminal_weight, want_dphi=False):
+        """Reverse sweep over the last run: returns (lambda0 (3,n), dJ/dphi_bed)."""
+        w = np.ascontiguousarray(terminal_weight, dtype=np.float64).reshape(-1)
+        fn = _lib.tg_functional()
+        fn.terminal_weight = _lib.dptr(w)
+        lam = np.empty(3 * self.problem.n)
+        dphi = C.c_double(0.0)
+        self._check(self.lib.tg_adjoint_run(self.ctx, C.byref(self.params), C.byref(fn),
+                                            _lib.dptr(lam),
+                                            C.cast(C.byref(dphi), _lib._dp) if want_dphi
+                                            else None))
+        return lam.reshape(3, -1), (dphi.value if want_dphi else None)
+
+    def adjoint_step(self, s, dt, lam_next):
+        ln = np.ascontiguousarray(lam_next, dtype=np.float64).reshape(-1)
+        out = np.empty_like(ln)
+        self._check(self.lib.tg_adjoint_step(self.ctx, C.byref(self.params), s, dt,
+                                             _lib.dptr(ln), _lib.dptr(out)))
+        return out.reshape(3, -1)
+
+    def time_kernel(self, which, reps=20):
+        """Average launch time (ms) of one kernel: 0 coef, 1 update, 2 adjoint."""
+        ms = C.c_double()
+        self._check(self.lib.tg_time_kernel(self.ctx, C.byref(self.params), which, reps,
+                                            C.cast(C.byref(ms), _lib._dp)))
+        return ms.value
+
+    def timing(self):
+        f, a, r = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.lib.tg_last_timing(self.ctx, C.cast(C.byref(f), _lib._dp),
+                                            C.cast(C.byref(a), _lib._dp),
+                                            C.cast(C.byref(r), _lib._dp)))
+        return f.value, a.value, r.value

@@ -1,0 +1,189 @@
+/*
+ * tg.h -- C ABI of the MI355X granular-flow hot path (forward finite-volume
+ * step + discrete-adjoint reverse sweep).  Plain C types only.
+ *
+ * Every entry point replaces a pass body of the reference (cited per call).
+ * Conventions (the reference has no status codes; these replace its
+ * printf+exit/assert error handling, residual.C:137-147):
+ *   - every call returns TG_OK (0) or a negative tg_status;
+ *   - one host thread per context; calls are ordered on the context's
+ *     stream; getters synchronise;
+ *   - host-side arrays are FP64 regardless of the internal precision;
+ *   - transcendentals of the material parameters are evaluated on the host
+ *     (tg_params_from_angles) so the device never re-derives them.
+ */
+#ifndef TG_H
+#define TG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tg_status {
+  TG_OK = 0,
+  TG_ERR_ARG = -1,          /* bad argument / size                           */
+  TG_ERR_HIP = -2,          /* HIP runtime error                             */
+  TG_ERR_STATE = -3,        /* call out of order (no mesh, no forward run)   */
+  TG_ERR_NOMEM = -4,        /* device allocation failed                      */
+  TG_ERR_NONFINITE = -5,    /* NaN/Inf in the state (residual.C:137-147)     */
+  TG_ERR_EIGEN = -6,        /* eigenvalue > 1e9 (get_coef_and_eigen.C:144)   */
+  TG_ERR_UNSUPPORTED = -7,  /* mesh kind / mode not built                    */
+  TG_ERR_NODEVICE = -8      /* no HIP device                                 */
+} tg_status;
+
+typedef enum tg_precision {
+  TG_FP64 = 0,   /* every value FP64: the accuracy reference                  */
+  TG_MIXED = 1   /* FP32 state and fluxes; FP64 thin-layer velocity recovery,
+                    CFL reduction, mass and adjoint sums                     */
+} tg_precision;
+
+/* Context configuration.  tg_config_default() fills the reference's values. */
+typedef struct tg_config {
+  int precision;          /* tg_precision                                      */
+  int checkpoint_every;   /* k: forward states stored every k steps (>= 1)     */
+  int max_steps;          /* capacity of the dt[] record                      */
+  int device;             /* HIP device ordinal                                */
+  double tiny;            /* GEOFLOW_TINY = 1e-4 (constant.h:68)               */
+  double gamma;           /* slope limiter gamma = 1 (hpfem.C:82)              */
+  double cfl_inner;       /* 0.5 (get_coef_and_eigen.C:168)                    */
+  double cfl_outer;       /* 0.5 (get_coef_and_eigen.C:175)                    */
+  double frict_tiny_early;/* 0.1  (hpfem.C:232-235)                            */
+  double frict_tiny_late; /* 1e-9 (hpfem.C:232-235)                            */
+  int frict_tiny_switch;  /* steps 0..49 use the early value (hpfem.C:232,
+                             calc_jacobain.C:81-84)                            */
+  int tile_x, tile_y;     /* 0 = use the autotuned / built-in tile shape       */
+} tg_config;
+
+/* Mesh.  TG_MESH_UNIFORM: an nx*ny block of equal cells; cell c = j*nx + i,
+ * centre (x0 + i*dx, y0 + j*dy); cells on the outer ring have the domain
+ * boundary (neigh_proc == INIT) on their outer sides.  Static per-cell
+ * coefficients (element2.C:3545-3703) are FP64 SoA arrays of n = nx*ny.
+ * TG_MESH_LEAVES (adaptive, flattened quadtree tables) is declared for the
+ * adaptive path and returns TG_ERR_UNSUPPORTED until that path is built. */
+typedef enum tg_mesh_kind { TG_MESH_UNIFORM = 0, TG_MESH_LEAVES = 1 } tg_mesh_kind;
+
+typedef struct tg_mesh {
+  int kind;
+  int nx, ny;
+  double dx, dy;
+  double x0, y0;
+  const double* grav;   /* [3*n]: gravity x | y | z  (calc_gravity_vector)   */
+  const double* dgrav;  /* [2*n]: d gravity_z / dx | dy (calc_d_gravity)     */
+  const double* curv;   /* [2*n]: curvature x | y (calc_topo_data)           */
+} tg_mesh;
+
+/* Material and time parameters of one run (MatProps, TimeProps). */
+typedef struct tg_params {
+  double sin_int;        /* sin(phi_int)           residual.C:208            */
+  double tan_bed;        /* tan(phi_bed)           residual.C:213            */
+  double kact_c1;        /* 2/cos^2(phi_int)       getcoef.f:40              */
+  double kact_root;      /* sqrt|1-(1+tan^2 phi_bed) cos^2 phi_int| getcoef.f:41 */
+  double dkroot_dphi;    /* d kact_root / d phi_bed (bed-friction gradient)  */
+  double epsilon;        /* HEIGHT_SCALE / LENGTH_SCALE                      */
+  double gravity_scale;  /* GRAVITY_SCALE                                    */
+  double maxtime;        /* non-dimensional end time (incrtime clamp)        */
+  double nextoutput;     /* next output time (incrtime clamp); +inf = none   */
+  double nextsave;       /* next save time (incrtime clamp); +inf = none     */
+} tg_params;
+
+/* Mass/height statistics of the last completed step (step.C:205 realvolume,
+ * get_coef_and_eigen.C:102-104 hmax); FP64 sums on the device. */
+typedef struct tg_stats {
+  double dt;       /* dt taken by the step                                   */
+  double time;     /* simulated non-dimensional time after the step          */
+  double mass;     /* sum dx*dy*h over all cells after the step              */
+  double hmax;     /* max h over wet cells after the step                    */
+  int step;        /* number of steps completed                              */
+} tg_stats;
+
+/* Functional J = sum_cells w . U^N (terminal weights, FP64 SoA [3*n]); the
+ * reference's per-step func_sens vector at the final step (calc_adjoint.C:93-97). */
+typedef struct tg_functional {
+  const double* terminal_weight;
+} tg_functional;
+
+typedef struct tg_ctx tg_ctx;
+
+/* ---- setup --------------------------------------------------------------- */
+
+/* Fills cfg with the reference's constants (constant.h, hpfem.C). */
+void tg_config_default(tg_config* cfg);
+
+/* MatProps setup: computes the host-side transcendentals once
+ * (MatProps ctor properties.h:578-608; getcoef.f:39-43).  Angles in radians. */
+int tg_params_from_angles(double phi_int, double phi_bed, double epsilon,
+                          double gravity_scale, tg_params* out);
+
+/* Creates a context on cfg->device (owns device buffers and stream). */
+int tg_create(const tg_config* cfg, tg_ctx** out);
+void tg_destroy(tg_ctx* ctx);
+const char* tg_last_error(const tg_ctx* ctx);
+
+/* Copies mesh tables and static coefficients to the device (replaces
+ * setup_geoflow's per-element data, setup_geoflow.C:24; called per re-mesh). */
+int tg_set_mesh(tg_ctx* ctx, const tg_mesh* mesh);
+
+/* State U = (h, hu, hv), FP64 host arrays of n (Element::state_vars). */
+int tg_set_state(tg_ctx* ctx, const double* h, const double* hu, const double* hv);
+int tg_get_state(tg_ctx* ctx, double* h, double* hu, double* hv);
+
+/* ---- forward ---------------------------------------------------------------- */
+
+/* One order-1 step: replaces step() step.C:29-275 (slopes, get_coef_and_eigen,
+ * incrtime, calc_edge_states, corrector).  Appends dt to the dt[] record. */
+int tg_forward_step(tg_ctx* ctx, const tg_params* prm, tg_stats* stats);
+
+/* nsteps steps (the hpfem.C:222-300 time loop without adaptation): stores a
+ * checkpoint of U every cfg->checkpoint_every steps and the dt[] record for
+ * the adjoint sweep.  Restarts the record (U at entry is checkpoint 0). */
+int tg_forward_run(tg_ctx* ctx, const tg_params* prm, int nsteps, tg_stats* stats);
+
+/* Same time loop with a prescribed dt sequence (frozen-dt replay, used for
+ * finite-difference checks of the adjoint; calc_jacobain.C:89 uses the
+ * recorded dt the same way). */
+int tg_forward_replay(tg_ctx* ctx, const tg_params* prm, int nsteps,
+                      const double* dt_seq, tg_stats* stats);
+
+/* Copies the dt[] record of the last run (TimeProps::dt). */
+int tg_get_dt_record(tg_ctx* ctx, double* dt_out, int capacity, int* count);
+
+/* ---- adjoint ---------------------------------------------------------------- */
+
+/* Reverse sweep over the last forward run: replaces dual_solver.C:183-257
+ * with calc_jacobian + calc_adjoint at every step (not only the MaxH step).
+ * Forward states are recomputed segment by segment from the checkpoints.
+ * lambda0_out: FP64 [3*n] adjoint at step 0 (= dJ/dU^0), may be NULL.
+ * dJ_dphibed_out: dJ/d phi_bed at frozen dt (may be NULL). */
+int tg_adjoint_run(tg_ctx* ctx, const tg_params* prm, const tg_functional* fn,
+                   double* lambda0_out, double* dJ_dphibed_out);
+
+/* One reverse step on the current state: given U^s (the context state) and
+ * lambda^{s+1}, writes lambda^s (FP64 SoA [3*n]); step index s selects
+ * frict_tiny; dt is the step's dt.  Mirrors calc_jacobian + calc_adjoint of
+ * one dual_solver iteration. */
+int tg_adjoint_step(tg_ctx* ctx, const tg_params* prm, int s, double dt,
+                    const double* lambda_next, double* lambda_out);
+
+/* ---- timing (HIP events around the context's kernels) ---------------------- */
+
+/* Kernel-only time of the last forward/adjoint run in milliseconds. */
+int tg_last_timing(tg_ctx* ctx, double* forward_ms, double* adjoint_ms,
+                   double* recompute_ms);
+
+/* Average duration of one launch of a single kernel, timed with HIP events
+ * on the context's stream over `reps` back-to-back launches on the current
+ * state (no reference counterpart: measurement only).  which: 0 = forward
+ * coefficient/CFL pass, 1 = forward update pass, 2 = adjoint gather.
+ * Requires a completed tg_forward_run (and tg_adjoint_run for which = 2). */
+typedef enum tg_kernel_id { TG_K_COEF = 0, TG_K_UPDATE = 1, TG_K_ADJOINT = 2 } tg_kernel_id;
+int tg_time_kernel(tg_ctx* ctx, const tg_params* prm, int which, int reps, double* avg_ms);
+
+/* Cells in the context's mesh (sum over instances). */
+int64_t tg_num_cells(const tg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,173 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the FP64 CPU restatement
+(oracle/titan_oracle.c).  Imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker; never by the product.
+"""
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_dp = C.POINTER(C.c_double)
+
+
+class or_mesh(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("x0", C.c_double), ("y0", C.c_double),
+                ("gx", _dp), ("gy", _dp), ("gz", _dp), ("dgx", _dp), ("dgy", _dp),
+                ("kx", _dp), ("ky", _dp)]
+
+
+class or_params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "sin_int", "tan_bed", "kact_c1", "kact_root", "dkroot_dphi", "epsilon", "gscale",
+        "tiny", "gamma", "cfl_inner", "cfl_outer", "maxtime", "nextoutput", "nextsave")]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_LIB = None
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = C.CDLL(LIB_PATH)
+        lib.or_step.restype = C.c_int
+        lib.or_step.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), C.c_double, _dp, _dp,
+                                _dp, C.c_double, _dp, _dp, _dp]
+        lib.or_compute_dt.restype = C.c_double
+        lib.or_compute_dt.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), _dp, _dp, _dp,
+                                      C.c_double]
+        lib.or_jacobian.restype = None
+        lib.or_jacobian.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), C.c_double, _dp,
+                                    _dp, _dp, C.c_double, _dp]
+        lib.or_adjoint_step.restype = None
+        lib.or_adjoint_step.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), C.c_double,
+                                        _dp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
+        lib.or_dstep_dphi.restype = None
+        lib.or_dstep_dphi.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), C.c_double, _dp,
+                                      _dp, _dp, C.c_double, _dp]
+        _LIB = lib
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def frict_tiny(s, switch=50, early=0.1, late=0.000000001):
+    """hpfem.C:232-235 schedule for step index s (iter before the step)."""
+    return early if s < switch else late
+
+
+class Oracle:
+    """FP64 CPU restatement bound to one uniform problem."""
+
+    def __init__(self, nx, ny, dx, dy, x0, y0, grav, dgrav, curv, phi_int_deg, phi_bed_deg,
+                 epsilon=1.0, gscale=1.0, tiny=1e-4, gamma=1.0):
+        self.lib = load()
+        self.n = nx * ny
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                      (grav[0], grav[1], grav[2], dgrav[0], dgrav[1], curv[0], curv[1])]
+        m = or_mesh()
+        m.nx, m.ny, m.dx, m.dy, m.x0, m.y0 = nx, ny, dx, dy, x0, y0
+        m.gx, m.gy, m.gz, m.dgx, m.dgy, m.kx, m.ky = [_p(a) for a in self._keep]
+        self.mesh = m
+        self.tiny, self.gamma, self.eps, self.gscale = tiny, gamma, epsilon, gscale
+        self.set_angles(phi_int_deg, phi_bed_deg)
+        self.time = 0.0
+
+    @classmethod
+    def from_problem(cls, pb):
+        return cls(pb.nx, pb.ny, pb.dx, pb.dy, pb.x0, pb.y0, pb.grav, pb.dgrav, pb.curv,
+                   pb.phi_int_deg, pb.phi_bed_deg, pb.epsilon, pb.gravity_scale)
+
+    def set_angles(self, phi_int_deg, phi_bed_deg):
+        """MatProps + getcoef.f:39-43 constants, evaluated with libm."""
+        pi_, pb_ = math.radians(phi_int_deg), math.radians(phi_bed_deg)
+        p = or_params()
+        cosphi, tandel = math.cos(pi_), math.tan(pb_)
+        p.sin_int = math.sin(pi_)
+        p.tan_bed = tandel
+        p.kact_c1 = 2.0 / (cosphi * cosphi)
+        w = 1.0 - (1.0 + tandel * tandel) * (cosphi * cosphi)
+        p.kact_root = math.sqrt(abs(w))
+        p.dkroot_dphi = (-(-1.0 if w < 0 else 1.0) * (cosphi * cosphi) * tandel *
+                         (1.0 + tandel * tandel) / p.kact_root) if p.kact_root > 0 else 0.0
+        p.epsilon, p.gscale, p.tiny, p.gamma = self.eps, self.gscale, self.tiny, self.gamma
+        p.cfl_inner = p.cfl_outer = 0.5
+        p.maxtime = p.nextoutput = p.nextsave = math.inf
+        self.params = p
+
+    def step(self, U, s, dt=None):
+        """One forward step in place on U = (h, hu, hv) (float64 arrays);
+        returns (dt, mass)."""
+        h, hu, hv = U
+        t = C.c_double(self.time)
+        dto, mass = C.c_double(), C.c_double()
+        rc = self.lib.or_step(C.byref(self.mesh), C.byref(self.params), frict_tiny(s), _p(h),
+                              _p(hu), _p(hv), -1.0 if dt is None else float(dt), C.byref(t),
+                              C.byref(dto), C.byref(mass))
+        if rc:
+            raise FloatingPointError("oracle: non-finite state")
+        self.time = t.value
+        return dto.value, mass.value
+
+    def run(self, U, nsteps, dt_seq=None):
+        dts = []
+        for s in range(nsteps):
+            dt, _ = self.step(U, s, None if dt_seq is None else dt_seq[s])
+            dts.append(dt)
+        return np.array(dts)
+
+    def compute_dt(self, U):
+        return self.lib.or_compute_dt(C.byref(self.mesh), C.byref(self.params), _p(U[0]),
+                                      _p(U[1]), _p(U[2]), self.time)
+
+    def jacobian(self, U, s, dt):
+        J = np.empty(self.n * 81)
+        self.lib.or_jacobian(C.byref(self.mesh), C.byref(self.params), frict_tiny(s), _p(U[0]),
+                             _p(U[1]), _p(U[2]), dt, _p(J))
+        return J.reshape(self.n, 9, 3, 3)
+
+    def adjoint_step(self, U, s, dt, lam_next, g=None, want_dphi=False):
+        out = np.empty(3 * self.n)
+        ln = np.ascontiguousarray(lam_next, dtype=np.float64).reshape(-1)
+        gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64).reshape(-1)
+        acc = C.c_double(0.0)
+        self.lib.or_adjoint_step(C.byref(self.mesh), C.byref(self.params), frict_tiny(s),
+                                 _p(U[0]), _p(U[1]), _p(U[2]), dt, _p(ln), _p(gg), _p(out),
+                                 C.cast(C.byref(acc), _dp) if want_dphi else None)
+        return out.reshape(3, -1), acc.value
+
+    def dstep_dphi(self, U, s, dt):
+        out = np.empty(3 * self.n)
+        self.lib.or_dstep_dphi(C.byref(self.mesh), C.byref(self.params), frict_tiny(s),
+                               _p(U[0]), _p(U[1]), _p(U[2]), dt, _p(out))
+        return out.reshape(3, -1)
+
+    def forward_adjoint(self, U0, nsteps, terminal_weight, dt_seq=None, want_dphi=False):
+        """Full store-everything forward + reverse sweep (SolRec semantics).
+        Returns (U_N, dt_seq, lambda0, dJ/dphi_bed)."""
+        U = [np.array(a, dtype=np.float64, copy=True) for a in U0]
+        traj = []
+        dts = []
+        for s in range(nsteps):
+            traj.append([a.copy() for a in U])
+            dt, _ = self.step(U, s, None if dt_seq is None else dt_seq[s])
+            dts.append(dt)
+        lam = np.ascontiguousarray(terminal_weight, dtype=np.float64).reshape(3, -1)
+        dphi = 0.0
+        for s in range(nsteps - 1, -1, -1):
+            lam, acc = self.adjoint_step(traj[s], s, dts[s], lam, want_dphi=want_dphi)
+            dphi += acc
+        return U, np.array(dts), lam, dphi
