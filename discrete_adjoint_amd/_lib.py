@@ -24,6 +24,7 @@ TG_ERR_NODEVICE = -8
 
 TG_FP64 = 0
 TG_MIXED = 1
+TG_FP32 = 2
 TG_MESH_UNIFORM = 0
 TG_MESH_LEAVES = 1
 
@@ -115,6 +116,16 @@ SIGNATURES = {
     "tg_set_mesh": (C.c_int, [C.c_void_p, C.POINTER(tg_mesh)]),
     "tg_set_state": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "tg_get_state": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "tg_set_ensemble": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(tg_mesh),
+                                  C.POINTER(tg_params)]),
+    "tg_num_members": (C.c_int, [C.c_void_p]),
+    "tg_member_cells": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64)]),
+    "tg_set_member_state": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "tg_get_member_state": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "tg_get_member_dt_record": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int,
+                                          C.POINTER(C.c_int)]),
+    "tg_get_member_dphi": (C.c_int, [C.c_void_p, _dp]),
     "tg_forward_step": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.POINTER(tg_stats)]),
     "tg_forward_run": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int,
                                  C.POINTER(tg_stats)]),

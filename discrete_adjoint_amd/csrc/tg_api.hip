@@ -1,5 +1,8 @@
 // C ABI of the hot path: context, mesh/state transfer, the forward time loop
 // with checkpoints, and the reverse (adjoint) sweep with segment recompute.
+// A context holds one or several uniform blocks ("members"); every kernel
+// launch covers all members (tile table), each member with its own material
+// parameters, CFL dt and time.
 #include "../../include/tg.h"
 #include "tg_kernels.h"
 
@@ -18,38 +21,37 @@ struct tg_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   bool have_mesh = false, have_state = false;
+  int mode = TG_FP64;
+  int es = 8;   // bytes per stored state / adjoint value
+  int ses = 8;  // bytes per static coefficient and kact value
 
-  // mesh (one instance)
+  // members
   std::vector<InstDev> inst_h;
   std::vector<TileDev> tiles_h;
   InstDev* inst_d = nullptr;
   TileDev* tiles_d = nullptr;
   int ninst = 0, ntiles = 0;
   long long ncell = 0;
-  double dx = 0, dy = 0;
 
-  // device fields
-  double* U[3] = {nullptr, nullptr, nullptr};
-  double* Ut[3] = {nullptr, nullptr, nullptr};
-  double* statics = nullptr;  // 7*n
-  double* kact = nullptr;
+  // device fields (element type double or float by es)
+  void* U[3] = {nullptr, nullptr, nullptr};
+  void* Ut[3] = {nullptr, nullptr, nullptr};
+  void* statics = nullptr;  // 7 * ncell
+  void* kact = nullptr;
+  void* lam[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   RedDev* red = nullptr;
-  double* dt_rec = nullptr;
-  double* time_d = nullptr;
+  double* dt_rec = nullptr;  // ninst * max_steps
+  double* time_d = nullptr;  // ninst
   int* err_d = nullptr;
   double* part_mass = nullptr;
   double* part_hmax = nullptr;
   double* dphi_part = nullptr;
-  double* lam[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-  double* wterm = nullptr;  // terminal weights 3n
 
   // forward record
-  int rec_steps = 0;        // steps in the dt record
-  int run_steps = 0;        // steps of the last tg_forward_run/replay (0 = none)
-  int run_start_step = 0;   // dt_rec index of the run's first step
-  std::vector<double*> ckpt;
-  std::vector<double*> seg;
-  double time_h = 0.;
+  int rec_steps = 0;  // steps in the dt record
+  int run_steps = 0;  // steps of the last forward run/replay (0 = none)
+  std::vector<void*> ckpt, seg;
+  std::vector<double> member_dphi;
 
   // timing
   hipEvent_t ev[6] = {};
@@ -58,13 +60,13 @@ struct tg_ctx {
 
 namespace {
 
-#define HIPCHK(ctx, call)                                                      \
-  do {                                                                         \
-    hipError_t e_ = (call);                                                    \
-    if (e_ != hipSuccess) {                                                    \
-      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
-      return TG_ERR_HIP;                                                       \
-    }                                                                          \
+#define HIPCHK(ctx, call)                                             \
+  do {                                                                \
+    hipError_t e_ = (call);                                           \
+    if (e_ != hipSuccess) {                                           \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_); \
+      return TG_ERR_HIP;                                              \
+    }                                                                 \
   } while (0)
 
 int set_err(tg_ctx* ctx, int code, const std::string& msg) {
@@ -74,7 +76,7 @@ int set_err(tg_ctx* ctx, int code, const std::string& msg) {
 
 template <class T>
 void dfree(T*& p) {
-  if (p) hipFree(p);
+  if (p) hipFree((void*)p);
   p = nullptr;
 }
 
@@ -101,9 +103,11 @@ void free_fields(tg_ctx* ctx) {
   dfree(ctx->part_mass);
   dfree(ctx->part_hmax);
   dfree(ctx->dphi_part);
-  dfree(ctx->wterm);
   dfree(ctx->inst_d);
   dfree(ctx->tiles_d);
+  dfree(ctx->red);
+  dfree(ctx->time_d);
+  dfree(ctx->dt_rec);
   for (auto& p : ctx->ckpt) dfree(p);
   for (auto& p : ctx->seg) dfree(p);
   ctx->ckpt.clear();
@@ -114,64 +118,133 @@ double frict_tiny(const tg_ctx* ctx, int s) {
   return (s < ctx->cfg.frict_tiny_switch) ? ctx->cfg.frict_tiny_early : ctx->cfg.frict_tiny_late;
 }
 
-void params_to_inst(tg_ctx* ctx, const tg_params* p) {
-  for (auto& I : ctx->inst_h) {
-    I.sin_int = p->sin_int;
-    I.tan_bed = p->tan_bed;
-    I.kc1 = p->kact_c1;
-    I.kroot = p->kact_root;
-    I.dkroot = p->dkroot_dphi;
-    I.eps = p->epsilon;
-    I.gscale = p->gravity_scale;
-    I.maxtime = p->maxtime;
-    I.nextoutput = p->nextoutput;
-    I.nextsave = p->nextsave;
-  }
+void params_into(InstDev& I, const tg_params* p) {
+  I.sin_int = p->sin_int;
+  I.tan_bed = p->tan_bed;
+  I.kc1 = p->kact_c1;
+  I.kroot = p->kact_root;
+  I.dkroot = p->dkroot_dphi;
+  I.eps = p->epsilon;
+  I.gscale = p->gravity_scale;
+  I.maxtime = p->maxtime;
+  I.nextoutput = p->nextoutput;
+  I.nextsave = p->nextsave;
 }
 
+// prm == nullptr keeps each member's own parameters (tg_set_ensemble)
 int upload_params(tg_ctx* ctx, const tg_params* p) {
-  if (!p) return set_err(ctx, TG_ERR_ARG, "null params");
-  params_to_inst(ctx, p);
+  if (p)
+    for (auto& I : ctx->inst_h) params_into(I, p);
   HIPCHK(ctx, hipMemcpyAsync(ctx->inst_d, ctx->inst_h.data(), sizeof(InstDev) * ctx->ninst,
                              hipMemcpyHostToDevice, ctx->stream));
   return TG_OK;
 }
 
-StepArgs step_args(tg_ctx* ctx, double* const in[3], double* const out[3], int s) {
-  StepArgs a{};
+template <class T>
+T* at(void* base, long long k, long long n) {
+  return (T*)base + k * n;
+}
+
+Statics32 statics32(tg_ctx* ctx) {
   const long long n = ctx->ncell;
-  a.inst = ctx->inst_d;
-  a.tiles = ctx->tiles_d;
-  a.ntiles = ctx->ntiles;
-  a.U = Fields{in[0], in[1], in[2]};
-  const double* st = ctx->statics;
-  a.S = Statics{st, st + n, st + 2 * n, st + 3 * n, st + 4 * n, st + 5 * n, st + 6 * n};
-  a.kact = ctx->kact;
-  a.red = ctx->red;
-  a.dt_rec = ctx->dt_rec;
-  a.time = ctx->time_d;
-  a.max_steps = ctx->cfg.max_steps;
-  a.step = s;
-  a.ft = frict_tiny(ctx, s);
-  a.tiny = ctx->cfg.tiny;
-  a.gamma = ctx->cfg.gamma;
-  a.cfl_inner = ctx->cfg.cfl_inner;
-  a.cfl_outer = ctx->cfg.cfl_outer;
-  a.h_out = out[0];
-  a.hu_out = out[1];
-  a.hv_out = out[2];
+  return Statics32{at<float>(ctx->statics, 0, n), at<float>(ctx->statics, 1, n),
+                   at<float>(ctx->statics, 2, n), at<float>(ctx->statics, 3, n),
+                   at<float>(ctx->statics, 4, n), at<float>(ctx->statics, 5, n),
+                   at<float>(ctx->statics, 6, n)};
+}
+
+template <class S>
+StepArgsM<S> step_args_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s) {
+  StepArgsM<S> a{};
+  a.inst = ctx->inst_d; a.tiles = ctx->tiles_d; a.ntiles = ctx->ntiles;
+  a.U = FieldsT<S>{(S*)in[0], (S*)in[1], (S*)in[2]};
+  a.S32 = statics32(ctx);
+  a.kact = (float*)ctx->kact; a.red = ctx->red; a.dt_rec = ctx->dt_rec; a.time = ctx->time_d;
+  a.max_steps = ctx->cfg.max_steps; a.step = s; a.ft = frict_tiny(ctx, s);
+  a.tiny = ctx->cfg.tiny; a.gamma = ctx->cfg.gamma;
+  a.h_out = (S*)out[0]; a.hu_out = (S*)out[1]; a.hv_out = (S*)out[2];
   a.err = ctx->err_d;
   return a;
 }
 
-// One forward step U=in (step index s) -> out.  compute_dt: CFL + incrtime,
-// else dt_rec[s] must hold the step's dt.
-void enqueue_step(tg_ctx* ctx, double* const in[3], double* const out[3], int s,
-                  bool compute_dt) {
-  StepArgs a = step_args(ctx, in, out, s);
-  launch_coef(a, compute_dt, ctx->stream);
-  if (compute_dt) launch_dt(a, ctx->ninst, ctx->stream);
-  launch_update(a, ctx->stream);
+template <class S>
+void enqueue_step_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s, bool compute_dt) {
+  StepArgsM<S> a = step_args_m<S>(ctx, in, out, s);
+  launch_coef_m<S>(a, compute_dt, ctx->stream);
+  if (compute_dt) {
+    StepArgs d{};
+    d.inst = ctx->inst_d; d.red = ctx->red; d.dt_rec = ctx->dt_rec; d.time = ctx->time_d;
+    d.max_steps = ctx->cfg.max_steps; d.step = s;
+    d.cfl_inner = ctx->cfg.cfl_inner; d.cfl_outer = ctx->cfg.cfl_outer;
+    launch_dt(d, ctx->ninst, ctx->stream);
+  }
+  launch_update_m<S>(a, ctx->stream);
+}
+
+template <class S>
+void enqueue_adjoint_m(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* const lout[3],
+                       int s, bool dphi) {
+  AdjArgsM<S> a{};
+  a.inst = ctx->inst_d; a.tiles = ctx->tiles_d; a.ntiles = ctx->ntiles;
+  a.U = FieldsT<S>{(S*)U[0], (S*)U[1], (S*)U[2]};
+  a.S32 = statics32(ctx);
+  a.lam_next = FieldsT<S>{(S*)lnext[0], (S*)lnext[1], (S*)lnext[2]};
+  a.dt_rec = ctx->dt_rec; a.max_steps = ctx->cfg.max_steps; a.step = s;
+  a.ft = frict_tiny(ctx, s); a.tiny = ctx->cfg.tiny;
+  a.gamma = ctx->cfg.gamma;
+  a.lam_h = (S*)lout[0]; a.lam_hu = (S*)lout[1]; a.lam_hv = (S*)lout[2];
+  a.dphi_part = dphi ? ctx->dphi_part : nullptr;
+  launch_adjoint_m<S>(a, ctx->stream);
+}
+
+void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, bool compute_dt) {
+  const long long n = ctx->ncell;
+  if (ctx->mode == TG_FP64) {
+    StepArgs a{};
+    a.inst = ctx->inst_d; a.tiles = ctx->tiles_d; a.ntiles = ctx->ntiles;
+    a.U = Fields{(double*)in[0], (double*)in[1], (double*)in[2]};
+    a.S = Statics{at<double>(ctx->statics, 0, n), at<double>(ctx->statics, 1, n),
+                  at<double>(ctx->statics, 2, n), at<double>(ctx->statics, 3, n),
+                  at<double>(ctx->statics, 4, n), at<double>(ctx->statics, 5, n),
+                  at<double>(ctx->statics, 6, n)};
+    a.kact = (double*)ctx->kact; a.red = ctx->red; a.dt_rec = ctx->dt_rec; a.time = ctx->time_d;
+    a.max_steps = ctx->cfg.max_steps; a.step = s; a.ft = frict_tiny(ctx, s);
+    a.tiny = ctx->cfg.tiny; a.gamma = ctx->cfg.gamma;
+    a.cfl_inner = ctx->cfg.cfl_inner; a.cfl_outer = ctx->cfg.cfl_outer;
+    a.h_out = (double*)out[0]; a.hu_out = (double*)out[1]; a.hv_out = (double*)out[2];
+    a.err = ctx->err_d;
+    launch_coef(a, compute_dt, ctx->stream);
+    if (compute_dt) launch_dt(a, ctx->ninst, ctx->stream);
+    launch_update(a, ctx->stream);
+  } else if (ctx->es == 8) {
+    enqueue_step_m<double>(ctx, in, out, s, compute_dt);
+  } else {
+    enqueue_step_m<float>(ctx, in, out, s, compute_dt);
+  }
+}
+
+void enqueue_adjoint(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* const lout[3],
+                     int s, bool dphi) {
+  const long long n = ctx->ncell;
+  if (ctx->mode == TG_FP64) {
+    AdjArgs a{};
+    a.inst = ctx->inst_d; a.tiles = ctx->tiles_d; a.ntiles = ctx->ntiles;
+    a.U = Fields{(double*)U[0], (double*)U[1], (double*)U[2]};
+    a.S = Statics{at<double>(ctx->statics, 0, n), at<double>(ctx->statics, 1, n),
+                  at<double>(ctx->statics, 2, n), at<double>(ctx->statics, 3, n),
+                  at<double>(ctx->statics, 4, n), at<double>(ctx->statics, 5, n),
+                  at<double>(ctx->statics, 6, n)};
+    a.lam_next = Fields{(double*)lnext[0], (double*)lnext[1], (double*)lnext[2]};
+    a.dt_rec = ctx->dt_rec; a.max_steps = ctx->cfg.max_steps; a.step = s;
+    a.ft = frict_tiny(ctx, s); a.tiny = ctx->cfg.tiny; a.gamma = ctx->cfg.gamma;
+    a.lam_h = (double*)lout[0]; a.lam_hu = (double*)lout[1]; a.lam_hv = (double*)lout[2];
+    a.dphi_part = dphi ? ctx->dphi_part : nullptr;
+    launch_adjoint(a, ctx->stream);
+  } else if (ctx->es == 8) {
+    enqueue_adjoint_m<double>(ctx, U, lnext, lout, s, dphi);
+  } else {
+    enqueue_adjoint_m<float>(ctx, U, lnext, lout, s, dphi);
+  }
 }
 
 int check_err(tg_ctx* ctx) {
@@ -183,77 +256,116 @@ int check_err(tg_ctx* ctx) {
   return TG_OK;
 }
 
-int fill_stats(tg_ctx* ctx, tg_stats* st) {
+// FP64 host array of one field of one member <-> device storage
+int h2d_field(tg_ctx* ctx, void* dev_base, long long off, long long count, const double* src,
+              int es = 0) {
+  if ((es ? es : ctx->es) == 8)
+    HIPCHK(ctx, hipMemcpyAsync((double*)dev_base + off, src, sizeof(double) * count,
+                               hipMemcpyHostToDevice, ctx->stream));
+  else {
+    std::vector<float> tmp(count);
+    for (long long i = 0; i < count; i++) tmp[i] = (float)src[i];
+    HIPCHK(ctx, hipMemcpyAsync((float*)dev_base + off, tmp.data(), sizeof(float) * count,
+                               hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // tmp goes out of scope
+  }
+  return TG_OK;
+}
+
+int d2h_field(tg_ctx* ctx, const void* dev_base, long long off, long long count, double* dst,
+              int es = 0) {
+  if ((es ? es : ctx->es) == 8) {
+    HIPCHK(ctx, hipMemcpyAsync(dst, (const double*)dev_base + off, sizeof(double) * count,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  } else {
+    std::vector<float> tmp(count);
+    HIPCHK(ctx, hipMemcpyAsync(tmp.data(), (const float*)dev_base + off, sizeof(float) * count,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (long long i = 0; i < count; i++) dst[i] = (double)tmp[i];
+  }
+  return TG_OK;
+}
+
+int fill_stats(tg_ctx* ctx, tg_stats* st, int nst) {
   if (!st) return TG_OK;
-  launch_stats(ctx->inst_d, ctx->tiles_d, ctx->ntiles, Fields{ctx->U[0], ctx->U[1], ctx->U[2]},
-               ctx->part_mass, ctx->part_hmax, ctx->cfg.tiny, ctx->stream);
-  std::vector<double> pm(ctx->ntiles), ph(ctx->ntiles);
+  if (ctx->es == 8)
+    launch_stats(ctx->inst_d, ctx->tiles_d, ctx->ntiles,
+                 Fields{(double*)ctx->U[0], (double*)ctx->U[1], (double*)ctx->U[2]},
+                 ctx->part_mass, ctx->part_hmax, ctx->cfg.tiny, ctx->stream);
+  else
+    launch_stats32(ctx->inst_d, ctx->tiles_d, ctx->ntiles,
+                   FieldsT<float>{(float*)ctx->U[0], (float*)ctx->U[1], (float*)ctx->U[2]},
+                   ctx->part_mass, ctx->part_hmax, ctx->cfg.tiny, ctx->stream);
+  std::vector<double> pm(ctx->ntiles), ph(ctx->ntiles), t(ctx->ninst);
   HIPCHK(ctx, hipMemcpyAsync(pm.data(), ctx->part_mass, sizeof(double) * ctx->ntiles,
                              hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ph.data(), ctx->part_hmax, sizeof(double) * ctx->ntiles,
                              hipMemcpyDeviceToHost, ctx->stream));
-  double dt = 0;
+  HIPCHK(ctx, hipMemcpyAsync(t.data(), ctx->time_d, sizeof(double) * ctx->ninst,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<double> dtl(ctx->ninst, 0.);
   if (ctx->rec_steps > 0)
-    HIPCHK(ctx, hipMemcpyAsync(&dt, ctx->dt_rec + ctx->rec_steps - 1, sizeof(double),
-                               hipMemcpyDeviceToHost, ctx->stream));
-  double t = 0;
-  HIPCHK(ctx, hipMemcpyAsync(&t, ctx->time_d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    for (int b = 0; b < ctx->ninst; b++)
+      HIPCHK(ctx, hipMemcpyAsync(&dtl[b], ctx->dt_rec + (long long)b * ctx->cfg.max_steps +
+                                              ctx->rec_steps - 1,
+                                 sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  double m = 0, hm = 0;
+  std::vector<double> m(ctx->ninst, 0.), hm(ctx->ninst, 0.);
   for (int k = 0; k < ctx->ntiles; k++) {
-    m += pm[k];
-    hm = (ph[k] > hm) ? ph[k] : hm;
+    const int b = ctx->tiles_h[k].inst;
+    m[b] += pm[k];
+    hm[b] = (ph[k] > hm[b]) ? ph[k] : hm[b];
   }
-  st->dt = dt;
-  st->time = t;
-  st->mass = m;
-  st->hmax = hm;
-  st->step = ctx->rec_steps;
+  for (int b = 0; b < nst && b < ctx->ninst; b++) {
+    st[b].dt = dtl[b];
+    st[b].time = t[b];
+    st[b].mass = m[b];
+    st[b].hmax = hm[b];
+    st[b].step = ctx->rec_steps;
+  }
   return TG_OK;
 }
 
-int ensure_store(tg_ctx* ctx, std::vector<double*>& v, size_t count) {
-  const size_t bytes = sizeof(double) * 3 * (size_t)ctx->ncell;
+int ensure_store(tg_ctx* ctx, std::vector<void*>& v, size_t count) {
+  const size_t bytes = (size_t)ctx->es * 3 * (size_t)ctx->ncell;
   while (v.size() < count) {
-    double* p = nullptr;
-    int rc = dalloc(ctx, (void**)&p, bytes);
+    void* p = nullptr;
+    int rc = dalloc(ctx, &p, bytes);
     if (rc) return rc;
     v.push_back(p);
   }
   return TG_OK;
 }
 
-int copy_state(tg_ctx* ctx, double* dst, double* const src[3]) {
-  const size_t b = sizeof(double) * (size_t)ctx->ncell;
+void split(tg_ctx* ctx, void* base, void* out[3]) {
+  for (int k = 0; k < 3; k++) out[k] = (char*)base + (size_t)k * ctx->es * ctx->ncell;
+}
+
+int copy_state(tg_ctx* ctx, void* dst, void* const src[3]) {
+  const size_t b = (size_t)ctx->es * ctx->ncell;
   for (int k = 0; k < 3; k++)
-    HIPCHK(ctx, hipMemcpyAsync(dst + k * ctx->ncell, src[k], b, hipMemcpyDeviceToDevice,
+    HIPCHK(ctx, hipMemcpyAsync((char*)dst + k * b, src[k], b, hipMemcpyDeviceToDevice,
                                ctx->stream));
   return TG_OK;
 }
 
-void split(tg_ctx* ctx, double* base, double* out[3]) {
-  for (int k = 0; k < 3; k++) out[k] = base + k * ctx->ncell;
-}
-
-// Shared body of tg_forward_run / tg_forward_replay.
 int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt_seq,
-                 tg_stats* stats) {
+                 tg_stats* stats, int nstats) {
   if (!ctx->have_mesh || !ctx->have_state) return set_err(ctx, TG_ERR_STATE, "mesh/state not set");
   if (nsteps < 0 || nsteps > ctx->cfg.max_steps)
     return set_err(ctx, TG_ERR_ARG, "nsteps outside [0, max_steps]");
   int rc = upload_params(ctx, prm);
   if (rc) return rc;
   const int k = ctx->cfg.checkpoint_every;
-  const size_t nck = (size_t)(nsteps + k - 1) / k + 1;
-  rc = ensure_store(ctx, ctx->ckpt, nck);
+  rc = ensure_store(ctx, ctx->ckpt, (size_t)(nsteps + k - 1) / k + 1);
   if (rc) return rc;
-  // the record restarts: dt_rec[0..nsteps) belongs to this run
-  ctx->rec_steps = 0;
-  ctx->run_steps = 0;
-  ctx->run_start_step = 0;
-  if (dt_seq)
-    HIPCHK(ctx, hipMemcpyAsync(ctx->dt_rec, dt_seq, sizeof(double) * nsteps,
-                               hipMemcpyHostToDevice, ctx->stream));
+  ctx->rec_steps = ctx->run_steps = 0;
+  if (dt_seq)  // the same prescribed sequence for every member
+    for (int b = 0; b < ctx->ninst; b++)
+      HIPCHK(ctx, hipMemcpyAsync(ctx->dt_rec + (long long)b * ctx->cfg.max_steps, dt_seq,
+                                 sizeof(double) * nsteps, hipMemcpyHostToDevice, ctx->stream));
   launch_reset_red(ctx->red, ctx->ninst, ctx->stream);
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
   for (int s = 0; s < nsteps; s++) {
@@ -271,16 +383,187 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
   float ms = 0;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
   ctx->fwd_ms = ms;
-  ctx->rec_steps = nsteps;
-  ctx->run_steps = nsteps;
-  if (dt_seq) {
-    double t = ctx->time_h;
-    for (int s = 0; s < nsteps; s++) t += dt_seq[s];
-    ctx->time_h = t;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->time_d, &ctx->time_h, sizeof(double), hipMemcpyHostToDevice,
-                               ctx->stream));
+  ctx->rec_steps = ctx->run_steps = nsteps;
+  if (dt_seq) {  // time advances by the prescribed sequence
+    std::vector<double> t(ctx->ninst);
+    HIPCHK(ctx, hipMemcpy(t.data(), ctx->time_d, sizeof(double) * ctx->ninst,
+                          hipMemcpyDeviceToHost));
+    for (auto& v : t)
+      for (int s = 0; s < nsteps; s++) v += dt_seq[s];
+    HIPCHK(ctx, hipMemcpy(ctx->time_d, t.data(), sizeof(double) * ctx->ninst,
+                          hipMemcpyHostToDevice));
   }
-  return fill_stats(ctx, stats);
+  return fill_stats(ctx, stats, nstats);
+}
+
+int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lambda0,
+                 double* dJ) {
+  if (ctx->run_steps <= 0) return set_err(ctx, TG_ERR_STATE, "no forward run recorded");
+  int rc = upload_params(ctx, prm);
+  if (rc) return rc;
+  const long long n = ctx->ncell;
+  const int N = ctx->run_steps, k = ctx->cfg.checkpoint_every;
+  rc = ensure_store(ctx, ctx->seg, (size_t)k);
+  if (rc) return rc;
+  const bool dphi = dJ != nullptr;
+  if (dphi)
+    HIPCHK(ctx, hipMemsetAsync(ctx->dphi_part, 0, sizeof(double) * ctx->ntiles, ctx->stream));
+  // lambda^N = func_sens at the final step (calc_adjoint.C:93-97)
+  for (int c = 0; c < 3; c++) {
+    rc = h2d_field(ctx, ctx->lam[0][c], 0, n, w + c * n);
+    if (rc) return rc;
+  }
+  int cur = 0;
+  float rec_ms = 0, ms = 0;
+  HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  const int nseg = (N + k - 1) / k;
+  for (int sg = nseg - 1; sg >= 0; sg--) {
+    const int s0 = sg * k, s1 = (s0 + k < N) ? s0 + k : N;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    void* base[3];
+    split(ctx, ctx->ckpt[sg], base);
+    rc = copy_state(ctx, ctx->seg[0], base);
+    if (rc) return rc;
+    for (int s = s0; s + 1 < s1; s++) {  // recompute U^{s0+1..s1-1} with the recorded dt
+      void* in[3];
+      void* out[3];
+      split(ctx, ctx->seg[s - s0], in);
+      split(ctx, ctx->seg[s - s0 + 1], out);
+      enqueue_step(ctx, in, out, s, false);
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    for (int s = s1 - 1; s >= s0; s--) {
+      void* Us[3];
+      split(ctx, ctx->seg[s - s0], Us);
+      enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi);
+      cur = 1 - cur;
+    }
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[5]));
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+    rec_ms += ms;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  HIPCHK(ctx, hipGetLastError());
+  rc = check_err(ctx);
+  if (rc) return rc;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+  ctx->adj_ms = ms;
+  ctx->rec_ms = rec_ms;
+  if (lambda0)
+    for (int c = 0; c < 3; c++) {
+      rc = d2h_field(ctx, ctx->lam[cur][c], 0, n, lambda0 + c * n);
+      if (rc) return rc;
+    }
+  ctx->member_dphi.assign(ctx->ninst, 0.);
+  if (dphi) {
+    std::vector<double> part(ctx->ntiles);
+    HIPCHK(ctx, hipMemcpyAsync(part.data(), ctx->dphi_part, sizeof(double) * ctx->ntiles,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    double acc = 0;
+    for (int t = 0; t < ctx->ntiles; t++) {
+      acc += part[t];
+      ctx->member_dphi[ctx->tiles_h[t].inst] += part[t];
+    }
+    *dJ = acc;
+  }
+  return TG_OK;
+}
+
+// Builds members, tiles and device buffers.
+int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* params) {
+  if (nm < 1 || !meshes) return set_err(ctx, TG_ERR_ARG, "no members");
+  for (int b = 0; b < nm; b++) {
+    const tg_mesh* m = &meshes[b];
+    if (m->kind != TG_MESH_UNIFORM)
+      return set_err(ctx, TG_ERR_UNSUPPORTED, "only TG_MESH_UNIFORM is built");
+    if (m->nx < 1 || m->ny < 1 || !(m->dx > 0) || !(m->dy > 0) || !m->grav || !m->dgrav ||
+        !m->curv)
+      return set_err(ctx, TG_ERR_ARG, "bad mesh");
+    // The kernels use dx, dy as the centre spacing; the reference takes
+    // differences of centre coordinates (element2.C:894-895).  Both agree
+    // only if those differences are exact (dyadic spacings and origins).
+    for (int i = 0; i + 1 < m->nx; i++)
+      if ((m->x0 + (i + 1) * m->dx) - (m->x0 + i * m->dx) != m->dx)
+        return set_err(ctx, TG_ERR_ARG, "x centre spacing is not exactly dx");
+    for (int j = 0; j + 1 < m->ny; j++)
+      if ((m->y0 + (j + 1) * m->dy) - (m->y0 + j * m->dy) != m->dy)
+        return set_err(ctx, TG_ERR_ARG, "y centre spacing is not exactly dy");
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  free_fields(ctx);
+  ctx->have_mesh = ctx->have_state = false;
+  ctx->rec_steps = ctx->run_steps = 0;
+  ctx->ninst = nm;
+  ctx->inst_h.assign(nm, InstDev{});
+  ctx->tiles_h.clear();
+  long long off = 0;
+  for (int b = 0; b < nm; b++) {
+    const tg_mesh* m = &meshes[b];
+    InstDev& I = ctx->inst_h[b];
+    I.nx = m->nx; I.ny = m->ny; I.off = off;
+    I.dx = m->dx; I.dy = m->dy;
+    I.inv_dx = 1 / m->dx; I.inv_sx = 1 / (m->dx + m->dx);
+    I.inv_dy = 1 / m->dy; I.inv_sy = 1 / (m->dy + m->dy);
+    I.maxtime = I.nextoutput = I.nextsave = HUGE_VAL;
+    if (params) params_into(I, &params[b]);
+    for (int ty = 0; ty < m->ny; ty += TILE_Y)
+      for (int tx = 0; tx < m->nx; tx += TILE_X) ctx->tiles_h.push_back(TileDev{b, tx, ty, 0});
+    off += (long long)m->nx * m->ny;
+  }
+  ctx->ncell = off;
+  ctx->ntiles = (int)ctx->tiles_h.size();
+  const long long n = ctx->ncell;
+  const size_t fb = (size_t)ctx->es * n;
+  int rc = 0;
+  for (int k = 0; k < 3 && !rc; k++) {
+    rc = dalloc(ctx, &ctx->U[k], fb);
+    if (!rc) rc = dalloc(ctx, &ctx->Ut[k], fb);
+    if (!rc) rc = dalloc(ctx, &ctx->lam[0][k], fb);
+    if (!rc) rc = dalloc(ctx, &ctx->lam[1][k], fb);
+  }
+  const size_t sb = (size_t)ctx->ses * n;
+  if (!rc) rc = dalloc(ctx, &ctx->statics, 7 * sb);
+  if (!rc) rc = dalloc(ctx, &ctx->kact, sb);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->part_mass, sizeof(double) * ctx->ntiles);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->part_hmax, sizeof(double) * ctx->ntiles);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->dphi_part, sizeof(double) * ctx->ntiles);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->inst_d, sizeof(InstDev) * nm);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->tiles_d, sizeof(TileDev) * ctx->ntiles);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->red, sizeof(RedDev) * nm);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->time_d, sizeof(double) * nm);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->dt_rec, sizeof(double) * nm * ctx->cfg.max_steps);
+  if (rc) {
+    free_fields(ctx);
+    return rc;
+  }
+  // static coefficients: field k of member b at statics[k*n + off_b]
+  for (int b = 0; b < nm && !rc; b++) {
+    const tg_mesh* m = &meshes[b];
+    const long long nb = (long long)m->nx * m->ny, o = ctx->inst_h[b].off;
+    const double* src[7] = {m->grav, m->grav + nb, m->grav + 2 * nb, m->dgrav, m->dgrav + nb,
+                            m->curv, m->curv + nb};
+    for (int k = 0; k < 7 && !rc; k++)
+      rc = h2d_field(ctx, (char*)ctx->statics + (size_t)k * sb, o, nb, src[k], ctx->ses);
+  }
+  if (rc) return rc;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->tiles_d, ctx->tiles_h.data(), sizeof(TileDev) * ctx->ntiles,
+                             hipMemcpyHostToDevice, ctx->stream));
+  rc = upload_params(ctx, nullptr);
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->have_mesh = true;
+  return TG_OK;
+}
+
+int reset_time(tg_ctx* ctx) {
+  std::vector<double> z(ctx->ninst, 0.);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->time_d, z.data(), sizeof(double) * ctx->ninst,
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(ctx->err_d, 0, sizeof(int), ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return TG_OK;
 }
 
 }  // namespace
@@ -334,14 +617,15 @@ int tg_create(const tg_config* cfg, tg_ctx** out) {
   tg_ctx* ctx = new tg_ctx();
   if (cfg) ctx->cfg = *cfg;
   else tg_config_default(&ctx->cfg);
-  if (ctx->cfg.checkpoint_every < 1 || ctx->cfg.max_steps < 1) {
+  if (ctx->cfg.checkpoint_every < 1 || ctx->cfg.max_steps < 1 ||
+      (ctx->cfg.precision != TG_FP64 && ctx->cfg.precision != TG_MIXED &&
+       ctx->cfg.precision != TG_FP32)) {
     delete ctx;
     return TG_ERR_ARG;
   }
-  if (ctx->cfg.precision != TG_FP64) {
-    delete ctx;
-    return TG_ERR_UNSUPPORTED;
-  }
+  ctx->mode = ctx->cfg.precision;
+  ctx->es = (ctx->mode == TG_FP32) ? 4 : 8;
+  ctx->ses = (ctx->mode == TG_FP64) ? 8 : 4;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= ctx->cfg.device) {
     delete ctx;
@@ -353,8 +637,7 @@ int tg_create(const tg_config* cfg, tg_ctx** out) {
     return TG_ERR_HIP;
   }
   for (auto& e : ctx->ev) hipEventCreate(&e);
-  if (dalloc(ctx, (void**)&ctx->err_d, sizeof(int)) ||
-      dalloc(ctx, (void**)&ctx->dt_rec, sizeof(double) * ctx->cfg.max_steps)) {
+  if (dalloc(ctx, (void**)&ctx->err_d, sizeof(int))) {
     tg_destroy(ctx);
     return TG_ERR_NOMEM;
   }
@@ -368,10 +651,7 @@ void tg_destroy(tg_ctx* ctx) {
   hipSetDevice(ctx->cfg.device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   free_fields(ctx);
-  dfree(ctx->red);
-  dfree(ctx->time_d);
   dfree(ctx->err_d);
-  dfree(ctx->dt_rec);
   for (auto& e : ctx->ev)
     if (e) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -382,89 +662,51 @@ const char* tg_last_error(const tg_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 
 int tg_set_mesh(tg_ctx* ctx, const tg_mesh* m) {
   if (!ctx || !m) return TG_ERR_ARG;
-  if (m->kind != TG_MESH_UNIFORM)
-    return set_err(ctx, TG_ERR_UNSUPPORTED, "only TG_MESH_UNIFORM is built");
-  if (m->nx < 1 || m->ny < 1 || !(m->dx > 0) || !(m->dy > 0) || !m->grav || !m->dgrav || !m->curv)
-    return set_err(ctx, TG_ERR_ARG, "bad mesh");
-  // The kernels use dx, dy for the centre spacing; the reference takes
-  // differences of centre coordinates (element2.C:894-895).  Both agree only
-  // if those differences are exact, which holds for dyadic spacings and
-  // origins; reject meshes where they do not.
-  for (int i = 0; i + 1 < m->nx; i++)
-    if ((m->x0 + (i + 1) * m->dx) - (m->x0 + i * m->dx) != m->dx)
-      return set_err(ctx, TG_ERR_ARG, "x centre spacing is not exactly dx");
-  for (int j = 0; j + 1 < m->ny; j++)
-    if ((m->y0 + (j + 1) * m->dy) - (m->y0 + j * m->dy) != m->dy)
-      return set_err(ctx, TG_ERR_ARG, "y centre spacing is not exactly dy");
-  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  free_fields(ctx);
-  ctx->have_mesh = ctx->have_state = false;
-  ctx->rec_steps = ctx->run_steps = 0;
-  const long long n = (long long)m->nx * m->ny;
-  ctx->ncell = n;
-  ctx->ninst = 1;
-  ctx->dx = m->dx;
-  ctx->dy = m->dy;
-  InstDev I{};
-  I.nx = m->nx;
-  I.ny = m->ny;
-  I.off = 0;
-  I.dx = m->dx;
-  I.dy = m->dy;
-  ctx->inst_h.assign(1, I);
-  ctx->tiles_h.clear();
-  for (int ty = 0; ty < m->ny; ty += TILE_Y)
-    for (int tx = 0; tx < m->nx; tx += TILE_X) ctx->tiles_h.push_back(TileDev{0, tx, ty, 0});
-  ctx->ntiles = (int)ctx->tiles_h.size();
-  int rc = 0;
-  const size_t nb = sizeof(double) * n;
-  for (int k = 0; k < 3 && !rc; k++) {
-    rc = dalloc(ctx, (void**)&ctx->U[k], nb);
-    if (!rc) rc = dalloc(ctx, (void**)&ctx->Ut[k], nb);
-    if (!rc) rc = dalloc(ctx, (void**)&ctx->lam[0][k], nb);
-    if (!rc) rc = dalloc(ctx, (void**)&ctx->lam[1][k], nb);
-  }
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->statics, 7 * nb);
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->kact, nb);
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->wterm, 3 * nb);
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->part_mass, sizeof(double) * ctx->ntiles);
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->part_hmax, sizeof(double) * ctx->ntiles);
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->dphi_part, sizeof(double) * ctx->ntiles);
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->inst_d, sizeof(InstDev));
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->tiles_d, sizeof(TileDev) * ctx->ntiles);
-  if (!rc && !ctx->red) rc = dalloc(ctx, (void**)&ctx->red, sizeof(RedDev) * 1);
-  if (!rc && !ctx->time_d) rc = dalloc(ctx, (void**)&ctx->time_d, sizeof(double) * 1);
-  if (rc) {
-    free_fields(ctx);
-    return rc;
-  }
-  HIPCHK(ctx, hipMemcpyAsync(ctx->statics, m->grav, 3 * nb, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->statics + 3 * n, m->dgrav, 2 * nb, hipMemcpyHostToDevice,
-                             ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->statics + 5 * n, m->curv, 2 * nb, hipMemcpyHostToDevice,
-                             ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->tiles_d, ctx->tiles_h.data(), sizeof(TileDev) * ctx->ntiles,
-                             hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->inst_d, ctx->inst_h.data(), sizeof(InstDev),
-                             hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  ctx->have_mesh = true;
+  return set_members(ctx, 1, m, nullptr);
+}
+
+int tg_set_ensemble(tg_ctx* ctx, int nmembers, const tg_mesh* meshes, const tg_params* params) {
+  if (!ctx || !meshes || !params || nmembers < 1) return TG_ERR_ARG;
+  return set_members(ctx, nmembers, meshes, params);
+}
+
+int tg_num_members(const tg_ctx* ctx) { return ctx ? ctx->ninst : 0; }
+
+int tg_member_cells(const tg_ctx* ctx, int member, int64_t* offset, int64_t* count) {
+  if (!ctx || member < 0 || member >= ctx->ninst) return TG_ERR_ARG;
+  const InstDev& I = ctx->inst_h[member];
+  if (offset) *offset = I.off;
+  if (count) *count = (int64_t)I.nx * I.ny;
   return TG_OK;
 }
 
 int tg_set_state(tg_ctx* ctx, const double* h, const double* hu, const double* hv) {
   if (!ctx || !h || !hu || !hv) return TG_ERR_ARG;
   if (!ctx->have_mesh) return set_err(ctx, TG_ERR_STATE, "no mesh");
-  const size_t nb = sizeof(double) * ctx->ncell;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->U[0], h, nb, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->U[1], hu, nb, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->U[2], hv, nb, hipMemcpyHostToDevice, ctx->stream));
-  ctx->time_h = 0.;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->time_d, &ctx->time_h, sizeof(double), hipMemcpyHostToDevice,
-                             ctx->stream));
-  HIPCHK(ctx, hipMemsetAsync(ctx->err_d, 0, sizeof(int), ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const double* src[3] = {h, hu, hv};
+  for (int k = 0; k < 3; k++) {
+    int rc = h2d_field(ctx, ctx->U[k], 0, ctx->ncell, src[k]);
+    if (rc) return rc;
+  }
+  int rc = reset_time(ctx);
+  if (rc) return rc;
+  ctx->rec_steps = ctx->run_steps = 0;
+  ctx->have_state = true;
+  return TG_OK;
+}
+
+int tg_set_member_state(tg_ctx* ctx, int member, const double* h, const double* hu,
+                        const double* hv) {
+  if (!ctx || !h || !hu || !hv || member < 0 || member >= ctx->ninst) return TG_ERR_ARG;
+  if (!ctx->have_mesh) return set_err(ctx, TG_ERR_STATE, "no mesh");
+  const InstDev& I = ctx->inst_h[member];
+  const double* src[3] = {h, hu, hv};
+  for (int k = 0; k < 3; k++) {
+    int rc = h2d_field(ctx, ctx->U[k], I.off, (long long)I.nx * I.ny, src[k]);
+    if (rc) return rc;
+  }
+  int rc = reset_time(ctx);
+  if (rc) return rc;
   ctx->rec_steps = ctx->run_steps = 0;
   ctx->have_state = true;
   return TG_OK;
@@ -473,11 +715,23 @@ int tg_set_state(tg_ctx* ctx, const double* h, const double* hu, const double* h
 int tg_get_state(tg_ctx* ctx, double* h, double* hu, double* hv) {
   if (!ctx || !h || !hu || !hv) return TG_ERR_ARG;
   if (!ctx->have_state) return set_err(ctx, TG_ERR_STATE, "no state");
-  const size_t nb = sizeof(double) * ctx->ncell;
-  HIPCHK(ctx, hipMemcpyAsync(h, ctx->U[0], nb, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(hu, ctx->U[1], nb, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(hv, ctx->U[2], nb, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  double* dst[3] = {h, hu, hv};
+  for (int k = 0; k < 3; k++) {
+    int rc = d2h_field(ctx, ctx->U[k], 0, ctx->ncell, dst[k]);
+    if (rc) return rc;
+  }
+  return TG_OK;
+}
+
+int tg_get_member_state(tg_ctx* ctx, int member, double* h, double* hu, double* hv) {
+  if (!ctx || !h || !hu || !hv || member < 0 || member >= ctx->ninst) return TG_ERR_ARG;
+  if (!ctx->have_state) return set_err(ctx, TG_ERR_STATE, "no state");
+  const InstDev& I = ctx->inst_h[member];
+  double* dst[3] = {h, hu, hv};
+  for (int k = 0; k < 3; k++) {
+    int rc = d2h_field(ctx, ctx->U[k], I.off, (long long)I.nx * I.ny, dst[k]);
+    if (rc) return rc;
+  }
   return TG_OK;
 }
 
@@ -494,136 +748,48 @@ int tg_forward_step(tg_ctx* ctx, const tg_params* prm, tg_stats* stats) {
   rc = check_err(ctx);
   if (rc) return rc;
   ctx->rec_steps++;
-  ctx->run_steps = 0;  // a single step does not keep checkpoints
-  HIPCHK(ctx, hipMemcpyAsync(&ctx->time_h, ctx->time_d, sizeof(double), hipMemcpyDeviceToHost,
-                             ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  return fill_stats(ctx, stats);
+  ctx->run_steps = 0;  // single steps keep no checkpoints
+  return fill_stats(ctx, stats, 1);
 }
 
 int tg_forward_run(tg_ctx* ctx, const tg_params* prm, int nsteps, tg_stats* stats) {
   if (!ctx) return TG_ERR_ARG;
-  int rc = forward_loop(ctx, prm, nsteps, nullptr, stats);
-  if (!rc) {
-    HIPCHK(ctx, hipMemcpyAsync(&ctx->time_h, ctx->time_d, sizeof(double), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  }
-  return rc;
+  return forward_loop(ctx, prm, nsteps, nullptr, stats, stats ? ctx->ninst : 0);
 }
 
 int tg_forward_replay(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt_seq,
                       tg_stats* stats) {
   if (!ctx || (!dt_seq && nsteps > 0)) return TG_ERR_ARG;
-  return forward_loop(ctx, prm, nsteps, dt_seq, stats);
+  return forward_loop(ctx, prm, nsteps, dt_seq, stats, stats ? ctx->ninst : 0);
 }
 
 int tg_get_dt_record(tg_ctx* ctx, double* dt_out, int capacity, int* count) {
-  if (!ctx || !count) return TG_ERR_ARG;
+  return tg_get_member_dt_record(ctx, 0, dt_out, capacity, count);
+}
+
+int tg_get_member_dt_record(tg_ctx* ctx, int member, double* dt_out, int capacity, int* count) {
+  if (!ctx || !count || member < 0 || member >= ctx->ninst) return TG_ERR_ARG;
   *count = ctx->rec_steps;
   if (dt_out && capacity > 0) {
     int m = capacity < ctx->rec_steps ? capacity : ctx->rec_steps;
-    HIPCHK(ctx, hipMemcpyAsync(dt_out, ctx->dt_rec, sizeof(double) * m, hipMemcpyDeviceToHost,
-                               ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(dt_out, ctx->dt_rec + (long long)member * ctx->cfg.max_steps,
+                               sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   }
   return TG_OK;
 }
 
-static AdjArgs adj_args(tg_ctx* ctx, double* const U[3], double* const lnext[3],
-                        double* const lout[3], int s, bool dphi) {
-  AdjArgs a{};
-  const long long n = ctx->ncell;
-  a.inst = ctx->inst_d;
-  a.tiles = ctx->tiles_d;
-  a.ntiles = ctx->ntiles;
-  a.U = Fields{U[0], U[1], U[2]};
-  const double* st = ctx->statics;
-  a.S = Statics{st, st + n, st + 2 * n, st + 3 * n, st + 4 * n, st + 5 * n, st + 6 * n};
-  a.lam_next = Fields{lnext[0], lnext[1], lnext[2]};
-  a.dt_rec = ctx->dt_rec;
-  a.max_steps = ctx->cfg.max_steps;
-  a.step = s;
-  a.ft = frict_tiny(ctx, s);
-  a.tiny = ctx->cfg.tiny;
-  a.gamma = ctx->cfg.gamma;
-  a.lam_h = lout[0];
-  a.lam_hu = lout[1];
-  a.lam_hv = lout[2];
-  a.dphi_part = dphi ? ctx->dphi_part : nullptr;
-  return a;
-}
-
 int tg_adjoint_run(tg_ctx* ctx, const tg_params* prm, const tg_functional* fn,
                    double* lambda0_out, double* dJ_dphibed_out) {
   if (!ctx || !fn || !fn->terminal_weight) return TG_ERR_ARG;
-  if (ctx->run_steps <= 0) return set_err(ctx, TG_ERR_STATE, "no forward run recorded");
-  int rc = upload_params(ctx, prm);
-  if (rc) return rc;
-  const long long n = ctx->ncell;
-  const int N = ctx->run_steps, k = ctx->cfg.checkpoint_every;
-  rc = ensure_store(ctx, ctx->seg, (size_t)k);
-  if (rc) return rc;
-  const bool dphi = dJ_dphibed_out != nullptr;
-  if (dphi) HIPCHK(ctx, hipMemsetAsync(ctx->dphi_part, 0, sizeof(double) * ctx->ntiles, ctx->stream));
-  // lambda^N = func_sens at the final step (calc_adjoint.C:93-97)
-  const size_t nb = sizeof(double) * n;
-  for (int c = 0; c < 3; c++)
-    HIPCHK(ctx, hipMemcpyAsync(ctx->lam[0][c], fn->terminal_weight + c * n, nb,
-                               hipMemcpyHostToDevice, ctx->stream));
-  int cur = 0;
-  float rec_ms = 0, ms = 0;
-  HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-  const int nseg = (N + k - 1) / k;
-  for (int sg = nseg - 1; sg >= 0; sg--) {
-    const int s0 = sg * k, s1 = (s0 + k < N) ? s0 + k : N;
-    // recompute U^{s0..s1-1} from the checkpoint with the recorded dt
-    HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-    double* base[3];
-    split(ctx, ctx->ckpt[sg], base);
-    rc = copy_state(ctx, ctx->seg[0], base);
-    if (rc) return rc;
-    for (int s = s0; s + 1 < s1; s++) {
-      double* in[3];
-      double* out[3];
-      split(ctx, ctx->seg[s - s0], in);
-      split(ctx, ctx->seg[s - s0 + 1], out);
-      enqueue_step(ctx, in, out, s, false);
-    }
-    HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-    // reverse steps s = s1-1 .. s0
-    for (int s = s1 - 1; s >= s0; s--) {
-      double* Us[3];
-      split(ctx, ctx->seg[s - s0], Us);
-      AdjArgs a = adj_args(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi);
-      launch_adjoint(a, ctx->stream);
-      cur = 1 - cur;
-    }
-    HIPCHK(ctx, hipEventSynchronize(ctx->ev[5]));
-    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
-    rec_ms += ms;
-  }
-  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-  HIPCHK(ctx, hipGetLastError());
-  rc = check_err(ctx);
-  if (rc) return rc;
-  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
-  ctx->adj_ms = ms;
-  ctx->rec_ms = rec_ms;
-  if (lambda0_out)
-    for (int c = 0; c < 3; c++)
-      HIPCHK(ctx, hipMemcpyAsync(lambda0_out + c * n, ctx->lam[cur][c], nb,
-                                 hipMemcpyDeviceToHost, ctx->stream));
-  if (dphi) {
-    std::vector<double> part(ctx->ntiles);
-    HIPCHK(ctx, hipMemcpyAsync(part.data(), ctx->dphi_part, sizeof(double) * ctx->ntiles,
-                               hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    double acc = 0;
-    for (double v : part) acc += v;
-    *dJ_dphibed_out = acc;
-  }
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return adjoint_loop(ctx, prm, fn->terminal_weight, lambda0_out, dJ_dphibed_out);
+}
+
+int tg_get_member_dphi(tg_ctx* ctx, double* dJ_dphibed) {
+  if (!ctx || !dJ_dphibed) return TG_ERR_ARG;
+  if ((int)ctx->member_dphi.size() != ctx->ninst)
+    return set_err(ctx, TG_ERR_STATE, "no adjoint run with the gradient");
+  for (int b = 0; b < ctx->ninst; b++) dJ_dphibed[b] = ctx->member_dphi[b];
   return TG_OK;
 }
 
@@ -635,20 +801,19 @@ int tg_adjoint_step(tg_ctx* ctx, const tg_params* prm, int s, double dt,
   int rc = upload_params(ctx, prm);
   if (rc) return rc;
   const long long n = ctx->ncell;
-  const size_t nb = sizeof(double) * n;
-  // the step's dt goes into the record slot the kernel reads
-  HIPCHK(ctx, hipMemcpyAsync(ctx->dt_rec + s, &dt, sizeof(double), hipMemcpyHostToDevice,
-                             ctx->stream));
-  for (int c = 0; c < 3; c++)
-    HIPCHK(ctx, hipMemcpyAsync(ctx->lam[0][c], lambda_next + c * n, nb, hipMemcpyHostToDevice,
-                               ctx->stream));
-  AdjArgs a = adj_args(ctx, ctx->U, ctx->lam[0], ctx->lam[1], s, false);
-  launch_adjoint(a, ctx->stream);
+  for (int b = 0; b < ctx->ninst; b++)  // the step's dt goes where the kernel reads it
+    HIPCHK(ctx, hipMemcpyAsync(ctx->dt_rec + (long long)b * ctx->cfg.max_steps + s, &dt,
+                               sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  for (int c = 0; c < 3; c++) {
+    rc = h2d_field(ctx, ctx->lam[0][c], 0, n, lambda_next + c * n);
+    if (rc) return rc;
+  }
+  enqueue_adjoint(ctx, ctx->U, ctx->lam[0], ctx->lam[1], s, false);
   HIPCHK(ctx, hipGetLastError());
-  for (int c = 0; c < 3; c++)
-    HIPCHK(ctx, hipMemcpyAsync(lambda_out + c * n, ctx->lam[1][c], nb, hipMemcpyDeviceToHost,
-                               ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int c = 0; c < 3; c++) {
+    rc = d2h_field(ctx, ctx->lam[1][c], 0, n, lambda_out + c * n);
+    if (rc) return rc;
+  }
   ctx->run_steps = 0;  // the dt record slot was overwritten
   return TG_OK;
 }
@@ -667,16 +832,38 @@ int tg_time_kernel(tg_ctx* ctx, const tg_params* prm, int which, int reps, doubl
     return set_err(ctx, TG_ERR_STATE, "needs a completed forward run");
   int rc = upload_params(ctx, prm);
   if (rc) return rc;
-  // every variant reads U and writes scratch (Ut or lam[1]); step 0's dt
-  StepArgs a = step_args(ctx, ctx->U, ctx->Ut, 0);
-  AdjArgs b = adj_args(ctx, ctx->U, ctx->lam[0], ctx->lam[1], 0, false);
-  launch_coef(a, false, ctx->stream);  // kact of the current state
+  // every variant reads U and writes scratch (Ut or lam[1]) with step 0's dt
+  auto one = [&](int w) {
+    if (w == 2) {
+      enqueue_adjoint(ctx, ctx->U, ctx->lam[0], ctx->lam[1], 0, false);
+    } else if (ctx->mode == TG_FP64) {
+      StepArgs a{};
+      const long long n = ctx->ncell;
+      a.inst = ctx->inst_d; a.tiles = ctx->tiles_d; a.ntiles = ctx->ntiles;
+      a.U = Fields{(double*)ctx->U[0], (double*)ctx->U[1], (double*)ctx->U[2]};
+      a.S = Statics{at<double>(ctx->statics, 0, n), at<double>(ctx->statics, 1, n),
+                    at<double>(ctx->statics, 2, n), at<double>(ctx->statics, 3, n),
+                    at<double>(ctx->statics, 4, n), at<double>(ctx->statics, 5, n),
+                    at<double>(ctx->statics, 6, n)};
+      a.kact = (double*)ctx->kact; a.dt_rec = ctx->dt_rec; a.max_steps = ctx->cfg.max_steps;
+      a.ft = frict_tiny(ctx, 0); a.tiny = ctx->cfg.tiny; a.gamma = ctx->cfg.gamma;
+      a.h_out = (double*)ctx->Ut[0]; a.hu_out = (double*)ctx->Ut[1]; a.hv_out = (double*)ctx->Ut[2];
+      a.err = ctx->err_d;
+      if (w == 0) launch_coef(a, false, ctx->stream);
+      else launch_update(a, ctx->stream);
+    } else if (ctx->es == 8) {
+      StepArgsM<double> a = step_args_m<double>(ctx, ctx->U, ctx->Ut, 0);
+      if (w == 0) launch_coef_m<double>(a, false, ctx->stream);
+      else launch_update_m<double>(a, ctx->stream);
+    } else {
+      StepArgsM<float> a = step_args_m<float>(ctx, ctx->U, ctx->Ut, 0);
+      if (w == 0) launch_coef_m<float>(a, false, ctx->stream);
+      else launch_update_m<float>(a, ctx->stream);
+    }
+  };
+  one(0);  // kact of the current state
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-  for (int r = 0; r < reps; r++) {
-    if (which == 0) launch_coef(a, false, ctx->stream);
-    else if (which == 1) launch_update(a, ctx->stream);
-    else launch_adjoint(b, ctx->stream);
-  }
+  for (int r = 0; r < reps; r++) one(which);
   HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));

@@ -12,6 +12,9 @@ struct InstDev {
   int nx, ny;
   long long off;
   double dx, dy;
+  // 1/dx, 1/(dx+dx), 1/dy, 1/(dy+dy): the dual slopes' reciprocals
+  // (dualmesh.C:1459), evaluated once on the host (same IEEE result)
+  double inv_dx, inv_sx, inv_dy, inv_sy;
   double sin_int, tan_bed, kc1, kroot, dkroot, eps, gscale;
   double maxtime, nextoutput, nextsave;
 };
@@ -86,6 +89,72 @@ struct AdjArgs {
   double* lam_hv;
   double* dphi_part;     // [ntiles] running sum of lambda^{s+1}.dU/dphi, or null
 };
+
+// Mixed-precision counterparts: FP32 arithmetic, FP32 static coefficients
+// and kact; the state and adjoint are stored as S (double: FP32 increments
+// applied to an FP64 state, float: FP32 storage).
+template <class S>
+struct FieldsT {
+  const S* h;
+  const S* hu;
+  const S* hv;
+};
+struct Statics32 {
+  const float* gx;
+  const float* gy;
+  const float* gz;
+  const float* dgx;
+  const float* dgy;
+  const float* kx;
+  const float* ky;
+};
+template <class S>
+struct StepArgsM {
+  const InstDev* inst;
+  const TileDev* tiles;
+  int ntiles;
+  FieldsT<S> U;
+  Statics32 S32;
+  float* kact;
+  RedDev* red;
+  double* dt_rec;
+  double* time;
+  int max_steps;
+  int step;
+  double ft;
+  double tiny, gamma;  // thresholds kept in FP64: the switches compare in W
+  S* h_out;
+  S* hu_out;
+  S* hv_out;
+  int* err;
+};
+template <class S>
+struct AdjArgsM {
+  const InstDev* inst;
+  const TileDev* tiles;
+  int ntiles;
+  FieldsT<S> U;
+  Statics32 S32;
+  FieldsT<S> lam_next;
+  const double* dt_rec;
+  int max_steps;
+  int step;
+  double ft;
+  double tiny, gamma;
+  S* lam_h;
+  S* lam_hu;
+  S* lam_hv;
+  double* dphi_part;
+};
+
+template <class S>
+void launch_coef_m(const StepArgsM<S>& a, bool reduce, hipStream_t s);
+template <class S>
+void launch_update_m(const StepArgsM<S>& a, hipStream_t s);
+template <class S>
+void launch_adjoint_m(const AdjArgsM<S>& a, hipStream_t s);
+void launch_stats32(const InstDev* inst, const TileDev* tiles, int ntiles, FieldsT<float> U,
+                    double* part_mass, double* part_hmax, double tiny, hipStream_t s);
 
 void launch_coef(const StepArgs& a, bool reduce, hipStream_t s);
 void launch_dt(const StepArgs& a, int ninst, hipStream_t s);

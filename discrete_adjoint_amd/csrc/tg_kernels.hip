@@ -243,8 +243,8 @@ __device__ __forceinline__ DualInfo dual_info(const AdjArgs& a, const InstDev& I
 #pragma unroll
   for (int k = 0; k < 3; k++) r.hsx[k] = r.hsy[k] = 0.;
   if (i + 1 < I.nx && i > 0 && j + 1 < I.ny && j > 0) {
-    const double ix = 1 / I.dx, isx = 1 / (I.dx + I.dx);
-    const double iy = 1 / I.dy, isy = 1 / (I.dy + I.dy);
+    const double ix = I.inv_dx, isx = I.inv_sx;
+    const double iy = I.inv_dy, isy = I.inv_sy;
     DualSlope s = slope_dual(a.U.h[c + 1], r.h, a.U.h[c - 1], I.dx, ix, isx, a.gamma, a.tiny, true);
     r.d[0] = s.d; r.hsx[0] = s.s0; r.hsx[1] = s.s1; r.hsx[2] = s.s2;
     r.d[1] = slope_dual(a.U.hu[c + 1], r.hu, a.U.hu[c - 1], I.dx, ix, isx, a.gamma, a.tiny, false).d;
@@ -264,16 +264,19 @@ __device__ __forceinline__ DualInfo dual_info(const AdjArgs& a, const InstDev& I
 }
 
 // J = dtdx*(px - nx) + dtdy*(py - ny), calc_jacobian_elem
-// (compute_jacobian_elem.C:21-24); null operands are ZERO_MATRIX.
-__device__ __forceinline__ M3 flux_block(const M3* px, const M3* nxm, const M3* py, const M3* nym,
+// (compute_jacobian_elem.C:21-24); an operand whose flag is false is
+// ZERO_MATRIX.  Operands are taken by value and selected element by element
+// so the blocks stay in registers.
+__device__ __forceinline__ M3 flux_block(const M3& px, bool ux, const M3& nxm, bool unx,
+                                         const M3& py, bool uy, const M3& nym, bool uny,
                                          double dtdx, double dtdy) {
   M3 J;
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      double a = px ? px->a[i][j] : 0., b = nxm ? nxm->a[i][j] : 0.;
-      double c = py ? py->a[i][j] : 0., d = nym ? nym->a[i][j] : 0.;
+      double a = ux ? px.a[i][j] : 0., b = unx ? nxm.a[i][j] : 0.;
+      double c = uy ? py.a[i][j] : 0., d = uny ? nym.a[i][j] : 0.;
       J.a[i][j] = +dtdx * (a - b) + dtdy * (c - d);
     }
   return J;
@@ -351,44 +354,47 @@ __device__ __forceinline__ void driemann(const Hfv& l, const Hfv& r, const doubl
   }
 }
 
+// face flux and its derivative along d kact/d phi_bed (both sides)
+__device__ __forceinline__ void face_dk(const AdjArgs& a, int side, const DualInfo& L,
+                                        const DualInfo& R, double flux[3], double dflux[3]) {
+  Hfv hl = hfv_fwd(side, L.h, L.hu, L.hv, L.k, L.g2, a.tiny);
+  Hfv hr = hfv_fwd(side, R.h, R.hu, R.hv, R.k, R.g2, a.tiny);
+  riemann(hl, hr, flux);
+  double dFl[3], dWl[3], dFr[3], dWr[3];
+  dhfv_dk(side, L.h, L.k, L.g2, a.tiny, dFl, dWl);
+  dhfv_dk(side, R.h, R.k, R.g2, a.tiny, dFr, dWr);
+#pragma unroll
+  for (int v = 0; v < 3; v++) {
+    dFl[v] *= L.dk; dWl[v] *= L.dk; dFr[v] *= R.dk; dWr[v] *= R.dk;
+  }
+  driemann(hl, hr, dFl, dWl, dFr, dWr, dflux);
+}
+
 // lambda^{s+1}_c . d U^{s+1}_c / d phi_bed at frozen dt (bed-friction
-// gradient; no reference counterpart).  ci[0..4] = self, E, N, W, S.
+// gradient; no reference counterpart).  Faces with no neighbour copy the
+// opposite face (outflow), as in the forward step.
 __device__ double dphi_term(const AdjArgs& a, const InstDev& I, const Cell& q,
-                            const DualInfo ci[5], const bool has[5], double dt,
+                            const DualInfo& C, const DualInfo& E, const DualInfo& N,
+                            const DualInfo& Wc, const DualInfo& Sc, double dt,
                             const double lam[3]) {
-  // faces E, N, W, S: (left, right) cell slots, with outflow copies
-  double flux[4][3], dflux[4][3];
-  for (int f = 0; f < 4; f++) {
-    const int side = f & 1;
-    int l, r;
-    bool ok = true;
-    if (f == 0) { if (has[1]) { l = 0; r = 1; } else if (has[3]) { l = 3; r = 0; } else ok = false; }
-    else if (f == 1) { if (has[2]) { l = 0; r = 2; } else if (has[4]) { l = 4; r = 0; } else ok = false; }
-    else if (f == 2) { if (has[3]) { l = 3; r = 0; } else if (has[1]) { l = 0; r = 1; } else ok = false; }
-    else { if (has[4]) { l = 4; r = 0; } else if (has[2]) { l = 0; r = 2; } else ok = false; }
-    if (!ok) {
-      for (int v = 0; v < 3; v++) flux[f][v] = dflux[f][v] = 0.;
-      continue;
-    }
-    const DualInfo& L = ci[l];
-    const DualInfo& R = ci[r];
-    Hfv hl = hfv_fwd(side, L.h, L.hu, L.hv, L.k, L.g2, a.tiny);
-    Hfv hr = hfv_fwd(side, R.h, R.hu, R.hv, R.k, R.g2, a.tiny);
-    riemann(hl, hr, flux[f]);
-    double dFl[3], dWl[3], dFr[3], dWr[3];
-    dhfv_dk(side, L.h, L.k, L.g2, a.tiny, dFl, dWl);
-    dhfv_dk(side, R.h, R.k, R.g2, a.tiny, dFr, dWr);
-    for (int v = 0; v < 3; v++) {
-      dFl[v] *= L.dk; dWl[v] *= L.dk; dFr[v] *= R.dk; dWr[v] *= R.dk;
-    }
-    driemann(hl, hr, dFl, dWl, dFr, dWr, dflux[f]);
+  double fE[3] = {0., 0., 0.}, fN[3] = {0., 0., 0.}, fW[3] = {0., 0., 0.}, fS[3] = {0., 0., 0.};
+  double dE[3] = {0., 0., 0.}, dN[3] = {0., 0., 0.}, dW[3] = {0., 0., 0.}, dS[3] = {0., 0., 0.};
+  if (q.hasE) face_dk(a, 0, C, E, fE, dE);
+  if (q.hasW) face_dk(a, 0, Wc, C, fW, dW);
+  if (q.hasN) face_dk(a, 1, C, N, fN, dN);
+  if (q.hasS) face_dk(a, 1, Sc, C, fS, dS);
+#pragma unroll
+  for (int v = 0; v < 3; v++) {
+    if (!q.hasE) { fE[v] = fW[v]; dE[v] = dW[v]; }
+    if (!q.hasW) { fW[v] = fE[v]; dW[v] = dE[v]; }
+    if (!q.hasN) { fN[v] = fS[v]; dN[v] = dS[v]; }
+    if (!q.hasS) { fS[v] = fN[v]; dS[v] = dN[v]; }
   }
   const double dtdx = dt / I.dx, dtdy = dt / I.dy;
   double out[3];
-  for (int v = 0; v < 3; v++)
-    out[v] = -dtdx * (dflux[0][v] - dflux[2][v]) - dtdy * (dflux[1][v] - dflux[3][v]);
-  const DualInfo& C = ci[0];
-  if (C.h - dtdx * (flux[0][0] - flux[2][0]) - dtdy * (flux[1][0] - flux[3][0]) < 0.) out[0] = 0.;
+#pragma unroll
+  for (int v = 0; v < 3; v++) out[v] = -dtdx * (dE[v] - dW[v]) - dtdy * (dN[v] - dS[v]);
+  if (C.h - dtdx * (fE[0] - fW[0]) - dtdy * (fN[0] - fS[0]) < 0.) out[0] = 0.;
   if (C.h > a.tiny) {
     const long long c = q.c;
     const double sec2 = 1. + I.tan_bed * I.tan_bed;
@@ -404,45 +410,114 @@ __device__ double dphi_term(const AdjArgs& a, const InstDev& I, const Cell& q,
   return lam[0] * out[0] + lam[1] * out[1] + lam[2] * out[2];
 }
 
+// The part of a cell's dual record its neighbours read, staged in LDS for the
+// tile plus a one-cell halo: each record is computed once per workgroup
+// instead of once per reading cell.
+constexpr int RX = TILE_X + 2, RY = TILE_Y + 2, RN = RX * RY, NHALO = RN - BLOCK;
+struct DualLds {
+  double h[RN], hu[RN], hv[RN], g2[RN], k[RN], dk[RN];
+  double hsx1[RN], hsx2[RN], hsy1[RN], hsy2[RN];
+  int flags[RN];  // sgn0+1 | (sgn1+1)<<2 | stop0<<4 | stop1<<5
+};
+
+__device__ __forceinline__ void lds_put(DualLds& L, int r, const DualInfo& d) {
+  L.h[r] = d.h; L.hu[r] = d.hu; L.hv[r] = d.hv; L.g2[r] = d.g2; L.k[r] = d.k; L.dk[r] = d.dk;
+  L.hsx1[r] = d.hsx[1]; L.hsx2[r] = d.hsx[2]; L.hsy1[r] = d.hsy[1]; L.hsy2[r] = d.hsy[2];
+  L.flags[r] = (int)(d.si.sgn[0] + 1.) | ((int)(d.si.sgn[1] + 1.) << 2) | (d.si.stop[0] << 4) |
+               (d.si.stop[1] << 5);
+}
+
+// Rebuilds the fields a neighbour block and the face Jacobians use; the
+// other fields (own slopes, sgn[2..3]) belong to the owning cell only.
+__device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
+  DualInfo d;
+  d.h = L.h[r]; d.hu = L.hu[r]; d.hv = L.hv[r]; d.g2 = L.g2[r]; d.k = L.k[r]; d.dk = L.dk[r];
+  d.hsx[0] = d.hsy[0] = 0.;
+  d.hsx[1] = L.hsx1[r]; d.hsx[2] = L.hsx2[r]; d.hsy[1] = L.hsy1[r]; d.hsy[2] = L.hsy2[r];
+  const int f = L.flags[r];
+  d.si.sgn[0] = (double)((f & 3) - 1);
+  d.si.sgn[1] = (double)(((f >> 2) & 3) - 1);
+  d.si.sgn[2] = d.si.sgn[3] = 0.;
+  d.si.stop[0] = (f >> 4) & 1;
+  d.si.stop[1] = (f >> 5) & 1;
+#pragma unroll
+  for (int k = 0; k < 6; k++) d.d[k] = 0.;
+  return d;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of(t, I);
   double dphi = 0.;
   const bool want_dphi = a.dphi_part != nullptr;
+  __shared__ DualLds L;
+  const int tx = (int)(threadIdx.x % TILE_X), ty = (int)(threadIdx.x / TILE_X);
+  const int rself = (ty + 1) * RX + tx + 1;
+  // phase 1: dual records of the tile (kept in registers) and of the halo
+  DualInfo self;
+  if (q.act) {
+    self = dual_info(a, I, q.i, q.j, want_dphi);
+    lds_put(L, rself, self);
+  }
+  if ((int)threadIdx.x < NHALO) {
+    const int h = (int)threadIdx.x;
+    int rx, ry;
+    if (h < RX) { rx = h; ry = 0; }
+    else if (h < 2 * RX) { rx = h - RX; ry = RY - 1; }
+    else if (h < 2 * RX + TILE_Y) { rx = 0; ry = h - 2 * RX + 1; }
+    else { rx = RX - 1; ry = h - 2 * RX - TILE_Y + 1; }
+    const int gi = t.x0 - 1 + rx, gj = t.y0 - 1 + ry;
+    if (gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny)
+      lds_put(L, ry * RX + rx, dual_info(a, I, gi, gj, want_dphi));
+  }
+  __syncthreads();
   if (q.act) {
     const long long c = q.c, nx = I.nx;
     const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
     const double dtdx = dt / I.dx, dtdy = dt / I.dy;
-    const double hc = a.U.h[c];
+    const double hc = self.h;
     double adj[3] = {0., 0., 0.};
     // slots: 0 self, 1 E, 2 N, 3 W, 4 S
     const bool has[5] = {true, q.hasE, q.hasN, q.hasW, q.hasS};
     const long long nb[5] = {c, c + 1, c + nx, c - 1, c - nx};
-    const int ni[5] = {q.i, q.i + 1, q.i, q.i - 1, q.i};
-    const int nj[5] = {q.j, q.j, q.j + 1, q.j, q.j - 1};
-    DualInfo ci[5];
-    for (int s = 0; s < 5; s++)
-      if (has[s]) ci[s] = dual_info(a, I, ni[s], nj[s], want_dphi);
+    const int rn[5] = {rself, rself + 1, rself + RX, rself - 1, rself - RX};
+    // Neighbour records are re-read from LDS at each use (not held in
+    // registers across the phases); the empty asm statements keep the
+    // compiler from merging the re-reads into long-lived registers.
+    auto nbr_hfv = [&](int dir, int r) {
+      return dual_hfv(dir, L.h[r], L.hu[r], L.hv[r], L.k[r], L.g2[r], a.tiny);
+    };
     if (hc != 0.) {  // a void cell receives nothing (calc_jacobain.C:117-122)
       // c-side Jacobians of the four faces (dual_riemannflux)
-      const DualHfv cx = dual_hfv(0, ci[0].h, ci[0].hu, ci[0].hv, ci[0].k, ci[0].g2, a.tiny);
-      const DualHfv cy = dual_hfv(1, ci[0].h, ci[0].hu, ci[0].hv, ci[0].k, ci[0].g2, a.tiny);
       M3 JE = zero3(), JN = zero3(), JW = zero3(), JS = zero3();
-      if (has[1]) JE = dual_riemann_side(cx, dual_hfv(0, ci[1].h, ci[1].hu, ci[1].hv, ci[1].k, ci[1].g2, a.tiny), true);
-      if (has[2]) JN = dual_riemann_side(cy, dual_hfv(1, ci[2].h, ci[2].hu, ci[2].hv, ci[2].k, ci[2].g2, a.tiny), true);
-      if (has[3]) JW = dual_riemann_side(dual_hfv(0, ci[3].h, ci[3].hu, ci[3].hv, ci[3].k, ci[3].g2, a.tiny), cx, false);
-      if (has[4]) JS = dual_riemann_side(dual_hfv(1, ci[4].h, ci[4].hu, ci[4].hv, ci[4].k, ci[4].g2, a.tiny), cy, false);
+      {
+        const DualHfv cx = dual_hfv(0, self.h, self.hu, self.hv, self.k, self.g2, a.tiny);
+        if (has[1]) JE = dual_riemann_side(cx, nbr_hfv(0, rn[1]), true);
+        if (has[3]) JW = dual_riemann_side(nbr_hfv(0, rn[3]), cx, false);
+      }
+      {
+        const DualHfv cy = dual_hfv(1, self.h, self.hu, self.hv, self.k, self.g2, a.tiny);
+        if (has[2]) JN = dual_riemann_side(cy, nbr_hfv(1, rn[2]), true);
+        if (has[4]) JS = dual_riemann_side(nbr_hfv(1, rn[4]), cy, false);
+      }
+      asm volatile("" ::: "memory");
 
       // ---- self block (effelement 0): fjc(.,.,0) with boundary copies
       {
-        const M3* pE = has[1] ? &JE : (has[3] ? &JW : nullptr);
-        const M3* nW = has[3] ? &JW : (has[1] ? &JE : nullptr);
-        const M3* pN = has[2] ? &JN : (has[4] ? &JS : nullptr);
-        const M3* nS = has[4] ? &JS : (has[2] ? &JN : nullptr);
-        M3 J = flux_block(pE, nW, pN, nS, dtdx, dtdy);
+        M3 J;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+          for (int j = 0; j < 3; j++) {
+            const double pE = has[1] ? JE.a[i][j] : (has[3] ? JW.a[i][j] : 0.);
+            const double nW = has[3] ? JW.a[i][j] : (has[1] ? JE.a[i][j] : 0.);
+            const double pN = has[2] ? JN.a[i][j] : (has[4] ? JS.a[i][j] : 0.);
+            const double nS = has[4] ? JS.a[i][j] : (has[2] ? JN.a[i][j] : 0.);
+            J.a[i][j] = +dtdx * (pE - nW) + dtdy * (pN - nS);
+          }
         for (int i = 0; i < 3; i++) J.a[i][i] -= 1.;
-        const DualInfo& C = ci[0];
+        const DualInfo& C = self;
         if (C.h > a.tiny) {
           const double alpha = I.sin_int * C.si.sgn[0] * C.k;
           const double betta = I.sin_int * C.si.sgn[1] * C.k;
@@ -470,27 +545,28 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
       // ---- neighbour blocks in the reference's visiting order S, E, N, W
       // (neighbor[0..3] with positive_x_side = 1).
       const int order[4] = {4, 1, 2, 3};
+#pragma unroll
       for (int o = 0; o < 4; o++) {
         const int s = order[o];
         if (!has[s]) continue;
-        const DualInfo& N = ci[s];
+        const DualInfo N = lds_get(L, rn[s]);
         const long long n = nb[s];
         M3 J;
         if (s == 1) {         // c is the xm neighbour of n
           const bool bnd = !(q.i + 2 < I.nx);
-          J = flux_block(bnd ? &JE : nullptr, &JE, nullptr, nullptr, dtdx, dtdy);
+          J = flux_block(JE, bnd, JE, true, JE, false, JE, false, dtdx, dtdy);
           nbr_source(J, N, I.sin_int, dt, N.hsx[1], 0., a.tiny);
         } else if (s == 3) {  // c is the xp neighbour of n
           const bool bnd = !(q.i - 2 >= 0);
-          J = flux_block(&JW, bnd ? &JW : nullptr, nullptr, nullptr, dtdx, dtdy);
+          J = flux_block(JW, true, JW, bnd, JW, false, JW, false, dtdx, dtdy);
           nbr_source(J, N, I.sin_int, dt, N.hsx[2], 0., a.tiny);
         } else if (s == 2) {  // c is the ym neighbour of n
           const bool bnd = !(q.j + 2 < I.ny);
-          J = flux_block(nullptr, nullptr, bnd ? &JN : nullptr, &JN, dtdx, dtdy);
+          J = flux_block(JN, false, JN, false, JN, bnd, JN, true, dtdx, dtdy);
           nbr_source(J, N, I.sin_int, dt, 0., N.hsy[1], a.tiny);
         } else {              // c is the yp neighbour of n
           const bool bnd = !(q.j - 2 >= 0);
-          J = flux_block(nullptr, nullptr, &JS, bnd ? &JS : nullptr, dtdx, dtdy);
+          J = flux_block(JS, false, JS, false, JS, true, JS, bnd, dtdx, dtdy);
           nbr_source(J, N, I.sin_int, dt, 0., N.hsy[2], a.tiny);
         }
         const double lam[3] = {a.lam_next.h[n], a.lam_next.hu[n], a.lam_next.hv[n]};
@@ -508,7 +584,11 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
     a.lam_hv[c] = out[2];
     if (want_dphi) {
       const double lam[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
-      dphi = dphi_term(a, I, q, ci, has, dt, lam);
+      asm volatile("" ::: "memory");
+      // every rn[] slot lies inside the LDS tile; records of absent
+      // neighbours are passed but not read (dphi_term checks q.has*)
+      dphi = dphi_term(a, I, q, self, lds_get(L, rn[1]), lds_get(L, rn[2]), lds_get(L, rn[3]),
+                       lds_get(L, rn[4]), dt, lam);
     }
   }
   if (want_dphi) {

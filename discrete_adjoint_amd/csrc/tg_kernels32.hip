@@ -1,0 +1,708 @@
+// Mixed-precision kernels.  Same tiles, tile table and algorithm as the
+// FP64 kernels (tg_kernels.hip).  Template parameter S is the storage type
+// of the state and adjoint:
+//   S = double (TG_MIXED): FP32 increments applied to FP64 state/adjoint;
+//                          switch arguments in FP64 (W = double)
+//   S = float  (TG_FP32):  FP32 storage, switches in FP32 (W = float)
+// Static coefficients and kact are FP32 in both; the CFL reduction, dt,
+// time, mass and the bed-friction gradient sums are FP64.
+#include <type_traits>
+
+#include "tg_kernels.h"
+#include "tg_math32.h"
+
+namespace tg {
+
+using namespace m32;
+
+namespace {
+
+template <class S>
+using Wt = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
+
+__device__ __forceinline__ unsigned long long dbits(double v) {
+  return (unsigned long long)__double_as_longlong(v);
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_xor(v, o, 64);
+    v = (w < v) ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_xor(v, o, 64);
+    v = (w > v) ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+struct Cell {
+  int i, j;
+  long long c;
+  bool act, hasE, hasW, hasN, hasS;
+};
+__device__ __forceinline__ Cell cell_of(const TileDev& t, const InstDev& I) {
+  Cell q;
+  q.i = t.x0 + (int)(threadIdx.x % TILE_X);
+  q.j = t.y0 + (int)(threadIdx.x / TILE_X);
+  q.act = q.i < I.nx && q.j < I.ny;
+  q.c = I.off + (long long)q.j * I.nx + q.i;
+  q.hasE = q.i + 1 < I.nx;
+  q.hasW = q.i > 0;
+  q.hasN = q.j + 1 < I.ny;
+  q.hasS = q.j > 0;
+  return q;
+}
+
+// ---------------------------------------------------------------- forward --
+
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k32_coef(StepArgsM<S> a, int reduce) {
+  using W = Wt<S>;
+  const TileDev t = a.tiles[blockIdx.x];
+  const InstDev I = a.inst[t.inst];
+  const Cell q = cell_of(t, I);
+  const W tiny = (W)a.tiny, gamma = (W)a.gamma;
+  double minev = HUGE_VAL, hmax = 0.;
+  if (q.act) {
+    const long long c = q.c, nx = I.nx;
+    const W h = (W)a.U.h[c], hu = (W)a.U.hu[c], hv = (W)a.U.hv[c];
+    const W ix = (W)I.inv_dx, iy = (W)I.inv_dy;
+    W hx = 0, hux = 0, hy = 0, hvy = 0;
+    if (q.hasE && q.hasW && q.hasN && q.hasS) {
+      hx = slope<W>((W)a.U.h[c + 1], h, (W)a.U.h[c - 1], ix, gamma, tiny);
+      hux = slope<W>((W)a.U.hu[c + 1], hu, (W)a.U.hu[c - 1], ix, gamma, tiny);
+      hy = slope<W>((W)a.U.h[c + nx], h, (W)a.U.h[c - nx], iy, gamma, tiny);
+      hvy = slope<W>((W)a.U.hv[c + nx], hv, (W)a.U.hv[c - nx], iy, gamma, tiny);
+    }
+    float k = (float)I.eps;
+    if (h > tiny) {
+      const W ih = inv_h(h);
+      const W u = vel(hu, h, ih), v = vel(hv, h, ih);
+      k = kact<W>(h, u, v, ih, hx, hux, hy, hvy, (float)I.kc1, (float)I.kroot, (float)I.eps,
+                  tiny);
+      if (reduce) {
+        const float c2 = fsqrt(k * a.S32.gz[c] * (float)h);
+        const double ev = (double)fmax(fabs(u), fabs(v)) + (double)c2;
+        if (ev > 1000000000.) atomicOr(a.err, 2);
+        minev = ((I.dx > I.dy) ? I.dy : I.dx) / ev;
+        hmax = (double)h;
+      }
+    }
+    a.kact[c] = k;
+  }
+  if (reduce) {
+    __shared__ double smin[BLOCK / 64], smax[BLOCK / 64];
+    minev = wave_min(minev);
+    hmax = wave_max(hmax);
+    if (threadIdx.x % 64 == 0) { smin[threadIdx.x / 64] = minev; smax[threadIdx.x / 64] = hmax; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < BLOCK / 64; k++) {
+        minev = (smin[k] < minev) ? smin[k] : minev;
+        hmax = (smax[k] > hmax) ? smax[k] : hmax;
+      }
+      if (minev < HUGE_VAL) atomicMin(&a.red[t.inst].min_ev, dbits(minev));
+      if (hmax > 0.) atomicMax(&a.red[t.inst].hmax, dbits(hmax));
+    }
+  }
+}
+
+template <class S>
+__device__ __forceinline__ Hfv hfv_at(const StepArgsM<S>& a, int dir, long long c) {
+  using W = Wt<S>;
+  const W h = (W)a.U.h[c], hu = (W)a.U.hu[c], hv = (W)a.U.hv[c];
+  const bool wet = !(h < (W)a.tiny);
+  const W un = wet ? vel(dir ? hv : hu, h, inv_h(h)) : W(0);
+  return hfv(dir, (float)h, (float)hu, (float)hv, (float)un, a.kact[c], a.S32.gz[c], wet);
+}
+
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k32_update(StepArgsM<S> a) {
+  using W = Wt<S>;
+  const TileDev t = a.tiles[blockIdx.x];
+  const InstDev I = a.inst[t.inst];
+  const Cell q = cell_of(t, I);
+  if (!q.act) return;
+  const long long c = q.c, nx = I.nx;
+  const W tiny = (W)a.tiny, gamma = (W)a.gamma;
+  const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
+  const float dtf = (float)dt, dtdx = (float)(dt * I.inv_dx), dtdy = (float)(dt * I.inv_dy);
+  const S hs = a.U.h[c], hus = a.U.hu[c], hvs = a.U.hv[c];
+  const W h = (W)hs, hu = (W)hus, hv = (W)hvs;
+  const W ix = (W)I.inv_dx, iy = (W)I.inv_dy;
+  W hx = 0, hvx = 0, hy = 0, huy = 0;
+  if (q.hasE && q.hasW && q.hasN && q.hasS) {
+    hx = slope<W>((W)a.U.h[c + 1], h, (W)a.U.h[c - 1], ix, gamma, tiny);
+    hvx = slope<W>((W)a.U.hv[c + 1], hv, (W)a.U.hv[c - 1], ix, gamma, tiny);
+    hy = slope<W>((W)a.U.h[c + nx], h, (W)a.U.h[c - nx], iy, gamma, tiny);
+    huy = slope<W>((W)a.U.hu[c + nx], hu, (W)a.U.hu[c - nx], iy, gamma, tiny);
+  }
+  float fE[3] = {0.f, 0.f, 0.f}, fW[3] = {0.f, 0.f, 0.f}, fN[3] = {0.f, 0.f, 0.f},
+        fS[3] = {0.f, 0.f, 0.f};
+  {
+    const Hfv self = hfv_at(a, 0, c);
+    if (q.hasE) riemann(self, hfv_at(a, 0, c + 1), fE);
+    if (q.hasW) riemann(hfv_at(a, 0, c - 1), self, fW);
+    if (!q.hasE) { fE[0] = fW[0]; fE[1] = fW[1]; fE[2] = fW[2]; }
+    if (!q.hasW) { fW[0] = fE[0]; fW[1] = fE[1]; fW[2] = fE[2]; }
+  }
+  {
+    const Hfv self = hfv_at(a, 1, c);
+    if (q.hasN) riemann(self, hfv_at(a, 1, c + nx), fN);
+    if (q.hasS) riemann(hfv_at(a, 1, c - nx), self, fS);
+    if (!q.hasN) { fN[0] = fS[0]; fN[1] = fS[1]; fN[2] = fS[2]; }
+    if (!q.hasS) { fS[0] = fN[0]; fS[1] = fN[1]; fS[2] = fN[2]; }
+  }
+  // increments in FP32, applied to the stored state type S
+  float inc[3];
+#pragma unroll
+  for (int v = 0; v < 3; v++) inc[v] = -dtdx * (fE[v] - fW[v]) - dtdy * (fN[v] - fS[v]);
+  if (h > tiny) {
+    const W ih = inv_h(h);
+    const W u = vel(hu, h, ih), v = vel(hv, h, ih);
+    const W ft = (W)a.ft;
+    const float sgn0 = (float)tiny_sgn<W>(ih * (huy - u * hy), ft);
+    const float sgn2 = (float)tiny_sgn<W>(u, ft);
+    const float sgn1 = (float)tiny_sgn<W>(ih * (hvx - v * hx), ft);
+    const float sgn3 = (float)tiny_sgn<W>(v, ft);
+    const float k = a.kact[c], g2 = a.S32.gz[c];
+    const float hf = (float)h, uf = (float)u, vf = (float)v, huf = (float)hu, hvf = (float)hv;
+    const float sint = (float)I.sin_int, tanb = (float)I.tan_bed;
+    float s1 = a.S32.gx[c] * hf;
+    float s2 = sgn0 * hf * k * (g2 * (float)hy + a.S32.dgy[c] * hf) * sint;
+    float s3 = sgn2 * fmaxf(g2 * hf + uf * huf * a.S32.kx[c], 0.f) * tanb;
+    inc[1] += dtf * (s1 - s2 - s3);
+    s1 = a.S32.gy[c] * hf;
+    s2 = sgn1 * hf * k * (g2 * (float)hx + a.S32.dgx[c] * hf) * sint;
+    s3 = sgn3 * fmaxf(g2 * hf + vf * hvf * a.S32.ky[c], 0.f) * tanb;
+    inc[2] += dtf * (s1 - s2 - s3);
+  }
+  S out0 = hs + (S)inc[0];
+  if (out0 < (S)0) out0 = (S)0;
+  const S out1 = hus + (S)inc[1], out2 = hvs + (S)inc[2];
+  if (!isfinite((double)out0) || !isfinite((double)out1) || !isfinite((double)out2))
+    atomicOr(a.err, 1);
+  a.h_out[c] = out0;
+  a.hu_out[c] = out1;
+  a.hv_out[c] = out2;
+}
+
+__global__ __launch_bounds__(BLOCK) void k32_stats(const InstDev* inst, const TileDev* tiles,
+                                                   FieldsT<float> U, double* part_mass,
+                                                   double* part_hmax, double tiny) {
+  const TileDev t = tiles[blockIdx.x];
+  const InstDev I = inst[t.inst];
+  const Cell q = cell_of(t, I);
+  double m = 0., hm = 0.;
+  if (q.act) {
+    double h = U.h[q.c];
+    m = I.dx * I.dy * h;
+    if (h > tiny) hm = h;
+  }
+  __shared__ double sm[BLOCK / 64], sh[BLOCK / 64];
+  m = wave_sum(m);
+  hm = wave_max(hm);
+  if (threadIdx.x % 64 == 0) { sm[threadIdx.x / 64] = m; sh[threadIdx.x / 64] = hm; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < BLOCK / 64; k++) { m += sm[k]; hm = (sh[k] > hm) ? sh[k] : hm; }
+    part_mass[blockIdx.x] = m;
+    part_hmax[blockIdx.x] = hm;
+  }
+}
+
+// ---------------------------------------------------------------- adjoint --
+
+struct M3f {
+  float a[3][3];
+};
+__device__ __forceinline__ M3f zero3f() {
+  M3f m;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) m.a[i][j] = 0.f;
+  return m;
+}
+
+// What the Jacobian blocks need about one cell at U^s.  Switch decisions
+// (slopes, hslope branch, kact sign, sgn, stop, wet, thick) are taken in W
+// before the values are narrowed to FP32.
+struct DualInfo {
+  float h, hu, hv, g2, ih, u, v, k, dk;
+  float hx, hy;           // own h slopes
+  float hsx[3], hsy[3];   // hslope_sens
+  float sgn[4];
+  int stop[2];
+  int wet;                // !(h < tiny)
+  int thick;              // h > tiny
+  int nz;                 // h != 0
+};
+
+struct DualHfv {
+  Hfv b;
+  M3f fj, sj;
+};
+__device__ __forceinline__ DualHfv dual_hfv(int dir, const DualInfo& d) {
+  DualHfv r;
+  const float u = d.u, v = d.v, ih = d.ih, h = d.h;
+  r.b = hfv(dir, h, d.hu, d.hv, dir ? v : u, d.k, d.g2, d.wet);
+  r.fj = zero3f();
+  r.sj = zero3f();
+  if (!d.wet) return r;
+  const float a2 = d.k * h * d.g2, a = fsqrt(a2);
+  if (dir == 0) {
+    r.fj.a[0][1] = 1.f;
+    r.fj.a[1][0] = a2 - u * u; r.fj.a[1][1] = 2.f * u;
+    r.fj.a[2][0] = -u * v; r.fj.a[2][1] = v; r.fj.a[2][2] = u;
+    r.sj.a[0][0] = (-0.5f * a - u) * ih; r.sj.a[0][1] = ih;
+    r.sj.a[1][0] = -u * ih; r.sj.a[1][1] = ih;
+    r.sj.a[2][0] = (0.5f * a - u) * ih; r.sj.a[2][1] = ih;
+  } else {
+    r.fj.a[0][2] = 1.f;
+    r.fj.a[1][0] = -v * u; r.fj.a[1][1] = v; r.fj.a[1][2] = u;
+    r.fj.a[2][0] = a2 - v * v; r.fj.a[2][2] = 2.f * v;
+    r.sj.a[0][0] = (-0.5f * a - v) * ih; r.sj.a[0][2] = ih;
+    r.sj.a[1][0] = -v * ih; r.sj.a[1][2] = ih;
+    r.sj.a[2][0] = (0.5f * a - v) * ih; r.sj.a[2][2] = ih;
+  }
+  return r;
+}
+
+// d(HLL face flux)/d(U of one side), branch logic of dual_riemannflux
+// (dualmesh.C:1628-1843)
+__device__ __forceinline__ M3f dual_riemann_side(const DualHfv& l, const DualHfv& r,
+                                                 bool want_left) {
+  M3f j = zero3f();
+  const Speeds s = speeds(l.b, r.b);
+  if (s.mode == 0) return j;
+  if (s.mode == 1) return want_left ? l.fj : j;
+  if (s.mode == 2) return want_left ? j : r.fj;
+  float dsl[3], dsr[3];
+  const DualHfv& me = want_left ? l : r;
+  const bool ldry = l.b.u[0] == 0.f, rdry = r.b.u[0] == 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; i++) { dsl[i] = 0.f; dsr[i] = 0.f; }
+  if (ldry || rdry) {
+    const bool mine_wet = want_left ? !ldry : !rdry;
+    if (mine_wet) {
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        dsl[i] = (s.sl == 0.f) ? 0.f : 2.f * me.sj.a[0][i] - me.sj.a[1][i];
+        dsr[i] = (s.sr == 0.f) ? 0.f : 2.f * me.sj.a[2][i] - me.sj.a[1][i];
+      }
+    }
+  } else {
+    const bool sl_left = s.sl == l.b.w[0], sr_left = s.sr == l.b.w[2];
+    const bool sl_mine = (s.sl != 0.f) && (want_left ? sl_left : !sl_left);
+    const bool sr_mine = (s.sr != 0.f) && (want_left ? sr_left : !sr_left);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      dsl[i] = sl_mine ? me.sj.a[0][i] : 0.f;
+      dsr[i] = sr_mine ? me.sj.a[2][i] : 0.f;
+    }
+  }
+  const float inv = rcp(s.sr - s.sl);
+  float dfdsr[3], dfdsl[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const float vd = r.b.u[i] - l.b.u[i];
+    const float flux = (s.sr * l.b.f[i] - s.sl * r.b.f[i] + s.sl * s.sr * vd) * inv;
+    dfdsr[i] = (l.b.f[i] + s.sl * vd - flux) * inv;
+    dfdsl[i] = (-r.b.f[i] + s.sr * vd + flux) * inv;
+  }
+  const float dF = want_left ? s.sr * inv : -s.sl * inv;
+  const float jv = want_left ? -s.sr * s.sl * inv : s.sr * s.sl * inv;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      j.a[i][k] = dfdsr[i] * dsr[k] + dfdsl[i] * dsl[k] + dF * me.fj.a[i][k] + ((i == k) ? jv : 0.f);
+  return j;
+}
+
+template <class S>
+__device__ __forceinline__ DualInfo dual_info(const AdjArgsM<S>& a, const InstDev& I, int i, int j,
+                                              bool want_dk) {
+  using W = Wt<S>;
+  DualInfo r;
+  const W tiny = (W)a.tiny, gamma = (W)a.gamma;
+  const long long nx = I.nx, c = I.off + (long long)j * nx + i;
+  const W h = (W)a.U.h[c], hu = (W)a.U.hu[c], hv = (W)a.U.hv[c];
+  W d[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 3; k++) r.hsx[k] = r.hsy[k] = 0.f;
+  if (i + 1 < I.nx && i > 0 && j + 1 < I.ny && j > 0) {
+    const W ix = (W)I.inv_dx, iy = (W)I.inv_dy;
+    Slope<W> s = slope_sens<W>((W)a.U.h[c + 1], h, (W)a.U.h[c - 1], ix, gamma, tiny);
+    d[0] = s.d; r.hsx[0] = (float)s.s0; r.hsx[1] = (float)s.s1; r.hsx[2] = (float)s.s2;
+    d[1] = slope<W>((W)a.U.hu[c + 1], hu, (W)a.U.hu[c - 1], ix, gamma, tiny);
+    d[2] = slope<W>((W)a.U.hv[c + 1], hv, (W)a.U.hv[c - 1], ix, gamma, tiny);
+    s = slope_sens<W>((W)a.U.h[c + nx], h, (W)a.U.h[c - nx], iy, gamma, tiny);
+    d[3] = s.d; r.hsy[0] = (float)s.s0; r.hsy[1] = (float)s.s1; r.hsy[2] = (float)s.s2;
+    d[4] = slope<W>((W)a.U.hu[c + nx], hu, (W)a.U.hu[c - nx], iy, gamma, tiny);
+    d[5] = slope<W>((W)a.U.hv[c + nx], hv, (W)a.U.hv[c - nx], iy, gamma, tiny);
+  }
+  const W ih = inv_h(h);
+  const W u = (h > W(0)) ? vel(hu, h, ih) : W(0), v = (h > W(0)) ? vel(hv, h, ih) : W(0);
+  r.h = (float)h; r.hu = (float)hu; r.hv = (float)hv; r.g2 = a.S32.gz[c];
+  r.ih = (float)ih; r.u = (float)u; r.v = (float)v;
+  r.wet = !(h < tiny);
+  r.thick = h > tiny;
+  r.nz = h != W(0);
+  r.hx = (float)d[0];
+  r.hy = (float)d[3];
+  r.k = kact<W>(h, u, v, ih, d[0], d[1], d[3], d[5], (float)I.kc1, (float)I.kroot, (float)I.eps,
+                tiny);
+  r.dk = want_dk ? dkact_dphi<W>(h, u, v, ih, d[0], d[1], d[3], d[5], (float)I.kc1,
+                                 (float)I.kroot, (float)I.dkroot, (float)I.eps, tiny)
+                 : 0.f;
+  r.sgn[0] = r.sgn[1] = r.sgn[2] = r.sgn[3] = 0.f;
+  r.stop[0] = r.stop[1] = 0;
+  if (r.thick) {
+    const W ft = (W)a.ft;
+    r.sgn[0] = (float)tiny_sgn<W>(ih * (d[4] - u * d[3]), ft);
+    r.sgn[2] = (float)tiny_sgn<W>(u, ft);
+    r.sgn[1] = (float)tiny_sgn<W>(ih * (d[2] - v * d[0]), ft);
+    r.sgn[3] = (float)tiny_sgn<W>(v, ft);
+    // stop: the Coulomb term's max(.,0) clamps to zero (or tan(phi_bed) = 0)
+    const W g2 = (W)r.g2, tanb = (W)I.tan_bed;
+    r.stop[0] = r.sgn[2] != 0.f && (!(g2 * h + u * hu * (W)a.S32.kx[c] > W(0)) || tanb == W(0));
+    r.stop[1] = r.sgn[3] != 0.f && (!(g2 * h + v * hv * (W)a.S32.ky[c] > W(0)) || tanb == W(0));
+  }
+  return r;
+}
+
+constexpr int RX = TILE_X + 2, RY = TILE_Y + 2, RN = RX * RY, NHALO = RN - BLOCK;
+struct DualLds {
+  float h[RN], hu[RN], hv[RN], g2[RN], ih[RN], u[RN], v[RN], k[RN], dk[RN];
+  float hsx1[RN], hsx2[RN], hsy1[RN], hsy2[RN];
+  int flags[RN];  // sgn0+1 | (sgn1+1)<<2 | stop0<<4 | stop1<<5 | wet<<6 | thick<<7 | nz<<8
+};
+__device__ __forceinline__ void lds_put(DualLds& L, int r, const DualInfo& d) {
+  L.h[r] = d.h; L.hu[r] = d.hu; L.hv[r] = d.hv; L.g2[r] = d.g2; L.ih[r] = d.ih;
+  L.u[r] = d.u; L.v[r] = d.v; L.k[r] = d.k; L.dk[r] = d.dk;
+  L.hsx1[r] = d.hsx[1]; L.hsx2[r] = d.hsx[2]; L.hsy1[r] = d.hsy[1]; L.hsy2[r] = d.hsy[2];
+  L.flags[r] = (int)(d.sgn[0] + 1.f) | ((int)(d.sgn[1] + 1.f) << 2) | (d.stop[0] << 4) |
+               (d.stop[1] << 5) | (d.wet << 6) | (d.thick << 7) | (d.nz << 8);
+}
+__device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
+  DualInfo d;
+  d.h = L.h[r]; d.hu = L.hu[r]; d.hv = L.hv[r]; d.g2 = L.g2[r]; d.ih = L.ih[r];
+  d.u = L.u[r]; d.v = L.v[r]; d.k = L.k[r]; d.dk = L.dk[r];
+  d.hsx[0] = d.hsy[0] = 0.f;
+  d.hsx[1] = L.hsx1[r]; d.hsx[2] = L.hsx2[r]; d.hsy[1] = L.hsy1[r]; d.hsy[2] = L.hsy2[r];
+  const int f = L.flags[r];
+  d.sgn[0] = (float)((f & 3) - 1);
+  d.sgn[1] = (float)(((f >> 2) & 3) - 1);
+  d.sgn[2] = d.sgn[3] = 0.f;
+  d.stop[0] = (f >> 4) & 1;
+  d.stop[1] = (f >> 5) & 1;
+  d.wet = (f >> 6) & 1;
+  d.thick = (f >> 7) & 1;
+  d.nz = (f >> 8) & 1;
+  d.hx = d.hy = 0.f;
+  return d;
+}
+
+__device__ __forceinline__ void accum_T(float adj[3], const M3f& J, const float lam[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) adj[k] += lam[l] * J.a[l][k];
+}
+
+// neighbour-block source part: slope sensitivity of the internal friction
+__device__ __forceinline__ void nbr_block(M3f& J, const DualInfo& n, float sin_int, float dt,
+                                          float dh0, float dh1) {
+  if (n.thick) {
+    const float alpha = sin_int * n.sgn[0] * n.k, betta = sin_int * n.sgn[1] * n.k;
+    if (!n.stop[0]) J.a[1][0] += dt * n.h * alpha * n.g2 * dh1;
+    if (!n.stop[1]) J.a[2][0] += dt * n.h * betta * n.g2 * dh0;
+  }
+}
+
+// d(face flux)/dk of one side
+__device__ __forceinline__ void dhfv_dk(int dir, const DualInfo& d, float dF[3], float dW[3]) {
+  dF[0] = dF[1] = dF[2] = dW[0] = dW[1] = dW[2] = 0.f;
+  if (!d.wet) return;
+  const float a = fsqrt(d.k * d.h * d.g2);
+  const float da = (a > 0.f) ? 0.5f * d.h * d.g2 * rcp(a) : 0.f;
+  dF[dir ? 2 : 1] = 0.5f * d.h * d.g2 * d.h;
+  dW[0] = -da;
+  dW[2] = da;
+}
+
+__device__ __forceinline__ void driemann(const Hfv& l, const Hfv& r, const float dFl[3],
+                                         const float dWl[3], const float dFr[3],
+                                         const float dWr[3], float out[3]) {
+  out[0] = out[1] = out[2] = 0.f;
+  const Speeds s = speeds(l, r);
+  if (s.mode == 0) return;
+  if (s.mode == 1) { for (int v = 0; v < 3; v++) out[v] = dFl[v]; return; }
+  if (s.mode == 2) { for (int v = 0; v < 3; v++) out[v] = dFr[v]; return; }
+  float dsl, dsr;
+  if (l.u[0] == 0.f) {
+    dsl = (s.sl < 0.f) ? 2.f * dWr[0] - dWr[1] : 0.f;
+    dsr = (s.sr > 0.f) ? 2.f * dWr[2] - dWr[1] : 0.f;
+  } else if (r.u[0] == 0.f) {
+    dsl = (s.sl < 0.f) ? 2.f * dWl[0] - dWl[1] : 0.f;
+    dsr = (s.sr > 0.f) ? 2.f * dWl[2] - dWl[1] : 0.f;
+  } else {
+    dsl = (s.sl < 0.f) ? ((l.w[0] > r.w[0]) ? dWr[0] : dWl[0]) : 0.f;
+    dsr = (s.sr > 0.f) ? ((l.w[2] > r.w[2]) ? dWl[2] : dWr[2]) : 0.f;
+  }
+  const float inv = rcp(s.sr - s.sl);
+#pragma unroll
+  for (int v = 0; v < 3; v++) {
+    const float vd = r.u[v] - l.u[v];
+    const float f = (s.sr * l.f[v] - s.sl * r.f[v] + s.sl * s.sr * vd) * inv;
+    const float dfdsr = (l.f[v] + s.sl * vd - f) * inv, dfdsl = (-r.f[v] + s.sr * vd + f) * inv;
+    out[v] = dfdsr * dsr + dfdsl * dsl + s.sr * inv * dFl[v] - s.sl * inv * dFr[v];
+  }
+}
+
+// face flux and its derivative along d kact/d phi_bed (both sides)
+__device__ __forceinline__ void face_dk(int side, const DualInfo& L, const DualInfo& R,
+                                        float flux[3], float dflux[3]) {
+  const Hfv hl = hfv(side, L.h, L.hu, L.hv, side ? L.v : L.u, L.k, L.g2, L.wet);
+  const Hfv hr = hfv(side, R.h, R.hu, R.hv, side ? R.v : R.u, R.k, R.g2, R.wet);
+  riemann(hl, hr, flux);
+  float dFl[3], dWl[3], dFr[3], dWr[3];
+  dhfv_dk(side, L, dFl, dWl);
+  dhfv_dk(side, R, dFr, dWr);
+#pragma unroll
+  for (int v = 0; v < 3; v++) {
+    dFl[v] *= L.dk; dWl[v] *= L.dk; dFr[v] *= R.dk; dWr[v] *= R.dk;
+  }
+  driemann(hl, hr, dFl, dWl, dFr, dWr, dflux);
+}
+
+// lambda^{s+1} . dF/d phi_bed of one cell at frozen dt
+template <class S>
+__device__ float dphi_term(const AdjArgsM<S>& a, const InstDev& I, long long c,
+                           const DualInfo& C, const DualInfo& E, const DualInfo& N,
+                           const DualInfo& Wc, const DualInfo& Sc, const bool has[5], float dt,
+                           const float lam[3]) {
+  float fE[3] = {0.f, 0.f, 0.f}, fN[3] = {0.f, 0.f, 0.f}, fW[3] = {0.f, 0.f, 0.f},
+        fS[3] = {0.f, 0.f, 0.f};
+  float dE[3] = {0.f, 0.f, 0.f}, dN[3] = {0.f, 0.f, 0.f}, dW[3] = {0.f, 0.f, 0.f},
+        dS[3] = {0.f, 0.f, 0.f};
+  if (has[1]) face_dk(0, C, E, fE, dE);
+  if (has[3]) face_dk(0, Wc, C, fW, dW);
+  if (has[2]) face_dk(1, C, N, fN, dN);
+  if (has[4]) face_dk(1, Sc, C, fS, dS);
+  // boundary faces copy the opposite face
+#pragma unroll
+  for (int v = 0; v < 3; v++) {
+    if (!has[1]) { fE[v] = fW[v]; dE[v] = dW[v]; }
+    if (!has[3]) { fW[v] = fE[v]; dW[v] = dE[v]; }
+    if (!has[2]) { fN[v] = fS[v]; dN[v] = dS[v]; }
+    if (!has[4]) { fS[v] = fN[v]; dS[v] = dN[v]; }
+  }
+  const float dtdx = (float)(dt * I.inv_dx), dtdy = (float)(dt * I.inv_dy);
+  float out[3];
+#pragma unroll
+  for (int v = 0; v < 3; v++) out[v] = -dtdx * (dE[v] - dW[v]) - dtdy * (dN[v] - dS[v]);
+  if (C.h - dtdx * (fE[0] - fW[0]) - dtdy * (fN[0] - fS[0]) < 0.f) out[0] = 0.f;
+  if (C.thick) {
+    const float sec2 = (float)(1. + I.tan_bed * I.tan_bed), sint = (float)I.sin_int;
+    const float g2 = C.g2;
+    float ds2x = C.sgn[0] * C.h * C.dk * (g2 * C.hy + a.S32.dgy[c] * C.h) * sint;
+    float ds3x = C.sgn[2] * fmaxf(g2 * C.h + C.u * C.hu * a.S32.kx[c], 0.f) * sec2;
+    float ds2y = C.sgn[1] * C.h * C.dk * (g2 * C.hx + a.S32.dgx[c] * C.h) * sint;
+    float ds3y = C.sgn[3] * fmaxf(g2 * C.h + C.v * C.hv * a.S32.ky[c], 0.f) * sec2;
+    out[1] -= dt * (ds2x + ds3x);
+    out[2] -= dt * (ds2y + ds3y);
+  }
+  return lam[0] * out[0] + lam[1] * out[1] + lam[2] * out[2];
+}
+
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k32_adjoint(AdjArgsM<S> a) {
+  const TileDev t = a.tiles[blockIdx.x];
+  const InstDev I = a.inst[t.inst];
+  const Cell q = cell_of(t, I);
+  const bool want_dphi = a.dphi_part != nullptr;
+  __shared__ DualLds L;
+  const int tx = (int)(threadIdx.x % TILE_X), ty = (int)(threadIdx.x / TILE_X);
+  const int rself = (ty + 1) * RX + tx + 1;
+  DualInfo self;
+  if (q.act) {
+    self = dual_info(a, I, q.i, q.j, want_dphi);
+    lds_put(L, rself, self);
+  }
+  if ((int)threadIdx.x < NHALO) {
+    const int h = (int)threadIdx.x;
+    int rx, ry;
+    if (h < RX) { rx = h; ry = 0; }
+    else if (h < 2 * RX) { rx = h - RX; ry = RY - 1; }
+    else if (h < 2 * RX + TILE_Y) { rx = 0; ry = h - 2 * RX + 1; }
+    else { rx = RX - 1; ry = h - 2 * RX - TILE_Y + 1; }
+    const int gi = t.x0 - 1 + rx, gj = t.y0 - 1 + ry;
+    if (gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny)
+      lds_put(L, ry * RX + rx, dual_info(a, I, gi, gj, want_dphi));
+  }
+  __syncthreads();
+  double dphi = 0.;
+  if (q.act) {
+    const long long c = q.c, nx = I.nx;
+    const double dtd = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
+    const float dt = (float)dtd, dtdx = (float)(dtd * I.inv_dx), dtdy = (float)(dtd * I.inv_dy);
+    float adj[3] = {0.f, 0.f, 0.f};
+    const bool has[5] = {true, q.hasE, q.hasN, q.hasW, q.hasS};
+    const long long nb[5] = {c, c + 1, c + nx, c - 1, c - nx};
+    const int rn[5] = {rself, rself + 1, rself + RX, rself - 1, rself - RX};
+    // Neighbour records are re-read from LDS at each use; the empty asm
+    // statements keep the compiler from merging the re-reads into
+    // long-lived registers.
+    if (self.nz) {
+      M3f JE = zero3f(), JN = zero3f(), JW = zero3f(), JS = zero3f();
+      {
+        const DualHfv cx = dual_hfv(0, self);
+        if (has[1]) JE = dual_riemann_side(cx, dual_hfv(0, lds_get(L, rn[1])), true);
+        if (has[3]) JW = dual_riemann_side(dual_hfv(0, lds_get(L, rn[3])), cx, false);
+      }
+      {
+        const DualHfv cy = dual_hfv(1, self);
+        if (has[2]) JN = dual_riemann_side(cy, dual_hfv(1, lds_get(L, rn[2])), true);
+        if (has[4]) JS = dual_riemann_side(dual_hfv(1, lds_get(L, rn[4])), cy, false);
+      }
+      asm volatile("" ::: "memory");
+      // self block without its identity (applied in the storage type below);
+      // boundary faces copy the opposite face
+      {
+        M3f J;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+          for (int j = 0; j < 3; j++) {
+            const float pE = has[1] ? JE.a[i][j] : JW.a[i][j];
+            const float nW = has[3] ? JW.a[i][j] : JE.a[i][j];
+            const float pN = has[2] ? JN.a[i][j] : JS.a[i][j];
+            const float nS = has[4] ? JS.a[i][j] : JN.a[i][j];
+            J.a[i][j] = dtdx * (pE - nW) + dtdy * (pN - nS);
+          }
+        if (self.thick) {
+          const float sint = (float)I.sin_int, tb = (float)I.tan_bed;
+          const float alpha = sint * self.sgn[0] * self.k, betta = sint * self.sgn[1] * self.k;
+          const float u = self.u, v = self.v;
+          const float g2 = self.g2, k0 = a.S32.kx[c], k1 = a.S32.ky[c];
+          J.a[1][0] -= dt * (a.S32.gx[c] - alpha * self.h * (2.f * a.S32.dgy[c] + g2 * self.hsy[0]) -
+                             alpha * g2 * self.hy);
+          if (self.sgn[2] != 0.f && !self.stop[0]) {
+            J.a[1][0] += dt * self.sgn[2] * tb * (g2 - u * u * k0);
+            J.a[1][1] += dt * self.sgn[2] * tb * (2.f * u * k0);
+          }
+          J.a[2][0] -= dt * (a.S32.gy[c] - betta * self.h * (2.f * a.S32.dgx[c] + g2 * self.hsx[0]) -
+                             betta * g2 * self.hx);
+          if (self.sgn[3] != 0.f && !self.stop[1]) {
+            J.a[2][0] += dt * self.sgn[3] * tb * (g2 - v * v * k1);
+            J.a[2][2] += dt * self.sgn[3] * tb * (2.f * v * k1);
+          }
+        }
+        const float lam[3] = {(float)a.lam_next.h[c], (float)a.lam_next.hu[c],
+                              (float)a.lam_next.hv[c]};
+        accum_T(adj, J, lam);
+      }
+      const int order[4] = {4, 1, 2, 3};  // S, E, N, W (reference visiting order)
+      const float sint = (float)I.sin_int;
+#pragma unroll
+      for (int o = 0; o < 4; o++) {
+        const int s = order[o];
+        if (!has[s]) continue;
+        const DualInfo N = lds_get(L, rn[s]);
+        const long long n = nb[s];
+        M3f J = zero3f();
+        if (s == 1) {
+          if (q.i + 2 < I.nx)
+            for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = -dtdx * JE.a[i][j];
+          nbr_block(J, N, sint, dt, N.hsx[1], 0.f);
+        } else if (s == 3) {
+          if (q.i - 2 >= 0)
+            for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = dtdx * JW.a[i][j];
+          nbr_block(J, N, sint, dt, N.hsx[2], 0.f);
+        } else if (s == 2) {
+          if (q.j + 2 < I.ny)
+            for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = -dtdy * JN.a[i][j];
+          nbr_block(J, N, sint, dt, 0.f, N.hsy[1]);
+        } else {
+          if (q.j - 2 >= 0)
+            for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = dtdy * JS.a[i][j];
+          nbr_block(J, N, sint, dt, 0.f, N.hsy[2]);
+        }
+        const float lam[3] = {(float)a.lam_next.h[n], (float)a.lam_next.hu[n],
+                              (float)a.lam_next.hv[n]};
+        accum_T(adj, J, lam);
+      }
+    }
+    // lambda^s = -(K_self - I)^T lambda_c - sum K_n^T lambda_n = lambda_c - adj
+    const S lc[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
+    S out[3];
+    for (int j = 0; j < 3; j++) {
+      S v = self.nz ? lc[j] - (S)adj[j] : (S)0;
+      if (fabs((double)v) < 1e-16) v = (S)0;
+      out[j] = v;
+    }
+    a.lam_h[c] = out[0];
+    a.lam_hu[c] = out[1];
+    a.lam_hv[c] = out[2];
+    if (want_dphi) {
+      const float lam[3] = {(float)lc[0], (float)lc[1], (float)lc[2]};
+      asm volatile("" ::: "memory");
+      // every rn[] slot lies inside the LDS tile; records of absent
+      // neighbours are passed but not read (dphi_term checks has[])
+      dphi = (double)dphi_term(a, I, c, self, lds_get(L, rn[1]), lds_get(L, rn[2]),
+                               lds_get(L, rn[3]), lds_get(L, rn[4]), has, dt, lam);
+    }
+  }
+  if (want_dphi) {
+    __shared__ double sp[BLOCK / 64];
+    dphi = wave_sum(dphi);
+    if (threadIdx.x % 64 == 0) sp[threadIdx.x / 64] = dphi;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < BLOCK / 64; k++) dphi += sp[k];
+      a.dphi_part[blockIdx.x] += dphi;
+    }
+  }
+}
+
+}  // namespace
+
+template <class S>
+void launch_coef_m(const StepArgsM<S>& a, bool reduce, hipStream_t s) {
+  hipLaunchKernelGGL(k32_coef<S>, dim3(a.ntiles), dim3(BLOCK), 0, s, a, reduce ? 1 : 0);
+}
+template <class S>
+void launch_update_m(const StepArgsM<S>& a, hipStream_t s) {
+  hipLaunchKernelGGL(k32_update<S>, dim3(a.ntiles), dim3(BLOCK), 0, s, a);
+}
+template <class S>
+void launch_adjoint_m(const AdjArgsM<S>& a, hipStream_t s) {
+  hipLaunchKernelGGL(k32_adjoint<S>, dim3(a.ntiles), dim3(BLOCK), 0, s, a);
+}
+template void launch_coef_m<float>(const StepArgsM<float>&, bool, hipStream_t);
+template void launch_coef_m<double>(const StepArgsM<double>&, bool, hipStream_t);
+template void launch_update_m<float>(const StepArgsM<float>&, hipStream_t);
+template void launch_update_m<double>(const StepArgsM<double>&, hipStream_t);
+template void launch_adjoint_m<float>(const AdjArgsM<float>&, hipStream_t);
+template void launch_adjoint_m<double>(const AdjArgsM<double>&, hipStream_t);
+
+void launch_stats32(const InstDev* inst, const TileDev* tiles, int ntiles, FieldsT<float> U,
+                    double* part_mass, double* part_hmax, double tiny, hipStream_t s) {
+  hipLaunchKernelGGL(k32_stats, dim3(ntiles), dim3(BLOCK), 0, s, inst, tiles, U, part_mass,
+                     part_hmax, tiny);
+}
+
+}  // namespace tg

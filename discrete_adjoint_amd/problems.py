@@ -156,6 +156,35 @@ def fractal_problem(n=512, dx=5.0, nsteps=1000, npiles=1, seed=3, dem_seed=1,
     return p
 
 
+def ensemble_problems(count=256, seed=2024, nsteps=500, sides=(64, 128, 256, 512), dx=5.0,
+                      checkpoint_every=25):
+    """A stochastic-study ensemble: each member a uniform side x side block
+    (side drawn from `sides`) of dx cells on its own seeded diamond-square DEM
+    (relief 0.156 * extent, H = 0.8), phi_bed ~ U(15, 30) deg,
+    phi_int ~ U(28, 38) deg, one paraboloid pile of volume
+    lognormal(ln 3e5 m^3, 0.7) at a seeded position.  The functional box is
+    the middle half of the block (h integrated at the final step)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(count):
+        side = int(rng.choice(sides))
+        L = side * dx
+        elev = fractal_dem(side + 1, 0.15625 * L, 0.8, int(rng.integers(1, 2**62)))
+        x0 = y0 = 0.5 * dx
+        grav, dgrav, curv = static_coefs((side + 1, side + 1, 0.0, L, dx, elev), side, side,
+                                         dx, dx, x0, y0)
+        vol = float(np.exp(rng.normal(math.log(3e5), 0.7)))
+        r = paraboloid_radius_for_volume(vol)
+        cx, cy = rng.uniform(0.35, 0.65) * L, rng.uniform(0.35, 0.65) * L
+        h, hu, hv = piles(side, side, dx, dx, x0, y0, [(cx, cy, r, r, 0.0, 0.1 * r)])
+        pb = Problem("member_%d" % b, side, side, dx, dx, x0, y0, grav, dgrav, curv, h, hu, hv,
+                     float(rng.uniform(28.0, 38.0)), float(rng.uniform(15.0, 30.0)), nsteps,
+                     checkpoint_every, box=(0.25 * L, 0.75 * L, 0.25 * L, 0.75 * L))
+        pb.meta["volume"] = vol
+        out.append(pb)
+    return out
+
+
 def target_box(problem, h_final, cells=20, seed=5):
     """Seeded cells x cells target region placed on the final deposit: the
     window centre is a seeded draw among cells with h above 10% of max."""

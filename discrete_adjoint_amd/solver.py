@@ -158,9 +158,118 @@ class Simulation:
                                             C.cast(C.byref(ms), _lib._dp)))
         return ms.value
 
+    def kernel_timing(self):
+        return self.timing()
+
     def timing(self):
         f, a, r = C.c_double(), C.c_double(), C.c_double()
         self._check(self.lib.tg_last_timing(self.ctx, C.cast(C.byref(f), _lib._dp),
                                             C.cast(C.byref(a), _lib._dp),
                                             C.cast(C.byref(r), _lib._dp)))
         return f.value, a.value, r.value
+
+
+def _mesh_of(problem, keep):
+    m = _lib.tg_mesh()
+    m.kind = _lib.TG_MESH_UNIFORM
+    m.nx, m.ny, m.dx, m.dy, m.x0, m.y0 = (problem.nx, problem.ny, problem.dx, problem.dy,
+                                          problem.x0, problem.y0)
+    g = np.ascontiguousarray(problem.grav.reshape(-1))
+    dg = np.ascontiguousarray(problem.dgrav.reshape(-1))
+    cv = np.ascontiguousarray(problem.curv.reshape(-1))
+    keep += [g, dg, cv]
+    m.grav, m.dgrav, m.curv = _lib.dptr(g), _lib.dptr(dg), _lib.dptr(cv)
+    return m
+
+
+class Ensemble(Simulation):
+    """Several uniform blocks (members) advanced together in every launch,
+    each with its own friction angles, CFL dt and time (tg_set_ensemble)."""
+
+    def __init__(self, problems, precision=_lib.TG_FP64, checkpoint_every=50, max_steps=None,
+                 device=0, **cfg_overrides):
+        self.lib = _lib.load()
+        self.problems = list(problems)
+        self.problem = self.problems[0]
+        ms = max_steps or max(p.nsteps for p in self.problems)
+        self.cfg = default_config(precision=precision, checkpoint_every=checkpoint_every,
+                                  max_steps=ms, device=device, **cfg_overrides)
+        self.ctx = C.c_void_p()
+        self._check(self.lib.tg_create(C.byref(self.cfg), C.byref(self.ctx)), nocontext=True)
+        self._arrays = []
+        nm = len(self.problems)
+        meshes = (_lib.tg_mesh * nm)(*[_mesh_of(p, self._arrays) for p in self.problems])
+        prms = (_lib.tg_params * nm)(*[params_from_angles(p.phi_int_deg, p.phi_bed_deg,
+                                                          p.epsilon, p.gravity_scale)
+                                       for p in self.problems])
+        self._check(self.lib.tg_set_ensemble(self.ctx, nm, meshes, prms))
+        self.params = None
+        self.offsets = []
+        for b in range(nm):
+            off, cnt = C.c_int64(), C.c_int64()
+            self._check(self.lib.tg_member_cells(self.ctx, b, C.byref(off), C.byref(cnt)))
+            self.offsets.append((off.value, cnt.value))
+        self.ncell = int(self.lib.tg_num_cells(self.ctx))
+        h = np.concatenate([p.h for p in self.problems])
+        hu = np.concatenate([p.hu for p in self.problems])
+        hv = np.concatenate([p.hv for p in self.problems])
+        self.set_state(h, hu, hv)
+
+    def _pp(self):
+        return None if self.params is None else C.byref(self.params)
+
+    def get_state(self):
+        n = self.ncell
+        h, hu, hv = np.empty(n), np.empty(n), np.empty(n)
+        self._check(self.lib.tg_get_state(self.ctx, _lib.dptr(h), _lib.dptr(hu), _lib.dptr(hv)))
+        return h, hu, hv
+
+    def member_state(self, b):
+        o, n = self.offsets[b]
+        h, hu, hv = self.get_state()
+        return h[o:o + n], hu[o:o + n], hv[o:o + n]
+
+    def run(self, nsteps=None):
+        nm = len(self.problems)
+        st = (_lib.tg_stats * nm)()
+        n = max(p.nsteps for p in self.problems) if nsteps is None else nsteps
+        self._check(self.lib.tg_forward_run(self.ctx, None, n, st))
+        return list(st)
+
+    def member_dt_record(self, b):
+        cnt = C.c_int()
+        self._check(self.lib.tg_get_member_dt_record(self.ctx, b, None, 0, C.byref(cnt)))
+        out = np.empty(max(cnt.value, 1))
+        self._check(self.lib.tg_get_member_dt_record(self.ctx, b, _lib.dptr(out), cnt.value,
+                                                     C.byref(cnt)))
+        return out[: cnt.value]
+
+    def concat(self, per_member):
+        """(3, n_b) arrays per member -> the concatenated (3, ncell) layout."""
+        return np.concatenate([np.asarray(a, dtype=np.float64).reshape(3, -1)
+                               for a in per_member], axis=1)
+
+    def dual_solver(self, weights, want_dphi=False):
+        """weights: list of (3, n_b) terminal weights.  Returns (list of
+        lambda0 (3, n_b), per-member dJ/dphi_bed or None)."""
+        w = np.ascontiguousarray(self.concat(weights)).reshape(-1)
+        fn = _lib.tg_functional()
+        fn.terminal_weight = _lib.dptr(w)
+        lam = np.empty(3 * self.ncell)
+        dphi = C.c_double(0.0)
+        self._check(self.lib.tg_adjoint_run(self.ctx, None, C.byref(fn), _lib.dptr(lam),
+                                            C.cast(C.byref(dphi), _lib._dp) if want_dphi
+                                            else None))
+        lam = lam.reshape(3, -1)
+        out = [lam[:, o:o + n] for o, n in self.offsets]
+        per = None
+        if want_dphi:
+            per = np.empty(len(self.problems))
+            self._check(self.lib.tg_get_member_dphi(self.ctx, _lib.dptr(per)))
+        return out, per
+
+    def time_kernel(self, which, reps=20):
+        ms = C.c_double()
+        self._check(self.lib.tg_time_kernel(self.ctx, None, which, reps,
+                                            C.cast(C.byref(ms), _lib._dp)))
+        return ms.value

@@ -35,8 +35,13 @@ typedef enum tg_status {
 
 typedef enum tg_precision {
   TG_FP64 = 0,   /* every value FP64: the accuracy reference                  */
-  TG_MIXED = 1   /* FP32 state and fluxes; FP64 thin-layer velocity recovery,
-                    CFL reduction, mass and adjoint sums                     */
+  TG_MIXED = 1,  /* FP32 fluxes, slopes, sources and Jacobians (increments);
+                    FP32 static coefficients and kact; state and adjoint kept
+                    in FP64 and advanced by the FP32 increments; FP64 thin-
+                    layer velocity recovery, CFL reduction, mass and
+                    gradient sums                                           */
+  TG_FP32 = 2    /* as TG_MIXED but the state and adjoint are stored in FP32
+                    (half the bytes; larger error over long runs)           */
 } tg_precision;
 
 /* Context configuration.  tg_config_default() fills the reference's values. */
@@ -125,9 +130,30 @@ const char* tg_last_error(const tg_ctx* ctx);
  * setup_geoflow's per-element data, setup_geoflow.C:24; called per re-mesh). */
 int tg_set_mesh(tg_ctx* ctx, const tg_mesh* mesh);
 
-/* State U = (h, hu, hv), FP64 host arrays of n (Element::state_vars). */
+/* State U = (h, hu, hv), FP64 host arrays of n (Element::state_vars).
+ * With several members, the arrays hold all cells: member b's cells at
+ * [offset_b, offset_b + count_b) (tg_member_cells). */
 int tg_set_state(tg_ctx* ctx, const double* h, const double* hu, const double* hv);
 int tg_get_state(tg_ctx* ctx, double* h, double* hu, double* hv);
+
+/* ---- ensembles ----------------------------------------------------------
+ * Several independent uniform blocks (members, e.g. samples of a stochastic
+ * study, lhsbed.C / lhsvol.C) advanced together: every kernel launch covers
+ * all members' cells, each member with its own parameters, CFL dt and time.
+ * The reference runs each sample as a separate MPI job (lhs.readme:6-14);
+ * this has no counterpart there.  After tg_set_ensemble, the prm argument
+ * of tg_forward_run / tg_adjoint_run may be NULL (keep per-member params);
+ * a non-NULL prm overrides every member.  Functional weights, lambda0 and
+ * states use the concatenated layout of tg_set_state. */
+int tg_set_ensemble(tg_ctx* ctx, int nmembers, const tg_mesh* meshes, const tg_params* params);
+int tg_num_members(const tg_ctx* ctx);
+int tg_member_cells(const tg_ctx* ctx, int member, int64_t* offset, int64_t* count);
+int tg_set_member_state(tg_ctx* ctx, int member, const double* h, const double* hu,
+                        const double* hv);
+int tg_get_member_state(tg_ctx* ctx, int member, double* h, double* hu, double* hv);
+int tg_get_member_dt_record(tg_ctx* ctx, int member, double* dt_out, int capacity, int* count);
+/* Per-member dJ/d phi_bed of the last tg_adjoint_run that asked for it. */
+int tg_get_member_dphi(tg_ctx* ctx, double* dJ_dphibed);
 
 /* ---- forward ---------------------------------------------------------------- */
 
