@@ -1,14 +1,29 @@
 """Benchmark of the hot path: forward finite-volume steps + the discrete
-adjoint sweep (checkpointed, recompute in reverse) on one MI355X.
+adjoint sweep (checkpointed, recomputed in reverse) on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--n 2048]
+    python bench.py [--steps K] [--warmup W] [--members 256] [--workload ensemble|grid]
 
-Workload: uniform 2048 x 2048 grid (2.5 m) on a seeded 1025^2 diamond-square
-DEM (5 m, 800 m relief, H = 0.8), three seeded paraboloid piles, phi_int 32 /
-phi_bed 22 deg; K forward steps, then the adjoint of "h integrated over a
-seeded 20 x 20-cell target region" back to step 0 with FP64 checkpoints every
-50 steps.  A "step" is one forward step plus its reverse step over every
-cell; value = cells * K / (forward + adjoint kernel time).
+Default workload ("ensemble"): 256 independent members advanced together in
+every launch.  Member b is a uniform side x side block (side drawn from
+{64, 128, 256, 512}, 5 m cells) on its own seeded diamond-square DEM, with
+phi_bed ~ U(15, 30) deg, phi_int ~ U(28, 38) deg and one paraboloid pile of
+volume lognormal(ln 3e5 m^3, 0.7); K = 500 forward steps, then the adjoint of
+"h integrated over the middle half of the block at the last step" back to
+step 0, checkpoints every 25 steps, and dJ/dphi_bed per member.
+
+The measured path is the batched mixed-precision one (FP32 flux/Jacobian
+arithmetic, FP64 state, switches, CFL and sums).  The same ensemble is then
+run (a) batched in FP64 and (b) member by member in FP64 (the plain FP64 GPU
+path); (a) is the accuracy reference for the per-member functional and
+gradient errors, (b) the speed reference.
+
+A "step" is one forward step plus its reverse step over every cell of every
+member; value = total cell-steps / wall time of the timed forward run plus
+reverse sweep (synchronised on both sides, inputs resident on the GPU).
+
+"grid" workload: one 2048 x 2048 block (2.5 m) with three piles, 1500 steps,
+mixed precision against FP64 (L2 / Linf errors of h, hu, hv and the
+functional).
 
 Prints one JSON line.  Diagnostics go to stderr.
 """
@@ -24,166 +39,364 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fwd+adjoint cell-steps/s & %HBM roofline, mixed/batched vs FP64 GPU at fixed err"
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per cell per launch (unique bytes, perfect neighbour
-# reuse): forward update = U in + U out + 7 static coefficients; adjoint
-# gather = U^s + lambda^{s+1} in, lambda^s out, 7 static coefficients
-ALGO_BYTES = {"coef": 24 + 8, "update": 24 + 24 + 56, "adjoint": 24 + 24 + 24 + 56}
+KERNELS = ("coef", "update", "adjoint")
+
+
+def algo_bytes(prec):
+    """Algorithmic HBM bytes per cell for one launch of each kernel (unique
+    bytes, perfect neighbour reuse).  es = state/adjoint element size,
+    ss = static-coefficient / kact element size.
+      coef:    read U (3), write kact (1)
+      update:  read U (3), write U (3), read 7 static coefficients
+      adjoint: read U^s (3), read lambda^{s+1} (3), write lambda^s (3), 7 statics
+    The forward step (coef + update) is charged the update's bytes: kact is
+    an intermediate."""
+    from discrete_adjoint_amd import _lib
+    es = 4 if prec == _lib.TG_FP32 else 8
+    ss = 8 if prec == _lib.TG_FP64 else 4
+    return {"coef": 3 * es + ss, "update": 6 * es + 7 * ss, "adjoint": 9 * es + 7 * ss,
+            "forward_step": 6 * es + 7 * ss}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pb, nsteps):
-    """The oracle (FP64 C restatement, one core) on a bounded sample: the
-    first `nsteps` forward steps of the same workload and their adjoint."""
+def stats(e):
+    import numpy as np
+    e = np.asarray(e, dtype=float)
+    return {"max": float(e.max()), "median": float(np.median(e)),
+            "p90": float(np.percentile(e, 90))}
+
+
+def cpu_baseline(problems_, nsteps):
+    """The oracle (FP64 C restatement of the reference, one core) on a
+    bounded sample: the first `nsteps` forward steps of each given member and
+    their adjoint steps.  Returns (cell-steps, forward s, adjoint s)."""
+    import numpy as np
+
+    import oracle
+    tf = ta = 0.0
+    cs = 0
+    for pb in problems_:
+        orc = oracle.Oracle.from_problem(pb)
+        lam = np.zeros((3, pb.n))
+        lam[0] = 1.0
+        U = [pb.h.copy(), pb.hu.copy(), pb.hv.copy()]
+        traj = [[a.copy() for a in U]]
+        dts = []
+        for s in range(nsteps):
+            t0 = time.perf_counter()
+            dt, _ = orc.step(U, s)
+            tf += time.perf_counter() - t0
+            dts.append(dt)
+            traj.append([a.copy() for a in U])   # trajectory store: bookkeeping, not timed
+        t0 = time.perf_counter()
+        for s in range(nsteps - 1, -1, -1):
+            lam, _ = orc.adjoint_step(traj[s], s, dts[s], lam)
+        ta += time.perf_counter() - t0
+        cs += pb.n * nsteps
+    return cs, tf, ta
+
+
+def warm_oracle(pb):
+    """Faults in the oracle's reusable scratch buffers (untimed)."""
     import numpy as np
 
     import oracle
     orc = oracle.Oracle.from_problem(pb)
-    lam = np.zeros((3, pb.n))
-    lam[0] = 1.0
-    # initialisation (not timed): fault in the oracle's scratch buffers
-    W = [pb.h.copy(), pb.hu.copy(), pb.hv.copy()]
-    dt0, _ = orc.step(W, 0)
-    orc.adjoint_step([pb.h, pb.hu, pb.hv], 0, dt0, lam)
-    orc.time = 0.0
     U = [pb.h.copy(), pb.hu.copy(), pb.hv.copy()]
-    traj = [[a.copy() for a in U]]
-    dts = []
-    tf = 0.0
-    for s in range(nsteps):
-        t0 = time.perf_counter()
-        dt, _ = orc.step(U, s)
-        tf += time.perf_counter() - t0
-        dts.append(dt)
-        traj.append([a.copy() for a in U])   # trajectory store: bookkeeping, not timed
-    t0 = time.perf_counter()
-    for s in range(nsteps - 1, -1, -1):
-        lam, _ = orc.adjoint_step(traj[s], s, dts[s], lam)
-    ta = time.perf_counter() - t0
-    return tf, ta
+    dt, _ = orc.step(U, 0)
+    lam = np.zeros((3, pb.n))
+    orc.adjoint_step([pb.h, pb.hu, pb.hv], 0, dt, lam)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--checkpoint-every", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel-reps", type=int, default=20)
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this hot path has no multi-GPU mode "
-              "(ensembles shard by instance, one mesh by SFC range; neither is built)",
-              file=sys.stderr)
-        sys.exit(2)
-
+def timed_run(sim, K, weights, h0, want_lambda=False):
+    """Forward K steps + reverse sweep, each timed by wall clock around a
+    synchronous library call.  Returns a dict."""
     import numpy as np
-
-    from discrete_adjoint_amd import problems
-    from discrete_adjoint_amd.solver import Simulation
-
-    n, K, W = args.n, args.steps, args.warmup
-    dx = 5120.0 / n
-    t_setup = time.perf_counter()
-    pb = problems.fractal_problem(n=n, dx=dx, nsteps=K, npiles=3, seed=11, dem_seed=1,
-                                  checkpoint_every=args.checkpoint_every)
-    log(f"setup {time.perf_counter() - t_setup:.1f}s: {n}x{n} cells, dx={dx}, "
-        f"volume {pb.h.sum() * dx * dx:.4g} m^3")
-
-    sim = Simulation(pb, checkpoint_every=args.checkpoint_every, max_steps=max(K, W, 1))
-    # warmup: W forward steps + their adjoint (untimed)
-    if W > 0:
-        sim.run(W)
-        sim.dual_solver(np.concatenate([np.full(pb.n, 1.0), np.zeros(2 * pb.n)]))
-        sim.set_state(pb.h, pb.hu, pb.hv)
-        log(f"warmup {W} steps done")
-
-    wall0 = time.perf_counter()
+    sim.set_state(*h0)
+    t0 = time.perf_counter()
     st = sim.run(K)
-    wall_f = time.perf_counter() - wall0
+    wall_f = time.perf_counter() - t0
     fwd_ms, _, _ = sim.timing()
-    h, _, _ = sim.get_state()
-    box = problems.target_box(pb, h, cells=20, seed=5)
-    w = pb.box_weights(box)
-    J = float(np.sum(w[: pb.n] * h))
-    wall0 = time.perf_counter()
-    lam0, dphi = sim.dual_solver(w, want_dphi=True)
-    wall_a = time.perf_counter() - wall0
+    H = sim.get_state()
+    t0 = time.perf_counter()
+    lam, dphi = sim.dual_solver(weights, want_dphi=True, want_lambda=want_lambda)
+    wall_a = time.perf_counter() - t0
     _, adj_ms, rec_ms = sim.timing()
-    total_s = (fwd_ms + adj_ms) / 1e3
-    value = pb.n * K / total_s
-    log(f"forward {K} steps: {fwd_ms:.1f} ms kernels ({wall_f * 1e3:.1f} ms wall), "
-        f"t_end={st.time:.3f}s, mass={st.mass:.6g}")
-    log(f"adjoint: {adj_ms:.1f} ms kernels incl. recompute {rec_ms:.1f} ms "
-        f"({wall_a * 1e3:.1f} ms wall); J={J:.6g} dJ/dphi_bed={dphi:.6g}")
-    if not (np.isfinite(lam0).all() and math.isfinite(dphi)):
-        raise RuntimeError("non-finite adjoint")
+    return dict(st=st, H=H, lam=lam, dphi=np.atleast_1d(np.asarray(dphi, dtype=float)),
+                wall_f=wall_f, wall_a=wall_a, fwd_ms=fwd_ms, adj_ms=adj_ms, rec_ms=rec_ms)
 
-    # dominant kernel, timed live with HIP events on the library's stream
-    kt = {name: sim.time_kernel(i, args.kernel_reps)
-          for i, name in enumerate(("coef", "update", "adjoint"))}
-    k = args.checkpoint_every
+
+def kernel_roofline(sim, prec, ncells, k, reps):
+    """Times each kernel live (HIP events on the library's stream) and returns
+    the roofline entry of the dominant one plus per-kernel figures."""
+    ab = algo_bytes(prec)
+    kt = {name: sim.time_kernel(i, reps) for i, name in enumerate(KERNELS)}
+    # per reverse step the forward kernels run again in the segment recompute
     per_step = {"coef": kt["coef"] * (2 - 1.0 / k), "update": kt["update"] * (2 - 1.0 / k),
                 "adjoint": kt["adjoint"]}
     dom = max(per_step, key=per_step.get)
-    achieved = ALGO_BYTES[dom] * pb.n / (kt[dom] * 1e-3) / 1e9
-    log("kernel avg ms: " + ", ".join(f"{a}={b:.3f}" for a, b in kt.items()) +
-        f"; dominant {dom}: {achieved:.0f} GB/s algorithmic")
-    for name in kt:
-        log(f"  {name}: {ALGO_BYTES[name] * pb.n / (kt[name] * 1e-3) / 1e9:.0f} GB/s")
-    sim.close()
+    gbs = {n: ab[n] * ncells / (kt[n] * 1e-3) / 1e9 for n in KERNELS}
+    log("kernel avg ms: " + ", ".join(f"{a}={b:.4f}" for a, b in kt.items()))
+    for n in KERNELS:
+        log(f"  {n}: {gbs[n]:.0f} GB/s algorithmic ({ab[n]} B/cell)")
+    return dom, kt, gbs
 
+
+def traffic_from_profile(kernel_name, workload):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE runs, scaled by a copy kernel's counters), read from the
+    committed summary for this kernel and workload; None when absent."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_name}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def bench_ensemble(args):
+    import numpy as np
+
+    from discrete_adjoint_amd import _lib, problems
+    from discrete_adjoint_amd.solver import Ensemble, Simulation
+
+    K, W, k = args.steps, args.warmup, args.checkpoint_every
+    t0 = time.perf_counter()
+    E = problems.ensemble_problems(args.members, seed=args.seed, nsteps=K, checkpoint_every=k)
+    ncells = sum(p.n for p in E)
+    weights = [p.box_weights().reshape(3, -1) for p in E]
+    h0 = tuple(np.concatenate([getattr(p, f) for p in E]) for f in ("h", "hu", "hv"))
+    log(f"setup {time.perf_counter() - t0:.1f}s: {len(E)} members, {ncells} cells")
+    prec = {"mixed": _lib.TG_MIXED, "fp32": _lib.TG_FP32, "fp64": _lib.TG_FP64}[args.precision]
+
+    def member_J(H):
+        out, o = [], 0
+        for b, p in enumerate(E):
+            out.append(float(np.sum(weights[b][0] * H[0][o:o + p.n])))
+            o += p.n
+        return np.array(out)
+
+    res = {}
+    roof = None
+    for mode in ([prec, _lib.TG_FP64] if prec != _lib.TG_FP64 else [prec]):
+        with Ensemble(E, precision=mode, checkpoint_every=k, max_steps=max(K, W, 1)) as ens:
+            if W > 0:  # untimed: first launches, allocation of the checkpoint store
+                ens.run(W)
+                ens.dual_solver(weights, want_lambda=False)
+            r = timed_run(ens, K, weights, h0)
+            r["J"] = member_J(r["H"])
+            del r["H"]
+            res[mode] = r
+            log(f"precision {mode}: forward {r['wall_f'] * 1e3:.1f} ms wall "
+                f"({r['fwd_ms']:.1f} ms kernels), reverse {r['wall_a'] * 1e3:.1f} ms wall "
+                f"({r['adj_ms']:.1f} ms kernels, recompute {r['rec_ms']:.1f})")
+            if mode == prec:
+                roof = kernel_roofline(ens, prec, ncells, k, args.kernel_reps)
+    # plain FP64 GPU path: one member at a time
+    plain_s = 0.0
+    for b, p in enumerate(E):
+        with Simulation(p, checkpoint_every=k, max_steps=max(K, 1)) as sim:
+            if b == 0 and W > 0:
+                sim.run(W)
+                sim.dual_solver(weights[b], want_lambda=False)
+                sim.set_state(p.h, p.hu, p.hv)
+            t0 = time.perf_counter()
+            sim.run(K)
+            sim.dual_solver(weights[b], want_dphi=True, want_lambda=False)
+            plain_s += time.perf_counter() - t0
+    cell_steps = ncells * K
+    plain = cell_steps / plain_s
+    log(f"plain FP64 per-member: {plain_s:.2f} s wall, {plain:.3g} cell-steps/s")
+
+    m, d = res[prec], res[_lib.TG_FP64]
+    wall = m["wall_f"] + m["wall_a"]
+    value = cell_steps / wall
+    errs = None
+    if prec != _lib.TG_FP64:
+        Jd, Jm = d["J"], m["J"]
+        okJ = np.abs(Jd) > 0
+        eJ = np.abs(Jm[okJ] - Jd[okJ]) / np.abs(Jd[okJ])
+        okg = np.abs(d["dphi"]) > 0
+        eg = np.abs(m["dphi"][okg] - d["dphi"][okg]) / np.abs(d["dphi"][okg])
+        errs = {"functional_rel": stats(eJ), "functional_members_le_1e-6": int((eJ <= 1e-6).sum()),
+                "dJ_dphibed_rel": stats(eg),
+                "dJ_dphibed_members_le_1e-6": int((eg <= 1e-6).sum()),
+                "members": len(E)}
+        log(f"per-member functional rel err: {errs['functional_rel']}, "
+            f"dJ/dphi_bed rel err: {errs['dJ_dphibed_rel']}")
+
+    dom, kt, gbs = roof
+    ab = algo_bytes(prec)
+    fwd_step_ms = m["fwd_ms"] / K
+    fwd_gbs = ab["forward_step"] * ncells / (fwd_step_ms * 1e-3) / 1e9
+    workload = "ensemble256"
     cpu = None
     if not args.no_cpu:
-        cs = args.cpu_steps
-        tf, ta = cpu_baseline(pb, cs)
-        cpu = {"value": pb.n * cs / (tf + ta), "unit": "cell-steps/s", "cores": 1,
-               "kind": "port",
-               "sample": f"first {cs} forward steps of the same {n}x{n} workload and their "
-                         f"{cs} adjoint steps (oracle, FP64 C, -O3, one thread); "
-                         f"forward {pb.n * cs / tf:.3g}, adjoint {pb.n * cs / ta:.3g} cell-steps/s"}
-        log(f"cpu baseline: fwd {tf:.2f}s adj {ta:.2f}s")
+        warm_oracle(E[0])
+        cs, tf, ta = cpu_baseline(E, args.cpu_steps)
+        cpu = {"value": cs / (tf + ta), "unit": "cell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"first {args.cpu_steps} forward steps of each of the {len(E)} members "
+                         f"and their {args.cpu_steps} adjoint steps (oracle: FP64 C restatement, "
+                         f"-O3, one thread); forward {cs / tf:.3g}, adjoint {cs / ta:.3g} "
+                         f"cell-steps/s"}
+        log(f"cpu baseline: fwd {tf:.2f}s adj {ta:.2f}s over {cs} cell-steps")
 
-    out = {
+    return {
         "metric": METRIC,
         "value": value,
         "unit": "cell-steps/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": (fwd_ms + adj_ms) / K,
+        "ms_per_step": wall * 1e3 / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"mixed": "f32+f64", "fp32": "f32", "fp64": "f64"}[args.precision],
         "data": "synthetic",
         "config": {
-            "workload": f"uniform {n}x{n} ({dx} m) fractal DEM, 3 piles, {K} forward steps + "
-                        f"discrete adjoint, FP64 checkpoints every {k}",
-            "cells": pb.n,
+            "workload": f"ensemble of {len(E)} members (uniform blocks, side in "
+                        f"{{64,128,256,512}}, 5 m, seeded fractal DEMs, one pile each), "
+                        f"{K} forward steps + discrete adjoint with dJ/dphi_bed, checkpoints "
+                        f"every {k}, all members in every launch",
+            "members": len(E),
+            "cells": ncells,
             "checkpoint_every": k,
-            "precision": "fp64",
-            "forward_cell_steps_per_s": pb.n * K / (fwd_ms / 1e3),
-            "adjoint_cell_steps_per_s": pb.n * K / (adj_ms / 1e3),
+            "precision": args.precision,
+            "forward_cell_steps_per_s": cell_steps / m["wall_f"],
+            "adjoint_cell_steps_per_s": cell_steps / m["wall_a"],
+            "kernel_ms": {"forward": m["fwd_ms"], "reverse": m["adj_ms"],
+                          "recompute": m["rec_ms"]},
+            "fp64_batched_cell_steps_per_s":
+                cell_steps / (d["wall_f"] + d["wall_a"]),
+            "fp64_plain_cell_steps_per_s": plain,
+            "speedup_vs_fp64_plain": value / plain,
+            "errors_vs_fp64": errs,
+            "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS,
+                                 "algorithmic_bytes_per_cell": ab["forward_step"]},
+            "kernel_gbs": gbs,
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": gbs[dom],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "frac": gbs[dom] / HBM_PEAK_GBS,
+            "traffic": traffic_from_profile(dom, workload),
             "kernel": dom,
-            "algorithmic_bytes_per_cell": ALGO_BYTES[dom],
+            "algorithmic_bytes_per_cell": ab[dom],
             "avg_launch_ms": kt[dom],
         },
         "cpu_baseline": cpu,
     }
+
+
+def bench_grid(args):
+    import numpy as np
+
+    from discrete_adjoint_amd import _lib, problems
+    from discrete_adjoint_amd.solver import Simulation
+
+    n, K, W, k = args.n, args.steps, args.warmup, args.checkpoint_every
+    dx = 5120.0 / n
+    t0 = time.perf_counter()
+    pb = problems.fractal_problem(n=n, dx=dx, nsteps=K, npiles=3, seed=11, dem_seed=1,
+                                  checkpoint_every=k)
+    log(f"setup {time.perf_counter() - t0:.1f}s: {n}x{n} cells, dx={dx}")
+    prec = {"mixed": _lib.TG_MIXED, "fp32": _lib.TG_FP32, "fp64": _lib.TG_FP64}[args.precision]
+    h0 = (pb.h, pb.hu, pb.hv)
+    res = {}
+    roof = None
+    w = None
+    for mode in ([_lib.TG_FP64, prec] if prec != _lib.TG_FP64 else [prec]):
+        with Simulation(pb, precision=mode, checkpoint_every=k, max_steps=max(K, W, 1)) as sim:
+            if W > 0:
+                sim.run(W)
+                sim.dual_solver(np.concatenate([np.ones(pb.n), np.zeros(2 * pb.n)]),
+                                want_lambda=False)
+            if w is None:  # target region placed on the FP64 deposit
+                sim.set_state(*h0)
+                sim.run(K)
+                box = problems.target_box(pb, sim.get_state()[0], cells=20, seed=5)
+                w = pb.box_weights(box)
+            r = timed_run(sim, K, w, h0, want_lambda=True)
+            r["J"] = float(np.sum(w[:pb.n] * r["H"][0]))
+            res[mode] = r
+            log(f"precision {mode}: forward {r['wall_f'] * 1e3:.1f} ms, reverse "
+                f"{r['wall_a'] * 1e3:.1f} ms wall")
+            if mode == prec:
+                roof = kernel_roofline(sim, prec, pb.n, k, args.kernel_reps)
+    m, d = res[prec], res[_lib.TG_FP64]
+    cell_steps = pb.n * K
+    wall = m["wall_f"] + m["wall_a"]
+    errs = None
+    if prec != _lib.TG_FP64:
+        errs = {}
+        for i, name in enumerate(("h", "hu", "hv")):
+            a, b = m["H"][i], d["H"][i]
+            errs[name] = {"l2_rel": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                          "linf_rel": float(np.abs(a - b).max() / np.abs(b).max())}
+        errs["functional_rel"] = abs(m["J"] - d["J"]) / abs(d["J"])
+        errs["dJ_dphibed_rel"] = float(abs(m["dphi"][0] - d["dphi"][0]) / abs(d["dphi"][0]))
+        errs["lambda0_l2_rel"] = float(np.linalg.norm(m["lam"] - d["lam"]) /
+                                       np.linalg.norm(d["lam"]))
+        log(f"errors vs fp64: {errs}")
+    dom, kt, gbs = roof
+    ab = algo_bytes(prec)
+    fwd_gbs = ab["forward_step"] * pb.n / (m["fwd_ms"] / K * 1e-3) / 1e9
+    return {
+        "metric": METRIC, "value": cell_steps / wall, "unit": "cell-steps/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": wall * 1e3 / K, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {"mixed": "f32+f64", "fp32": "f32", "fp64": "f64"}[args.precision],
+        "data": "synthetic",
+        "config": {
+            "workload": f"uniform {n}x{n} ({dx} m) fractal DEM, 3 piles, {K} forward steps + "
+                        f"discrete adjoint, checkpoints every {k}",
+            "cells": pb.n, "precision": args.precision,
+            "fp64_cell_steps_per_s": cell_steps / (d["wall_f"] + d["wall_a"]),
+            "errors_vs_fp64": errs,
+            "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS},
+            "kernel_gbs": gbs,
+        },
+        "roofline": {"bound": "hbm", "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs[dom] / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(dom, f"grid{n}"), "kernel": dom,
+                     "algorithmic_bytes_per_cell": ab[dom], "avg_launch_ms": kt[dom]},
+        "cpu_baseline": None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("ensemble", "grid"), default="ensemble")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--precision", choices=("mixed", "fp32", "fp64"), default="mixed")
+    ap.add_argument("--checkpoint-every", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this hot path has no multi-GPU mode "
+              "(ensembles would shard by member, one large block by SFC range; neither is "
+              "built)", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "ensemble":
+        args.steps = args.steps or 500
+        args.checkpoint_every = args.checkpoint_every or 25
+        out = bench_ensemble(args)
+    else:
+        args.steps = args.steps or 1500
+        args.checkpoint_every = args.checkpoint_every or 50
+        out = bench_grid(args)
     print(json.dumps(out), flush=True)
 
 

@@ -105,6 +105,23 @@ class tg_dem(C.Structure):
     ]
 
 
+class tg_leaf_table(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("order", C.POINTER(C.c_int32)),
+        ("key", C.POINTER(C.c_uint32)),
+        ("gen", C.POINTER(C.c_int32)),
+        ("nbr", C.POINTER(C.c_int32)),
+        ("nkind", C.POINTER(C.c_int32)),
+        ("ngen", C.POINTER(C.c_int32)),
+        ("dx", _dp),
+        ("xc", _dp),
+    ]
+
+
+_i32p = C.POINTER(C.c_int32)
+_u32p = C.POINTER(C.c_uint32)
+
 # name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {
     "tg_config_default": (None, [C.POINTER(tg_config)]),
@@ -154,6 +171,14 @@ SIGNATURES = {
     "tg_host_box_weights": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_double, _dp]),
+    "tg_host_hsfc2d": (C.c_int, [C.c_int, _dp, _u32p]),
+    "tg_host_hash_bucket": (C.c_int, [C.c_int, _u32p, C.c_double, C.c_double, C.c_int,
+                                      _i32p]),
+    "tg_host_traversal": (C.c_int, [C.c_int, _u32p, C.c_double, C.c_double, C.c_int,
+                                    _i32p]),
+    "tg_host_flatten": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_int, _i32p, _i32p, _i32p, C.c_int,
+                                  C.POINTER(tg_leaf_table)]),
 }
 
 _LIB = None
@@ -182,6 +207,16 @@ def dptr(a):
         return None
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "need contiguous float64"
     return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], "need contiguous int32"
+    return a.ctypes.data_as(_i32p)
+
+
+def uptr(a):
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"], "need contiguous uint32"
+    return a.ctypes.data_as(_u32p)
 
 
 def fptr(a):

@@ -76,7 +76,7 @@ int set_err(tg_ctx* ctx, int code, const std::string& msg) {
 
 template <class T>
 void dfree(T*& p) {
-  if (p) hipFree((void*)p);
+  if (p) (void)hipFree((void*)p);  // teardown: nothing to report to
   p = nullptr;
 }
 
@@ -636,25 +636,35 @@ int tg_create(const tg_config* cfg, tg_ctx** out) {
     delete ctx;
     return TG_ERR_HIP;
   }
-  for (auto& e : ctx->ev) hipEventCreate(&e);
+  for (auto& e : ctx->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+      tg_destroy(ctx);
+      return TG_ERR_HIP;
+    }
+  }
   if (dalloc(ctx, (void**)&ctx->err_d, sizeof(int))) {
     tg_destroy(ctx);
     return TG_ERR_NOMEM;
   }
-  hipMemset(ctx->err_d, 0, sizeof(int));
+  if (hipMemset(ctx->err_d, 0, sizeof(int)) != hipSuccess) {
+    tg_destroy(ctx);
+    return TG_ERR_HIP;
+  }
   *out = ctx;
   return TG_OK;
 }
 
 void tg_destroy(tg_ctx* ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->cfg.device);
-  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  // teardown: errors here have no caller to report to
+  (void)hipSetDevice(ctx->cfg.device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_fields(ctx);
   dfree(ctx->err_d);
   for (auto& e : ctx->ev)
-    if (e) hipEventDestroy(e);
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 

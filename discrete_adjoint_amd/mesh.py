@@ -1,0 +1,130 @@
+"""Adaptive-mesh flattening: numpy views of tg_host_hsfc2d,
+tg_host_hash_bucket and tg_host_flatten (include/tg_host.h)."""
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+# element table size of the reference (datread.C:560: EL_TABLE_SIZE)
+EL_TABLE_SIZE = 100000
+KEY_RANGE = 4294967296.0 + 1.0
+
+
+def hsfc2d(norm_xy):
+    """Hilbert keys (n, 2) uint32 of normalised coordinates (n, 2)."""
+    xy = np.ascontiguousarray(norm_xy, dtype=np.float64).reshape(-1, 2)
+    keys = np.zeros((len(xy), 2), dtype=np.uint32)
+    rc = _lib.load().tg_host_hsfc2d(len(xy), _lib.dptr(xy.reshape(-1)), _lib.uptr(keys))
+    if rc:
+        raise ValueError(f"tg_host_hsfc2d returned {rc}")
+    return keys
+
+
+def hash_bucket(keys, nbuckets=EL_TABLE_SIZE, keyrange=(KEY_RANGE, KEY_RANGE)):
+    k = np.ascontiguousarray(keys, dtype=np.uint32).reshape(-1, 2)
+    out = np.zeros(len(k), dtype=np.int32)
+    rc = _lib.load().tg_host_hash_bucket(len(k), _lib.uptr(k), keyrange[0], keyrange[1],
+                                         nbuckets, _lib.iptr(out))
+    if rc:
+        raise ValueError(f"tg_host_hash_bucket returned {rc}")
+    return out
+
+
+def traversal(keys, nbuckets=EL_TABLE_SIZE, keyrange=(KEY_RANGE, KEY_RANGE)):
+    """Element-table traversal order of keys added in index order."""
+    k = np.ascontiguousarray(keys, dtype=np.uint32).reshape(-1, 2)
+    out = np.zeros(len(k), dtype=np.int32)
+    rc = _lib.load().tg_host_traversal(len(k), _lib.uptr(k), keyrange[0], keyrange[1],
+                                       nbuckets, _lib.iptr(out))
+    if rc:
+        raise ValueError(f"tg_host_traversal returned {rc}")
+    return out
+
+
+@dataclass
+class LeafTable:
+    order: np.ndarray   # (n,) flattened position -> input leaf index
+    key: np.ndarray     # (n, 2) uint32
+    gen: np.ndarray     # (n,)
+    nbr: np.ndarray     # (n, 8) flattened neighbour index, -1 boundary
+    nkind: np.ndarray   # (n, 8) 0 present, -1 boundary, -2 no second neighbour
+    ngen: np.ndarray    # (n, 8)
+    dx: np.ndarray      # (n, 2)
+    xc: np.ndarray      # (n, 2)
+
+
+FLATTEN_ERRORS = {-1: "bad argument", -2: "overlapping or out-of-range leaves",
+                  -3: "leaves do not cover the base grid",
+                  -4: "neighbours more than one level apart"}
+
+
+def flatten(nx0, ny0, xmin, ymin, dx0, dy0, lev, ix, iy, nbuckets=EL_TABLE_SIZE):
+    lev, ix, iy = (np.ascontiguousarray(a, dtype=np.int32) for a in (lev, ix, iy))
+    n = len(lev)
+    t = LeafTable(order=np.zeros(n, np.int32), key=np.zeros((n, 2), np.uint32),
+                  gen=np.zeros(n, np.int32), nbr=np.zeros((n, 8), np.int32),
+                  nkind=np.zeros((n, 8), np.int32), ngen=np.zeros((n, 8), np.int32),
+                  dx=np.zeros((n, 2)), xc=np.zeros((n, 2)))
+    c = _lib.tg_leaf_table()
+    c.order, c.key, c.gen = _lib.iptr(t.order), _lib.uptr(t.key), _lib.iptr(t.gen)
+    c.nbr, c.nkind, c.ngen = _lib.iptr(t.nbr), _lib.iptr(t.nkind), _lib.iptr(t.ngen)
+    c.dx, c.xc = _lib.dptr(t.dx), _lib.dptr(t.xc)
+    import ctypes as C
+    rc = _lib.load().tg_host_flatten(nx0, ny0, xmin, ymin, dx0, dy0, n, _lib.iptr(lev),
+                                     _lib.iptr(ix), _lib.iptr(iy), nbuckets, C.byref(c))
+    if rc:
+        raise ValueError(f"tg_host_flatten: {FLATTEN_ERRORS.get(rc, rc)}")
+    return t
+
+
+def random_balanced_tree(nx0, ny0, max_level, frac, seed):
+    """A seeded 2:1-balanced quadtree over an nx0 x ny0 base grid: leaves are
+    refined at random (probability frac per leaf and level) and neighbours
+    are refined until no two side-adjacent leaves differ by more than one
+    level.  Returns (lev, ix, iy): base cells row by row, each refined cell
+    replaced in place by its sons 0..3 (SW, SE, NE, NW)."""
+    rng = np.random.default_rng(seed)
+    leaves = [(0, i, j) for j in range(ny0) for i in range(nx0)]
+
+    def split(c):
+        L, i, j = c
+        return [(L + 1, 2 * i, 2 * j), (L + 1, 2 * i + 1, 2 * j), (L + 1, 2 * i + 1, 2 * j + 1),
+                (L + 1, 2 * i, 2 * j + 1)]
+
+    for L in range(max_level):
+        out = []
+        for c in leaves:
+            out.extend(split(c) if c[0] == L and rng.random() < frac else [c])
+        leaves = out
+    # balance: refine any leaf with a neighbour two or more levels finer
+    while True:
+        S = set(leaves)
+
+        def finest_at(L, i, j, depth):
+            # is any leaf strictly inside cell (L, i, j) at level >= L + depth?
+            for d in range(depth, max_level - L + 1):
+                s = 1 << d
+                for a in range(i * s, (i + 1) * s):
+                    for b in range(j * s, (j + 1) * s):
+                        if (L + d, a, b) in S:
+                            return True
+            return False
+
+        todo = set()
+        for (L, i, j) in leaves:
+            W, H = nx0 << L, ny0 << L
+            for di, dj in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                a, b = i + di, j + dj
+                if 0 <= a < W and 0 <= b < H and finest_at(L, a, b, 2):
+                    todo.add((L, i, j))
+        if not todo:
+            break
+        out = []
+        for c in leaves:
+            out.extend(split(c) if c in todo else [c])
+        leaves = out
+    lev = np.array([c[0] for c in leaves], np.int32)
+    ix = np.array([c[1] for c in leaves], np.int32)
+    iy = np.array([c[2] for c in leaves], np.int32)
+    return lev, ix, iy

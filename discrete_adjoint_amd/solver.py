@@ -131,18 +131,19 @@ class Simulation:
         return out[: cnt.value]
 
     # -- adjoint ----------------------------------------------------------------
-    def dual_solver(self, terminal_weight, want_dphi=False):
-        """Reverse sweep over the last run: returns (lambda0 (3,n), dJ/dphi_bed)."""
+    def dual_solver(self, terminal_weight, want_dphi=False, want_lambda=True):
+        """Reverse sweep over the last run: returns (lambda0 (3,n) or None,
+        dJ/dphi_bed or None)."""
         w = np.ascontiguousarray(terminal_weight, dtype=np.float64).reshape(-1)
         fn = _lib.tg_functional()
         fn.terminal_weight = _lib.dptr(w)
-        lam = np.empty(3 * self.problem.n)
+        lam = np.empty(3 * self.problem.n) if want_lambda else None
         dphi = C.c_double(0.0)
         self._check(self.lib.tg_adjoint_run(self.ctx, C.byref(self.params), C.byref(fn),
                                             _lib.dptr(lam),
                                             C.cast(C.byref(dphi), _lib._dp) if want_dphi
                                             else None))
-        return lam.reshape(3, -1), (dphi.value if want_dphi else None)
+        return (lam.reshape(3, -1) if want_lambda else None), (dphi.value if want_dphi else None)
 
     def adjoint_step(self, s, dt, lam_next):
         ln = np.ascontiguousarray(lam_next, dtype=np.float64).reshape(-1)
@@ -249,19 +250,21 @@ class Ensemble(Simulation):
         return np.concatenate([np.asarray(a, dtype=np.float64).reshape(3, -1)
                                for a in per_member], axis=1)
 
-    def dual_solver(self, weights, want_dphi=False):
+    def dual_solver(self, weights, want_dphi=False, want_lambda=True):
         """weights: list of (3, n_b) terminal weights.  Returns (list of
-        lambda0 (3, n_b), per-member dJ/dphi_bed or None)."""
+        lambda0 (3, n_b) or None, per-member dJ/dphi_bed or None)."""
         w = np.ascontiguousarray(self.concat(weights)).reshape(-1)
         fn = _lib.tg_functional()
         fn.terminal_weight = _lib.dptr(w)
-        lam = np.empty(3 * self.ncell)
+        lam = np.empty(3 * self.ncell) if want_lambda else None
         dphi = C.c_double(0.0)
         self._check(self.lib.tg_adjoint_run(self.ctx, None, C.byref(fn), _lib.dptr(lam),
                                             C.cast(C.byref(dphi), _lib._dp) if want_dphi
                                             else None))
-        lam = lam.reshape(3, -1)
-        out = [lam[:, o:o + n] for o, n in self.offsets]
+        out = None
+        if want_lambda:
+            lam = lam.reshape(3, -1)
+            out = [lam[:, o:o + n] for o, n in self.offsets]
         per = None
         if want_dphi:
             per = np.empty(len(self.problems))

@@ -58,6 +58,62 @@ int tg_host_piles(int nx, int ny, double dx, double dy, double x0, double y0,
 int tg_host_box_weights(int nx, int ny, double dx, double dy, double x0, double y0,
                         double xmin, double xmax, double ymin, double ymax, double* w);
 
+/* ---- adaptive-mesh flattening ------------------------------------------- */
+
+/* Hilbert keys of normalised coordinates: replaces fhsfc2d_ (hilbert.C:267-278
+ * with hsfc2d :76-145, two 32-bit words, most significant word first).
+ * norm_xy [2*n] interleaved (x, y) in [0, 1]; keys [2*n] interleaved. */
+int tg_host_hsfc2d(int n, const double* norm_xy, uint32_t* keys);
+
+/* Element-table bucket of each key: HashTable::hash (hashtab.h:181-189) with
+ * hashconstant = 8*nbuckets/(keyrange0*keyrange1 + keyrange1)
+ * (hashtab2.C:85). */
+int tg_host_hash_bucket(int n, const uint32_t* keys, double keyrange0, double keyrange1,
+                        int nbuckets, int32_t* bucket);
+
+/* Element-table traversal order of n keys added in index order: buckets
+ * 0..nbuckets-1 (tg_host_hash_bucket), each bucket's list as
+ * HashTable::addElement (hashtab2.C:185-217) builds it -- entries walked past
+ * while the new first key word is larger; on an equal first word, walked past
+ * while the second word is larger, without re-checking the first word.
+ * order[p] = index of the p-th entry visited. */
+int tg_host_traversal(int n, const uint32_t* keys, double keyrange0, double keyrange1,
+                      int nbuckets, int32_t* order);
+
+/* Flattened leaf tables, in the reference's element-table traversal order
+ * (buckets 0..nbuckets-1, each bucket's list in HashTable::addElement order,
+ * hashtab2.C:185-217).  Caller-owned arrays of the sizes given. */
+typedef struct tg_leaf_table {
+  int n;             /* number of leaves (set by tg_host_flatten)              */
+  int32_t* order;    /* [n]   flattened position -> input leaf index            */
+  uint32_t* key;     /* [2n]  element key (Element::key)                       */
+  int32_t* gen;      /* [n]   refinement generation (Element::generation)      */
+  int32_t* nbr;      /* [8n]  flattened index of the neighbour in slot s; -1 on
+                        the domain boundary (Element::neighbor[8])             */
+  int32_t* nkind;    /* [8n]  0 neighbour present, -1 domain boundary, -2 no
+                        second neighbour on that side (Element::neigh_proc
+                        with one process)                                      */
+  int32_t* ngen;     /* [8n]  neighbour generation (Element::neigh_gen)        */
+  double* dx;        /* [2n]  cell size                                        */
+  double* xc;        /* [2n]  cell centre                                      */
+} tg_leaf_table;
+
+/* Flattens a 2:1-balanced quadtree over an nx0 x ny0 base grid of cells
+ * (dx0, dy0) with lower-left corner (xmin, ymin).  Leaf e is the cell
+ * (ix[e], iy[e]) of level lev[e] (cell size dx0/2^lev); e is also the order
+ * in which the leaves were added to the element table (it decides the order
+ * inside a bucket).  Side s: 0 south, 1 east, 2 north, 3 west; with two finer
+ * neighbours on side s, slot s holds the one next to corner s and slot s+4
+ * the one next to corner s+1 (corners 0 SW, 1 SE, 2 NE, 3 NW), as the son
+ * construction in refine2.C:589-700 assigns them.  Element keys are the
+ * Hilbert keys of the centres normalised by the domain extent, with key
+ * ranges 2^32+1 (preprocess.C:561-562, one process).
+ * Returns 0; -1 bad argument; -2 overlapping / out-of-range leaves; -3 leaves
+ * that do not cover the base grid; -4 neighbours more than one level apart. */
+int tg_host_flatten(int nx0, int ny0, double xmin, double ymin, double dx0, double dy0, int n,
+                    const int32_t* lev, const int32_t* ix, const int32_t* iy, int nbuckets,
+                    tg_leaf_table* out);
+
 #ifdef __cplusplus
 }
 #endif
