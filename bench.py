@@ -145,6 +145,15 @@ def kernel_roofline(sim, prec, ncells, k, reps):
     return dom, kt, gbs
 
 
+def tile_shape(v):
+    import ctypes as C
+
+    from discrete_adjoint_amd import _lib
+    tx, ty, band = C.c_int(), C.c_int(), C.c_int()
+    _lib.load().tg_tile_variant(v, C.byref(tx), C.byref(ty), C.byref(band))
+    return f"{tx.value}x{ty.value}" + (" xcd-banded" if band.value else " row-major")
+
+
 def traffic_from_profile(kernel_name, workload):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE runs, scaled by a copy kernel's counters), read from the
@@ -185,7 +194,11 @@ def bench_ensemble(args):
             if W > 0:  # untimed: first launches, allocation of the checkpoint store
                 ens.run(W)
                 ens.dual_solver(weights, want_lambda=False)
+                if not args.no_autotune:  # tile variant per kernel (untimed)
+                    var, _ = ens.autotune(reps=5)
+                    log(f"precision {mode}: autotuned tile variants {var}")
             r = timed_run(ens, K, weights, h0)
+            r["tiles"] = ens.tile_selection()
             r["J"] = member_J(r["H"])
             del r["H"]
             res[mode] = r
@@ -277,6 +290,8 @@ def bench_ensemble(args):
             "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS,
                                  "algorithmic_bytes_per_cell": ab["forward_step"]},
             "kernel_gbs": gbs,
+            "tile_variants": {"coef_update_adjoint": m["tiles"],
+                              "shapes": [tile_shape(v) for v in m["tiles"]]},
         },
         "roofline": {
             "bound": "hbm",
@@ -383,6 +398,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this hot path has no multi-GPU mode "

@@ -25,6 +25,7 @@ TG_ERR_NODEVICE = -8
 TG_FP64 = 0
 TG_MIXED = 1
 TG_FP32 = 2
+TG_NUM_TILE_VARIANTS = 4
 TG_MESH_UNIFORM = 0
 TG_MESH_LEAVES = 1
 
@@ -156,6 +157,11 @@ SIGNATURES = {
     "tg_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "tg_time_kernel": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_int, _dp]),
     "tg_num_cells": (C.c_int64, [C.c_void_p]),
+    "tg_tile_variant": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int)]),
+    "tg_autotune": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.POINTER(C.c_int),
+                              _dp]),
+    "tg_get_tile_selection": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     # tg_host.h
     "tg_host_dem_plane_runout": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double,
                                            C.c_double, C.c_double, C.c_double, C.c_double,

@@ -27,9 +27,13 @@ struct tg_ctx {
 
   // members
   std::vector<InstDev> inst_h;
-  std::vector<TileDev> tiles_h;
+  // one tile table per variant (SHAPE_TX / SHAPE_XCD); sel[k] = variant
+  // used by kernel k (0 coef, 1 update, 2 adjoint); ntiles = largest table
+  std::vector<TileDev> tiles_h[NSHAPE];
   InstDev* inst_d = nullptr;
-  TileDev* tiles_d = nullptr;
+  TileDev* tiles_d[NSHAPE] = {};
+  int nt[NSHAPE] = {};
+  int sel[3] = {0, 0, 0};
   int ninst = 0, ntiles = 0;
   long long ncell = 0;
 
@@ -104,7 +108,7 @@ void free_fields(tg_ctx* ctx) {
   dfree(ctx->part_hmax);
   dfree(ctx->dphi_part);
   dfree(ctx->inst_d);
-  dfree(ctx->tiles_d);
+  for (auto& t : ctx->tiles_d) dfree(t);
   dfree(ctx->red);
   dfree(ctx->time_d);
   dfree(ctx->dt_rec);
@@ -153,10 +157,19 @@ Statics32 statics32(tg_ctx* ctx) {
                    at<float>(ctx->statics, 6, n)};
 }
 
+// points a kernel's arguments at the tile table of its selected variant
+template <class A>
+void use_tiles(const tg_ctx* ctx, int kernel, A& a) {
+  const int v = ctx->sel[kernel];
+  a.tiles = ctx->tiles_d[v];
+  a.ntiles = ctx->nt[v];
+  a.shape = v;
+}
+
 template <class S>
 StepArgsM<S> step_args_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s) {
   StepArgsM<S> a{};
-  a.inst = ctx->inst_d; a.tiles = ctx->tiles_d; a.ntiles = ctx->ntiles;
+  a.inst = ctx->inst_d;
   a.U = FieldsT<S>{(S*)in[0], (S*)in[1], (S*)in[2]};
   a.S32 = statics32(ctx);
   a.kact = (float*)ctx->kact; a.red = ctx->red; a.dt_rec = ctx->dt_rec; a.time = ctx->time_d;
@@ -170,6 +183,7 @@ StepArgsM<S> step_args_m(tg_ctx* ctx, void* const in[3], void* const out[3], int
 template <class S>
 void enqueue_step_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s, bool compute_dt) {
   StepArgsM<S> a = step_args_m<S>(ctx, in, out, s);
+  use_tiles(ctx, 0, a);
   launch_coef_m<S>(a, compute_dt, ctx->stream);
   if (compute_dt) {
     StepArgs d{};
@@ -178,6 +192,7 @@ void enqueue_step_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s, b
     d.cfl_inner = ctx->cfg.cfl_inner; d.cfl_outer = ctx->cfg.cfl_outer;
     launch_dt(d, ctx->ninst, ctx->stream);
   }
+  use_tiles(ctx, 1, a);
   launch_update_m<S>(a, ctx->stream);
 }
 
@@ -185,7 +200,8 @@ template <class S>
 void enqueue_adjoint_m(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* const lout[3],
                        int s, bool dphi) {
   AdjArgsM<S> a{};
-  a.inst = ctx->inst_d; a.tiles = ctx->tiles_d; a.ntiles = ctx->ntiles;
+  a.inst = ctx->inst_d;
+  use_tiles(ctx, 2, a);
   a.U = FieldsT<S>{(S*)U[0], (S*)U[1], (S*)U[2]};
   a.S32 = statics32(ctx);
   a.lam_next = FieldsT<S>{(S*)lnext[0], (S*)lnext[1], (S*)lnext[2]};
@@ -201,7 +217,7 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
   const long long n = ctx->ncell;
   if (ctx->mode == TG_FP64) {
     StepArgs a{};
-    a.inst = ctx->inst_d; a.tiles = ctx->tiles_d; a.ntiles = ctx->ntiles;
+    a.inst = ctx->inst_d;
     a.U = Fields{(double*)in[0], (double*)in[1], (double*)in[2]};
     a.S = Statics{at<double>(ctx->statics, 0, n), at<double>(ctx->statics, 1, n),
                   at<double>(ctx->statics, 2, n), at<double>(ctx->statics, 3, n),
@@ -213,8 +229,10 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
     a.cfl_inner = ctx->cfg.cfl_inner; a.cfl_outer = ctx->cfg.cfl_outer;
     a.h_out = (double*)out[0]; a.hu_out = (double*)out[1]; a.hv_out = (double*)out[2];
     a.err = ctx->err_d;
+    use_tiles(ctx, 0, a);
     launch_coef(a, compute_dt, ctx->stream);
     if (compute_dt) launch_dt(a, ctx->ninst, ctx->stream);
+    use_tiles(ctx, 1, a);
     launch_update(a, ctx->stream);
   } else if (ctx->es == 8) {
     enqueue_step_m<double>(ctx, in, out, s, compute_dt);
@@ -228,7 +246,8 @@ void enqueue_adjoint(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* 
   const long long n = ctx->ncell;
   if (ctx->mode == TG_FP64) {
     AdjArgs a{};
-    a.inst = ctx->inst_d; a.tiles = ctx->tiles_d; a.ntiles = ctx->ntiles;
+    a.inst = ctx->inst_d;
+    use_tiles(ctx, 2, a);
     a.U = Fields{(double*)U[0], (double*)U[1], (double*)U[2]};
     a.S = Statics{at<double>(ctx->statics, 0, n), at<double>(ctx->statics, 1, n),
                   at<double>(ctx->statics, 2, n), at<double>(ctx->statics, 3, n),
@@ -291,17 +310,18 @@ int d2h_field(tg_ctx* ctx, const void* dev_base, long long off, long long count,
 int fill_stats(tg_ctx* ctx, tg_stats* st, int nst) {
   if (!st) return TG_OK;
   if (ctx->es == 8)
-    launch_stats(ctx->inst_d, ctx->tiles_d, ctx->ntiles,
+    launch_stats(ctx->inst_d, ctx->tiles_d[0], ctx->nt[0],
                  Fields{(double*)ctx->U[0], (double*)ctx->U[1], (double*)ctx->U[2]},
                  ctx->part_mass, ctx->part_hmax, ctx->cfg.tiny, ctx->stream);
   else
-    launch_stats32(ctx->inst_d, ctx->tiles_d, ctx->ntiles,
+    launch_stats32(ctx->inst_d, ctx->tiles_d[0], ctx->nt[0],
                    FieldsT<float>{(float*)ctx->U[0], (float*)ctx->U[1], (float*)ctx->U[2]},
                    ctx->part_mass, ctx->part_hmax, ctx->cfg.tiny, ctx->stream);
-  std::vector<double> pm(ctx->ntiles), ph(ctx->ntiles), t(ctx->ninst);
-  HIPCHK(ctx, hipMemcpyAsync(pm.data(), ctx->part_mass, sizeof(double) * ctx->ntiles,
+  const int nt0 = ctx->nt[0];
+  std::vector<double> pm(nt0), ph(nt0), t(ctx->ninst);
+  HIPCHK(ctx, hipMemcpyAsync(pm.data(), ctx->part_mass, sizeof(double) * nt0,
                              hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ph.data(), ctx->part_hmax, sizeof(double) * ctx->ntiles,
+  HIPCHK(ctx, hipMemcpyAsync(ph.data(), ctx->part_hmax, sizeof(double) * nt0,
                              hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(t.data(), ctx->time_d, sizeof(double) * ctx->ninst,
                              hipMemcpyDeviceToHost, ctx->stream));
@@ -313,8 +333,8 @@ int fill_stats(tg_ctx* ctx, tg_stats* st, int nst) {
                                  sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<double> m(ctx->ninst, 0.), hm(ctx->ninst, 0.);
-  for (int k = 0; k < ctx->ntiles; k++) {
-    const int b = ctx->tiles_h[k].inst;
+  for (int k = 0; k < nt0; k++) {
+    const int b = ctx->tiles_h[0][k].inst;
     m[b] += pm[k];
     hm[b] = (ph[k] > hm[b]) ? ph[k] : hm[b];
   }
@@ -461,9 +481,10 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
                                hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     double acc = 0;
-    for (int t = 0; t < ctx->ntiles; t++) {
+    const std::vector<TileDev>& tl = ctx->tiles_h[ctx->sel[2]];
+    for (int t = 0; t < (int)tl.size(); t++) {
       acc += part[t];
-      ctx->member_dphi[ctx->tiles_h[t].inst] += part[t];
+      ctx->member_dphi[tl[t].inst] += part[t];
     }
     *dJ = acc;
   }
@@ -471,6 +492,21 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
 }
 
 // Builds members, tiles and device buffers.
+// Reorders a row-major tile list so that workgroup b (which runs on XCD
+// b % NXCD) takes tile start[x] + b / NXCD of band x = b % NXCD: band x is a
+// contiguous run of the list, of q or q+1 tiles (q = n / NXCD, the first
+// n % NXCD bands one longer).  A bijection on [0, n).
+void band_by_xcd(std::vector<TileDev>& t) {
+  const int n = (int)t.size(), q = n / NXCD, r = n % NXCD;
+  std::vector<TileDev> out(n);
+  for (int b = 0; b < n; b++) {
+    const int x = b % NXCD, slot = b / NXCD;
+    const int start = x * q + (x < r ? x : r);
+    out[b] = t[start + slot];
+  }
+  t.swap(out);
+}
+
 int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* params) {
   if (nm < 1 || !meshes) return set_err(ctx, TG_ERR_ARG, "no members");
   for (int b = 0; b < nm; b++) {
@@ -497,7 +533,7 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   ctx->rec_steps = ctx->run_steps = 0;
   ctx->ninst = nm;
   ctx->inst_h.assign(nm, InstDev{});
-  ctx->tiles_h.clear();
+  for (auto& t : ctx->tiles_h) t.clear();
   long long off = 0;
   for (int b = 0; b < nm; b++) {
     const tg_mesh* m = &meshes[b];
@@ -508,12 +544,20 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     I.inv_dy = 1 / m->dy; I.inv_sy = 1 / (m->dy + m->dy);
     I.maxtime = I.nextoutput = I.nextsave = HUGE_VAL;
     if (params) params_into(I, &params[b]);
-    for (int ty = 0; ty < m->ny; ty += TILE_Y)
-      for (int tx = 0; tx < m->nx; tx += TILE_X) ctx->tiles_h.push_back(TileDev{b, tx, ty, 0});
+    for (int v = 0; v < NSHAPE; v++) {
+      const int TX = SHAPE_TX[v], TY = BLOCK / SHAPE_TX[v];
+      for (int ty = 0; ty < m->ny; ty += TY)
+        for (int tx = 0; tx < m->nx; tx += TX) ctx->tiles_h[v].push_back(TileDev{b, tx, ty, 0});
+    }
     off += (long long)m->nx * m->ny;
   }
   ctx->ncell = off;
-  ctx->ntiles = (int)ctx->tiles_h.size();
+  ctx->ntiles = 0;
+  for (int v = 0; v < NSHAPE; v++) {
+    if (SHAPE_XCD[v]) band_by_xcd(ctx->tiles_h[v]);
+    ctx->nt[v] = (int)ctx->tiles_h[v].size();
+    if (ctx->nt[v] > ctx->ntiles) ctx->ntiles = ctx->nt[v];
+  }
   const long long n = ctx->ncell;
   const size_t fb = (size_t)ctx->es * n;
   int rc = 0;
@@ -530,7 +574,8 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   if (!rc) rc = dalloc(ctx, (void**)&ctx->part_hmax, sizeof(double) * ctx->ntiles);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->dphi_part, sizeof(double) * ctx->ntiles);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->inst_d, sizeof(InstDev) * nm);
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->tiles_d, sizeof(TileDev) * ctx->ntiles);
+  for (int v = 0; v < NSHAPE && !rc; v++)
+    rc = dalloc(ctx, (void**)&ctx->tiles_d[v], sizeof(TileDev) * ctx->nt[v]);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->red, sizeof(RedDev) * nm);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->time_d, sizeof(double) * nm);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->dt_rec, sizeof(double) * nm * ctx->cfg.max_steps);
@@ -548,8 +593,9 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
       rc = h2d_field(ctx, (char*)ctx->statics + (size_t)k * sb, o, nb, src[k], ctx->ses);
   }
   if (rc) return rc;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->tiles_d, ctx->tiles_h.data(), sizeof(TileDev) * ctx->ntiles,
-                             hipMemcpyHostToDevice, ctx->stream));
+  for (int v = 0; v < NSHAPE; v++)
+    HIPCHK(ctx, hipMemcpyAsync(ctx->tiles_d[v], ctx->tiles_h[v].data(),
+                               sizeof(TileDev) * ctx->nt[v], hipMemcpyHostToDevice, ctx->stream));
   rc = upload_params(ctx, nullptr);
   if (rc) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -622,6 +668,22 @@ int tg_create(const tg_config* cfg, tg_ctx** out) {
        ctx->cfg.precision != TG_FP32)) {
     delete ctx;
     return TG_ERR_ARG;
+  }
+  // tile shape: 0 x 0 keeps the built-in default variant (tg_autotune may
+  // change it); an explicit shape selects its XCD-banded table for all kernels
+  if (ctx->cfg.tile_x != 0 || ctx->cfg.tile_y != 0) {
+    int v = -1;
+    for (int k = 0; k < NSHAPE; k++)
+      if (SHAPE_XCD[k] && SHAPE_TX[k] == ctx->cfg.tile_x &&
+          BLOCK / SHAPE_TX[k] == ctx->cfg.tile_y)
+        v = k;
+    if (v < 0) {
+      delete ctx;
+      return TG_ERR_UNSUPPORTED;
+    }
+    ctx->sel[0] = ctx->sel[1] = ctx->sel[2] = v;
+  } else {
+    ctx->sel[0] = ctx->sel[1] = ctx->sel[2] = DEFAULT_VARIANT;
   }
   ctx->mode = ctx->cfg.precision;
   ctx->es = (ctx->mode == TG_FP32) ? 4 : 8;
@@ -836,12 +898,11 @@ int tg_last_timing(tg_ctx* ctx, double* forward_ms, double* adjoint_ms, double* 
   return TG_OK;
 }
 
-int tg_time_kernel(tg_ctx* ctx, const tg_params* prm, int which, int reps, double* avg_ms) {
-  if (!ctx || !avg_ms || reps < 1 || which < 0 || which > 2) return TG_ERR_ARG;
-  if (!ctx->have_mesh || !ctx->have_state || ctx->rec_steps < 1)
-    return set_err(ctx, TG_ERR_STATE, "needs a completed forward run");
-  int rc = upload_params(ctx, prm);
-  if (rc) return rc;
+// Average launch time of kernel `which` with tile variant v (-1: the
+// selected one), over reps back-to-back launches on the current state.
+int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms) {
+  const int keep[3] = {ctx->sel[0], ctx->sel[1], ctx->sel[2]};
+  if (v >= 0) ctx->sel[which] = v;
   // every variant reads U and writes scratch (Ut or lam[1]) with step 0's dt
   auto one = [&](int w) {
     if (w == 2) {
@@ -849,7 +910,8 @@ int tg_time_kernel(tg_ctx* ctx, const tg_params* prm, int which, int reps, doubl
     } else if (ctx->mode == TG_FP64) {
       StepArgs a{};
       const long long n = ctx->ncell;
-      a.inst = ctx->inst_d; a.tiles = ctx->tiles_d; a.ntiles = ctx->ntiles;
+      a.inst = ctx->inst_d;
+      use_tiles(ctx, w, a);
       a.U = Fields{(double*)ctx->U[0], (double*)ctx->U[1], (double*)ctx->U[2]};
       a.S = Statics{at<double>(ctx->statics, 0, n), at<double>(ctx->statics, 1, n),
                     at<double>(ctx->statics, 2, n), at<double>(ctx->statics, 3, n),
@@ -863,24 +925,78 @@ int tg_time_kernel(tg_ctx* ctx, const tg_params* prm, int which, int reps, doubl
       else launch_update(a, ctx->stream);
     } else if (ctx->es == 8) {
       StepArgsM<double> a = step_args_m<double>(ctx, ctx->U, ctx->Ut, 0);
+      use_tiles(ctx, w, a);
       if (w == 0) launch_coef_m<double>(a, false, ctx->stream);
       else launch_update_m<double>(a, ctx->stream);
     } else {
       StepArgsM<float> a = step_args_m<float>(ctx, ctx->U, ctx->Ut, 0);
+      use_tiles(ctx, w, a);
       if (w == 0) launch_coef_m<float>(a, false, ctx->stream);
       else launch_update_m<float>(a, ctx->stream);
     }
   };
   one(0);  // kact of the current state
-  HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-  for (int r = 0; r < reps; r++) one(which);
-  HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-  HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+  one(which);  // untimed first launch of this variant
+  int rc = TG_OK;
   float ms = 0;
-  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  if (hipEventRecord(ctx->ev[0], ctx->stream) != hipSuccess) rc = TG_ERR_HIP;
+  for (int r = 0; r < reps && !rc; r++) one(which);
+  if (!rc && (hipEventRecord(ctx->ev[1], ctx->stream) != hipSuccess ||
+              hipGetLastError() != hipSuccess || hipEventSynchronize(ctx->ev[1]) != hipSuccess ||
+              hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) != hipSuccess))
+    rc = set_err(ctx, TG_ERR_HIP, "kernel timing failed");
+  for (int k = 0; k < 3; k++) ctx->sel[k] = keep[k];
+  if (rc) return rc;
   *avg_ms = ms / reps;
   return check_err(ctx);
+}
+
+int tg_time_kernel(tg_ctx* ctx, const tg_params* prm, int which, int reps, double* avg_ms) {
+  if (!ctx || !avg_ms || reps < 1 || which < 0 || which > 2) return TG_ERR_ARG;
+  if (!ctx->have_mesh || !ctx->have_state || ctx->rec_steps < 1)
+    return set_err(ctx, TG_ERR_STATE, "needs a completed forward run");
+  int rc = upload_params(ctx, prm);
+  if (rc) return rc;
+  return time_kernel_variant(ctx, which, -1, reps, avg_ms);
+}
+
+int tg_autotune(tg_ctx* ctx, const tg_params* prm, int reps, int* variant_out, double* ms_out) {
+  if (!ctx || reps < 1) return TG_ERR_ARG;
+  if (!ctx->have_mesh || !ctx->have_state || ctx->rec_steps < 1)
+    return set_err(ctx, TG_ERR_STATE, "needs a completed forward run");
+  int rc = upload_params(ctx, prm);
+  if (rc) return rc;
+  // the adjoint variant reads lam[0]: give it defined values (zeros)
+  const size_t fb = (size_t)ctx->es * ctx->ncell;
+  for (int c = 0; c < 3; c++) HIPCHK(ctx, hipMemsetAsync(ctx->lam[0][c], 0, fb, ctx->stream));
+  for (int w = 0; w < 3; w++) {
+    double best = HUGE_VAL;
+    int bv = ctx->sel[w];
+    for (int v = 0; v < NSHAPE; v++) {
+      double ms = 0;
+      rc = time_kernel_variant(ctx, w, v, reps, &ms);
+      if (rc) return rc;
+      if (ms_out) ms_out[w * NSHAPE + v] = ms;
+      if (ms < best) { best = ms; bv = v; }
+    }
+    ctx->sel[w] = bv;
+    if (variant_out) variant_out[w] = bv;
+  }
+  return TG_OK;
+}
+
+int tg_tile_variant(int v, int* tile_x, int* tile_y, int* xcd_banded) {
+  if (v < 0 || v >= NSHAPE) return TG_ERR_ARG;
+  if (tile_x) *tile_x = SHAPE_TX[v];
+  if (tile_y) *tile_y = BLOCK / SHAPE_TX[v];
+  if (xcd_banded) *xcd_banded = SHAPE_XCD[v] ? 1 : 0;
+  return TG_OK;
+}
+
+int tg_get_tile_selection(const tg_ctx* ctx, int* variant_out) {
+  if (!ctx || !variant_out) return TG_ERR_ARG;
+  for (int k = 0; k < 3; k++) variant_out[k] = ctx->sel[k];
+  return TG_OK;
 }
 
 int64_t tg_num_cells(const tg_ctx* ctx) { return ctx ? (int64_t)ctx->ncell : 0; }

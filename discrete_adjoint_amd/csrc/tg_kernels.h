@@ -19,7 +19,7 @@ struct InstDev {
   double maxtime, nextoutput, nextsave;
 };
 
-// A launch tile: TILE_X x TILE_Y cells of instance `inst` starting at
+// A launch tile: TX x (BLOCK / TX) cells of instance `inst` starting at
 // (x0, y0).  One workgroup per tile.
 struct TileDev {
   int inst, x0, y0, pad;
@@ -33,9 +33,17 @@ struct RedDev {
   unsigned long long hmax;    // max over wet cells of h
 };
 
-constexpr int TILE_X = 64;
-constexpr int TILE_Y = 4;
-constexpr int BLOCK = TILE_X * TILE_Y;
+constexpr int BLOCK = 256;
+// Tile variants (one tile table each): tile width TX (height BLOCK / TX)
+// and whether the table is banded by XCD -- workgroup b runs on XCD b % 8,
+// so a banded table hands XCD k a contiguous run of tiles (rows of the
+// block), keeping halo rows in one L2.  Variant 0 is the plain row-major
+// order.  The autotuner picks a variant per kernel.
+constexpr int NSHAPE = 4;
+constexpr int SHAPE_TX[NSHAPE] = {64, 64, 32, 16};
+constexpr bool SHAPE_XCD[NSHAPE] = {false, true, true, true};
+constexpr int NXCD = 8;
+constexpr int DEFAULT_VARIANT = 0;
 
 struct Fields {
   const double* h;
@@ -56,6 +64,7 @@ struct StepArgs {
   const InstDev* inst;
   const TileDev* tiles;
   int ntiles;
+  int shape;             // tile variant of `tiles`
   Fields U;
   Statics S;
   double* kact;          // [n] forward kact of U
@@ -76,6 +85,7 @@ struct AdjArgs {
   const InstDev* inst;
   const TileDev* tiles;
   int ntiles;
+  int shape;             // tile variant of `tiles`
   Fields U;              // U^s
   Statics S;
   Fields lam_next;       // lambda^{s+1}
@@ -113,6 +123,7 @@ struct StepArgsM {
   const InstDev* inst;
   const TileDev* tiles;
   int ntiles;
+  int shape;             // tile variant of `tiles`
   FieldsT<S> U;
   Statics32 S32;
   float* kact;
@@ -133,6 +144,7 @@ struct AdjArgsM {
   const InstDev* inst;
   const TileDev* tiles;
   int ntiles;
+  int shape;             // tile variant of `tiles`
   FieldsT<S> U;
   Statics32 S32;
   FieldsT<S> lam_next;

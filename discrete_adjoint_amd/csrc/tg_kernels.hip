@@ -53,10 +53,11 @@ struct Cell {
   bool hasE, hasW, hasN, hasS;
 };
 
+template <int TX>
 __device__ __forceinline__ Cell cell_of(const TileDev& t, const InstDev& I) {
   Cell q;
-  q.i = t.x0 + (int)(threadIdx.x % TILE_X);
-  q.j = t.y0 + (int)(threadIdx.x / TILE_X);
+  q.i = t.x0 + (int)(threadIdx.x % TX);
+  q.j = t.y0 + (int)(threadIdx.x / TX);
   q.act = q.i < I.nx && q.j < I.ny;
   q.c = I.off + (long long)q.j * I.nx + q.i;
   q.hasE = q.i + 1 < I.nx;
@@ -68,10 +69,11 @@ __device__ __forceinline__ Cell cell_of(const TileDev& t, const InstDev& I) {
 
 // ---------------------------------------------------------------- forward --
 
+template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
-  const Cell q = cell_of(t, I);
+  const Cell q = cell_of<TX>(t, I);
   double minev = HUGE_VAL, hmax = 0.;
   if (q.act) {
     const long long c = q.c, nx = I.nx;
@@ -149,10 +151,11 @@ __device__ __forceinline__ Hfv hfv_at(const StepArgs& a, int dir, long long c) {
   return hfv_fwd(dir, a.U.h[c], a.U.hu[c], a.U.hv[c], a.kact[c], a.S.gz[c], a.tiny);
 }
 
+template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_update(StepArgs a) {
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
-  const Cell q = cell_of(t, I);
+  const Cell q = cell_of<TX>(t, I);
   if (!q.act) return;
   const long long c = q.c, nx = I.nx;
   const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(BLOCK) void k_stats(const InstDev* inst, const Tile
                                                  double* part_hmax, double tiny) {
   const TileDev t = tiles[blockIdx.x];
   const InstDev I = inst[t.inst];
-  const Cell q = cell_of(t, I);
+  const Cell q = cell_of<64>(t, I);
   double m = 0., hm = 0.;
   if (q.act) {
     double h = U.h[q.c];
@@ -413,7 +416,9 @@ __device__ double dphi_term(const AdjArgs& a, const InstDev& I, const Cell& q,
 // The part of a cell's dual record its neighbours read, staged in LDS for the
 // tile plus a one-cell halo: each record is computed once per workgroup
 // instead of once per reading cell.
-constexpr int RX = TILE_X + 2, RY = TILE_Y + 2, RN = RX * RY, NHALO = RN - BLOCK;
+// LDS record slots: the largest tile plus halo over the built tile widths
+// ((TX + 2) * (BLOCK / TX + 2) is largest at TX = 64)
+constexpr int RN = (64 + 2) * (BLOCK / 64 + 2);
 struct DualLds {
   double h[RN], hu[RN], hv[RN], g2[RN], k[RN], dk[RN];
   double hsx1[RN], hsx2[RN], hsy1[RN], hsy2[RN];
@@ -445,14 +450,17 @@ __device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
   return d;
 }
 
+template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
-  const Cell q = cell_of(t, I);
+  const Cell q = cell_of<TX>(t, I);
   double dphi = 0.;
   const bool want_dphi = a.dphi_part != nullptr;
+  constexpr int TY = BLOCK / TX, RX = TX + 2, RY = TY + 2, NHALO = RX * RY - BLOCK;
+  static_assert(RX * RY <= RN && NHALO <= BLOCK, "tile does not fit the LDS records");
   __shared__ DualLds L;
-  const int tx = (int)(threadIdx.x % TILE_X), ty = (int)(threadIdx.x / TILE_X);
+  const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
   const int rself = (ty + 1) * RX + tx + 1;
   // phase 1: dual records of the tile (kept in registers) and of the halo
   DualInfo self;
@@ -465,8 +473,8 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
     int rx, ry;
     if (h < RX) { rx = h; ry = 0; }
     else if (h < 2 * RX) { rx = h - RX; ry = RY - 1; }
-    else if (h < 2 * RX + TILE_Y) { rx = 0; ry = h - 2 * RX + 1; }
-    else { rx = RX - 1; ry = h - 2 * RX - TILE_Y + 1; }
+    else if (h < 2 * RX + TY) { rx = 0; ry = h - 2 * RX + 1; }
+    else { rx = RX - 1; ry = h - 2 * RX - TY + 1; }
     const int gi = t.x0 - 1 + rx, gj = t.y0 - 1 + ry;
     if (gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny)
       lds_put(L, ry * RX + rx, dual_info(a, I, gi, gj, want_dphi));
@@ -605,17 +613,27 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
 
 }  // namespace
 
+// one instantiation per built tile width; a.shape selects it
+#define TG_LAUNCH_TX(shape, KERNEL, GRID, ...)                                      \
+  do {                                                                             \
+    switch (SHAPE_TX[shape]) {                                                     \
+      case 64: hipLaunchKernelGGL(KERNEL<64>, GRID, dim3(BLOCK), 0, __VA_ARGS__); break; \
+      case 32: hipLaunchKernelGGL(KERNEL<32>, GRID, dim3(BLOCK), 0, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL(KERNEL<16>, GRID, dim3(BLOCK), 0, __VA_ARGS__); break; \
+    }                                                                              \
+  } while (0)
+
 void launch_coef(const StepArgs& a, bool reduce, hipStream_t s) {
-  hipLaunchKernelGGL(k_coef, dim3(a.ntiles), dim3(BLOCK), 0, s, a, reduce ? 1 : 0);
+  TG_LAUNCH_TX(a.shape, k_coef, dim3(a.ntiles), s, a, reduce ? 1 : 0);
 }
 void launch_dt(const StepArgs& a, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_dt, dim3((ninst + 63) / 64), dim3(64), 0, s, a, ninst);
 }
 void launch_update(const StepArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_update, dim3(a.ntiles), dim3(BLOCK), 0, s, a);
+  TG_LAUNCH_TX(a.shape, k_update, dim3(a.ntiles), s, a);
 }
 void launch_adjoint(const AdjArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_adjoint, dim3(a.ntiles), dim3(BLOCK), 0, s, a);
+  TG_LAUNCH_TX(a.shape, k_adjoint, dim3(a.ntiles), s, a);
 }
 void launch_reset_red(RedDev* red, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_red, dim3((ninst + 63) / 64), dim3(64), 0, s, red, ninst);

@@ -50,10 +50,11 @@ struct Cell {
   long long c;
   bool act, hasE, hasW, hasN, hasS;
 };
+template <int TX>
 __device__ __forceinline__ Cell cell_of(const TileDev& t, const InstDev& I) {
   Cell q;
-  q.i = t.x0 + (int)(threadIdx.x % TILE_X);
-  q.j = t.y0 + (int)(threadIdx.x / TILE_X);
+  q.i = t.x0 + (int)(threadIdx.x % TX);
+  q.j = t.y0 + (int)(threadIdx.x / TX);
   q.act = q.i < I.nx && q.j < I.ny;
   q.c = I.off + (long long)q.j * I.nx + q.i;
   q.hasE = q.i + 1 < I.nx;
@@ -65,12 +66,12 @@ __device__ __forceinline__ Cell cell_of(const TileDev& t, const InstDev& I) {
 
 // ---------------------------------------------------------------- forward --
 
-template <class S>
+template <class S, int TX>
 __global__ __launch_bounds__(BLOCK) void k32_coef(StepArgsM<S> a, int reduce) {
   using W = Wt<S>;
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
-  const Cell q = cell_of(t, I);
+  const Cell q = cell_of<TX>(t, I);
   const W tiny = (W)a.tiny, gamma = (W)a.gamma;
   double minev = HUGE_VAL, hmax = 0.;
   if (q.act) {
@@ -126,12 +127,12 @@ __device__ __forceinline__ Hfv hfv_at(const StepArgsM<S>& a, int dir, long long 
   return hfv(dir, (float)h, (float)hu, (float)hv, (float)un, a.kact[c], a.S32.gz[c], wet);
 }
 
-template <class S>
+template <class S, int TX>
 __global__ __launch_bounds__(BLOCK) void k32_update(StepArgsM<S> a) {
   using W = Wt<S>;
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
-  const Cell q = cell_of(t, I);
+  const Cell q = cell_of<TX>(t, I);
   if (!q.act) return;
   const long long c = q.c, nx = I.nx;
   const W tiny = (W)a.tiny, gamma = (W)a.gamma;
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(BLOCK) void k32_stats(const InstDev* inst, const Ti
                                                    double* part_hmax, double tiny) {
   const TileDev t = tiles[blockIdx.x];
   const InstDev I = inst[t.inst];
-  const Cell q = cell_of(t, I);
+  const Cell q = cell_of<64>(t, I);
   double m = 0., hm = 0.;
   if (q.act) {
     double h = U.h[q.c];
@@ -383,7 +384,8 @@ __device__ __forceinline__ DualInfo dual_info(const AdjArgsM<S>& a, const InstDe
   return r;
 }
 
-constexpr int RX = TILE_X + 2, RY = TILE_Y + 2, RN = RX * RY, NHALO = RN - BLOCK;
+// LDS record slots: the largest tile plus halo over the built tile widths
+constexpr int RN = (64 + 2) * (BLOCK / 64 + 2);
 struct DualLds {
   float h[RN], hu[RN], hv[RN], g2[RN], ih[RN], u[RN], v[RN], k[RN], dk[RN];
   float hsx1[RN], hsx2[RN], hsy1[RN], hsy2[RN];
@@ -528,14 +530,16 @@ __device__ float dphi_term(const AdjArgsM<S>& a, const InstDev& I, long long c,
   return lam[0] * out[0] + lam[1] * out[1] + lam[2] * out[2];
 }
 
-template <class S>
+template <class S, int TX>
 __global__ __launch_bounds__(BLOCK) void k32_adjoint(AdjArgsM<S> a) {
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
-  const Cell q = cell_of(t, I);
+  const Cell q = cell_of<TX>(t, I);
   const bool want_dphi = a.dphi_part != nullptr;
+  constexpr int TY = BLOCK / TX, RX = TX + 2, RY = TY + 2, NHALO = RX * RY - BLOCK;
+  static_assert(RX * RY <= RN && NHALO <= BLOCK, "tile does not fit the LDS records");
   __shared__ DualLds L;
-  const int tx = (int)(threadIdx.x % TILE_X), ty = (int)(threadIdx.x / TILE_X);
+  const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
   const int rself = (ty + 1) * RX + tx + 1;
   DualInfo self;
   if (q.act) {
@@ -547,8 +551,8 @@ __global__ __launch_bounds__(BLOCK) void k32_adjoint(AdjArgsM<S> a) {
     int rx, ry;
     if (h < RX) { rx = h; ry = 0; }
     else if (h < 2 * RX) { rx = h - RX; ry = RY - 1; }
-    else if (h < 2 * RX + TILE_Y) { rx = 0; ry = h - 2 * RX + 1; }
-    else { rx = RX - 1; ry = h - 2 * RX - TILE_Y + 1; }
+    else if (h < 2 * RX + TY) { rx = 0; ry = h - 2 * RX + 1; }
+    else { rx = RX - 1; ry = h - 2 * RX - TY + 1; }
     const int gi = t.x0 - 1 + rx, gj = t.y0 - 1 + ry;
     if (gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny)
       lds_put(L, ry * RX + rx, dual_info(a, I, gi, gj, want_dphi));
@@ -680,17 +684,26 @@ __global__ __launch_bounds__(BLOCK) void k32_adjoint(AdjArgsM<S> a) {
 
 }  // namespace
 
+#define TG_LAUNCH_TX(shape, KERNEL, S, GRID, ...)                                       \
+  do {                                                                                \
+    switch (SHAPE_TX[shape]) {                                                        \
+      case 64: hipLaunchKernelGGL((KERNEL<S, 64>), GRID, dim3(BLOCK), 0, __VA_ARGS__); break; \
+      case 32: hipLaunchKernelGGL((KERNEL<S, 32>), GRID, dim3(BLOCK), 0, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((KERNEL<S, 16>), GRID, dim3(BLOCK), 0, __VA_ARGS__); break; \
+    }                                                                                 \
+  } while (0)
+
 template <class S>
 void launch_coef_m(const StepArgsM<S>& a, bool reduce, hipStream_t s) {
-  hipLaunchKernelGGL(k32_coef<S>, dim3(a.ntiles), dim3(BLOCK), 0, s, a, reduce ? 1 : 0);
+  TG_LAUNCH_TX(a.shape, k32_coef, S, dim3(a.ntiles), s, a, reduce ? 1 : 0);
 }
 template <class S>
 void launch_update_m(const StepArgsM<S>& a, hipStream_t s) {
-  hipLaunchKernelGGL(k32_update<S>, dim3(a.ntiles), dim3(BLOCK), 0, s, a);
+  TG_LAUNCH_TX(a.shape, k32_update, S, dim3(a.ntiles), s, a);
 }
 template <class S>
 void launch_adjoint_m(const AdjArgsM<S>& a, hipStream_t s) {
-  hipLaunchKernelGGL(k32_adjoint<S>, dim3(a.ntiles), dim3(BLOCK), 0, s, a);
+  TG_LAUNCH_TX(a.shape, k32_adjoint, S, dim3(a.ntiles), s, a);
 }
 template void launch_coef_m<float>(const StepArgsM<float>&, bool, hipStream_t);
 template void launch_coef_m<double>(const StepArgsM<double>&, bool, hipStream_t);

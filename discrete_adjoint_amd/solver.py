@@ -159,6 +159,22 @@ class Simulation:
                                             C.cast(C.byref(ms), _lib._dp)))
         return ms.value
 
+    def _pp(self):
+        return C.byref(self.params)
+
+    def autotune(self, reps=10):
+        """Times every tile variant per kernel and keeps the fastest.
+        Returns (variants [coef, update, adjoint], ms (3, variants))."""
+        var = (C.c_int * 3)()
+        ms = np.zeros(3 * _lib.TG_NUM_TILE_VARIANTS)
+        self._check(self.lib.tg_autotune(self.ctx, self._pp(), reps, var, _lib.dptr(ms)))
+        return list(var), ms.reshape(3, -1)
+
+    def tile_selection(self):
+        var = (C.c_int * 3)()
+        self._check(self.lib.tg_get_tile_selection(self.ctx, var))
+        return list(var)
+
     def kernel_timing(self):
         return self.timing()
 

@@ -58,7 +58,9 @@ typedef struct tg_config {
   double frict_tiny_late; /* 1e-9 (hpfem.C:232-235)                            */
   int frict_tiny_switch;  /* steps 0..49 use the early value (hpfem.C:232,
                              calc_jacobain.C:81-84)                            */
-  int tile_x, tile_y;     /* 0 = use the autotuned / built-in tile shape       */
+  int tile_x, tile_y;     /* 0 x 0: built-in tile variant (tg_autotune may
+                             re-select); 64x4, 32x8 or 16x16: that shape,
+                             XCD-banded, for every kernel                     */
 } tg_config;
 
 /* Mesh.  TG_MESH_UNIFORM: an nx*ny block of equal cells; cell c = j*nx + i,
@@ -205,6 +207,27 @@ int tg_last_timing(tg_ctx* ctx, double* forward_ms, double* adjoint_ms,
  * Requires a completed tg_forward_run (and tg_adjoint_run for which = 2). */
 typedef enum tg_kernel_id { TG_K_COEF = 0, TG_K_UPDATE = 1, TG_K_ADJOINT = 2 } tg_kernel_id;
 int tg_time_kernel(tg_ctx* ctx, const tg_params* prm, int which, int reps, double* avg_ms);
+
+/* ---- tile shapes (no reference counterpart: launch configuration) ----------
+ * Every kernel runs one 256-thread workgroup per tile of cells.  Built tile
+ * variants v = 0..TG_NUM_TILE_VARIANTS-1: a tile shape (tile_x x tile_y
+ * cells) and a tile order -- row-major, or banded by XCD (workgroup b runs
+ * on XCD b % 8, and a banded table gives each XCD a contiguous run of tile
+ * rows so halo rows stay in that XCD's L2).  Results do not depend on the
+ * variant except for the order of FP64 partial sums (mass statistics,
+ * dJ/dphi_bed), which is fixed for a fixed variant. */
+#define TG_NUM_TILE_VARIANTS 4
+int tg_tile_variant(int variant, int* tile_x, int* tile_y, int* xcd_banded);
+
+/* Times every variant for each kernel (coef, update, adjoint; `reps` launches
+ * each on the current state, after a completed tg_forward_run) and selects
+ * the fastest per kernel for the rest of the context's life.
+ * variant_out [3] (may be NULL); ms_out [3 * TG_NUM_TILE_VARIANTS] average
+ * launch times (may be NULL).  Overwrites the adjoint buffers. */
+int tg_autotune(tg_ctx* ctx, const tg_params* prm, int reps, int* variant_out, double* ms_out);
+
+/* Variants in use per kernel (coef, update, adjoint). */
+int tg_get_tile_selection(const tg_ctx* ctx, int* variant_out);
 
 /* Cells in the context's mesh (sum over instances). */
 int64_t tg_num_cells(const tg_ctx* ctx);

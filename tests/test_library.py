@@ -82,6 +82,25 @@ def test_no_device_is_reported_not_crashed():
     assert rc in (_lib.TG_ERR_NODEVICE, _lib.TG_ERR_HIP)
 
 
+def test_tile_variants_are_described_and_validated():
+    lib = _lib.load()
+    shapes = []
+    for v in range(_lib.TG_NUM_TILE_VARIANTS):
+        tx, ty, band = C.c_int(), C.c_int(), C.c_int()
+        assert lib.tg_tile_variant(v, C.byref(tx), C.byref(ty), C.byref(band)) == 0
+        assert tx.value * ty.value == 256
+        shapes.append((tx.value, ty.value, band.value))
+    assert shapes[0] == (64, 4, 0)       # the plain row-major table
+    assert {(a, b) for a, b, c in shapes if c} == {(64, 4), (32, 8), (16, 16)}
+    assert lib.tg_tile_variant(_lib.TG_NUM_TILE_VARIANTS, None, None, None) == _lib.TG_ERR_ARG
+    # an unbuilt tile shape is refused before any device is touched
+    cfg = _lib.tg_config()
+    lib.tg_config_default(C.byref(cfg))
+    cfg.tile_x, cfg.tile_y = 128, 2
+    ctx = C.c_void_p()
+    assert lib.tg_create(C.byref(cfg), C.byref(ctx)) == _lib.TG_ERR_UNSUPPORTED
+
+
 def test_horn_slope_is_exact_on_a_plane():
     # z = a x + b y sampled on the raster: Horn's operator and the bilinear
     # sampling return (a, b); gravity is the plane's; curvature and
