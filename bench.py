@@ -189,14 +189,20 @@ def bench_ensemble(args):
 
     res = {}
     roof = None
+    graph = 0 if args.no_graph else 1
     for mode in ([prec, _lib.TG_FP64] if prec != _lib.TG_FP64 else [prec]):
-        with Ensemble(E, precision=mode, checkpoint_every=k, max_steps=max(K, W, 1)) as ens:
+        with Ensemble(E, precision=mode, checkpoint_every=k, max_steps=max(K, W, 1),
+                      use_graph=graph) as ens:
             if W > 0:  # untimed: first launches, allocation of the checkpoint store
                 ens.run(W)
                 ens.dual_solver(weights, want_lambda=False)
                 if not args.no_autotune:  # tile variant per kernel (untimed)
                     var, _ = ens.autotune(reps=5)
                     log(f"precision {mode}: autotuned tile variants {var}")
+                if graph:  # capture the K-step sweeps (untimed)
+                    ens.set_state(*h0)
+                    ens.run(K)
+                    ens.dual_solver(weights, want_lambda=False)
             r = timed_run(ens, K, weights, h0)
             r["tiles"] = ens.tile_selection()
             r["J"] = member_J(r["H"])
@@ -290,6 +296,7 @@ def bench_ensemble(args):
             "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS,
                                  "algorithmic_bytes_per_cell": ab["forward_step"]},
             "kernel_gbs": gbs,
+            "graph_replay": bool(graph),
             "tile_variants": {"coef_update_adjoint": m["tiles"],
                               "shapes": [tile_shape(v) for v in m["tiles"]]},
         },
@@ -384,6 +391,42 @@ def bench_grid(args):
     }
 
 
+def probe(args):
+    """Short run for hardware-counter profiles (rocprofv3 --pmc): the same
+    ensemble, W forward steps and their adjoint, then `kernel_reps` launches
+    of each kernel on the resulting state (the launches the counters are read
+    from), then a 1 GiB device copy whose counters calibrate FETCH_SIZE /
+    WRITE_SIZE against a known byte count."""
+    import numpy as np
+    import torch
+
+    from discrete_adjoint_amd import _lib, problems
+    from discrete_adjoint_amd.solver import Ensemble
+
+    K, k = args.steps, args.checkpoint_every
+    E = problems.ensemble_problems(args.members, seed=args.seed, nsteps=K, checkpoint_every=k)
+    weights = [p.box_weights().reshape(3, -1) for p in E]
+    prec = {"mixed": _lib.TG_MIXED, "fp32": _lib.TG_FP32, "fp64": _lib.TG_FP64}[args.precision]
+    tiles = [int(v) for v in args.tiles.split(",")] if args.tiles else None
+    with Ensemble(E, precision=prec, checkpoint_every=k, max_steps=max(K, 1)) as ens:
+        ens.run(K)
+        ens.dual_solver(weights, want_lambda=False)
+        if tiles:  # reproduce a tile selection (e.g. the bench's autotuned one)
+            ens.select_tiles(tiles)
+        for i, name in enumerate(KERNELS):
+            ms = ens.time_kernel(i, args.kernel_reps)
+            log(f"probe {name}: {ms:.4f} ms/launch over {args.kernel_reps} launches")
+        log(f"probe cells {ens.ncell} tiles {ens.tile_selection()}")
+    n = 1 << 28  # 2^28 FP32 = 1 GiB read + 1 GiB written
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"probe copy: {4 * n} bytes read, {4 * n} bytes written")
+    assert float(y[-1]) == 1.0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -399,12 +442,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for hardware-counter profiles (see probe())")
+    ap.add_argument("--tiles", default="",
+                    help="with --probe: tile variants coef,update,adjoint")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch kernel by kernel instead of replaying captured sweeps")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this hot path has no multi-GPU mode "
               "(ensembles would shard by member, one large block by SFC range; neither is "
               "built)", file=sys.stderr)
         sys.exit(2)
+    if args.probe:
+        args.steps = args.steps or 5
+        args.checkpoint_every = args.checkpoint_every or 25
+        probe(args)
+        return
     if args.workload == "ensemble":
         args.steps = args.steps or 500
         args.checkpoint_every = args.checkpoint_every or 25

@@ -48,6 +48,7 @@ class tg_config(C.Structure):
         ("frict_tiny_switch", C.c_int),
         ("tile_x", C.c_int),
         ("tile_y", C.c_int),
+        ("use_graph", C.c_int),
     ]
 
 
@@ -162,6 +163,7 @@ SIGNATURES = {
     "tg_autotune": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.POINTER(C.c_int),
                               _dp]),
     "tg_get_tile_selection": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "tg_set_tile_selection": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     # tg_host.h
     "tg_host_dem_plane_runout": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double,
                                            C.c_double, C.c_double, C.c_double, C.c_double,

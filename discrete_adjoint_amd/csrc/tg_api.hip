@@ -16,6 +16,15 @@
 
 using namespace tg;
 
+// An instantiated capture of a whole sweep and the inputs it was captured
+// for (step count, buffer parity, tile variants, ...).
+struct SweepGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  long long key[8] = {};
+  bool valid = false;
+};
+
 struct tg_ctx {
   tg_config cfg{};
   hipStream_t stream = nullptr;
@@ -40,6 +49,7 @@ struct tg_ctx {
   // device fields (element type double or float by es)
   void* U[3] = {nullptr, nullptr, nullptr};
   void* Ut[3] = {nullptr, nullptr, nullptr};
+  void* U_home = nullptr;  // U[0] as allocated
   void* statics = nullptr;  // 7 * ncell
   void* kact = nullptr;
   void* lam[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
@@ -56,6 +66,10 @@ struct tg_ctx {
   int run_steps = 0;  // steps of the last forward run/replay (0 = none)
   std::vector<void*> ckpt, seg;
   std::vector<double> member_dphi;
+
+  // captured sweeps (cfg.use_graph): one forward and one reverse graph,
+  // re-captured when their key changes
+  SweepGraph gf, ga;
 
   // timing
   hipEvent_t ev[6] = {};
@@ -95,7 +109,15 @@ int dalloc(tg_ctx* ctx, void** p, size_t bytes) {
   return TG_OK;
 }
 
+void drop_graph(SweepGraph& G) {
+  if (G.x) (void)hipGraphExecDestroy(G.x);  // teardown: nothing to report to
+  if (G.g) (void)hipGraphDestroy(G.g);
+  G = SweepGraph{};
+}
+
 void free_fields(tg_ctx* ctx) {
+  drop_graph(ctx->gf);
+  drop_graph(ctx->ga);
   for (int k = 0; k < 3; k++) {
     dfree(ctx->U[k]);
     dfree(ctx->Ut[k]);
@@ -363,11 +385,33 @@ void split(tg_ctx* ctx, void* base, void* out[3]) {
   for (int k = 0; k < 3; k++) out[k] = (char*)base + (size_t)k * ctx->es * ctx->ncell;
 }
 
-int copy_state(tg_ctx* ctx, void* dst, void* const src[3]) {
+void copy_state(tg_ctx* ctx, void* dst, void* const src[3]) {
   const size_t b = (size_t)ctx->es * ctx->ncell;
-  for (int k = 0; k < 3; k++)
-    HIPCHK(ctx, hipMemcpyAsync((char*)dst + k * b, src[k], b, hipMemcpyDeviceToDevice,
-                               ctx->stream));
+  for (int k = 0; k < 3; k++) launch_copy(src[k], (char*)dst + k * b, b, ctx->stream);
+}
+
+bool same_key(const SweepGraph& G, const long long key[8]) {
+  if (!G.valid) return false;
+  for (int i = 0; i < 8; i++)
+    if (G.key[i] != key[i]) return false;
+  return true;
+}
+
+// Captures what `body` enqueues on the context's stream into G (replacing
+// what G held) and instantiates it.
+template <class F>
+int capture(tg_ctx* ctx, SweepGraph& G, const long long key[8], F&& body) {
+  drop_graph(G);
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  body();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (e != hipSuccess || !g)
+    return set_err(ctx, TG_ERR_HIP, std::string("stream capture failed: ") + hipGetErrorString(e));
+  G.g = g;
+  HIPCHK(ctx, hipGraphInstantiate(&G.x, G.g, nullptr, nullptr, 0));
+  for (int i = 0; i < 8; i++) G.key[i] = key[i];
+  G.valid = true;
   return TG_OK;
 }
 
@@ -386,17 +430,36 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
     for (int b = 0; b < ctx->ninst; b++)
       HIPCHK(ctx, hipMemcpyAsync(ctx->dt_rec + (long long)b * ctx->cfg.max_steps, dt_seq,
                                  sizeof(double) * nsteps, hipMemcpyHostToDevice, ctx->stream));
-  launch_reset_red(ctx->red, ctx->ninst, ctx->stream);
-  HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-  for (int s = 0; s < nsteps; s++) {
-    if (s % k == 0) {
-      rc = copy_state(ctx, ctx->ckpt[s / k], ctx->U);
+  // the sweep: checkpoint every k steps, one step per iteration; swaps the
+  // U / Ut roles once per step
+  auto body = [&]() {
+    launch_reset_red(ctx->red, ctx->ninst, ctx->stream);
+    for (int s = 0; s < nsteps; s++) {
+      if (s % k == 0) copy_state(ctx, ctx->ckpt[s / k], ctx->U);
+      enqueue_step(ctx, ctx->U, ctx->Ut, s, dt_seq == nullptr);
+      for (int c = 0; c < 3; c++) std::swap(ctx->U[c], ctx->Ut[c]);
+    }
+  };
+  if (ctx->cfg.use_graph) {
+    const long long key[8] = {nsteps, dt_seq != nullptr, (long long)(intptr_t)ctx->U[0],
+                              ctx->sel[0], ctx->sel[1], k, 0, 0};
+    if (!same_key(ctx->gf, key)) {
+      void* U0[3] = {ctx->U[0], ctx->U[1], ctx->U[2]};
+      void* T0[3] = {ctx->Ut[0], ctx->Ut[1], ctx->Ut[2]};
+      rc = capture(ctx, ctx->gf, key, body);
+      for (int c = 0; c < 3; c++) { ctx->U[c] = U0[c]; ctx->Ut[c] = T0[c]; }
       if (rc) return rc;
     }
-    enqueue_step(ctx, ctx->U, ctx->Ut, s, dt_seq == nullptr);
-    for (int c = 0; c < 3; c++) std::swap(ctx->U[c], ctx->Ut[c]);
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    HIPCHK(ctx, hipGraphLaunch(ctx->gf.x, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    if (nsteps % 2)
+      for (int c = 0; c < 3; c++) std::swap(ctx->U[c], ctx->Ut[c]);
+  } else {
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    body();
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   }
-  HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   HIPCHK(ctx, hipGetLastError());
   rc = check_err(ctx);
   if (rc) return rc;
@@ -426,41 +489,67 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
   rc = ensure_store(ctx, ctx->seg, (size_t)k);
   if (rc) return rc;
   const bool dphi = dJ != nullptr;
-  if (dphi)
-    HIPCHK(ctx, hipMemsetAsync(ctx->dphi_part, 0, sizeof(double) * ctx->ntiles, ctx->stream));
   // lambda^N = func_sens at the final step (calc_adjoint.C:93-97)
   for (int c = 0; c < 3; c++) {
     rc = h2d_field(ctx, ctx->lam[0][c], 0, n, w + c * n);
     if (rc) return rc;
   }
-  int cur = 0;
-  float rec_ms = 0, ms = 0;
-  HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
   const int nseg = (N + k - 1) / k;
-  for (int sg = nseg - 1; sg >= 0; sg--) {
+  // one reverse segment: restore U^{s0} from its checkpoint, recompute
+  // U^{s0+1..s1-1} with the recorded dt, then the adjoint steps s1-1..s0
+  auto recompute = [&](int sg) {
     const int s0 = sg * k, s1 = (s0 + k < N) ? s0 + k : N;
-    HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     void* base[3];
     split(ctx, ctx->ckpt[sg], base);
-    rc = copy_state(ctx, ctx->seg[0], base);
-    if (rc) return rc;
-    for (int s = s0; s + 1 < s1; s++) {  // recompute U^{s0+1..s1-1} with the recorded dt
+    copy_state(ctx, ctx->seg[0], base);
+    for (int s = s0; s + 1 < s1; s++) {
       void* in[3];
       void* out[3];
       split(ctx, ctx->seg[s - s0], in);
       split(ctx, ctx->seg[s - s0 + 1], out);
       enqueue_step(ctx, in, out, s, false);
     }
-    HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+  };
+  auto reverse = [&](int sg, int& cur) {
+    const int s0 = sg * k, s1 = (s0 + k < N) ? s0 + k : N;
     for (int s = s1 - 1; s >= s0; s--) {
       void* Us[3];
       split(ctx, ctx->seg[s - s0], Us);
       enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi);
       cur = 1 - cur;
     }
-    HIPCHK(ctx, hipEventSynchronize(ctx->ev[5]));
-    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
-    rec_ms += ms;
+  };
+  int cur = 0;
+  float rec_ms = 0, ms = 0;
+  if (ctx->cfg.use_graph) {
+    const long long key[8] = {N, k, dphi, ctx->sel[0], ctx->sel[1], ctx->sel[2], 0, 0};
+    if (!same_key(ctx->ga, key)) {
+      rc = capture(ctx, ctx->ga, key, [&]() {
+        int c = 0;
+        if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
+        for (int sg = nseg - 1; sg >= 0; sg--) {
+          recompute(sg);
+          reverse(sg, c);
+        }
+      });
+      if (rc) return rc;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    HIPCHK(ctx, hipGraphLaunch(ctx->ga.x, ctx->stream));
+    cur = N % 2;
+    rec_ms = -1.f;  // not separable inside one launch
+  } else {
+    if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
+    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+      HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+      recompute(sg);
+      HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+      reverse(sg, cur);
+      HIPCHK(ctx, hipEventSynchronize(ctx->ev[5]));
+      HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+      rec_ms += ms;
+    }
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   HIPCHK(ctx, hipGetLastError());
@@ -568,6 +657,7 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     if (!rc) rc = dalloc(ctx, &ctx->lam[1][k], fb);
   }
   const size_t sb = (size_t)ctx->ses * n;
+  ctx->U_home = ctx->U[0];
   if (!rc) rc = dalloc(ctx, &ctx->statics, 7 * sb);
   if (!rc) rc = dalloc(ctx, &ctx->kact, sb);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->part_mass, sizeof(double) * ctx->ntiles);
@@ -632,6 +722,7 @@ void tg_config_default(tg_config* cfg) {
   cfg->frict_tiny_switch = 50;
   cfg->tile_x = 0;
   cfg->tile_y = 0;
+  cfg->use_graph = 0;
 }
 
 int tg_params_from_angles(double phi_int, double phi_bed, double epsilon, double gravity_scale,
@@ -755,6 +846,10 @@ int tg_member_cells(const tg_ctx* ctx, int member, int64_t* offset, int64_t* cou
 int tg_set_state(tg_ctx* ctx, const double* h, const double* hu, const double* hv) {
   if (!ctx || !h || !hu || !hv) return TG_ERR_ARG;
   if (!ctx->have_mesh) return set_err(ctx, TG_ERR_STATE, "no mesh");
+  // a full state restores the buffers' home roles (U in the first of the
+  // pair), so repeated runs from set states reuse one captured sweep
+  if (ctx->U[0] != ctx->U_home)
+    for (int c = 0; c < 3; c++) std::swap(ctx->U[c], ctx->Ut[c]);
   const double* src[3] = {h, hu, hv};
   for (int k = 0; k < 3; k++) {
     int rc = h2d_field(ctx, ctx->U[k], 0, ctx->ncell, src[k]);
@@ -990,6 +1085,14 @@ int tg_tile_variant(int v, int* tile_x, int* tile_y, int* xcd_banded) {
   if (tile_x) *tile_x = SHAPE_TX[v];
   if (tile_y) *tile_y = BLOCK / SHAPE_TX[v];
   if (xcd_banded) *xcd_banded = SHAPE_XCD[v] ? 1 : 0;
+  return TG_OK;
+}
+
+int tg_set_tile_selection(tg_ctx* ctx, const int* variant) {
+  if (!ctx || !variant) return TG_ERR_ARG;
+  for (int k = 0; k < 3; k++)
+    if (variant[k] < 0 || variant[k] >= NSHAPE) return TG_ERR_ARG;
+  for (int k = 0; k < 3; k++) ctx->sel[k] = variant[k];
   return TG_OK;
 }
 

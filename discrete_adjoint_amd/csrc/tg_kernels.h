@@ -173,6 +173,9 @@ void launch_dt(const StepArgs& a, int ninst, hipStream_t s);
 void launch_update(const StepArgs& a, hipStream_t s);
 void launch_adjoint(const AdjArgs& a, hipStream_t s);
 void launch_reset_red(RedDev* red, int ninst, hipStream_t s);
+// bytes must be a multiple of 4 (fields are FP32 or FP64)
+void launch_zero(void* p, size_t bytes, hipStream_t s);
+void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 void launch_stats(const InstDev* inst, const TileDev* tiles, int ntiles, Fields U,
                   double* part_mass, double* part_hmax, double tiny, hipStream_t s);
 

@@ -126,6 +126,20 @@ __global__ void k_reset_red(RedDev* red, int ninst) {
   }
 }
 
+// Plain fill / copy kernels: used instead of memset / memcpy nodes so a
+// captured sweep holds kernel nodes only.
+__global__ void k_zero_u32(uint32_t* p, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+__global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                           long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_dt(StepArgs a, int ninst) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ninst) return;
@@ -634,6 +648,19 @@ void launch_update(const StepArgs& a, hipStream_t s) {
 }
 void launch_adjoint(const AdjArgs& a, hipStream_t s) {
   TG_LAUNCH_TX(a.shape, k_adjoint, dim3(a.ntiles), s, a);
+}
+static unsigned grid_for(long long n) {
+  const long long b = (n + 255) / 256;
+  return (unsigned)(b < 4096 ? (b < 1 ? 1 : b) : 4096);
+}
+void launch_zero(void* p, size_t bytes, hipStream_t s) {
+  const long long n = (long long)(bytes / 4);
+  hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n)), dim3(256), 0, s, (uint32_t*)p, n);
+}
+void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const long long n = (long long)(bytes / 4);
+  hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n)), dim3(256), 0, s, (const uint32_t*)src,
+                     (uint32_t*)dst, n);
 }
 void launch_reset_red(RedDev* red, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_red, dim3((ninst + 63) / 64), dim3(64), 0, s, red, ninst);

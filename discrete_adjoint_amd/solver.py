@@ -175,6 +175,10 @@ class Simulation:
         self._check(self.lib.tg_get_tile_selection(self.ctx, var))
         return list(var)
 
+    def select_tiles(self, variants):
+        v = (C.c_int * 3)(*variants)
+        self._check(self.lib.tg_set_tile_selection(self.ctx, v))
+
     def kernel_timing(self):
         return self.timing()
 

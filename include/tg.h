@@ -61,6 +61,11 @@ typedef struct tg_config {
   int tile_x, tile_y;     /* 0 x 0: built-in tile variant (tg_autotune may
                              re-select); 64x4, 32x8 or 16x16: that shape,
                              XCD-banded, for every kernel                     */
+  int use_graph;          /* 1: tg_forward_run / tg_forward_replay and tg_adjoint_run
+                             replay a hipGraph of the whole sweep (captured on
+                             the first call with a given step count, tile
+                             selection and buffer parity); 0: launch kernel
+                             by kernel.  Results are identical.              */
 } tg_config;
 
 /* Mesh.  TG_MESH_UNIFORM: an nx*ny block of equal cells; cell c = j*nx + i,
@@ -226,8 +231,10 @@ int tg_tile_variant(int variant, int* tile_x, int* tile_y, int* xcd_banded);
  * launch times (may be NULL).  Overwrites the adjoint buffers. */
 int tg_autotune(tg_ctx* ctx, const tg_params* prm, int reps, int* variant_out, double* ms_out);
 
-/* Variants in use per kernel (coef, update, adjoint). */
+/* Variants in use per kernel (coef, update, adjoint); set them explicitly
+ * (e.g. to reproduce an earlier autotune). */
 int tg_get_tile_selection(const tg_ctx* ctx, int* variant_out);
+int tg_set_tile_selection(tg_ctx* ctx, const int* variant);
 
 /* Cells in the context's mesh (sum over instances). */
 int64_t tg_num_cells(const tg_ctx* ctx);

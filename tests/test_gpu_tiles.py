@@ -71,3 +71,28 @@ def test_autotune_selects_valid_variants_and_keeps_results():
         for b in range(2):
             assert np.array_equal(lam[b], ref[2][b])
         assert dphi == pytest.approx(ref[3], rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.parametrize("precision", [_lib.TG_FP64, _lib.TG_MIXED])
+def test_graph_replay_equals_eager(precision):
+    # the captured sweeps are replayed several times with eager work (and
+    # synchronisations) in between; every replay must equal the eager run
+    print(f"case graph replay precision {precision}", flush=True)
+    ref = _run(precision)
+    mem = _members()
+    h0 = [np.concatenate([getattr(p, f) for p in mem]) for f in ("h", "hu", "hv")]
+    with Ensemble(mem, precision=precision, checkpoint_every=6, use_graph=1) as ens:
+        for rep in range(3):
+            ens.set_state(*h0)
+            ens.run(25)
+            for b in range(2):
+                for x, y in zip(ens.member_state(b), ref[0][b]):
+                    assert np.array_equal(x, y), rep
+                assert np.array_equal(ens.member_dt_record(b), ref[1][b])
+            lam, dphi = ens.dual_solver([_box(p) for p in mem], want_dphi=True)
+            for b in range(2):
+                assert np.array_equal(lam[b], ref[2][b]), rep
+            assert dphi == pytest.approx(ref[3], rel=1e-12, abs=1e-300)
+            # eager work between replays: the next set_state uploads the state
+            # and clears the error flag on the same stream, then synchronises
+            ens.get_state()
