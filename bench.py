@@ -53,9 +53,21 @@ def algo_bytes(prec):
     an intermediate."""
     from discrete_adjoint_amd import _lib
     es = 4 if prec == _lib.TG_FP32 else 8
-    ss = 8 if prec == _lib.TG_FP64 else 4
-    return {"coef": 3 * es + ss, "update": 6 * es + 7 * ss, "adjoint": 9 * es + 7 * ss,
+    ss = 8 if prec in (_lib.TG_FP64, _lib.TG_FWD64_ADJ32) else 4
+    sa = 8 if prec == _lib.TG_FP64 else 4  # statics read by the adjoint
+    return {"coef": 3 * es + ss, "update": 6 * es + 7 * ss, "adjoint": 9 * es + 7 * sa,
             "forward_step": 6 * es + 7 * ss}
+
+
+PRECISIONS = {"adj32": "TG_FWD64_ADJ32", "mixed": "TG_MIXED", "fp32": "TG_FP32",
+              "fp64": "TG_FP64"}
+DTYPES = {"adj32": "f64 forward + f32 adjoint", "mixed": "f32 increments on f64 state",
+          "fp32": "f32", "fp64": "f64"}
+
+
+def prec_of(name):
+    from discrete_adjoint_amd import _lib
+    return getattr(_lib, PRECISIONS[name])
 
 
 def log(*a):
@@ -178,7 +190,7 @@ def bench_ensemble(args):
     weights = [p.box_weights().reshape(3, -1) for p in E]
     h0 = tuple(np.concatenate([getattr(p, f) for p in E]) for f in ("h", "hu", "hv"))
     log(f"setup {time.perf_counter() - t0:.1f}s: {len(E)} members, {ncells} cells")
-    prec = {"mixed": _lib.TG_MIXED, "fp32": _lib.TG_FP32, "fp64": _lib.TG_FP64}[args.precision]
+    prec = prec_of(args.precision)
 
     def member_J(H):
         out, o = [], 0
@@ -190,7 +202,12 @@ def bench_ensemble(args):
     res = {}
     roof = None
     graph = 0 if args.no_graph else 1
-    for mode in ([prec, _lib.TG_FP64] if prec != _lib.TG_FP64 else [prec]):
+    names = [args.precision]
+    if not args.product_only:
+        names += ["fp64"] if args.precision != "fp64" else []
+        names += [c for c in args.compare.split(",") if c and c not in names]
+    for name in names:
+        mode = prec_of(name)
         with Ensemble(E, precision=mode, checkpoint_every=k, max_steps=max(K, W, 1),
                       use_graph=graph) as ens:
             if W > 0:  # untimed: first launches, allocation of the checkpoint store
@@ -215,7 +232,7 @@ def bench_ensemble(args):
                 roof = kernel_roofline(ens, prec, ncells, k, args.kernel_reps)
     # plain FP64 GPU path: one member at a time
     plain_s = 0.0
-    for b, p in enumerate(E):
+    for b, p in enumerate([] if args.product_only else E):
         with Simulation(p, checkpoint_every=k, max_steps=max(K, 1)) as sim:
             if b == 0 and W > 0:
                 sim.run(W)
@@ -226,25 +243,41 @@ def bench_ensemble(args):
             sim.dual_solver(weights[b], want_dphi=True, want_lambda=False)
             plain_s += time.perf_counter() - t0
     cell_steps = ncells * K
-    plain = cell_steps / plain_s
-    log(f"plain FP64 per-member: {plain_s:.2f} s wall, {plain:.3g} cell-steps/s")
+    plain = cell_steps / plain_s if plain_s > 0 else None
+    if plain:
+        log(f"plain FP64 per-member: {plain_s:.2f} s wall, {plain:.3g} cell-steps/s")
 
-    m, d = res[prec], res[_lib.TG_FP64]
+    m, d = res[prec], res.get(_lib.TG_FP64)
     wall = m["wall_f"] + m["wall_a"]
     value = cell_steps / wall
-    errs = None
-    if prec != _lib.TG_FP64:
-        Jd, Jm = d["J"], m["J"]
+
+    def errors(r):
+        Jd, Jm = d["J"], r["J"]
         okJ = np.abs(Jd) > 0
         eJ = np.abs(Jm[okJ] - Jd[okJ]) / np.abs(Jd[okJ])
         okg = np.abs(d["dphi"]) > 0
-        eg = np.abs(m["dphi"][okg] - d["dphi"][okg]) / np.abs(d["dphi"][okg])
-        errs = {"functional_rel": stats(eJ), "functional_members_le_1e-6": int((eJ <= 1e-6).sum()),
+        eg = np.abs(r["dphi"][okg] - d["dphi"][okg]) / np.abs(d["dphi"][okg])
+        gmax = np.abs(d["dphi"]).max()
+        return {"functional_rel": stats(eJ),
+                "functional_members_le_1e-6": int((eJ <= 1e-6).sum()),
                 "dJ_dphibed_rel": stats(eg),
                 "dJ_dphibed_members_le_1e-6": int((eg <= 1e-6).sum()),
+                "dJ_dphibed_abs_err_over_max": float(np.abs(r["dphi"] - d["dphi"]).max() / gmax),
                 "members": len(E)}
+
+    errs = errors(m) if (d is not None and prec != _lib.TG_FP64) else None
+    if errs:
         log(f"per-member functional rel err: {errs['functional_rel']}, "
             f"dJ/dphi_bed rel err: {errs['dJ_dphibed_rel']}")
+    others = {}
+    for name in names:
+        mode = prec_of(name)
+        if mode in (prec, _lib.TG_FP64):
+            continue
+        r = res[mode]
+        others[name] = {"cell_steps_per_s": cell_steps / (r["wall_f"] + r["wall_a"]),
+                        "errors_vs_fp64": errors(r)}
+        log(f"{name}: {others[name]}")
 
     dom, kt, gbs = roof
     ab = algo_bytes(prec)
@@ -252,7 +285,7 @@ def bench_ensemble(args):
     fwd_gbs = ab["forward_step"] * ncells / (fwd_step_ms * 1e-3) / 1e9
     workload = "ensemble256"
     cpu = None
-    if not args.no_cpu:
+    if not (args.no_cpu or args.product_only):
         warm_oracle(E[0])
         cs, tf, ta = cpu_baseline(E, args.cpu_steps)
         cpu = {"value": cs / (tf + ta), "unit": "cell-steps/s", "cores": 1, "kind": "port",
@@ -273,7 +306,7 @@ def bench_ensemble(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"mixed": "f32+f64", "fp32": "f32", "fp64": "f64"}[args.precision],
+        "dtype": DTYPES[args.precision],
         "data": "synthetic",
         "config": {
             "workload": f"ensemble of {len(E)} members (uniform blocks, side in "
@@ -289,10 +322,11 @@ def bench_ensemble(args):
             "kernel_ms": {"forward": m["fwd_ms"], "reverse": m["adj_ms"],
                           "recompute": m["rec_ms"]},
             "fp64_batched_cell_steps_per_s":
-                cell_steps / (d["wall_f"] + d["wall_a"]),
+                cell_steps / (d["wall_f"] + d["wall_a"]) if d is not None else None,
             "fp64_plain_cell_steps_per_s": plain,
-            "speedup_vs_fp64_plain": value / plain,
+            "speedup_vs_fp64_plain": value / plain if plain else None,
             "errors_vs_fp64": errs,
+            "other_precisions": others,
             "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS,
                                  "algorithmic_bytes_per_cell": ab["forward_step"]},
             "kernel_gbs": gbs,
@@ -327,7 +361,7 @@ def bench_grid(args):
     pb = problems.fractal_problem(n=n, dx=dx, nsteps=K, npiles=3, seed=11, dem_seed=1,
                                   checkpoint_every=k)
     log(f"setup {time.perf_counter() - t0:.1f}s: {n}x{n} cells, dx={dx}")
-    prec = {"mixed": _lib.TG_MIXED, "fp32": _lib.TG_FP32, "fp64": _lib.TG_FP64}[args.precision]
+    prec = prec_of(args.precision)
     h0 = (pb.h, pb.hu, pb.hv)
     res = {}
     roof = None
@@ -372,7 +406,7 @@ def bench_grid(args):
         "metric": METRIC, "value": cell_steps / wall, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": wall * 1e3 / K, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": {"mixed": "f32+f64", "fp32": "f32", "fp64": "f64"}[args.precision],
+        "dtype": DTYPES[args.precision],
         "data": "synthetic",
         "config": {
             "workload": f"uniform {n}x{n} ({dx} m) fractal DEM, 3 piles, {K} forward steps + "
@@ -406,7 +440,7 @@ def probe(args):
     K, k = args.steps, args.checkpoint_every
     E = problems.ensemble_problems(args.members, seed=args.seed, nsteps=K, checkpoint_every=k)
     weights = [p.box_weights().reshape(3, -1) for p in E]
-    prec = {"mixed": _lib.TG_MIXED, "fp32": _lib.TG_FP32, "fp64": _lib.TG_FP64}[args.precision]
+    prec = prec_of(args.precision)
     tiles = [int(v) for v in args.tiles.split(",")] if args.tiles else None
     with Ensemble(E, precision=prec, checkpoint_every=k, max_steps=max(K, 1)) as ens:
         ens.run(K)
@@ -436,12 +470,16 @@ def main():
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--precision", choices=("mixed", "fp32", "fp64"), default="mixed")
+    ap.add_argument("--precision", choices=tuple(PRECISIONS), default="adj32")
+    ap.add_argument("--compare", default="mixed,fp32",
+                    help="further precisions run on the same ensemble (errors and speed)")
     ap.add_argument("--checkpoint-every", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--product-only", action="store_true",
+                    help="only the measured path (no FP64 / plain / CPU legs): for profiling")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter profiles (see probe())")
     ap.add_argument("--tiles", default="",

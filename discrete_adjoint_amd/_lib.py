@@ -25,6 +25,7 @@ TG_ERR_NODEVICE = -8
 TG_FP64 = 0
 TG_MIXED = 1
 TG_FP32 = 2
+TG_FWD64_ADJ32 = 3
 TG_NUM_TILE_VARIANTS = 4
 TG_MESH_UNIFORM = 0
 TG_MESH_LEAVES = 1
@@ -155,6 +156,8 @@ SIGNATURES = {
                                  _dp, _dp]),
     "tg_adjoint_step": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_double,
                                   _dp, _dp]),
+    "tg_adjoint_step_grad": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_double,
+                                       _dp, _dp, _dp]),
     "tg_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "tg_time_kernel": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_int, _dp]),
     "tg_num_cells": (C.c_int64, [C.c_void_p]),

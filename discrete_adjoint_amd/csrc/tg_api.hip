@@ -50,7 +50,8 @@ struct tg_ctx {
   void* U[3] = {nullptr, nullptr, nullptr};
   void* Ut[3] = {nullptr, nullptr, nullptr};
   void* U_home = nullptr;  // U[0] as allocated
-  void* statics = nullptr;  // 7 * ncell
+  void* statics = nullptr;  // 7 * ncell, element size ses
+  void* statics_f = nullptr;  // 7 * ncell FP32 copy for the FP32 adjoint of TG_FWD64_ADJ32
   void* kact = nullptr;
   void* lam[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   RedDev* red = nullptr;
@@ -60,6 +61,8 @@ struct tg_ctx {
   double* part_mass = nullptr;
   double* part_hmax = nullptr;
   double* dphi_part = nullptr;
+  int2* thin_list = nullptr;  // FP32 adjoint modes: cells gathered in FP64
+  int* thin_count = nullptr;
 
   // forward record
   int rec_steps = 0;  // steps in the dt record
@@ -125,10 +128,13 @@ void free_fields(tg_ctx* ctx) {
     dfree(ctx->lam[1][k]);
   }
   dfree(ctx->statics);
+  dfree(ctx->statics_f);
   dfree(ctx->kact);
   dfree(ctx->part_mass);
   dfree(ctx->part_hmax);
   dfree(ctx->dphi_part);
+  dfree(ctx->thin_list);
+  dfree(ctx->thin_count);
   dfree(ctx->inst_d);
   for (auto& t : ctx->tiles_d) dfree(t);
   dfree(ctx->red);
@@ -171,12 +177,15 @@ T* at(void* base, long long k, long long n) {
   return (T*)base + k * n;
 }
 
+// FP64 forward (bit-identical to TG_FP64) in these modes
+bool fwd64(const tg_ctx* ctx) { return ctx->mode == TG_FP64 || ctx->mode == TG_FWD64_ADJ32; }
+
 Statics32 statics32(tg_ctx* ctx) {
   const long long n = ctx->ncell;
-  return Statics32{at<float>(ctx->statics, 0, n), at<float>(ctx->statics, 1, n),
-                   at<float>(ctx->statics, 2, n), at<float>(ctx->statics, 3, n),
-                   at<float>(ctx->statics, 4, n), at<float>(ctx->statics, 5, n),
-                   at<float>(ctx->statics, 6, n)};
+  void* b = ctx->statics_f ? ctx->statics_f : ctx->statics;
+  return Statics32{at<float>(b, 0, n), at<float>(b, 1, n), at<float>(b, 2, n),
+                   at<float>(b, 3, n), at<float>(b, 4, n), at<float>(b, 5, n),
+                   at<float>(b, 6, n)};
 }
 
 // points a kernel's arguments at the tile table of its selected variant
@@ -232,12 +241,16 @@ void enqueue_adjoint_m(tg_ctx* ctx, void* const U[3], void* const lnext[3], void
   a.gamma = ctx->cfg.gamma;
   a.lam_h = (S*)lout[0]; a.lam_hu = (S*)lout[1]; a.lam_hv = (S*)lout[2];
   a.dphi_part = dphi ? ctx->dphi_part : nullptr;
+  a.thin_list = ctx->thin_list;
+  a.thin_count = ctx->thin_count;
+  a.s64 = (ctx->ses == 8) ? (const double*)ctx->statics : nullptr;
+  a.ncell = ctx->ncell;
   launch_adjoint_m<S>(a, ctx->stream);
 }
 
 void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, bool compute_dt) {
   const long long n = ctx->ncell;
-  if (ctx->mode == TG_FP64) {
+  if (fwd64(ctx)) {
     StepArgs a{};
     a.inst = ctx->inst_d;
     a.U = Fields{(double*)in[0], (double*)in[1], (double*)in[2]};
@@ -659,10 +672,15 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   const size_t sb = (size_t)ctx->ses * n;
   ctx->U_home = ctx->U[0];
   if (!rc) rc = dalloc(ctx, &ctx->statics, 7 * sb);
+  if (!rc && ctx->mode == TG_FWD64_ADJ32)
+    rc = dalloc(ctx, &ctx->statics_f, 7 * sizeof(float) * n);
   if (!rc) rc = dalloc(ctx, &ctx->kact, sb);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->part_mass, sizeof(double) * ctx->ntiles);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->part_hmax, sizeof(double) * ctx->ntiles);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->dphi_part, sizeof(double) * ctx->ntiles);
+  if (!rc && ctx->mode != TG_FP64)
+    rc = dalloc(ctx, (void**)&ctx->thin_list, sizeof(int2) * (size_t)n);
+  if (!rc && ctx->mode != TG_FP64) rc = dalloc(ctx, (void**)&ctx->thin_count, sizeof(int));
   if (!rc) rc = dalloc(ctx, (void**)&ctx->inst_d, sizeof(InstDev) * nm);
   for (int v = 0; v < NSHAPE && !rc; v++)
     rc = dalloc(ctx, (void**)&ctx->tiles_d[v], sizeof(TileDev) * ctx->nt[v]);
@@ -679,8 +697,12 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     const long long nb = (long long)m->nx * m->ny, o = ctx->inst_h[b].off;
     const double* src[7] = {m->grav, m->grav + nb, m->grav + 2 * nb, m->dgrav, m->dgrav + nb,
                             m->curv, m->curv + nb};
-    for (int k = 0; k < 7 && !rc; k++)
+    for (int k = 0; k < 7 && !rc; k++) {
       rc = h2d_field(ctx, (char*)ctx->statics + (size_t)k * sb, o, nb, src[k], ctx->ses);
+      if (!rc && ctx->statics_f)
+        rc = h2d_field(ctx, (char*)ctx->statics_f + (size_t)k * sizeof(float) * n, o, nb, src[k],
+                       4);
+    }
   }
   if (rc) return rc;
   for (int v = 0; v < NSHAPE; v++)
@@ -756,7 +778,7 @@ int tg_create(const tg_config* cfg, tg_ctx** out) {
   else tg_config_default(&ctx->cfg);
   if (ctx->cfg.checkpoint_every < 1 || ctx->cfg.max_steps < 1 ||
       (ctx->cfg.precision != TG_FP64 && ctx->cfg.precision != TG_MIXED &&
-       ctx->cfg.precision != TG_FP32)) {
+       ctx->cfg.precision != TG_FP32 && ctx->cfg.precision != TG_FWD64_ADJ32)) {
     delete ctx;
     return TG_ERR_ARG;
   }
@@ -778,7 +800,7 @@ int tg_create(const tg_config* cfg, tg_ctx** out) {
   }
   ctx->mode = ctx->cfg.precision;
   ctx->es = (ctx->mode == TG_FP32) ? 4 : 8;
-  ctx->ses = (ctx->mode == TG_FP64) ? 8 : 4;
+  ctx->ses = fwd64(ctx) ? 8 : 4;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= ctx->cfg.device) {
     delete ctx;
@@ -960,8 +982,8 @@ int tg_get_member_dphi(tg_ctx* ctx, double* dJ_dphibed) {
   return TG_OK;
 }
 
-int tg_adjoint_step(tg_ctx* ctx, const tg_params* prm, int s, double dt,
-                    const double* lambda_next, double* lambda_out) {
+int adjoint_step_impl(tg_ctx* ctx, const tg_params* prm, int s, double dt,
+                      const double* lambda_next, double* lambda_out, double* dphi_member) {
   if (!ctx || !lambda_next || !lambda_out || s < 0 || s >= ctx->cfg.max_steps)
     return TG_ERR_ARG;
   if (!ctx->have_state) return set_err(ctx, TG_ERR_STATE, "no state");
@@ -975,14 +997,36 @@ int tg_adjoint_step(tg_ctx* ctx, const tg_params* prm, int s, double dt,
     rc = h2d_field(ctx, ctx->lam[0][c], 0, n, lambda_next + c * n);
     if (rc) return rc;
   }
-  enqueue_adjoint(ctx, ctx->U, ctx->lam[0], ctx->lam[1], s, false);
+  const bool dphi = dphi_member != nullptr;
+  if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
+  enqueue_adjoint(ctx, ctx->U, ctx->lam[0], ctx->lam[1], s, dphi);
   HIPCHK(ctx, hipGetLastError());
   for (int c = 0; c < 3; c++) {
     rc = d2h_field(ctx, ctx->lam[1][c], 0, n, lambda_out + c * n);
     if (rc) return rc;
   }
+  if (dphi) {
+    const std::vector<TileDev>& tl = ctx->tiles_h[ctx->sel[2]];
+    std::vector<double> part(tl.size());
+    HIPCHK(ctx, hipMemcpyAsync(part.data(), ctx->dphi_part, sizeof(double) * tl.size(),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int b = 0; b < ctx->ninst; b++) dphi_member[b] = 0.;
+    for (size_t t = 0; t < tl.size(); t++) dphi_member[tl[t].inst] += part[t];
+  }
   ctx->run_steps = 0;  // the dt record slot was overwritten
   return TG_OK;
+}
+
+int tg_adjoint_step(tg_ctx* ctx, const tg_params* prm, int s, double dt,
+                    const double* lambda_next, double* lambda_out) {
+  return adjoint_step_impl(ctx, prm, s, dt, lambda_next, lambda_out, nullptr);
+}
+
+int tg_adjoint_step_grad(tg_ctx* ctx, const tg_params* prm, int s, double dt,
+                         const double* lambda_next, double* lambda_out, double* dJ_dphibed) {
+  if (!dJ_dphibed) return TG_ERR_ARG;
+  return adjoint_step_impl(ctx, prm, s, dt, lambda_next, lambda_out, dJ_dphibed);
 }
 
 int tg_last_timing(tg_ctx* ctx, double* forward_ms, double* adjoint_ms, double* recompute_ms) {
@@ -1002,7 +1046,7 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
   auto one = [&](int w) {
     if (w == 2) {
       enqueue_adjoint(ctx, ctx->U, ctx->lam[0], ctx->lam[1], 0, false);
-    } else if (ctx->mode == TG_FP64) {
+    } else if (fwd64(ctx)) {
       StepArgs a{};
       const long long n = ctx->ncell;
       a.inst = ctx->inst_d;

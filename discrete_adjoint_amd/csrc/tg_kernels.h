@@ -157,7 +157,13 @@ struct AdjArgsM {
   S* lam_hu;
   S* lam_hv;
   double* dphi_part;
+  int2* thin_list;       // [ncell] (tile, thread) of cells gathered in FP64
+  int* thin_count;
+  const double* s64;     // FP64 static coefficients [7 * ncell] or null
+  long long ncell;
 };
+// workgroups of the FP64 thin-layer adjoint pass (strides over its list)
+constexpr int THIN_GRID = 1024;
 
 template <class S>
 void launch_coef_m(const StepArgsM<S>& a, bool reduce, hipStream_t s);

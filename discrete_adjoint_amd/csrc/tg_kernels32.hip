@@ -223,163 +223,186 @@ __global__ __launch_bounds__(BLOCK) void k32_stats(const InstDev* inst, const Ti
 }
 
 // ---------------------------------------------------------------- adjoint --
+//
+// The Jacobian blocks are formed in arithmetic type F: float for the bulk;
+// double for cells next to a thin layer (0 < h < THIN in the cell or a
+// neighbour).  At wetting fronts the reverse recursion has growing modes
+// (1/h terms of the thin cell's momentum Jacobian) that amplify whatever
+// rounding seeds them; FP32 seeds them some 1e8 times larger than FP64.
 
-struct M3f {
-  float a[3][3];
+template <class F>
+struct M3T {
+  F a[3][3];
 };
-__device__ __forceinline__ M3f zero3f() {
-  M3f m;
+template <class F>
+__device__ __forceinline__ M3T<F> zero3() {
+  M3T<F> m;
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) m.a[i][j] = 0.f;
+    for (int j = 0; j < 3; j++) m.a[i][j] = F(0);
   return m;
+}
+
+// Static coefficient k (0 gx, 1 gy, 2 gz, 3 dgx, 4 dgy, 5 kx, 6 ky) of cell c
+// in arithmetic type F: the FP64 copy when F is double and one exists (the
+// thin-layer pass of TG_FWD64_ADJ32), else the FP32 array.
+template <class F, class S>
+__device__ __forceinline__ F stat(const AdjArgsM<S>& a, int k, long long c) {
+  if constexpr (sizeof(F) == 8) {
+    if (a.s64) return a.s64[(long long)k * a.ncell + c];
+  }
+  const float* p[7] = {a.S32.gx, a.S32.gy, a.S32.gz, a.S32.dgx, a.S32.dgy, a.S32.kx, a.S32.ky};
+  return (F)p[k][c];
 }
 
 // What the Jacobian blocks need about one cell at U^s.  Switch decisions
 // (slopes, hslope branch, kact sign, sgn, stop, wet, thick) are taken in W
-// before the values are narrowed to FP32.
-struct DualInfo {
-  float h, hu, hv, g2, ih, u, v, k, dk;
-  float hx, hy;           // own h slopes
-  float hsx[3], hsy[3];   // hslope_sens
-  float sgn[4];
+// before the values are narrowed to F.
+template <class F>
+struct DualInfoT {
+  F h, hu, hv, g2, ih, u, v, k, dk;
+  F hx, hy;           // own h slopes
+  F hsx[3], hsy[3];   // hslope_sens
+  F sgn[4];
   int stop[2];
-  int wet;                // !(h < tiny)
-  int thick;              // h > tiny
-  int nz;                 // h != 0
+  int wet;            // !(h < tiny)
+  int thick;          // h > tiny
+  int nz;             // h != 0
 };
 
-struct DualHfv {
-  Hfv b;
-  M3f fj, sj;
+template <class F>
+struct DualHfvT {
+  HfvT<F> b;
+  M3T<F> fj, sj;
 };
-__device__ __forceinline__ DualHfv dual_hfv(int dir, const DualInfo& d) {
-  DualHfv r;
-  const float u = d.u, v = d.v, ih = d.ih, h = d.h;
-  r.b = hfv(dir, h, d.hu, d.hv, dir ? v : u, d.k, d.g2, d.wet);
-  r.fj = zero3f();
-  r.sj = zero3f();
+template <class F>
+__device__ __forceinline__ DualHfvT<F> dual_hfv(int dir, const DualInfoT<F>& d) {
+  DualHfvT<F> r;
+  const F u = d.u, v = d.v, ih = d.ih, h = d.h;
+  r.b = hfv<F>(dir, h, d.hu, d.hv, dir ? v : u, d.k, d.g2, d.wet);
+  r.fj = zero3<F>();
+  r.sj = zero3<F>();
   if (!d.wet) return r;
-  const float a2 = d.k * h * d.g2, a = fsqrt(a2);
+  const F a2 = d.k * h * d.g2, a = fsqrt(a2);
   if (dir == 0) {
-    r.fj.a[0][1] = 1.f;
-    r.fj.a[1][0] = a2 - u * u; r.fj.a[1][1] = 2.f * u;
+    r.fj.a[0][1] = F(1);
+    r.fj.a[1][0] = a2 - u * u; r.fj.a[1][1] = F(2) * u;
     r.fj.a[2][0] = -u * v; r.fj.a[2][1] = v; r.fj.a[2][2] = u;
-    r.sj.a[0][0] = (-0.5f * a - u) * ih; r.sj.a[0][1] = ih;
+    r.sj.a[0][0] = (F(-0.5) * a - u) * ih; r.sj.a[0][1] = ih;
     r.sj.a[1][0] = -u * ih; r.sj.a[1][1] = ih;
-    r.sj.a[2][0] = (0.5f * a - u) * ih; r.sj.a[2][1] = ih;
+    r.sj.a[2][0] = (F(0.5) * a - u) * ih; r.sj.a[2][1] = ih;
   } else {
-    r.fj.a[0][2] = 1.f;
+    r.fj.a[0][2] = F(1);
     r.fj.a[1][0] = -v * u; r.fj.a[1][1] = v; r.fj.a[1][2] = u;
-    r.fj.a[2][0] = a2 - v * v; r.fj.a[2][2] = 2.f * v;
-    r.sj.a[0][0] = (-0.5f * a - v) * ih; r.sj.a[0][2] = ih;
+    r.fj.a[2][0] = a2 - v * v; r.fj.a[2][2] = F(2) * v;
+    r.sj.a[0][0] = (F(-0.5) * a - v) * ih; r.sj.a[0][2] = ih;
     r.sj.a[1][0] = -v * ih; r.sj.a[1][2] = ih;
-    r.sj.a[2][0] = (0.5f * a - v) * ih; r.sj.a[2][2] = ih;
+    r.sj.a[2][0] = (F(0.5) * a - v) * ih; r.sj.a[2][2] = ih;
   }
   return r;
 }
 
 // d(HLL face flux)/d(U of one side), branch logic of dual_riemannflux
 // (dualmesh.C:1628-1843)
-__device__ __forceinline__ M3f dual_riemann_side(const DualHfv& l, const DualHfv& r,
-                                                 bool want_left) {
-  M3f j = zero3f();
-  const Speeds s = speeds(l.b, r.b);
+template <class F>
+__device__ __forceinline__ M3T<F> dual_riemann_side(const DualHfvT<F>& l, const DualHfvT<F>& r,
+                                                    bool want_left) {
+  M3T<F> j = zero3<F>();
+  const SpeedsT<F> s = speeds(l.b, r.b);
   if (s.mode == 0) return j;
   if (s.mode == 1) return want_left ? l.fj : j;
   if (s.mode == 2) return want_left ? j : r.fj;
-  float dsl[3], dsr[3];
-  const DualHfv& me = want_left ? l : r;
-  const bool ldry = l.b.u[0] == 0.f, rdry = r.b.u[0] == 0.f;
+  F dsl[3], dsr[3];
+  const DualHfvT<F>& me = want_left ? l : r;
+  const bool ldry = l.b.u[0] == F(0), rdry = r.b.u[0] == F(0);
 #pragma unroll
-  for (int i = 0; i < 3; i++) { dsl[i] = 0.f; dsr[i] = 0.f; }
+  for (int i = 0; i < 3; i++) { dsl[i] = F(0); dsr[i] = F(0); }
   if (ldry || rdry) {
     const bool mine_wet = want_left ? !ldry : !rdry;
     if (mine_wet) {
 #pragma unroll
       for (int i = 0; i < 3; i++) {
-        dsl[i] = (s.sl == 0.f) ? 0.f : 2.f * me.sj.a[0][i] - me.sj.a[1][i];
-        dsr[i] = (s.sr == 0.f) ? 0.f : 2.f * me.sj.a[2][i] - me.sj.a[1][i];
+        dsl[i] = (s.sl == F(0)) ? F(0) : F(2) * me.sj.a[0][i] - me.sj.a[1][i];
+        dsr[i] = (s.sr == F(0)) ? F(0) : F(2) * me.sj.a[2][i] - me.sj.a[1][i];
       }
     }
   } else {
     const bool sl_left = s.sl == l.b.w[0], sr_left = s.sr == l.b.w[2];
-    const bool sl_mine = (s.sl != 0.f) && (want_left ? sl_left : !sl_left);
-    const bool sr_mine = (s.sr != 0.f) && (want_left ? sr_left : !sr_left);
+    const bool sl_mine = (s.sl != F(0)) && (want_left ? sl_left : !sl_left);
+    const bool sr_mine = (s.sr != F(0)) && (want_left ? sr_left : !sr_left);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-      dsl[i] = sl_mine ? me.sj.a[0][i] : 0.f;
-      dsr[i] = sr_mine ? me.sj.a[2][i] : 0.f;
+      dsl[i] = sl_mine ? me.sj.a[0][i] : F(0);
+      dsr[i] = sr_mine ? me.sj.a[2][i] : F(0);
     }
   }
-  const float inv = rcp(s.sr - s.sl);
-  float dfdsr[3], dfdsl[3];
+  const F inv = rcp(s.sr - s.sl);
+  F dfdsr[3], dfdsl[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    const float vd = r.b.u[i] - l.b.u[i];
-    const float flux = (s.sr * l.b.f[i] - s.sl * r.b.f[i] + s.sl * s.sr * vd) * inv;
+    const F vd = r.b.u[i] - l.b.u[i];
+    const F flux = (s.sr * l.b.f[i] - s.sl * r.b.f[i] + s.sl * s.sr * vd) * inv;
     dfdsr[i] = (l.b.f[i] + s.sl * vd - flux) * inv;
     dfdsl[i] = (-r.b.f[i] + s.sr * vd + flux) * inv;
   }
-  const float dF = want_left ? s.sr * inv : -s.sl * inv;
-  const float jv = want_left ? -s.sr * s.sl * inv : s.sr * s.sl * inv;
+  const F dF = want_left ? s.sr * inv : -s.sl * inv;
+  const F jv = want_left ? -s.sr * s.sl * inv : s.sr * s.sl * inv;
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int k = 0; k < 3; k++)
-      j.a[i][k] = dfdsr[i] * dsr[k] + dfdsl[i] * dsl[k] + dF * me.fj.a[i][k] + ((i == k) ? jv : 0.f);
+      j.a[i][k] = dfdsr[i] * dsr[k] + dfdsl[i] * dsl[k] + dF * me.fj.a[i][k] + ((i == k) ? jv : F(0));
   return j;
 }
 
-template <class S>
-__device__ __forceinline__ DualInfo dual_info(const AdjArgsM<S>& a, const InstDev& I, int i, int j,
-                                              bool want_dk) {
+template <class S, class F>
+__device__ __forceinline__ DualInfoT<F> dual_info(const AdjArgsM<S>& a, const InstDev& I, int i,
+                                                  int j, bool want_dk) {
   using W = Wt<S>;
-  DualInfo r;
+  DualInfoT<F> r;
   const W tiny = (W)a.tiny, gamma = (W)a.gamma;
   const long long nx = I.nx, c = I.off + (long long)j * nx + i;
   const W h = (W)a.U.h[c], hu = (W)a.U.hu[c], hv = (W)a.U.hv[c];
   W d[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < 3; k++) r.hsx[k] = r.hsy[k] = 0.f;
+  for (int k = 0; k < 3; k++) r.hsx[k] = r.hsy[k] = F(0);
   if (i + 1 < I.nx && i > 0 && j + 1 < I.ny && j > 0) {
     const W ix = (W)I.inv_dx, iy = (W)I.inv_dy;
     Slope<W> s = slope_sens<W>((W)a.U.h[c + 1], h, (W)a.U.h[c - 1], ix, gamma, tiny);
-    d[0] = s.d; r.hsx[0] = (float)s.s0; r.hsx[1] = (float)s.s1; r.hsx[2] = (float)s.s2;
+    d[0] = s.d; r.hsx[0] = (F)s.s0; r.hsx[1] = (F)s.s1; r.hsx[2] = (F)s.s2;
     d[1] = slope<W>((W)a.U.hu[c + 1], hu, (W)a.U.hu[c - 1], ix, gamma, tiny);
     d[2] = slope<W>((W)a.U.hv[c + 1], hv, (W)a.U.hv[c - 1], ix, gamma, tiny);
     s = slope_sens<W>((W)a.U.h[c + nx], h, (W)a.U.h[c - nx], iy, gamma, tiny);
-    d[3] = s.d; r.hsy[0] = (float)s.s0; r.hsy[1] = (float)s.s1; r.hsy[2] = (float)s.s2;
+    d[3] = s.d; r.hsy[0] = (F)s.s0; r.hsy[1] = (F)s.s1; r.hsy[2] = (F)s.s2;
     d[4] = slope<W>((W)a.U.hu[c + nx], hu, (W)a.U.hu[c - nx], iy, gamma, tiny);
     d[5] = slope<W>((W)a.U.hv[c + nx], hv, (W)a.U.hv[c - nx], iy, gamma, tiny);
   }
   const W ih = inv_h(h);
   const W u = (h > W(0)) ? vel(hu, h, ih) : W(0), v = (h > W(0)) ? vel(hv, h, ih) : W(0);
-  r.h = (float)h; r.hu = (float)hu; r.hv = (float)hv; r.g2 = a.S32.gz[c];
-  r.ih = (float)ih; r.u = (float)u; r.v = (float)v;
+  r.h = (F)h; r.hu = (F)hu; r.hv = (F)hv; r.g2 = stat<F>(a, 2, c);
+  r.ih = (F)ih; r.u = (F)u; r.v = (F)v;
   r.wet = !(h < tiny);
   r.thick = h > tiny;
   r.nz = h != W(0);
-  r.hx = (float)d[0];
-  r.hy = (float)d[3];
-  r.k = kact<W>(h, u, v, ih, d[0], d[1], d[3], d[5], (float)I.kc1, (float)I.kroot, (float)I.eps,
-                tiny);
-  r.dk = want_dk ? dkact_dphi<W>(h, u, v, ih, d[0], d[1], d[3], d[5], (float)I.kc1,
-                                 (float)I.kroot, (float)I.dkroot, (float)I.eps, tiny)
-                 : 0.f;
-  r.sgn[0] = r.sgn[1] = r.sgn[2] = r.sgn[3] = 0.f;
+  r.hx = (F)d[0];
+  r.hy = (F)d[3];
+  r.k = kact<W, F>(h, u, v, ih, d[0], d[1], d[3], d[5], (F)I.kc1, (F)I.kroot, (F)I.eps, tiny);
+  r.dk = want_dk ? dkact_dphi<W, F>(h, u, v, ih, d[0], d[1], d[3], d[5], (F)I.kc1, (F)I.kroot,
+                                    (F)I.dkroot, (F)I.eps, tiny)
+                 : F(0);
+  r.sgn[0] = r.sgn[1] = r.sgn[2] = r.sgn[3] = F(0);
   r.stop[0] = r.stop[1] = 0;
   if (r.thick) {
     const W ft = (W)a.ft;
-    r.sgn[0] = (float)tiny_sgn<W>(ih * (d[4] - u * d[3]), ft);
-    r.sgn[2] = (float)tiny_sgn<W>(u, ft);
-    r.sgn[1] = (float)tiny_sgn<W>(ih * (d[2] - v * d[0]), ft);
-    r.sgn[3] = (float)tiny_sgn<W>(v, ft);
+    r.sgn[0] = (F)tiny_sgn<W>(ih * (d[4] - u * d[3]), ft);
+    r.sgn[2] = (F)tiny_sgn<W>(u, ft);
+    r.sgn[1] = (F)tiny_sgn<W>(ih * (d[2] - v * d[0]), ft);
+    r.sgn[3] = (F)tiny_sgn<W>(v, ft);
     // stop: the Coulomb term's max(.,0) clamps to zero (or tan(phi_bed) = 0)
-    const W g2 = (W)r.g2, tanb = (W)I.tan_bed;
-    r.stop[0] = r.sgn[2] != 0.f && (!(g2 * h + u * hu * (W)a.S32.kx[c] > W(0)) || tanb == W(0));
-    r.stop[1] = r.sgn[3] != 0.f && (!(g2 * h + v * hv * (W)a.S32.ky[c] > W(0)) || tanb == W(0));
+    const W g2 = (W)stat<F>(a, 2, c), tanb = (W)I.tan_bed;
+    r.stop[0] = r.sgn[2] != F(0) && (!(g2 * h + u * hu * (W)stat<F>(a, 5, c) > W(0)) || tanb == W(0));
+    r.stop[1] = r.sgn[3] != F(0) && (!(g2 * h + v * hv * (W)stat<F>(a, 6, c) > W(0)) || tanb == W(0));
   }
   return r;
 }
@@ -391,15 +414,15 @@ struct DualLds {
   float hsx1[RN], hsx2[RN], hsy1[RN], hsy2[RN];
   int flags[RN];  // sgn0+1 | (sgn1+1)<<2 | stop0<<4 | stop1<<5 | wet<<6 | thick<<7 | nz<<8
 };
-__device__ __forceinline__ void lds_put(DualLds& L, int r, const DualInfo& d) {
+__device__ __forceinline__ void lds_put(DualLds& L, int r, const DualInfoT<float>& d) {
   L.h[r] = d.h; L.hu[r] = d.hu; L.hv[r] = d.hv; L.g2[r] = d.g2; L.ih[r] = d.ih;
   L.u[r] = d.u; L.v[r] = d.v; L.k[r] = d.k; L.dk[r] = d.dk;
   L.hsx1[r] = d.hsx[1]; L.hsx2[r] = d.hsx[2]; L.hsy1[r] = d.hsy[1]; L.hsy2[r] = d.hsy[2];
   L.flags[r] = (int)(d.sgn[0] + 1.f) | ((int)(d.sgn[1] + 1.f) << 2) | (d.stop[0] << 4) |
                (d.stop[1] << 5) | (d.wet << 6) | (d.thick << 7) | (d.nz << 8);
 }
-__device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
-  DualInfo d;
+__device__ __forceinline__ DualInfoT<float> lds_get(const DualLds& L, int r) {
+  DualInfoT<float> d;
   d.h = L.h[r]; d.hu = L.hu[r]; d.hv = L.hv[r]; d.g2 = L.g2[r]; d.ih = L.ih[r];
   d.u = L.u[r]; d.v = L.v[r]; d.k = L.k[r]; d.dk = L.dk[r];
   d.hsx[0] = d.hsy[0] = 0.f;
@@ -417,7 +440,8 @@ __device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
   return d;
 }
 
-__device__ __forceinline__ void accum_T(float adj[3], const M3f& J, const float lam[3]) {
+template <class F>
+__device__ __forceinline__ void accum_T(F adj[3], const M3T<F>& J, const F lam[3]) {
 #pragma unroll
   for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -425,62 +449,66 @@ __device__ __forceinline__ void accum_T(float adj[3], const M3f& J, const float 
 }
 
 // neighbour-block source part: slope sensitivity of the internal friction
-__device__ __forceinline__ void nbr_block(M3f& J, const DualInfo& n, float sin_int, float dt,
-                                          float dh0, float dh1) {
+template <class F>
+__device__ __forceinline__ void nbr_block(M3T<F>& J, const DualInfoT<F>& n, F sin_int, F dt,
+                                          F dh0, F dh1) {
   if (n.thick) {
-    const float alpha = sin_int * n.sgn[0] * n.k, betta = sin_int * n.sgn[1] * n.k;
+    const F alpha = sin_int * n.sgn[0] * n.k, betta = sin_int * n.sgn[1] * n.k;
     if (!n.stop[0]) J.a[1][0] += dt * n.h * alpha * n.g2 * dh1;
     if (!n.stop[1]) J.a[2][0] += dt * n.h * betta * n.g2 * dh0;
   }
 }
 
 // d(face flux)/dk of one side
-__device__ __forceinline__ void dhfv_dk(int dir, const DualInfo& d, float dF[3], float dW[3]) {
-  dF[0] = dF[1] = dF[2] = dW[0] = dW[1] = dW[2] = 0.f;
+template <class F>
+__device__ __forceinline__ void dhfv_dk(int dir, const DualInfoT<F>& d, F dF[3], F dW[3]) {
+  dF[0] = dF[1] = dF[2] = dW[0] = dW[1] = dW[2] = F(0);
   if (!d.wet) return;
-  const float a = fsqrt(d.k * d.h * d.g2);
-  const float da = (a > 0.f) ? 0.5f * d.h * d.g2 * rcp(a) : 0.f;
-  dF[dir ? 2 : 1] = 0.5f * d.h * d.g2 * d.h;
+  const F a = fsqrt(d.k * d.h * d.g2);
+  const F da = (a > F(0)) ? F(0.5) * d.h * d.g2 * rcp(a) : F(0);
+  dF[dir ? 2 : 1] = F(0.5) * d.h * d.g2 * d.h;
   dW[0] = -da;
   dW[2] = da;
 }
 
-__device__ __forceinline__ void driemann(const Hfv& l, const Hfv& r, const float dFl[3],
-                                         const float dWl[3], const float dFr[3],
-                                         const float dWr[3], float out[3]) {
-  out[0] = out[1] = out[2] = 0.f;
-  const Speeds s = speeds(l, r);
+template <class F>
+__device__ __forceinline__ void driemann(const HfvT<F>& l, const HfvT<F>& r, const F dFl[3],
+                                         const F dWl[3], const F dFr[3], const F dWr[3],
+                                         F out[3]) {
+  out[0] = out[1] = out[2] = F(0);
+  const SpeedsT<F> s = speeds(l, r);
   if (s.mode == 0) return;
   if (s.mode == 1) { for (int v = 0; v < 3; v++) out[v] = dFl[v]; return; }
   if (s.mode == 2) { for (int v = 0; v < 3; v++) out[v] = dFr[v]; return; }
-  float dsl, dsr;
-  if (l.u[0] == 0.f) {
-    dsl = (s.sl < 0.f) ? 2.f * dWr[0] - dWr[1] : 0.f;
-    dsr = (s.sr > 0.f) ? 2.f * dWr[2] - dWr[1] : 0.f;
-  } else if (r.u[0] == 0.f) {
-    dsl = (s.sl < 0.f) ? 2.f * dWl[0] - dWl[1] : 0.f;
-    dsr = (s.sr > 0.f) ? 2.f * dWl[2] - dWl[1] : 0.f;
+  F dsl, dsr;
+  if (l.u[0] == F(0)) {
+    dsl = (s.sl < F(0)) ? F(2) * dWr[0] - dWr[1] : F(0);
+    dsr = (s.sr > F(0)) ? F(2) * dWr[2] - dWr[1] : F(0);
+  } else if (r.u[0] == F(0)) {
+    dsl = (s.sl < F(0)) ? F(2) * dWl[0] - dWl[1] : F(0);
+    dsr = (s.sr > F(0)) ? F(2) * dWl[2] - dWl[1] : F(0);
   } else {
-    dsl = (s.sl < 0.f) ? ((l.w[0] > r.w[0]) ? dWr[0] : dWl[0]) : 0.f;
-    dsr = (s.sr > 0.f) ? ((l.w[2] > r.w[2]) ? dWl[2] : dWr[2]) : 0.f;
+    dsl = (s.sl < F(0)) ? ((l.w[0] > r.w[0]) ? dWr[0] : dWl[0]) : F(0);
+    dsr = (s.sr > F(0)) ? ((l.w[2] > r.w[2]) ? dWl[2] : dWr[2]) : F(0);
   }
-  const float inv = rcp(s.sr - s.sl);
+  const F inv = rcp(s.sr - s.sl);
 #pragma unroll
   for (int v = 0; v < 3; v++) {
-    const float vd = r.u[v] - l.u[v];
-    const float f = (s.sr * l.f[v] - s.sl * r.f[v] + s.sl * s.sr * vd) * inv;
-    const float dfdsr = (l.f[v] + s.sl * vd - f) * inv, dfdsl = (-r.f[v] + s.sr * vd + f) * inv;
+    const F vd = r.u[v] - l.u[v];
+    const F f = (s.sr * l.f[v] - s.sl * r.f[v] + s.sl * s.sr * vd) * inv;
+    const F dfdsr = (l.f[v] + s.sl * vd - f) * inv, dfdsl = (-r.f[v] + s.sr * vd + f) * inv;
     out[v] = dfdsr * dsr + dfdsl * dsl + s.sr * inv * dFl[v] - s.sl * inv * dFr[v];
   }
 }
 
 // face flux and its derivative along d kact/d phi_bed (both sides)
-__device__ __forceinline__ void face_dk(int side, const DualInfo& L, const DualInfo& R,
-                                        float flux[3], float dflux[3]) {
-  const Hfv hl = hfv(side, L.h, L.hu, L.hv, side ? L.v : L.u, L.k, L.g2, L.wet);
-  const Hfv hr = hfv(side, R.h, R.hu, R.hv, side ? R.v : R.u, R.k, R.g2, R.wet);
+template <class F>
+__device__ __forceinline__ void face_dk(int side, const DualInfoT<F>& L, const DualInfoT<F>& R,
+                                        F flux[3], F dflux[3]) {
+  const HfvT<F> hl = hfv<F>(side, L.h, L.hu, L.hv, side ? L.v : L.u, L.k, L.g2, L.wet);
+  const HfvT<F> hr = hfv<F>(side, R.h, R.hu, R.hv, side ? R.v : R.u, R.k, R.g2, R.wet);
   riemann(hl, hr, flux);
-  float dFl[3], dWl[3], dFr[3], dWr[3];
+  F dFl[3], dWl[3], dFr[3], dWr[3];
   dhfv_dk(side, L, dFl, dWl);
   dhfv_dk(side, R, dFr, dWr);
 #pragma unroll
@@ -491,15 +519,13 @@ __device__ __forceinline__ void face_dk(int side, const DualInfo& L, const DualI
 }
 
 // lambda^{s+1} . dF/d phi_bed of one cell at frozen dt
-template <class S>
-__device__ float dphi_term(const AdjArgsM<S>& a, const InstDev& I, long long c,
-                           const DualInfo& C, const DualInfo& E, const DualInfo& N,
-                           const DualInfo& Wc, const DualInfo& Sc, const bool has[5], float dt,
-                           const float lam[3]) {
-  float fE[3] = {0.f, 0.f, 0.f}, fN[3] = {0.f, 0.f, 0.f}, fW[3] = {0.f, 0.f, 0.f},
-        fS[3] = {0.f, 0.f, 0.f};
-  float dE[3] = {0.f, 0.f, 0.f}, dN[3] = {0.f, 0.f, 0.f}, dW[3] = {0.f, 0.f, 0.f},
-        dS[3] = {0.f, 0.f, 0.f};
+template <class S, class F>
+__device__ __forceinline__ F dphi_term(const AdjArgsM<S>& a, const InstDev& I, long long c,
+                       const DualInfoT<F>& C, const DualInfoT<F>& E, const DualInfoT<F>& N,
+                       const DualInfoT<F>& Wc, const DualInfoT<F>& Sc, const bool has[5], F dt,
+                       const F lam[3]) {
+  F fE[3] = {0, 0, 0}, fN[3] = {0, 0, 0}, fW[3] = {0, 0, 0}, fS[3] = {0, 0, 0};
+  F dE[3] = {0, 0, 0}, dN[3] = {0, 0, 0}, dW[3] = {0, 0, 0}, dS[3] = {0, 0, 0};
   if (has[1]) face_dk(0, C, E, fE, dE);
   if (has[3]) face_dk(0, Wc, C, fW, dW);
   if (has[2]) face_dk(1, C, N, fN, dN);
@@ -512,22 +538,121 @@ __device__ float dphi_term(const AdjArgsM<S>& a, const InstDev& I, long long c,
     if (!has[2]) { fN[v] = fS[v]; dN[v] = dS[v]; }
     if (!has[4]) { fS[v] = fN[v]; dS[v] = dN[v]; }
   }
-  const float dtdx = (float)(dt * I.inv_dx), dtdy = (float)(dt * I.inv_dy);
-  float out[3];
+  const F dtdx = (F)(dt * I.inv_dx), dtdy = (F)(dt * I.inv_dy);
+  F out[3];
 #pragma unroll
   for (int v = 0; v < 3; v++) out[v] = -dtdx * (dE[v] - dW[v]) - dtdy * (dN[v] - dS[v]);
-  if (C.h - dtdx * (fE[0] - fW[0]) - dtdy * (fN[0] - fS[0]) < 0.f) out[0] = 0.f;
+  if (C.h - dtdx * (fE[0] - fW[0]) - dtdy * (fN[0] - fS[0]) < F(0)) out[0] = F(0);
   if (C.thick) {
-    const float sec2 = (float)(1. + I.tan_bed * I.tan_bed), sint = (float)I.sin_int;
-    const float g2 = C.g2;
-    float ds2x = C.sgn[0] * C.h * C.dk * (g2 * C.hy + a.S32.dgy[c] * C.h) * sint;
-    float ds3x = C.sgn[2] * fmaxf(g2 * C.h + C.u * C.hu * a.S32.kx[c], 0.f) * sec2;
-    float ds2y = C.sgn[1] * C.h * C.dk * (g2 * C.hx + a.S32.dgx[c] * C.h) * sint;
-    float ds3y = C.sgn[3] * fmaxf(g2 * C.h + C.v * C.hv * a.S32.ky[c], 0.f) * sec2;
+    const F sec2 = (F)(1. + I.tan_bed * I.tan_bed), sint = (F)I.sin_int;
+    const F g2 = C.g2;
+    F ds2x = C.sgn[0] * C.h * C.dk * (g2 * C.hy + stat<F>(a, 4, c) * C.h) * sint;
+    F ds3x = C.sgn[2] * fmax(g2 * C.h + C.u * C.hu * stat<F>(a, 5, c), F(0)) * sec2;
+    F ds2y = C.sgn[1] * C.h * C.dk * (g2 * C.hx + stat<F>(a, 3, c) * C.h) * sint;
+    F ds3y = C.sgn[3] * fmax(g2 * C.h + C.v * C.hv * stat<F>(a, 6, c), F(0)) * sec2;
     out[1] -= dt * (ds2x + ds3x);
     out[2] -= dt * (ds2y + ds3y);
   }
   return lam[0] * out[0] + lam[1] * out[1] + lam[2] * out[2];
+}
+
+// The gather of one cell in arithmetic type F: adj = sum of the transposed
+// Jacobian blocks (the self block without its identity) applied to
+// lambda^{s+1}; nbr(s) returns the record of neighbour slot s (1 E, 2 N,
+// 3 W, 4 S).  Returns the cell's dJ/dphi_bed term (0 unless want_dphi).
+template <class S, class F, class NBR>
+__device__ __forceinline__ double gather(const AdjArgsM<S>& a, const InstDev& I, const Cell& q,
+                                         const DualInfoT<F>& self, NBR&& nbr, double dtd,
+                                         F adj[3], bool want_dphi) {
+  const long long c = q.c, nx = I.nx;
+  const F dt = (F)dtd, dtdx = (F)(dtd * I.inv_dx), dtdy = (F)(dtd * I.inv_dy);
+  const bool has[5] = {true, q.hasE, q.hasN, q.hasW, q.hasS};
+  const long long nb[5] = {c, c + 1, c + nx, c - 1, c - nx};
+  adj[0] = adj[1] = adj[2] = F(0);
+  if (self.nz) {
+    M3T<F> JE = zero3<F>(), JN = zero3<F>(), JW = zero3<F>(), JS = zero3<F>();
+    {
+      const DualHfvT<F> cx = dual_hfv(0, self);
+      if (has[1]) JE = dual_riemann_side(cx, dual_hfv(0, nbr(1)), true);
+      if (has[3]) JW = dual_riemann_side(dual_hfv(0, nbr(3)), cx, false);
+    }
+    {
+      const DualHfvT<F> cy = dual_hfv(1, self);
+      if (has[2]) JN = dual_riemann_side(cy, dual_hfv(1, nbr(2)), true);
+      if (has[4]) JS = dual_riemann_side(dual_hfv(1, nbr(4)), cy, false);
+    }
+    asm volatile("" ::: "memory");
+    // self block without its identity (applied in the storage type by the
+    // caller); boundary faces copy the opposite face
+    {
+      M3T<F> J;
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          const F pE = has[1] ? JE.a[i][j] : JW.a[i][j];
+          const F nW = has[3] ? JW.a[i][j] : JE.a[i][j];
+          const F pN = has[2] ? JN.a[i][j] : JS.a[i][j];
+          const F nS = has[4] ? JS.a[i][j] : JN.a[i][j];
+          J.a[i][j] = dtdx * (pE - nW) + dtdy * (pN - nS);
+        }
+      if (self.thick) {
+        const F sint = (F)I.sin_int, tb = (F)I.tan_bed;
+        const F alpha = sint * self.sgn[0] * self.k, betta = sint * self.sgn[1] * self.k;
+        const F u = self.u, v = self.v;
+        const F g2 = self.g2, k0 = stat<F>(a, 5, c), k1 = stat<F>(a, 6, c);
+        J.a[1][0] -= dt * (stat<F>(a, 0, c) -
+                           alpha * self.h * (F(2) * stat<F>(a, 4, c) + g2 * self.hsy[0]) -
+                           alpha * g2 * self.hy);
+        if (self.sgn[2] != F(0) && !self.stop[0]) {
+          J.a[1][0] += dt * self.sgn[2] * tb * (g2 - u * u * k0);
+          J.a[1][1] += dt * self.sgn[2] * tb * (F(2) * u * k0);
+        }
+        J.a[2][0] -= dt * (stat<F>(a, 1, c) -
+                           betta * self.h * (F(2) * stat<F>(a, 3, c) + g2 * self.hsx[0]) -
+                           betta * g2 * self.hx);
+        if (self.sgn[3] != F(0) && !self.stop[1]) {
+          J.a[2][0] += dt * self.sgn[3] * tb * (g2 - v * v * k1);
+          J.a[2][2] += dt * self.sgn[3] * tb * (F(2) * v * k1);
+        }
+      }
+      const F lam[3] = {(F)a.lam_next.h[c], (F)a.lam_next.hu[c], (F)a.lam_next.hv[c]};
+      accum_T(adj, J, lam);
+    }
+    const int order[4] = {4, 1, 2, 3};  // S, E, N, W (reference visiting order)
+    const F sint = (F)I.sin_int;
+#pragma unroll
+    for (int o = 0; o < 4; o++) {
+      const int s = order[o];
+      if (!has[s]) continue;
+      const DualInfoT<F> N = nbr(s);
+      const long long n = nb[s];
+      M3T<F> J = zero3<F>();
+      if (s == 1) {
+        if (q.i + 2 < I.nx)
+          for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = -dtdx * JE.a[i][j];
+        nbr_block(J, N, sint, dt, N.hsx[1], F(0));
+      } else if (s == 3) {
+        if (q.i - 2 >= 0)
+          for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = dtdx * JW.a[i][j];
+        nbr_block(J, N, sint, dt, N.hsx[2], F(0));
+      } else if (s == 2) {
+        if (q.j + 2 < I.ny)
+          for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = -dtdy * JN.a[i][j];
+        nbr_block(J, N, sint, dt, F(0), N.hsy[1]);
+      } else {
+        if (q.j - 2 >= 0)
+          for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = dtdy * JS.a[i][j];
+        nbr_block(J, N, sint, dt, F(0), N.hsy[2]);
+      }
+      const F lam[3] = {(F)a.lam_next.h[n], (F)a.lam_next.hu[n], (F)a.lam_next.hv[n]};
+      accum_T(adj, J, lam);
+    }
+  }
+  if (!want_dphi) return 0.;
+  asm volatile("" ::: "memory");
+  const F lam[3] = {(F)a.lam_next.h[c], (F)a.lam_next.hu[c], (F)a.lam_next.hv[c]};
+  return (double)dphi_term<S, F>(a, I, c, self, nbr(1), nbr(2), nbr(3), nbr(4), has, dt, lam);
 }
 
 template <class S, int TX>
@@ -541,9 +666,9 @@ __global__ __launch_bounds__(BLOCK) void k32_adjoint(AdjArgsM<S> a) {
   __shared__ DualLds L;
   const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
   const int rself = (ty + 1) * RX + tx + 1;
-  DualInfo self;
+  DualInfoT<float> self;
   if (q.act) {
-    self = dual_info(a, I, q.i, q.j, want_dphi);
+    self = dual_info<S, float>(a, I, q.i, q.j, want_dphi);
     lds_put(L, rself, self);
   }
   if ((int)threadIdx.x < NHALO) {
@@ -555,119 +680,43 @@ __global__ __launch_bounds__(BLOCK) void k32_adjoint(AdjArgsM<S> a) {
     else { rx = RX - 1; ry = h - 2 * RX - TY + 1; }
     const int gi = t.x0 - 1 + rx, gj = t.y0 - 1 + ry;
     if (gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny)
-      lds_put(L, ry * RX + rx, dual_info(a, I, gi, gj, want_dphi));
+      lds_put(L, ry * RX + rx, dual_info<S, float>(a, I, gi, gj, want_dphi));
   }
   __syncthreads();
   double dphi = 0.;
   if (q.act) {
-    const long long c = q.c, nx = I.nx;
+    const long long c = q.c;
     const double dtd = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
-    const float dt = (float)dtd, dtdx = (float)(dtd * I.inv_dx), dtdy = (float)(dtd * I.inv_dy);
-    float adj[3] = {0.f, 0.f, 0.f};
-    const bool has[5] = {true, q.hasE, q.hasN, q.hasW, q.hasS};
-    const long long nb[5] = {c, c + 1, c + nx, c - 1, c - nx};
-    const int rn[5] = {rself, rself + 1, rself + RX, rself - 1, rself - RX};
-    // Neighbour records are re-read from LDS at each use; the empty asm
-    // statements keep the compiler from merging the re-reads into
-    // long-lived registers.
-    if (self.nz) {
-      M3f JE = zero3f(), JN = zero3f(), JW = zero3f(), JS = zero3f();
-      {
-        const DualHfv cx = dual_hfv(0, self);
-        if (has[1]) JE = dual_riemann_side(cx, dual_hfv(0, lds_get(L, rn[1])), true);
-        if (has[3]) JW = dual_riemann_side(dual_hfv(0, lds_get(L, rn[3])), cx, false);
+    // LDS slot of neighbour s (1 E, 2 N, 3 W, 4 S)
+    auto rn = [&](int s) {
+      return s == 1 ? rself + 1 : (s == 2 ? rself + RX : (s == 3 ? rself - 1 : rself - RX));
+    };
+    auto thin_at = [&](bool present, int r) {
+      const float hn = L.h[r];
+      return present && hn > 0.f && hn < THIN;
+    };
+    // a thin layer in the stencil: the FP64 pass takes the cell
+    const bool thin = (self.h > 0.f && self.h < THIN) || thin_at(q.hasE, rn(1)) ||
+                      thin_at(q.hasN, rn(2)) || thin_at(q.hasW, rn(3)) || thin_at(q.hasS, rn(4));
+    if (thin) {
+      // handed to k32_adjoint_thin (FP64), which writes this cell's lambda
+      const int slot = atomicAdd(a.thin_count, 1);
+      a.thin_list[slot] = make_int2((int)blockIdx.x, (int)threadIdx.x);
+    } else {
+      const S lc[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
+      float adj[3];
+      dphi = gather<S, float>(a, I, q, self, [&](int s) { return lds_get(L, rn(s)); }, dtd, adj,
+                              want_dphi);
+      // lambda^s = -(K_self - I)^T lambda_c - sum K_n^T lambda_n = lambda_c - adj;
+      // flush |lambda| < 1e-16 (calc_adjoint.C:150-152)
+      S out[3];
+      for (int j = 0; j < 3; j++) {
+        out[j] = self.nz ? lc[j] - (S)adj[j] : (S)0;
+        if (fabs((double)out[j]) < 1e-16) out[j] = (S)0;
       }
-      {
-        const DualHfv cy = dual_hfv(1, self);
-        if (has[2]) JN = dual_riemann_side(cy, dual_hfv(1, lds_get(L, rn[2])), true);
-        if (has[4]) JS = dual_riemann_side(dual_hfv(1, lds_get(L, rn[4])), cy, false);
-      }
-      asm volatile("" ::: "memory");
-      // self block without its identity (applied in the storage type below);
-      // boundary faces copy the opposite face
-      {
-        M3f J;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-          for (int j = 0; j < 3; j++) {
-            const float pE = has[1] ? JE.a[i][j] : JW.a[i][j];
-            const float nW = has[3] ? JW.a[i][j] : JE.a[i][j];
-            const float pN = has[2] ? JN.a[i][j] : JS.a[i][j];
-            const float nS = has[4] ? JS.a[i][j] : JN.a[i][j];
-            J.a[i][j] = dtdx * (pE - nW) + dtdy * (pN - nS);
-          }
-        if (self.thick) {
-          const float sint = (float)I.sin_int, tb = (float)I.tan_bed;
-          const float alpha = sint * self.sgn[0] * self.k, betta = sint * self.sgn[1] * self.k;
-          const float u = self.u, v = self.v;
-          const float g2 = self.g2, k0 = a.S32.kx[c], k1 = a.S32.ky[c];
-          J.a[1][0] -= dt * (a.S32.gx[c] - alpha * self.h * (2.f * a.S32.dgy[c] + g2 * self.hsy[0]) -
-                             alpha * g2 * self.hy);
-          if (self.sgn[2] != 0.f && !self.stop[0]) {
-            J.a[1][0] += dt * self.sgn[2] * tb * (g2 - u * u * k0);
-            J.a[1][1] += dt * self.sgn[2] * tb * (2.f * u * k0);
-          }
-          J.a[2][0] -= dt * (a.S32.gy[c] - betta * self.h * (2.f * a.S32.dgx[c] + g2 * self.hsx[0]) -
-                             betta * g2 * self.hx);
-          if (self.sgn[3] != 0.f && !self.stop[1]) {
-            J.a[2][0] += dt * self.sgn[3] * tb * (g2 - v * v * k1);
-            J.a[2][2] += dt * self.sgn[3] * tb * (2.f * v * k1);
-          }
-        }
-        const float lam[3] = {(float)a.lam_next.h[c], (float)a.lam_next.hu[c],
-                              (float)a.lam_next.hv[c]};
-        accum_T(adj, J, lam);
-      }
-      const int order[4] = {4, 1, 2, 3};  // S, E, N, W (reference visiting order)
-      const float sint = (float)I.sin_int;
-#pragma unroll
-      for (int o = 0; o < 4; o++) {
-        const int s = order[o];
-        if (!has[s]) continue;
-        const DualInfo N = lds_get(L, rn[s]);
-        const long long n = nb[s];
-        M3f J = zero3f();
-        if (s == 1) {
-          if (q.i + 2 < I.nx)
-            for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = -dtdx * JE.a[i][j];
-          nbr_block(J, N, sint, dt, N.hsx[1], 0.f);
-        } else if (s == 3) {
-          if (q.i - 2 >= 0)
-            for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = dtdx * JW.a[i][j];
-          nbr_block(J, N, sint, dt, N.hsx[2], 0.f);
-        } else if (s == 2) {
-          if (q.j + 2 < I.ny)
-            for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = -dtdy * JN.a[i][j];
-          nbr_block(J, N, sint, dt, 0.f, N.hsy[1]);
-        } else {
-          if (q.j - 2 >= 0)
-            for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) J.a[i][j] = dtdy * JS.a[i][j];
-          nbr_block(J, N, sint, dt, 0.f, N.hsy[2]);
-        }
-        const float lam[3] = {(float)a.lam_next.h[n], (float)a.lam_next.hu[n],
-                              (float)a.lam_next.hv[n]};
-        accum_T(adj, J, lam);
-      }
-    }
-    // lambda^s = -(K_self - I)^T lambda_c - sum K_n^T lambda_n = lambda_c - adj
-    const S lc[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
-    S out[3];
-    for (int j = 0; j < 3; j++) {
-      S v = self.nz ? lc[j] - (S)adj[j] : (S)0;
-      if (fabs((double)v) < 1e-16) v = (S)0;
-      out[j] = v;
-    }
-    a.lam_h[c] = out[0];
-    a.lam_hu[c] = out[1];
-    a.lam_hv[c] = out[2];
-    if (want_dphi) {
-      const float lam[3] = {(float)lc[0], (float)lc[1], (float)lc[2]};
-      asm volatile("" ::: "memory");
-      // every rn[] slot lies inside the LDS tile; records of absent
-      // neighbours are passed but not read (dphi_term checks has[])
-      dphi = (double)dphi_term(a, I, c, self, lds_get(L, rn[1]), lds_get(L, rn[2]),
-                               lds_get(L, rn[3]), lds_get(L, rn[4]), has, dt, lam);
+      a.lam_h[c] = out[0];
+      a.lam_hu[c] = out[1];
+      a.lam_hv[c] = out[2];
     }
   }
   if (want_dphi) {
@@ -679,6 +728,54 @@ __global__ __launch_bounds__(BLOCK) void k32_adjoint(AdjArgsM<S> a) {
       for (int k = 1; k < BLOCK / 64; k++) dphi += sp[k];
       a.dphi_part[blockIdx.x] += dphi;
     }
+  }
+}
+
+// The cells k32_adjoint listed (a thin layer in their stencil), gathered in
+// FP64 from global memory.  A fixed grid strides over the list, so the
+// launch does not depend on the count (graph capture).
+template <class S, int TX>
+__global__ __launch_bounds__(BLOCK) void k32_adjoint_thin(AdjArgsM<S> a) {
+  const bool want_dphi = a.dphi_part != nullptr;
+  const int n = *a.thin_count;
+  for (int e = blockIdx.x * BLOCK + threadIdx.x; e < n; e += gridDim.x * BLOCK) {
+    const int2 item = a.thin_list[e];
+    const TileDev t = a.tiles[item.x];
+    const InstDev I = a.inst[t.inst];
+    Cell q;
+    q.i = t.x0 + item.y % TX;
+    q.j = t.y0 + item.y / TX;
+    q.act = true;
+    q.c = I.off + (long long)q.j * I.nx + q.i;
+    q.hasE = q.i + 1 < I.nx;
+    q.hasW = q.i > 0;
+    q.hasN = q.j + 1 < I.ny;
+    q.hasS = q.j > 0;
+    const long long c = q.c;
+    const double dtd = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
+    const DualInfoT<double> self = dual_info<S, double>(a, I, q.i, q.j, want_dphi);
+    double adj[3];
+    // neighbour s (1 E, 2 N, 3 W, 4 S); an absent neighbour (domain edge)
+    // is never read by the gather -- return the cell's own record instead of
+    // touching memory outside the block
+    auto nbr = [&](int s) {
+      const bool present = s == 1 ? q.hasE : (s == 2 ? q.hasN : (s == 3 ? q.hasW : q.hasS));
+      if (!present) return self;
+      const int i = q.i + (s == 1 ? 1 : (s == 3 ? -1 : 0));
+      const int j = q.j + (s == 2 ? 1 : (s == 4 ? -1 : 0));
+      return dual_info<S, double>(a, I, i, j, want_dphi);
+    };
+    const double dphi = gather<S, double>(a, I, q, self, nbr, dtd, adj, want_dphi);
+    const S lc[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
+    S out[3];
+    for (int j = 0; j < 3; j++) {
+      out[j] = self.nz ? (S)((double)lc[j] - adj[j]) : (S)0;
+      if (fabs((double)out[j]) < 1e-16) out[j] = (S)0;
+    }
+    a.lam_h[c] = out[0];
+    a.lam_hu[c] = out[1];
+    a.lam_hv[c] = out[2];
+    if (want_dphi) atomicAdd(&a.dphi_part[item.x], dphi);
   }
 }
 
@@ -703,7 +800,10 @@ void launch_update_m(const StepArgsM<S>& a, hipStream_t s) {
 }
 template <class S>
 void launch_adjoint_m(const AdjArgsM<S>& a, hipStream_t s) {
+  launch_zero(a.thin_count, sizeof(int), s);
   TG_LAUNCH_TX(a.shape, k32_adjoint, S, dim3(a.ntiles), s, a);
+  const int g = a.ntiles < THIN_GRID ? a.ntiles : THIN_GRID;
+  TG_LAUNCH_TX(a.shape, k32_adjoint_thin, S, dim3(g), s, a);
 }
 template void launch_coef_m<float>(const StepArgsM<float>&, bool, hipStream_t);
 template void launch_coef_m<double>(const StepArgsM<double>&, bool, hipStream_t);

@@ -22,6 +22,9 @@ constexpr float THIN = 1e-3f;
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// FP64 counterparts (IEEE division / square root) for the thin-layer path
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
 
 template <class W>
 __device__ __forceinline__ W c_sgn(W z, W tiny) {
@@ -77,78 +80,85 @@ __device__ __forceinline__ Slope<W> slope_sens(W up, W uc, W um, W inv_dx, W gam
   return r;
 }
 
-// getcoef: kact (times epsilon); sign decision in W
-template <class W>
-__device__ __forceinline__ float kact(W h, W u, W v, W ih, W hx, W hux, W hy, W hvy, float kc1,
-                                      float kroot, float eps, W tiny) {
+// getcoef: kact (times epsilon) in arithmetic type F; sign decision in W
+template <class W, class F = float>
+__device__ __forceinline__ F kact(W h, W u, W v, W ih, W hx, W hux, W hy, W hvy, F kc1, F kroot,
+                                  F eps, W tiny) {
   if (!(h > tiny)) return eps;
   W velg = (hux - u * hx + hvy - v * hy) * ih;
-  float k = kc1 * (1.f - (float)f_sgn(velg, tiny) * kroot) - 1.f;
-  if (fabs(u) < tiny && fabs(v) < tiny) k = 1.f;
-  return fmaxf(k, 0.f) * eps;
+  F k = kc1 * (F(1) - (F)f_sgn(velg, tiny) * kroot) - F(1);
+  if (fabs(u) < tiny && fabs(v) < tiny) k = F(1);
+  return fmax(k, F(0)) * eps;
 }
 
-template <class W>
-__device__ __forceinline__ float dkact_dphi(W h, W u, W v, W ih, W hx, W hux, W hy, W hvy,
-                                            float kc1, float kroot, float dkroot, float eps,
-                                            W tiny) {
-  if (!(h > tiny)) return 0.f;
+template <class W, class F = float>
+__device__ __forceinline__ F dkact_dphi(W h, W u, W v, W ih, W hx, W hux, W hy, W hvy, F kc1,
+                                        F kroot, F dkroot, F eps, W tiny) {
+  if (!(h > tiny)) return F(0);
   W velg = (hux - u * hx + hvy - v * hy) * ih;
-  float s = (float)f_sgn(velg, tiny);
-  float k = kc1 * (1.f - s * kroot) - 1.f;
-  if (fabs(u) < tiny && fabs(v) < tiny) return 0.f;
-  if (k < 0.f) return 0.f;
+  F s = (F)f_sgn(velg, tiny);
+  F k = kc1 * (F(1) - s * kroot) - F(1);
+  if (fabs(u) < tiny && fabs(v) < tiny) return F(0);
+  if (k < F(0)) return F(0);
   return -kc1 * s * dkroot * eps;
 }
 
-struct Hfv {
-  float u[3], f[3], w[3];
+// Physical flux of one side, its state and wave speeds, in arithmetic type F
+// (float for the bulk, double for the thin-layer path of the adjoint).
+template <class F>
+struct HfvT {
+  F u[3], f[3], w[3];
 };
+using Hfv = HfvT<float>;
+
 // physical flux of one side in direction dir (0 x, 1 y); un = normal velocity
-__device__ __forceinline__ Hfv hfv(int dir, float h, float hu, float hv, float un, float k,
-                                   float g3, bool wet) {
-  Hfv r;
+template <class F>
+__device__ __forceinline__ HfvT<F> hfv(int dir, F h, F hu, F hv, F un, F k, F g3, bool wet) {
+  HfvT<F> r;
   if (!wet) {
 #pragma unroll
-    for (int i = 0; i < 3; i++) r.u[i] = r.f[i] = r.w[i] = 0.f;
+    for (int i = 0; i < 3; i++) r.u[i] = r.f[i] = r.w[i] = F(0);
     return r;
   }
-  float a2 = k * h * g3, a = fsqrt(a2);
+  F a2 = k * h * g3, a = fsqrt(a2);
   r.u[0] = h; r.u[1] = hu; r.u[2] = hv;
   r.f[0] = dir ? hv : hu;
-  r.f[1] = hu * un + (dir ? 0.f : 0.5f * a2 * h);
-  r.f[2] = hv * un + (dir ? 0.5f * a2 * h : 0.f);
+  r.f[1] = hu * un + (dir ? F(0) : F(0.5) * a2 * h);
+  r.f[2] = hv * un + (dir ? F(0.5) * a2 * h : F(0));
   r.w[0] = un - a; r.w[1] = un; r.w[2] = un + a;
   return r;
 }
 
-struct Speeds {
-  float sl, sr;
+template <class F>
+struct SpeedsT {
+  F sl, sr;
   int mode;  // 0 zero flux, 1 left upwind, 2 right upwind, 3 HLL
 };
-__device__ __forceinline__ Speeds speeds(const Hfv& l, const Hfv& r) {
-  Speeds s;
-  if (l.u[0] == 0.f && r.u[0] == 0.f) { s.sl = s.sr = 0.f; s.mode = 0; return s; }
-  if (l.u[0] == 0.f) {
-    s.sl = fminf(0.f, 2.f * r.w[0] - r.w[1]);
-    s.sr = fmaxf(0.f, 2.f * r.w[2] - r.w[1]);
-  } else if (r.u[0] == 0.f) {
-    s.sl = fminf(0.f, 2.f * l.w[0] - l.w[1]);
-    s.sr = fmaxf(0.f, 2.f * l.w[2] - l.w[1]);
+template <class F>
+__device__ __forceinline__ SpeedsT<F> speeds(const HfvT<F>& l, const HfvT<F>& r) {
+  SpeedsT<F> s;
+  if (l.u[0] == F(0) && r.u[0] == F(0)) { s.sl = s.sr = F(0); s.mode = 0; return s; }
+  if (l.u[0] == F(0)) {
+    s.sl = fmin(F(0), F(2) * r.w[0] - r.w[1]);
+    s.sr = fmax(F(0), F(2) * r.w[2] - r.w[1]);
+  } else if (r.u[0] == F(0)) {
+    s.sl = fmin(F(0), F(2) * l.w[0] - l.w[1]);
+    s.sr = fmax(F(0), F(2) * l.w[2] - l.w[1]);
   } else {
-    s.sl = fminf(0.f, fminf(l.w[0], r.w[0]));
-    s.sr = fmaxf(0.f, fmaxf(l.w[2], r.w[2]));
+    s.sl = fmin(F(0), fmin(l.w[0], r.w[0]));
+    s.sr = fmax(F(0), fmax(l.w[2], r.w[2]));
   }
-  s.mode = (s.sl >= 0.f) ? 1 : ((s.sr <= 0.f) ? 2 : 3);
+  s.mode = (s.sl >= F(0)) ? 1 : ((s.sr <= F(0)) ? 2 : 3);
   return s;
 }
 
-__device__ __forceinline__ void riemann(const Hfv& l, const Hfv& r, float flux[3]) {
-  Speeds s = speeds(l, r);
-  if (s.mode == 0) { flux[0] = flux[1] = flux[2] = 0.f; return; }
+template <class F>
+__device__ __forceinline__ void riemann(const HfvT<F>& l, const HfvT<F>& r, F flux[3]) {
+  SpeedsT<F> s = speeds(l, r);
+  if (s.mode == 0) { flux[0] = flux[1] = flux[2] = F(0); return; }
   if (s.mode == 1) { for (int v = 0; v < 3; v++) flux[v] = l.f[v]; return; }
   if (s.mode == 2) { for (int v = 0; v < 3; v++) flux[v] = r.f[v]; return; }
-  float inv = rcp(s.sr - s.sl);
+  F inv = rcp(s.sr - s.sl);
 #pragma unroll
   for (int v = 0; v < 3; v++)
     flux[v] = (s.sr * l.f[v] - s.sl * r.f[v] + s.sl * s.sr * (r.u[v] - l.u[v])) * inv;

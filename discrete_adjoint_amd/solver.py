@@ -152,6 +152,16 @@ class Simulation:
                                              _lib.dptr(ln), _lib.dptr(out)))
         return out.reshape(3, -1)
 
+    def adjoint_step_grad(self, s, dt, lam_next):
+        """One reverse step and its dJ/dphi_bed contribution."""
+        ln = np.ascontiguousarray(lam_next, dtype=np.float64).reshape(-1)
+        out = np.empty_like(ln)
+        g = C.c_double(0.0)
+        self._check(self.lib.tg_adjoint_step_grad(self.ctx, C.byref(self.params), s, dt,
+                                                  _lib.dptr(ln), _lib.dptr(out),
+                                                  C.cast(C.byref(g), _lib._dp)))
+        return out.reshape(3, -1), g.value
+
     def time_kernel(self, which, reps=20):
         """Average launch time (ms) of one kernel: 0 coef, 1 update, 2 adjoint."""
         ms = C.c_double()

@@ -40,8 +40,14 @@ typedef enum tg_precision {
                     in FP64 and advanced by the FP32 increments; FP64 thin-
                     layer velocity recovery, CFL reduction, mass and
                     gradient sums                                           */
-  TG_FP32 = 2    /* as TG_MIXED but the state and adjoint are stored in FP32
+  TG_FP32 = 2,   /* as TG_MIXED but the state and adjoint are stored in FP32
                     (half the bytes; larger error over long runs)           */
+  TG_FWD64_ADJ32 = 3  /* forward sweep and its reverse recompute exactly as
+                    TG_FP64 (bit-identical states, dt and functional); the
+                    adjoint's Jacobian and gradient arithmetic in FP32 with
+                    every branch decision taken in FP64 on the FP64 state,
+                    adjoint stored and accumulated in FP64, FP32 copy of the
+                    static coefficients for the adjoint                     */
 } tg_precision;
 
 /* Context configuration.  tg_config_default() fills the reference's values. */
@@ -198,6 +204,12 @@ int tg_adjoint_run(tg_ctx* ctx, const tg_params* prm, const tg_functional* fn,
  * one dual_solver iteration. */
 int tg_adjoint_step(tg_ctx* ctx, const tg_params* prm, int s, double dt,
                     const double* lambda_next, double* lambda_out);
+
+/* As tg_adjoint_step, and also returns this step's contribution
+ * lambda^{s+1} . dU^{s+1}/d phi_bed to dJ/d phi_bed, per member
+ * (dJ_dphibed [number of members]); the terms tg_adjoint_run sums. */
+int tg_adjoint_step_grad(tg_ctx* ctx, const tg_params* prm, int s, double dt,
+                         const double* lambda_next, double* lambda_out, double* dJ_dphibed);
 
 /* ---- timing (HIP events around the context's kernels) ---------------------- */
 

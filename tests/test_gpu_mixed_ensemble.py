@@ -96,3 +96,69 @@ def test_mixed_ensemble_equals_individual_mixed_runs():
             ref = sim.get_state()
         for x, y in zip(states[b], ref):
             assert np.array_equal(x, y)
+
+
+def test_fwd64_adj32_forward_is_fp64_and_adjoint_tracks():
+    # FP64 forward (bit-identical states, dt, functional); FP32 adjoint
+    # arithmetic with FP64 branch decisions on the identical states, so the
+    # adjoint differs by rounding only (no switch flips)
+    mem = _members()
+    print("case fwd64-adj32 vs fp64 ensemble", flush=True)
+    ws = [_box(p).reshape(3, -1) for p in mem]
+    out = {}
+    for prec in (_lib.TG_FP64, _lib.TG_FWD64_ADJ32):
+        with Ensemble(mem, precision=prec, checkpoint_every=7) as ens:
+            ens.run(30)
+            st = [ens.member_state(b) for b in range(3)]
+            dts = [ens.member_dt_record(b) for b in range(3)]
+            lam, dphi = ens.dual_solver(ws, want_dphi=True)
+            out[prec] = (st, dts, lam, dphi)
+    a, b = out[_lib.TG_FWD64_ADJ32], out[_lib.TG_FP64]
+    for m in range(3):
+        for x, y in zip(a[0][m], b[0][m]):
+            assert np.array_equal(x, y)
+        assert np.array_equal(a[1][m], b[1][m])
+        # lambda^0 within FP32 rounding of the Jacobian arithmetic
+        assert _rel(a[2][m], b[2][m]) < 1e-5
+    assert np.allclose(a[3], b[3], rtol=1e-4, atol=0)
+
+
+def _edge_members():
+    # piles against the domain edges: thin fronts reach the outer cells
+    # (thin-layer FP64 pass at the boundary, with the gradient requested)
+    from helpers import analytic_problem, paraboloid
+    a = analytic_problem(30, 22, 10.0, slope_x_deg=-20.0, slope_y_deg=-10.0,
+                         h=paraboloid(40.0, 35.0, 60.0, 50.0, 6.0), phi_int=32.0, phi_bed=14.0,
+                         nsteps=40, curv=(1e-3, 0.0))
+    b = analytic_problem(26, 34, 10.0, slope_x_deg=22.0, slope_y_deg=12.0,
+                         h=paraboloid(230.0, 300.0, 50.0, 60.0, 5.0), phi_int=30.0,
+                         phi_bed=16.0, nsteps=40)
+    return [a, b]
+
+
+@pytest.mark.parametrize("mode", [_lib.TG_FWD64_ADJ32, _lib.TG_MIXED, _lib.TG_FP32])
+def test_thin_fronts_at_the_domain_edge(mode):
+    mem = _edge_members()
+    print(f"case thin fronts at the edge, precision {mode}", flush=True)
+    ws = [np.concatenate([np.full(p.n, p.dx * p.dy), np.zeros(2 * p.n)]).reshape(3, -1)
+          for p in mem]
+    out = {}
+    for prec in (_lib.TG_FP64, mode):
+        with Ensemble(mem, precision=prec, checkpoint_every=9) as ens:
+            ens.run(40)
+            h = [ens.member_state(b)[0] for b in range(2)]
+            lam, dphi = ens.dual_solver(ws, want_dphi=True)
+            out[prec] = (h, lam, dphi)
+    a, b = out[mode], out[_lib.TG_FP64]
+    for m in range(2):
+        # the flow reached the edge ring with thin layers
+        p = mem[m]
+        ring = np.zeros((p.ny, p.nx), bool)
+        ring[0, :] = ring[-1, :] = ring[:, 0] = ring[:, -1] = True
+        hb = b[0][m].reshape(p.ny, p.nx)[ring]
+        assert ((hb > 0) & (hb < 1e-3)).any() or (hb > 0).any()
+        assert np.isfinite(a[1][m]).all()
+        if mode == _lib.TG_FWD64_ADJ32:
+            assert np.array_equal(a[0][m], b[0][m])
+            assert _rel(a[1][m], b[1][m]) < 1e-4
+    assert np.all(np.isfinite(a[2]))

@@ -73,15 +73,18 @@ def test_autotune_selects_valid_variants_and_keeps_results():
         assert dphi == pytest.approx(ref[3], rel=1e-12, abs=1e-300)
 
 
-@pytest.mark.parametrize("precision", [_lib.TG_FP64, _lib.TG_MIXED])
-def test_graph_replay_equals_eager(precision):
+@pytest.mark.parametrize("precision,tiles", [(_lib.TG_FP64, (0, 0)), (_lib.TG_MIXED, (0, 0)),
+                                             (_lib.TG_FP64, (16, 16)),
+                                             (_lib.TG_FWD64_ADJ32, (32, 8))])
+def test_graph_replay_equals_eager(precision, tiles):
     # the captured sweeps are replayed several times with eager work (and
     # synchronisations) in between; every replay must equal the eager run
-    print(f"case graph replay precision {precision}", flush=True)
+    print(f"case graph replay precision {precision} tiles {tiles}", flush=True)
     ref = _run(precision)
     mem = _members()
     h0 = [np.concatenate([getattr(p, f) for p in mem]) for f in ("h", "hu", "hv")]
-    with Ensemble(mem, precision=precision, checkpoint_every=6, use_graph=1) as ens:
+    with Ensemble(mem, precision=precision, checkpoint_every=6, use_graph=1,
+                  tile_x=tiles[0], tile_y=tiles[1]) as ens:
         for rep in range(3):
             ens.set_state(*h0)
             ens.run(25)
