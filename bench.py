@@ -141,13 +141,15 @@ def timed_run(sim, K, weights, h0, want_lambda=False):
                 wall_f=wall_f, wall_a=wall_a, fwd_ms=fwd_ms, adj_ms=adj_ms, rec_ms=rec_ms)
 
 
-def kernel_roofline(sim, prec, ncells, k, reps):
+def kernel_roofline(sim, prec, ncells, k, reps, recompute=None):
     """Times each kernel live (HIP events on the library's stream) and returns
-    the roofline entry of the dominant one plus per-kernel figures."""
+    the roofline entry of the dominant one plus per-kernel figures.
+    recompute: share of steps whose forward kernels run again in the reverse
+    sweep (default: every step outside the checkpoints)."""
     ab = algo_bytes(prec)
     kt = {name: sim.time_kernel(i, reps) for i, name in enumerate(KERNELS)}
-    # per reverse step the forward kernels run again in the segment recompute
-    per_step = {"coef": kt["coef"] * (2 - 1.0 / k), "update": kt["update"] * (2 - 1.0 / k),
+    r = (1 - 1.0 / k) if recompute is None else recompute
+    per_step = {"coef": kt["coef"] * (1 + r), "update": kt["update"] * (1 + r),
                 "adjoint": kt["adjoint"]}
     dom = max(per_step, key=per_step.get)
     gbs = {n: ab[n] * ncells / (kt[n] * 1e-3) / 1e9 for n in KERNELS}
@@ -177,6 +179,22 @@ def traffic_from_profile(kernel_name, workload):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def store_budget(args):
+    """Bytes of device memory each context may use to keep the last forward
+    states whole (the reverse sweep recomputes only below them): a fraction
+    of the free HBM, 0 with --no-store."""
+    if args.no_store:
+        return 0
+    import ctypes as C
+
+    from discrete_adjoint_amd import _lib
+    free, total = C.c_int64(), C.c_int64()
+    rc = _lib.load().tg_device_memory(0, C.byref(free), C.byref(total))
+    if rc:
+        raise RuntimeError(f"tg_device_memory returned {rc}")
+    return int(args.store_fraction * free.value)
+
+
 def bench_ensemble(args):
     import numpy as np
 
@@ -202,6 +220,8 @@ def bench_ensemble(args):
     res = {}
     roof = None
     graph = 0 if args.no_graph else 1
+    store = store_budget(args)
+    log(f"stored-trajectory budget {store / 2**30:.1f} GiB per context")
     names = [args.precision]
     if not args.product_only:
         names += ["fp64"] if args.precision != "fp64" else []
@@ -209,7 +229,7 @@ def bench_ensemble(args):
     for name in names:
         mode = prec_of(name)
         with Ensemble(E, precision=mode, checkpoint_every=k, max_steps=max(K, W, 1),
-                      use_graph=graph) as ens:
+                      use_graph=graph, store_bytes=store) as ens:
             if W > 0:  # untimed: first launches, allocation of the checkpoint store
                 ens.run(W)
                 ens.dual_solver(weights, want_lambda=False)
@@ -229,11 +249,14 @@ def bench_ensemble(args):
                 f"({r['fwd_ms']:.1f} ms kernels), reverse {r['wall_a'] * 1e3:.1f} ms wall "
                 f"({r['adj_ms']:.1f} ms kernels, recompute {r['rec_ms']:.1f})")
             if mode == prec:
-                roof = kernel_roofline(ens, prec, ncells, k, args.kernel_reps)
+                es = 4 if prec == _lib.TG_FP32 else 8
+                kept = min(K, store // (es * 3 * ncells))
+                roof = kernel_roofline(ens, prec, ncells, k, args.kernel_reps,
+                                       recompute=(K - kept) / K * (1 - 1.0 / k))
     # plain FP64 GPU path: one member at a time
     plain_s = 0.0
     for b, p in enumerate([] if args.product_only else E):
-        with Simulation(p, checkpoint_every=k, max_steps=max(K, 1)) as sim:
+        with Simulation(p, checkpoint_every=k, max_steps=max(K, 1), store_bytes=store) as sim:
             if b == 0 and W > 0:
                 sim.run(W)
                 sim.dual_solver(weights[b], want_lambda=False)
@@ -331,6 +354,7 @@ def bench_ensemble(args):
                                  "algorithmic_bytes_per_cell": ab["forward_step"]},
             "kernel_gbs": gbs,
             "graph_replay": bool(graph),
+            "stored_trajectory_bytes": store,
             "tile_variants": {"coef_update_adjoint": m["tiles"],
                               "shapes": [tile_shape(v) for v in m["tiles"]]},
         },
@@ -478,6 +502,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-store", action="store_true",
+                    help="checkpoints only: recompute every reverse segment")
+    ap.add_argument("--store-fraction", type=float, default=0.7,
+                    help="share of free HBM for the stored trajectory")
     ap.add_argument("--product-only", action="store_true",
                     help="only the measured path (no FP64 / plain / CPU legs): for profiling")
     ap.add_argument("--probe", action="store_true",

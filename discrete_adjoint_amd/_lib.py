@@ -50,6 +50,7 @@ class tg_config(C.Structure):
         ("tile_x", C.c_int),
         ("tile_y", C.c_int),
         ("use_graph", C.c_int),
+        ("store_bytes", C.c_int64),
     ]
 
 
@@ -161,6 +162,7 @@ SIGNATURES = {
     "tg_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "tg_time_kernel": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_int, _dp]),
     "tg_num_cells": (C.c_int64, [C.c_void_p]),
+    "tg_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "tg_tile_variant": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int)]),
     "tg_autotune": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.POINTER(C.c_int),

@@ -68,11 +68,17 @@ struct tg_ctx {
   int rec_steps = 0;  // steps in the dt record
   int run_steps = 0;  // steps of the last forward run/replay (0 = none)
   std::vector<void*> ckpt, seg;
+  // stored trajectory: U^{traj_s0 .. run_steps-1} of the last run, kept
+  // whole (cfg.store_bytes) so the reverse sweep recomputes only below it
+  void* traj_block = nullptr;  // one allocation of traj_cap states
+  size_t traj_cap = 0;
+  int traj_s0 = 0;
   std::vector<double> member_dphi;
 
   // captured sweeps (cfg.use_graph): one forward and one reverse graph,
   // re-captured when their key changes
   SweepGraph gf, ga;
+  bool gf_end_is_b = false;  // the captured forward sweep leaves U^N in Ut
 
   // timing
   hipEvent_t ev[6] = {};
@@ -142,6 +148,8 @@ void free_fields(tg_ctx* ctx) {
   dfree(ctx->dt_rec);
   for (auto& p : ctx->ckpt) dfree(p);
   for (auto& p : ctx->seg) dfree(p);
+  dfree(ctx->traj_block);
+  ctx->traj_cap = 0;
   ctx->ckpt.clear();
   ctx->seg.clear();
 }
@@ -394,6 +402,26 @@ int ensure_store(tg_ctx* ctx, std::vector<void*>& v, size_t count) {
   return TG_OK;
 }
 
+// The stored trajectory lives in one allocation (fast to size per run);
+// growing it frees the old block, so captured sweeps that point into it are
+// dropped.
+int ensure_traj(tg_ctx* ctx, size_t count) {
+  if (count <= ctx->traj_cap) return TG_OK;
+  drop_graph(ctx->gf);
+  drop_graph(ctx->ga);
+  dfree(ctx->traj_block);
+  ctx->traj_cap = 0;
+  const size_t bytes = (size_t)ctx->es * 3 * (size_t)ctx->ncell * count;
+  int rc = dalloc(ctx, &ctx->traj_block, bytes);
+  if (rc) return rc;
+  ctx->traj_cap = count;
+  return TG_OK;
+}
+
+void* traj_state(tg_ctx* ctx, int i) {
+  return (char*)ctx->traj_block + (size_t)i * ctx->es * 3 * (size_t)ctx->ncell;
+}
+
 void split(tg_ctx* ctx, void* base, void* out[3]) {
   for (int k = 0; k < 3; k++) out[k] = (char*)base + (size_t)k * ctx->es * ctx->ncell;
 }
@@ -428,6 +456,16 @@ int capture(tg_ctx* ctx, SweepGraph& G, const long long key[8], F&& body) {
   return TG_OK;
 }
 
+// Where each state of a forward sweep lives: U^s for s in [s0, N) is kept
+// in traj[s - s0] (written there directly by the step that produces it);
+// below s0 the states alternate between the two working buffers and U^s is
+// checkpointed every k steps.  U^N ends in a working buffer.
+int stored_steps(const tg_ctx* ctx, int nsteps) {
+  const double state = (double)ctx->es * 3 * (double)ctx->ncell;
+  const long long m = (long long)((double)ctx->cfg.store_bytes / state);
+  return (int)(m < nsteps ? (m < 0 ? 0 : m) : nsteps);
+}
+
 int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt_seq,
                  tg_stats* stats, int nstats) {
   if (!ctx->have_mesh || !ctx->have_state) return set_err(ctx, TG_ERR_STATE, "mesh/state not set");
@@ -436,43 +474,64 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
   int rc = upload_params(ctx, prm);
   if (rc) return rc;
   const int k = ctx->cfg.checkpoint_every;
-  rc = ensure_store(ctx, ctx->ckpt, (size_t)(nsteps + k - 1) / k + 1);
+  int M = stored_steps(ctx, nsteps);
+  if (ensure_traj(ctx, (size_t)M) != TG_OK) M = 0;  // no room: checkpoints only
+  const int s0 = nsteps - M;
+  rc = ensure_store(ctx, ctx->ckpt, (size_t)(s0 + k - 1) / k + 1);
   if (rc) return rc;
   ctx->rec_steps = ctx->run_steps = 0;
   if (dt_seq)  // the same prescribed sequence for every member
     for (int b = 0; b < ctx->ninst; b++)
       HIPCHK(ctx, hipMemcpyAsync(ctx->dt_rec + (long long)b * ctx->cfg.max_steps, dt_seq,
                                  sizeof(double) * nsteps, hipMemcpyHostToDevice, ctx->stream));
-  // the sweep: checkpoint every k steps, one step per iteration; swaps the
-  // U / Ut roles once per step
-  auto body = [&]() {
+  // the sweep: U^s -> U^{s+1} for s = 0..nsteps-1 (pointer roles planned up front)
+  void* A[3] = {ctx->U[0], ctx->U[1], ctx->U[2]};
+  void* B[3] = {ctx->Ut[0], ctx->Ut[1], ctx->Ut[2]};
+  auto body = [&]() -> void** {
     launch_reset_red(ctx->red, ctx->ninst, ctx->stream);
+    void** cur = A;
+    void* slot[3];
     for (int s = 0; s < nsteps; s++) {
-      if (s % k == 0) copy_state(ctx, ctx->ckpt[s / k], ctx->U);
-      enqueue_step(ctx, ctx->U, ctx->Ut, s, dt_seq == nullptr);
-      for (int c = 0; c < 3; c++) std::swap(ctx->U[c], ctx->Ut[c]);
+      if (s < s0 && s % k == 0) copy_state(ctx, ctx->ckpt[s / k], cur);
+      if (s == s0 && s0 == 0 && M > 0) copy_state(ctx, traj_state(ctx, 0), cur);
+      void* dst[3];
+      void** dstp;
+      if (s + 1 >= s0 && s + 1 < nsteps) {  // U^{s+1} is kept: write it in place
+        split(ctx, traj_state(ctx, s + 1 - s0), dst);
+        dstp = dst;
+      } else {
+        dstp = (cur == A) ? B : A;
+      }
+      enqueue_step(ctx, cur, dstp, s, dt_seq == nullptr);
+      if (dstp == dst) {
+        for (int c = 0; c < 3; c++) slot[c] = dst[c];
+        cur = slot;
+      } else {
+        cur = dstp;
+      }
     }
+    return cur;  // A or B (U^nsteps never goes to the stored trajectory)
   };
+  void** endU = nullptr;
   if (ctx->cfg.use_graph) {
     const long long key[8] = {nsteps, dt_seq != nullptr, (long long)(intptr_t)ctx->U[0],
-                              ctx->sel[0], ctx->sel[1], k, 0, 0};
+                              ctx->sel[0], ctx->sel[1], k, M, 0};
     if (!same_key(ctx->gf, key)) {
-      void* U0[3] = {ctx->U[0], ctx->U[1], ctx->U[2]};
-      void* T0[3] = {ctx->Ut[0], ctx->Ut[1], ctx->Ut[2]};
-      rc = capture(ctx, ctx->gf, key, body);
-      for (int c = 0; c < 3; c++) { ctx->U[c] = U0[c]; ctx->Ut[c] = T0[c]; }
+      rc = capture(ctx, ctx->gf, key, [&]() { endU = body(); });
       if (rc) return rc;
+      ctx->gf_end_is_b = (endU == B);
     }
+    endU = ctx->gf_end_is_b ? B : A;
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     HIPCHK(ctx, hipGraphLaunch(ctx->gf.x, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    if (nsteps % 2)
-      for (int c = 0; c < 3; c++) std::swap(ctx->U[c], ctx->Ut[c]);
   } else {
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    body();
+    endU = body();
     HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   }
+  if (endU == B)
+    for (int c = 0; c < 3; c++) std::swap(ctx->U[c], ctx->Ut[c]);
   HIPCHK(ctx, hipGetLastError());
   rc = check_err(ctx);
   if (rc) return rc;
@@ -480,6 +539,7 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
   ctx->fwd_ms = ms;
   ctx->rec_steps = ctx->run_steps = nsteps;
+  ctx->traj_s0 = s0;
   if (dt_seq) {  // time advances by the prescribed sequence
     std::vector<double> t(ctx->ninst);
     HIPCHK(ctx, hipMemcpy(t.data(), ctx->time_d, sizeof(double) * ctx->ninst,
@@ -498,8 +558,8 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
   int rc = upload_params(ctx, prm);
   if (rc) return rc;
   const long long n = ctx->ncell;
-  const int N = ctx->run_steps, k = ctx->cfg.checkpoint_every;
-  rc = ensure_store(ctx, ctx->seg, (size_t)k);
+  const int N = ctx->run_steps, k = ctx->cfg.checkpoint_every, s0 = ctx->traj_s0;
+  rc = ensure_store(ctx, ctx->seg, (size_t)(s0 > 0 ? k : 1));
   if (rc) return rc;
   const bool dphi = dJ != nullptr;
   // lambda^N = func_sens at the final step (calc_adjoint.C:93-97)
@@ -507,27 +567,36 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
     rc = h2d_field(ctx, ctx->lam[0][c], 0, n, w + c * n);
     if (rc) return rc;
   }
-  const int nseg = (N + k - 1) / k;
-  // one reverse segment: restore U^{s0} from its checkpoint, recompute
-  // U^{s0+1..s1-1} with the recorded dt, then the adjoint steps s1-1..s0
+  // steps N-1 .. s0 read the stored trajectory; below s0 the sweep works in
+  // segments of k: restore U^{a} from its checkpoint, recompute U^{a+1..b-1}
+  // with the recorded dt, then the adjoint steps b-1 .. a
+  const int nseg = (s0 + k - 1) / k;
+  auto stored = [&](int& cur) {
+    for (int s = N - 1; s >= s0; s--) {
+      void* Us[3];
+      split(ctx, traj_state(ctx, s - s0), Us);
+      enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi);
+      cur = 1 - cur;
+    }
+  };
   auto recompute = [&](int sg) {
-    const int s0 = sg * k, s1 = (s0 + k < N) ? s0 + k : N;
+    const int a = sg * k, b = (a + k < s0) ? a + k : s0;
     void* base[3];
     split(ctx, ctx->ckpt[sg], base);
     copy_state(ctx, ctx->seg[0], base);
-    for (int s = s0; s + 1 < s1; s++) {
+    for (int s = a; s + 1 < b; s++) {
       void* in[3];
       void* out[3];
-      split(ctx, ctx->seg[s - s0], in);
-      split(ctx, ctx->seg[s - s0 + 1], out);
+      split(ctx, ctx->seg[s - a], in);
+      split(ctx, ctx->seg[s - a + 1], out);
       enqueue_step(ctx, in, out, s, false);
     }
   };
   auto reverse = [&](int sg, int& cur) {
-    const int s0 = sg * k, s1 = (s0 + k < N) ? s0 + k : N;
-    for (int s = s1 - 1; s >= s0; s--) {
+    const int a = sg * k, b = (a + k < s0) ? a + k : s0;
+    for (int s = b - 1; s >= a; s--) {
       void* Us[3];
-      split(ctx, ctx->seg[s - s0], Us);
+      split(ctx, ctx->seg[s - a], Us);
       enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi);
       cur = 1 - cur;
     }
@@ -535,11 +604,12 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
   int cur = 0;
   float rec_ms = 0, ms = 0;
   if (ctx->cfg.use_graph) {
-    const long long key[8] = {N, k, dphi, ctx->sel[0], ctx->sel[1], ctx->sel[2], 0, 0};
+    const long long key[8] = {N, k, dphi, ctx->sel[0], ctx->sel[1], ctx->sel[2], s0, 0};
     if (!same_key(ctx->ga, key)) {
       rc = capture(ctx, ctx->ga, key, [&]() {
         int c = 0;
         if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
+        stored(c);
         for (int sg = nseg - 1; sg >= 0; sg--) {
           recompute(sg);
           reverse(sg, c);
@@ -554,6 +624,7 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
   } else {
     if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    stored(cur);
     for (int sg = nseg - 1; sg >= 0; sg--) {
       HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
       recompute(sg);
@@ -745,6 +816,7 @@ void tg_config_default(tg_config* cfg) {
   cfg->tile_x = 0;
   cfg->tile_y = 0;
   cfg->use_graph = 0;
+  cfg->store_bytes = 0;
 }
 
 int tg_params_from_angles(double phi_int, double phi_bed, double epsilon, double gravity_scale,
@@ -1121,6 +1193,18 @@ int tg_autotune(tg_ctx* ctx, const tg_params* prm, int reps, int* variant_out, d
     ctx->sel[w] = bv;
     if (variant_out) variant_out[w] = bv;
   }
+  return TG_OK;
+}
+
+int tg_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return TG_ERR_NODEVICE;
+  size_t f = 0, t = 0;
+  if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&f, &t) != hipSuccess)
+    return TG_ERR_HIP;
+  if (free_bytes) *free_bytes = (int64_t)f;
+  if (total_bytes) *total_bytes = (int64_t)t;
   return TG_OK;
 }
 

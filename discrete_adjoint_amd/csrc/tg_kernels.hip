@@ -69,6 +69,17 @@ __device__ __forceinline__ Cell cell_of(const TileDev& t, const InstDev& I) {
 
 // ---------------------------------------------------------------- forward --
 
+// Limited slope from its two one-sided differences dp = (u+ - u)/dx and
+// dm = (u - u-)/dx (Element::get_slopes, element2.C:886-897; same operands
+// as slope_fwd).
+__device__ __forceinline__ double slope_from(double dp, double dm, double dx, double gamma,
+                                             double tiny) {
+  double dc = (dp * dx + dm * dx) / (dx + dx);
+  return .5 * (c_sgn(dp, tiny) + c_sgn(dm, tiny)) *
+         dmin3(gamma * dabs(dp), gamma * dabs(dm), dabs(dc));
+}
+
+
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
   const TileDev t = a.tiles[blockIdx.x];
@@ -165,40 +176,74 @@ __device__ __forceinline__ Hfv hfv_at(const StepArgs& a, int dir, long long c) {
   return hfv_fwd(dir, a.U.h[c], a.U.hu[c], a.U.hv[c], a.kact[c], a.S.gz[c], a.tiny);
 }
 
+// Face records of one tile: every face flux and every one-sided difference
+// the tile's cells use is computed once (a face is shared by two cells) and
+// staged in LDS.  x-face fi of row fj lies between tile columns fi-1 and fi.
+template <int TX>
+struct FaceLds {
+  static constexpr int TY = BLOCK / TX;
+  double fx[3][TY][TX + 1], fy[3][TY + 1][TX];
+  double dxh[TY][TX + 1], dxhv[TY][TX + 1];  // (U_right - U_left) / dx of h, hv
+  double dyh[TY + 1][TX], dyhu[TY + 1][TX];  // (U_top - U_bottom) / dy of h, hu
+};
+
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_update(StepArgs a) {
+  constexpr int TY = BLOCK / TX;
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
+  const long long nx = I.nx;
+  __shared__ FaceLds<TX> L;
+  // phase 1: faces with a cell of the member on both sides
+  // (calc_edge_states, element2.C:2744-3072)
+  for (int f = (int)threadIdx.x; f < (TX + 1) * TY; f += BLOCK) {
+    const int fi = f % (TX + 1), fj = f / (TX + 1);
+    const int gi = t.x0 + fi, gj = t.y0 + fj;
+    if (gi > 0 && gi < I.nx && gj < I.ny) {
+      const long long r = I.off + (long long)gj * nx + gi;
+      double fl[3];
+      riemann(hfv_at(a, 0, r - 1), hfv_at(a, 0, r), fl);
+      for (int v = 0; v < 3; v++) L.fx[v][fj][fi] = fl[v];
+      L.dxh[fj][fi] = (a.U.h[r] - a.U.h[r - 1]) / I.dx;
+      L.dxhv[fj][fi] = (a.U.hv[r] - a.U.hv[r - 1]) / I.dx;
+    }
+  }
+  for (int f = (int)threadIdx.x; f < TX * (TY + 1); f += BLOCK) {
+    const int fi = f % TX, fj = f / TX;
+    const int gi = t.x0 + fi, gj = t.y0 + fj;
+    if (gj > 0 && gj < I.ny && gi < I.nx) {
+      const long long r = I.off + (long long)gj * nx + gi;
+      double fl[3];
+      riemann(hfv_at(a, 1, r - nx), hfv_at(a, 1, r), fl);
+      for (int v = 0; v < 3; v++) L.fy[v][fj][fi] = fl[v];
+      L.dyh[fj][fi] = (a.U.h[r] - a.U.h[r - nx]) / I.dy;
+      L.dyhu[fj][fi] = (a.U.hu[r] - a.U.hu[r - nx]) / I.dy;
+    }
+  }
+  __syncthreads();
   if (!q.act) return;
-  const long long c = q.c, nx = I.nx;
+  const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
+  const long long c = q.c;
   const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
   const double prev[3] = {a.U.h[c], a.U.hu[c], a.U.hv[c]};
   // Element::get_slopes, element2.C:830-939 (only the components the
-  // corrector reads: hx, hvx, hy, huy)
+  // corrector reads: hx, hvx, hy, huy); boundary cells keep zero slopes
   double hx = 0., hvx = 0., hy = 0., huy = 0.;
   if (q.hasE && q.hasW && q.hasN && q.hasS) {
-    hx = slope_fwd(a.U.h[c + 1], prev[0], a.U.h[c - 1], I.dx, a.gamma, a.tiny);
-    hvx = slope_fwd(a.U.hv[c + 1], prev[2], a.U.hv[c - 1], I.dx, a.gamma, a.tiny);
-    hy = slope_fwd(a.U.h[c + nx], prev[0], a.U.h[c - nx], I.dy, a.gamma, a.tiny);
-    huy = slope_fwd(a.U.hu[c + nx], prev[1], a.U.hu[c - nx], I.dy, a.gamma, a.tiny);
+    hx = slope_from(L.dxh[ty][tx + 1], L.dxh[ty][tx], I.dx, a.gamma, a.tiny);
+    hvx = slope_from(L.dxhv[ty][tx + 1], L.dxhv[ty][tx], I.dx, a.gamma, a.tiny);
+    hy = slope_from(L.dyh[ty + 1][tx], L.dyh[ty][tx], I.dy, a.gamma, a.tiny);
+    huy = slope_from(L.dyhu[ty + 1][tx], L.dyhu[ty][tx], I.dy, a.gamma, a.tiny);
   }
-  // calc_edge_states + boundary_flux (element2.C:2744-3072, 3374-3411): a
-  // face on the domain boundary takes the flux of the opposite face.
-  double fE[3] = {0., 0., 0.}, fW[3] = {0., 0., 0.}, fN[3] = {0., 0., 0.}, fS[3] = {0., 0., 0.};
-  {
-    const Hfv self = hfv_at(a, 0, c);
-    if (q.hasE) riemann(self, hfv_at(a, 0, c + 1), fE);
-    if (q.hasW) riemann(hfv_at(a, 0, c - 1), self, fW);
-    if (!q.hasE) { fE[0] = fW[0]; fE[1] = fW[1]; fE[2] = fW[2]; }
-    if (!q.hasW) { fW[0] = fE[0]; fW[1] = fE[1]; fW[2] = fE[2]; }
-  }
-  {
-    const Hfv self = hfv_at(a, 1, c);
-    if (q.hasN) riemann(self, hfv_at(a, 1, c + nx), fN);
-    if (q.hasS) riemann(hfv_at(a, 1, c - nx), self, fS);
-    if (!q.hasN) { fN[0] = fS[0]; fN[1] = fS[1]; fN[2] = fS[2]; }
-    if (!q.hasS) { fS[0] = fN[0]; fS[1] = fN[1]; fS[2] = fN[2]; }
+  // a face on the domain boundary takes the flux of the opposite face
+  // (boundary_flux, element2.C:3374-3411)
+  double fE[3], fW[3], fN[3], fS[3];
+  for (int v = 0; v < 3; v++) {
+    fE[v] = q.hasE ? L.fx[v][ty][tx + 1] : (q.hasW ? L.fx[v][ty][tx] : 0.);
+    fW[v] = q.hasW ? L.fx[v][ty][tx] : fE[v];
+    fN[v] = q.hasN ? L.fy[v][ty + 1][tx] : (q.hasS ? L.fy[v][ty][tx] : 0.);
+    fS[v] = q.hasS ? L.fy[v][ty][tx] : fN[v];
   }
   // correct() -> update_states (corrector.C:36-140, residual.C:162-275)
   const double kc = (prev[0] > a.tiny) ? a.kact[c] : I.eps;

@@ -72,6 +72,10 @@ typedef struct tg_config {
                              the first call with a given step count, tile
                              selection and buffer parity); 0: launch kernel
                              by kernel.  Results are identical.              */
+  int64_t store_bytes;    /* device memory for keeping the last states of a
+                             forward run whole (0: checkpoints only).  The
+                             reverse sweep reads those states and recomputes
+                             only below them.  Results are identical.        */
 } tg_config;
 
 /* Mesh.  TG_MESH_UNIFORM: an nx*ny block of equal cells; cell c = j*nx + i,
@@ -247,6 +251,9 @@ int tg_autotune(tg_ctx* ctx, const tg_params* prm, int reps, int* variant_out, d
  * (e.g. to reproduce an earlier autotune). */
 int tg_get_tile_selection(const tg_ctx* ctx, int* variant_out);
 int tg_set_tile_selection(tg_ctx* ctx, const int* variant);
+
+/* Free and total device memory of a HIP device (for sizing store_bytes). */
+int tg_device_memory(int device, int64_t* free_bytes, int64_t* total_bytes);
 
 /* Cells in the context's mesh (sum over instances). */
 int64_t tg_num_cells(const tg_ctx* ctx);

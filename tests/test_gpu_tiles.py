@@ -99,3 +99,26 @@ def test_graph_replay_equals_eager(precision, tiles):
             # eager work between replays: the next set_state uploads the state
             # and clears the error flag on the same stream, then synchronises
             ens.get_state()
+
+
+@pytest.mark.parametrize("frac,graph", [(1.0, 0), (0.4, 0), (0.4, 1), (0.04, 1)])
+def test_stored_trajectory_equals_checkpointing(frac, graph):
+    # keeping the last states of the run whole (no recompute for them) gives
+    # the same adjoint and gradient bit for bit
+    print(f"case stored trajectory fraction {frac} graph {graph}", flush=True)
+    ref = _run(_lib.TG_FP64)
+    mem = _members()
+    state = 8 * 3 * sum(p.n for p in mem)
+    with Ensemble(mem, precision=_lib.TG_FP64, checkpoint_every=6, use_graph=graph,
+                  store_bytes=int(frac * 25 * state)) as ens:
+        for rep in range(2):
+            h0 = [np.concatenate([getattr(p, f) for p in mem]) for f in ("h", "hu", "hv")]
+            ens.set_state(*h0)
+            ens.run(25)
+            for b in range(2):
+                for x, y in zip(ens.member_state(b), ref[0][b]):
+                    assert np.array_equal(x, y)
+            lam, dphi = ens.dual_solver([_box(p) for p in mem], want_dphi=True)
+            for b in range(2):
+                assert np.array_equal(lam[b], ref[2][b]), (frac, rep)
+            assert np.array_equal(np.asarray(dphi), np.asarray(ref[3]))
