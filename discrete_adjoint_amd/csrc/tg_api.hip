@@ -1114,10 +1114,13 @@ int tg_last_timing(tg_ctx* ctx, double* forward_ms, double* adjoint_ms, double* 
 int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms) {
   const int keep[3] = {ctx->sel[0], ctx->sel[1], ctx->sel[2]};
   if (v >= 0) ctx->sel[which] = v;
-  // every variant reads U and writes scratch (Ut or lam[1]) with step 0's dt
+  // every variant reads U and writes scratch (Ut or lam[1]) with step 0's
+  // dt; the adjoint runs with its bed-friction gradient term, as in
+  // tg_adjoint_run with dJ_dphibed requested (its partial sums are scratch
+  // here and cleared by the next tg_adjoint_run)
   auto one = [&](int w) {
     if (w == 2) {
-      enqueue_adjoint(ctx, ctx->U, ctx->lam[0], ctx->lam[1], 0, false);
+      enqueue_adjoint(ctx, ctx->U, ctx->lam[0], ctx->lam[1], 0, true);
     } else if (fwd64(ctx)) {
       StepArgs a{};
       const long long n = ctx->ncell;

@@ -224,7 +224,9 @@ int tg_last_timing(tg_ctx* ctx, double* forward_ms, double* adjoint_ms,
 /* Average duration of one launch of a single kernel, timed with HIP events
  * on the context's stream over `reps` back-to-back launches on the current
  * state (no reference counterpart: measurement only).  which: 0 = forward
- * coefficient/CFL pass, 1 = forward update pass, 2 = adjoint gather.
+ * coefficient/CFL pass, 1 = forward update pass, 2 = adjoint gather (with
+ * its bed-friction gradient term and, in the FP32-adjoint modes, its
+ * thin-layer pass).
  * Requires a completed tg_forward_run (and tg_adjoint_run for which = 2). */
 typedef enum tg_kernel_id { TG_K_COEF = 0, TG_K_UPDATE = 1, TG_K_ADJOINT = 2 } tg_kernel_id;
 int tg_time_kernel(tg_ctx* ctx, const tg_params* prm, int which, int reps, double* avg_ms);
