@@ -32,6 +32,8 @@ TG_MESH_LEAVES = 1
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
+_u32p = C.POINTER(C.c_uint32)
 
 
 class tg_config(C.Structure):
@@ -66,6 +68,12 @@ class tg_mesh(C.Structure):
         ("grav", _dp),
         ("dgrav", _dp),
         ("curv", _dp),
+        ("n", C.c_int),
+        ("nbr", _i32p),
+        ("nproc", _i32p),
+        ("gen", _i32p),
+        ("cdx", _dp),
+        ("xc", _dp),
     ]
 
 
@@ -123,8 +131,6 @@ class tg_leaf_table(C.Structure):
     ]
 
 
-_i32p = C.POINTER(C.c_int32)
-_u32p = C.POINTER(C.c_uint32)
 
 # name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {

@@ -128,3 +128,46 @@ def random_balanced_tree(nx0, ny0, max_level, frac, seed):
     ix = np.array([c[1] for c in leaves], np.int32)
     iy = np.array([c[2] for c in leaves], np.int32)
     return lev, ix, iy
+
+
+def _dilate4(a):
+    d = a.copy()
+    d[1:, :] |= a[:-1, :]
+    d[:-1, :] |= a[1:, :]
+    d[:, 1:] |= a[:, :-1]
+    d[:, :-1] |= a[:, 1:]
+    return d
+
+
+def _up(a):
+    return np.repeat(np.repeat(a, 2, axis=0), 2, axis=1)
+
+
+def refine_tree(nx0, ny0, max_level, want):
+    """A 2:1-balanced quadtree over an nx0 x ny0 base grid.  want(L, I, J)
+    returns, for the level-L grid of cell indices (I, J), which cells to split
+    (consulted only where the cell exists).  Splits are then added from fine
+    to coarse until side-adjacent leaves differ by at most one level: a split
+    level-(L+1) cell needs its four side neighbours to exist, i.e. their
+    level-L parents split.  Returns (lev, ix, iy), coarse leaves first, row by
+    row within a level."""
+    R = []
+    for L in range(max_level):
+        J, I = np.mgrid[0:(ny0 << L), 0:(nx0 << L)]
+        r = np.asarray(want(L, I, J), dtype=bool).copy()
+        if L > 0:
+            r &= _up(R[L - 1])
+        R.append(r)
+    for L in range(max_level - 1, 0, -1):
+        d = _dilate4(R[L])
+        H, W = d.shape
+        R[L - 1] |= d.reshape(H // 2, 2, W // 2, 2).any(axis=(1, 3))
+    lev, ix, iy = [], [], []
+    for L in range(max_level + 1):
+        exists = np.ones((ny0, nx0), bool) if L == 0 else _up(R[L - 1])
+        leaf = exists & ~R[L] if L < max_level else exists
+        jj, ii = np.nonzero(leaf)
+        lev.append(np.full(len(ii), L, np.int32))
+        ix.append(ii.astype(np.int32))
+        iy.append(jj.astype(np.int32))
+    return np.concatenate(lev), np.concatenate(ix), np.concatenate(iy)

@@ -198,3 +198,105 @@ def target_box(problem, h_final, cells=20, seed=5):
     xa = problem.x0 + (i0 - 0.5) * problem.dx
     ya = problem.y0 + (j0 - 0.5) * problem.dy
     return (xa, xa + cells * problem.dx, ya, ya + cells * problem.dy)
+
+
+@dataclass
+class LeafProblem:
+    """An adaptive (2:1-balanced) mesh problem: the leaf tables of
+    mesh.flatten (the element-table traversal order), per-leaf FP64 static
+    coefficients, the initial state and the material angles."""
+    name: str
+    nx0: int
+    ny0: int
+    dx0: float
+    dy0: float
+    xmin: float
+    ymin: float
+    table: object             # mesh.LeafTable
+    grav: np.ndarray          # (3, n)
+    dgrav: np.ndarray         # (2, n)
+    curv: np.ndarray          # (2, n)
+    h: np.ndarray
+    hu: np.ndarray
+    hv: np.ndarray
+    phi_int_deg: float
+    phi_bed_deg: float
+    nsteps: int
+    checkpoint_every: int
+    box: tuple = None
+    epsilon: float = 1.0
+    gravity_scale: float = 1.0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return len(self.table.gen)
+
+    def box_weights(self, box=None):
+        """Terminal weights of sum over leaves with centre inside the box of
+        h * dx * dy (the uniform tg_host_box_weights rule, per leaf)."""
+        box = box or self.box
+        x, y = self.table.xc[:, 0], self.table.xc[:, 1]
+        inside = (x > box[0]) & (x < box[1]) & (y > box[2]) & (y < box[3])
+        w = np.zeros((3, self.n))
+        w[0, inside] = (self.table.dx[:, 0] * self.table.dx[:, 1])[inside]
+        return w.reshape(-1)
+
+
+def adaptive_problem(nx0=32, ny0=32, dx0=40.0, max_level=3, nsteps=200, refine_radius=1.6,
+                     pile=None, dem=None, phi_int=30.0, phi_bed=20.0, checkpoint_every=50,
+                     nbuckets=None, name=None):
+    """Adaptive mesh around one paraboloid pile: base nx0 x ny0 cells of dx0,
+    cells whose centre lies within refine_radius * pile radius of the pile
+    centre refined down to max_level, then 2:1-balanced and flattened into the
+    element-table order.  dem: None for the 35-degree plane / runout DEM of
+    plane_runout_problem over the same extent, or a (nrows, ncols, xmin, ymax,
+    res, elev) raster.  Static coefficients of a leaf are those of its own
+    level's uniform grid (tg_host_static_coefs at that level's spacing);
+    the initial state samples the paraboloid at the leaf centres."""
+    from . import mesh as _mesh
+    Lx, Ly = nx0 * dx0, ny0 * dx0
+    if pile is None:
+        pile = (0.3 * Lx, 0.5 * Ly, 150.0, 100.0, 0.0, 30.0)
+    cx, cy, a, b = pile[0], pile[1], pile[2], pile[3]
+    rr = refine_radius * max(a, b)
+
+    def want(L, I, J):
+        s = dx0 / (1 << L)
+        X, Y = (I + 0.5) * s, (J + 0.5) * s
+        return np.hypot(X - cx, Y - cy) < rr + s
+
+    lev, ix, iy = _mesh.refine_tree(nx0, ny0, max_level, want)
+    nb = nbuckets or _mesh.EL_TABLE_SIZE
+    t = _mesh.flatten(nx0, ny0, 0.0, 0.0, dx0, dx0, lev, ix, iy, nbuckets=nb)
+    if dem is None:
+        res = dx0 / (1 << max_level) / 2.0
+        margin = 4 * dx0
+        xmin, ymax = -margin, max(Lx, Ly) + margin
+        ncols = nrows = int(round((max(Lx, Ly) + 2 * margin) / res)) + 1
+        elev = np.zeros(nrows * ncols, dtype=np.float32)
+        rc = _lib.load().tg_host_dem_plane_runout(nrows, ncols, xmin, ymax, res, 35.0, 0.6 * Lx,
+                                                  100.0, _lib.fptr(elev))
+        assert rc == 0
+        dem = (nrows, ncols, xmin, ymax, res, elev)
+    n = len(lev)
+    grav, dgrav, curv = np.zeros((3, n)), np.zeros((2, n)), np.zeros((2, n))
+    h, hu, hv = np.zeros(n), np.zeros(n), np.zeros(n)
+    o = t.order
+    for L in range(max_level + 1):
+        sel = np.nonzero(lev[o] == L)[0]          # flattened positions at level L
+        if len(sel) == 0:
+            continue
+        nxl, nyl, s = nx0 << L, ny0 << L, dx0 / (1 << L)
+        g, dg, cv = static_coefs(dem, nxl, nyl, s, s, 0.5 * s, 0.5 * s)
+        hh, hhu, hhv = piles(nxl, nyl, s, s, 0.5 * s, 0.5 * s, [pile])
+        cell = iy[o[sel]].astype(np.int64) * nxl + ix[o[sel]]
+        grav[:, sel], dgrav[:, sel], curv[:, sel] = g[:, cell], dg[:, cell], cv[:, cell]
+        h[sel], hu[sel], hv[sel] = hh[cell], hhu[cell], hhv[cell]
+    xb = 0.6 * Lx + 100.0
+    box = (xb, xb + 200.0, 0.5 * Ly - 100.0, 0.5 * Ly + 100.0)
+    pb = LeafProblem(name or f"adaptive_{nx0}x{ny0}_L{max_level}", nx0, ny0, dx0, dx0, 0.0, 0.0,
+                     t, grav, dgrav, curv, h, hu, hv, phi_int, phi_bed, nsteps, checkpoint_every,
+                     box=box)
+    pb.meta["levels"] = np.bincount(lev, minlength=max_level + 1).tolist()
+    return pb

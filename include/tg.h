@@ -81,9 +81,19 @@ typedef struct tg_config {
 /* Mesh.  TG_MESH_UNIFORM: an nx*ny block of equal cells; cell c = j*nx + i,
  * centre (x0 + i*dx, y0 + j*dy); cells on the outer ring have the domain
  * boundary (neigh_proc == INIT) on their outer sides.  Static per-cell
- * coefficients (element2.C:3545-3703) are FP64 SoA arrays of n = nx*ny.
- * TG_MESH_LEAVES (adaptive, flattened quadtree tables) is declared for the
- * adaptive path and returns TG_ERR_UNSUPPORTED until that path is built. */
+ * coefficients (element2.C:3545-3703) are FP64 SoA arrays of n cells.
+ *
+ * TG_MESH_LEAVES: the n leaves of a 2:1-balanced adaptive mesh, in the
+ * caller's order (e.g. the element-table traversal of tg_host_flatten), as
+ * the reference's Element neighbour fields (element2.h:652-667): sides
+ * 0 south, 1 east, 2 north, 3 west (positive_x_side = 1); slot s is the
+ * neighbour on side s (with two finer neighbours, the one next to corner s)
+ * and slot s+4 the second neighbour (next to corner s+1) or a copy of slot
+ * s.  nproc is neigh_proc of one process: -1 domain boundary (nbr -1), -2 no
+ * second neighbour, 0 present.  A leaf may not have the domain boundary on
+ * two opposite sides.  nx, ny, dx, dy, x0, y0 are ignored; the leaf path is
+ * built for TG_FP64 (other precisions return TG_ERR_UNSUPPORTED) and for a
+ * single member. */
 typedef enum tg_mesh_kind { TG_MESH_UNIFORM = 0, TG_MESH_LEAVES = 1 } tg_mesh_kind;
 
 typedef struct tg_mesh {
@@ -94,6 +104,13 @@ typedef struct tg_mesh {
   const double* grav;   /* [3*n]: gravity x | y | z  (calc_gravity_vector)   */
   const double* dgrav;  /* [2*n]: d gravity_z / dx | dy (calc_d_gravity)     */
   const double* curv;   /* [2*n]: curvature x | y (calc_topo_data)           */
+  /* TG_MESH_LEAVES only */
+  int n;                /* number of leaves                                  */
+  const int32_t* nbr;   /* [8*n] Element::neighbor as leaf indices           */
+  const int32_t* nproc; /* [8*n] Element::neigh_proc                         */
+  const int32_t* gen;   /* [n]   Element::generation                         */
+  const double* cdx;    /* [2*n] Element::dx (cell size x, y)                */
+  const double* xc;     /* [2*n] Element::coord (centre x, y)                */
 } tg_mesh;
 
 /* Material and time parameters of one run (MatProps, TimeProps). */

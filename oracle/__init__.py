@@ -171,3 +171,99 @@ class Oracle:
             lam, acc = self.adjoint_step(traj[s], s, dts[s], lam, want_dphi=want_dphi)
             dphi += acc
         return U, np.array(dts), lam, dphi
+
+
+_ip = C.POINTER(C.c_int)
+
+
+class or_leaf_mesh(C.Structure):
+    _fields_ = [("n", C.c_int), ("nbr", _ip), ("nproc", _ip), ("gen", _ip), ("cdx", _dp),
+                ("xc", _dp), ("gx", _dp), ("gy", _dp), ("gz", _dp), ("dgx", _dp), ("dgy", _dp),
+                ("kx", _dp), ("ky", _dp)]
+
+
+def _leaf_sigs(lib):
+    M, P = C.POINTER(or_leaf_mesh), C.POINTER(or_params)
+    lib.orl_step.restype = C.c_int
+    lib.orl_step.argtypes = [M, P, C.c_double, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]
+    lib.orl_jacobian.restype = None
+    lib.orl_jacobian.argtypes = [M, P, C.c_double, _dp, _dp, _dp, C.c_double, _dp]
+    lib.orl_adjoint_step.restype = None
+    lib.orl_adjoint_step.argtypes = [M, P, C.c_double, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp,
+                                     _dp]
+    lib.orl_dstep_dphi.restype = None
+    lib.orl_dstep_dphi.argtypes = [M, P, C.c_double, _dp, _dp, _dp, C.c_double, _dp]
+    lib.orl_slopes.restype = None
+    lib.orl_slopes.argtypes = [M, P, _dp, _dp, _dp, _dp]
+
+
+class LeafOracle(Oracle):
+    """FP64 CPU restatement bound to one adaptive-mesh problem (leaf tables:
+    nbr / nproc (n, 8), gen (n,), cdx / xc (n, 2))."""
+
+    def __init__(self, nbr, nproc, gen, cdx, xc, grav, dgrav, curv, phi_int_deg, phi_bed_deg,
+                 epsilon=1.0, gscale=1.0, tiny=1e-4, gamma=1.0):
+        self.lib = load()
+        _leaf_sigs(self.lib)
+        self.n = len(gen)
+        ints = [np.ascontiguousarray(a, dtype=np.intc).reshape(-1) for a in (nbr, nproc, gen)]
+        dbl = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in
+               (cdx, xc, grav[0], grav[1], grav[2], dgrav[0], dgrav[1], curv[0], curv[1])]
+        self._keep = ints + dbl
+        m = or_leaf_mesh()
+        m.n = self.n
+        m.nbr, m.nproc, m.gen = [a.ctypes.data_as(_ip) for a in ints]
+        (m.cdx, m.xc, m.gx, m.gy, m.gz, m.dgx, m.dgy, m.kx, m.ky) = [_p(a) for a in dbl]
+        self.mesh = m
+        self.tiny, self.gamma, self.eps, self.gscale = tiny, gamma, epsilon, gscale
+        self.set_angles(phi_int_deg, phi_bed_deg)
+        self.time = 0.0
+
+    @classmethod
+    def from_problem(cls, pb):
+        t = pb.table
+        return cls(t.nbr, t.nkind, t.gen, t.dx, t.xc, pb.grav, pb.dgrav, pb.curv,
+                   pb.phi_int_deg, pb.phi_bed_deg, pb.epsilon, pb.gravity_scale)
+
+    def step(self, U, s, dt=None):
+        h, hu, hv = U
+        t = C.c_double(self.time)
+        dto, mass = C.c_double(), C.c_double()
+        rc = self.lib.orl_step(C.byref(self.mesh), C.byref(self.params), frict_tiny(s), _p(h),
+                               _p(hu), _p(hv), -1.0 if dt is None else float(dt), C.byref(t),
+                               C.byref(dto), C.byref(mass))
+        if rc:
+            raise FloatingPointError("oracle: non-finite state")
+        self.time = t.value
+        return dto.value, mass.value
+
+    def compute_dt(self, U):
+        raise NotImplementedError
+
+    def slopes(self, U):
+        d = np.empty(6 * self.n)
+        self.lib.orl_slopes(C.byref(self.mesh), C.byref(self.params), _p(U[0]), _p(U[1]),
+                            _p(U[2]), _p(d))
+        return d.reshape(self.n, 6)
+
+    def jacobian(self, U, s, dt):
+        J = np.empty(self.n * 81)
+        self.lib.orl_jacobian(C.byref(self.mesh), C.byref(self.params), frict_tiny(s),
+                              _p(U[0]), _p(U[1]), _p(U[2]), dt, _p(J))
+        return J.reshape(self.n, 9, 3, 3)
+
+    def adjoint_step(self, U, s, dt, lam_next, g=None, want_dphi=False):
+        out = np.empty(3 * self.n)
+        ln = np.ascontiguousarray(lam_next, dtype=np.float64).reshape(-1)
+        gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64).reshape(-1)
+        acc = C.c_double(0.0)
+        self.lib.orl_adjoint_step(C.byref(self.mesh), C.byref(self.params), frict_tiny(s),
+                                  _p(U[0]), _p(U[1]), _p(U[2]), dt, _p(ln), _p(gg), _p(out),
+                                  C.cast(C.byref(acc), _dp) if want_dphi else None)
+        return out.reshape(3, -1), acc.value
+
+    def dstep_dphi(self, U, s, dt):
+        out = np.empty(3 * self.n)
+        self.lib.orl_dstep_dphi(C.byref(self.mesh), C.byref(self.params), frict_tiny(s),
+                                _p(U[0]), _p(U[1]), _p(U[2]), dt, _p(out))
+        return out.reshape(3, -1)

@@ -57,3 +57,50 @@ def small_flow_problem(nx=24, ny=20, nsteps=30, slope=25.0, phi_bed=15.0):
     return analytic_problem(nx, ny, 10.0, slope_x_deg=slope, slope_y_deg=3.0,
                             h=paraboloid(0.35 * L, 0.5 * ny * 10.0, 60.0, 45.0, 8.0),
                             phi_int=32.0, phi_bed=phi_bed, nsteps=nsteps, curv=(1e-3, -5e-4))
+
+
+def analytic_leaf_problem(nx0, ny0, dx0, max_level, want, slope_x_deg=0.0, slope_y_deg=0.0,
+                          h=None, hu=None, hv=None, phi_int=30.0, phi_bed=20.0, nsteps=10,
+                          curv=(0.0, 0.0), checkpoint_every=50, nbuckets=None):
+    """An adaptive mesh (refine_tree(want), 2:1 balanced, flattened into the
+    element-table order) on a plane: constant gravity vector as in
+    analytic_problem; h/hu/hv callables f(X, Y) of the leaf centres."""
+    from discrete_adjoint_amd import mesh
+    from discrete_adjoint_amd.problems import LeafProblem
+    lev, ix, iy = mesh.refine_tree(nx0, ny0, max_level, want)
+    t = mesh.flatten(nx0, ny0, 0.0, 0.0, dx0, dx0, lev, ix, iy,
+                     nbuckets=nbuckets or mesh.EL_TABLE_SIZE)
+    n = len(lev)
+    base = analytic_problem(1, 1, dx0, slope_x_deg=slope_x_deg, slope_y_deg=slope_y_deg,
+                            curv=curv)
+    g = np.repeat(base.grav, n, axis=1)
+    dg = np.zeros((2, n))
+    cv = np.repeat(base.curv, n, axis=1)
+    X, Y = t.xc[:, 0], t.xc[:, 1]
+
+    def field(f):
+        return np.zeros(n) if f is None else np.ascontiguousarray(f(X, Y), dtype=np.float64)
+
+    return LeafProblem("analytic_leaves", nx0, ny0, dx0, dx0, 0.0, 0.0, t, g, dg, cv, field(h),
+                       field(hu), field(hv), phi_int, phi_bed, nsteps, checkpoint_every)
+
+
+def disc(cx, cy, r):
+    """want() of refine_tree: split cells whose centre is within r (+ one
+    cell) of (cx, cy), r in base-cell units."""
+    def want(L, I, J):
+        s = 1.0 / (1 << L)
+        return np.hypot((I + 0.5) * s - cx, (J + 0.5) * s - cy) < r + s
+    return want
+
+
+def small_leaf_flow_problem(nx0=16, ny0=12, max_level=2, nsteps=30, slope=25.0, phi_bed=15.0):
+    """The small_flow_problem pile on an adaptive mesh refined around it:
+    moving fronts cross hanging sides of both kinds."""
+    dx0 = 20.0
+    L = nx0 * dx0
+    return analytic_leaf_problem(nx0, ny0, dx0, max_level, disc(0.35 * nx0, 0.5 * ny0, 3.0),
+                                 slope_x_deg=slope, slope_y_deg=3.0,
+                                 h=paraboloid(0.35 * L, 0.5 * ny0 * dx0, 60.0, 45.0, 8.0),
+                                 phi_int=32.0, phi_bed=phi_bed, nsteps=nsteps,
+                                 curv=(1e-3, -5e-4))
