@@ -64,6 +64,15 @@ struct tg_ctx {
   int2* thin_list = nullptr;  // FP32 adjoint modes: cells gathered in FP64
   int* thin_count = nullptr;
 
+  // adaptive mesh (TG_MESH_LEAVES): leaf tables and reverse-step buffers
+  bool leaves = false;
+  LeafDev leaf{};
+  int* leaf_nbr = nullptr;
+  uint32_t* leaf_meta = nullptr;
+  double* leaf_cdx = nullptr;
+  double* leaf_xc = nullptr;
+  LeafBufs lb{};
+
   // forward record
   int rec_steps = 0;  // steps in the dt record
   int run_steps = 0;  // steps of the last forward run/replay (0 = none)
@@ -141,6 +150,15 @@ void free_fields(tg_ctx* ctx) {
   dfree(ctx->dphi_part);
   dfree(ctx->thin_list);
   dfree(ctx->thin_count);
+  dfree(ctx->leaf_nbr);
+  dfree(ctx->leaf_meta);
+  dfree(ctx->leaf_cdx);
+  dfree(ctx->leaf_xc);
+  dfree(ctx->lb.kd);
+  dfree(ctx->lb.dkd);
+  dfree(ctx->lb.jac);
+  ctx->leaf = LeafDev{};
+  ctx->leaves = false;
   dfree(ctx->inst_d);
   for (auto& t : ctx->tiles_d) dfree(t);
   dfree(ctx->red);
@@ -272,6 +290,12 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
     a.cfl_inner = ctx->cfg.cfl_inner; a.cfl_outer = ctx->cfg.cfl_outer;
     a.h_out = (double*)out[0]; a.hu_out = (double*)out[1]; a.hv_out = (double*)out[2];
     a.err = ctx->err_d;
+    if (ctx->leaves) {
+      launch_leaf_coef(a, ctx->leaf, compute_dt, ctx->stream);
+      if (compute_dt) launch_dt(a, 1, ctx->stream);
+      launch_leaf_update(a, ctx->leaf, ctx->stream);
+      return;
+    }
     use_tiles(ctx, 0, a);
     launch_coef(a, compute_dt, ctx->stream);
     if (compute_dt) launch_dt(a, ctx->ninst, ctx->stream);
@@ -301,7 +325,8 @@ void enqueue_adjoint(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* 
     a.ft = frict_tiny(ctx, s); a.tiny = ctx->cfg.tiny; a.gamma = ctx->cfg.gamma;
     a.lam_h = (double*)lout[0]; a.lam_hu = (double*)lout[1]; a.lam_hv = (double*)lout[2];
     a.dphi_part = dphi ? ctx->dphi_part : nullptr;
-    launch_adjoint(a, ctx->stream);
+    if (ctx->leaves) launch_leaf_adjoint(a, ctx->leaf, ctx->lb, ctx->stream);
+    else launch_adjoint(a, ctx->stream);
   } else if (ctx->es == 8) {
     enqueue_adjoint_m<double>(ctx, U, lnext, lout, s, dphi);
   } else {
@@ -352,7 +377,11 @@ int d2h_field(tg_ctx* ctx, const void* dev_base, long long off, long long count,
 
 int fill_stats(tg_ctx* ctx, tg_stats* st, int nst) {
   if (!st) return TG_OK;
-  if (ctx->es == 8)
+  if (ctx->leaves)
+    launch_leaf_stats(ctx->leaf,
+                      Fields{(double*)ctx->U[0], (double*)ctx->U[1], (double*)ctx->U[2]},
+                      ctx->part_mass, ctx->part_hmax, ctx->cfg.tiny, ctx->stream);
+  else if (ctx->es == 8)
     launch_stats(ctx->inst_d, ctx->tiles_d[0], ctx->nt[0],
                  Fields{(double*)ctx->U[0], (double*)ctx->U[1], (double*)ctx->U[2]},
                  ctx->part_mass, ctx->part_hmax, ctx->cfg.tiny, ctx->stream);
@@ -680,12 +709,96 @@ void band_by_xcd(std::vector<TileDev>& t) {
   t.swap(out);
 }
 
+// Checks the leaf tables of a TG_MESH_LEAVES mesh and builds the device
+// tables: slot-major neighbours and the per-leaf meta word (side kinds,
+// which_neighbor back slots).
+int leaf_tables(tg_ctx* ctx, const tg_mesh* m, std::vector<int>& nbrT,
+                std::vector<uint32_t>& meta) {
+  const int n = m->n;
+  if (n < 1 || !m->nbr || !m->nproc || !m->gen || !m->cdx || !m->xc || !m->grav || !m->dgrav ||
+      !m->curv)
+    return set_err(ctx, TG_ERR_ARG, "bad leaf mesh");
+  nbrT.assign((size_t)8 * n, -1);
+  meta.assign(n, 0u);
+  const int32_t* nb = m->nbr;
+  const int32_t* np = m->nproc;
+  auto bad = [&](int e, const char* why) {
+    return set_err(ctx, TG_ERR_ARG, std::string("leaf ") + std::to_string(e) + ": " + why);
+  };
+  for (int e = 0; e < n; e++) {
+    if (!(m->cdx[2 * e] > 0) || !(m->cdx[2 * e + 1] > 0)) return bad(e, "cell size <= 0");
+    uint32_t kinds = 0;
+    for (int s = 0; s < 4; s++) {
+      const int p1 = np[8 * e + s], p2 = np[8 * e + s + 4];
+      const int f1 = nb[8 * e + s], f2 = nb[8 * e + s + 4];
+      uint32_t k;
+      if (p1 == -1) {
+        if (p2 >= 0) return bad(e, "second neighbour on a boundary side");
+        k = LEAF_BOUNDARY;
+      } else if (p1 >= 0) {
+        if (f1 < 0 || f1 >= n || f1 == e) return bad(e, "neighbour index out of range");
+        if (p2 >= 0) {
+          if (f2 < 0 || f2 >= n || f2 == e || f2 == f1)
+            return bad(e, "second neighbour index out of range");
+          if (m->gen[f1] != m->gen[e] + 1 || m->gen[f2] != m->gen[e] + 1)
+            return bad(e, "two neighbours on a side must be one generation finer");
+          k = LEAF_FINER;
+          nbrT[(size_t)(s + 4) * n + e] = f2;
+        } else if (p2 == -2) {
+          if (m->gen[f1] == m->gen[e]) k = LEAF_SAME;
+          else if (m->gen[f1] == m->gen[e] - 1) k = LEAF_COARSER;
+          else return bad(e, "single neighbour more than one generation apart");
+        } else {
+          return bad(e, "bad neigh_proc of a second slot");
+        }
+        nbrT[(size_t)s * n + e] = f1;
+      } else {
+        return bad(e, "bad neigh_proc");
+      }
+      kinds |= k << (2 * s);
+    }
+    if ((kinds & 3u) == LEAF_BOUNDARY && ((kinds >> 4) & 3u) == LEAF_BOUNDARY)
+      return bad(e, "domain boundary on both y sides");
+    if (((kinds >> 2) & 3u) == LEAF_BOUNDARY && ((kinds >> 6) & 3u) == LEAF_BOUNDARY)
+      return bad(e, "domain boundary on both x sides");
+    meta[e] = kinds;
+  }
+  // back slots: where each present neighbour lists this leaf
+  for (int e = 0; e < n; e++)
+    for (int k = 0; k < 8; k++) {
+      const uint32_t ks = (meta[e] >> (2 * (k & 3))) & 3u;
+      const bool present = (k < 4) ? ks != LEAF_BOUNDARY : ks == LEAF_FINER;
+      if (!present) continue;
+      const int f = nbrT[(size_t)k * n + e];
+      int w = -1;
+      for (int i = 0; i < 8 && w < 0; i++)
+        if (nb[8 * f + i] == e && np[8 * f + i] >= 0) w = i;
+      if (w < 0) return bad(e, "a neighbour does not list the leaf back");
+      meta[e] |= (uint32_t)w << (8 + 3 * k);
+    }
+  return TG_OK;
+}
+
+long long cells_of(const tg_mesh* m) {
+  return m->kind == TG_MESH_LEAVES ? (long long)m->n : (long long)m->nx * m->ny;
+}
+
 int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* params) {
   if (nm < 1 || !meshes) return set_err(ctx, TG_ERR_ARG, "no members");
-  for (int b = 0; b < nm; b++) {
+  const bool leaves = meshes[0].kind == TG_MESH_LEAVES;
+  std::vector<int> nbrT;
+  std::vector<uint32_t> meta;
+  if (leaves) {
+    if (nm != 1) return set_err(ctx, TG_ERR_UNSUPPORTED, "TG_MESH_LEAVES takes one member");
+    if (ctx->mode != TG_FP64)
+      return set_err(ctx, TG_ERR_UNSUPPORTED, "TG_MESH_LEAVES is built for TG_FP64");
+    int rc = leaf_tables(ctx, &meshes[0], nbrT, meta);
+    if (rc) return rc;
+  }
+  for (int b = 0; b < nm && !leaves; b++) {
     const tg_mesh* m = &meshes[b];
     if (m->kind != TG_MESH_UNIFORM)
-      return set_err(ctx, TG_ERR_UNSUPPORTED, "only TG_MESH_UNIFORM is built");
+      return set_err(ctx, TG_ERR_UNSUPPORTED, "members must all be TG_MESH_UNIFORM");
     if (m->nx < 1 || m->ny < 1 || !(m->dx > 0) || !(m->dy > 0) || !m->grav || !m->dgrav ||
         !m->curv)
       return set_err(ctx, TG_ERR_ARG, "bad mesh");
@@ -708,7 +821,16 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   ctx->inst_h.assign(nm, InstDev{});
   for (auto& t : ctx->tiles_h) t.clear();
   long long off = 0;
-  for (int b = 0; b < nm; b++) {
+  for (int b = 0; b < nm && leaves; b++) {  // one member, one thread per leaf
+    InstDev& I = ctx->inst_h[0];
+    I.nx = meshes[0].n; I.ny = 1; I.off = 0;
+    I.maxtime = I.nextoutput = I.nextsave = HUGE_VAL;
+    if (params) params_into(I, &params[0]);
+    for (int v = 0; v < NSHAPE; v++)
+      for (int x = 0; x < meshes[0].n; x += BLOCK) ctx->tiles_h[v].push_back(TileDev{0, x, 0, 0});
+    off = meshes[0].n;
+  }
+  for (int b = 0; b < nm && !leaves; b++) {
     const tg_mesh* m = &meshes[b];
     InstDev& I = ctx->inst_h[b];
     I.nx = m->nx; I.ny = m->ny; I.off = off;
@@ -727,7 +849,7 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   ctx->ncell = off;
   ctx->ntiles = 0;
   for (int v = 0; v < NSHAPE; v++) {
-    if (SHAPE_XCD[v]) band_by_xcd(ctx->tiles_h[v]);
+    if (SHAPE_XCD[v] && !leaves) band_by_xcd(ctx->tiles_h[v]);
     ctx->nt[v] = (int)ctx->tiles_h[v].size();
     if (ctx->nt[v] > ctx->ntiles) ctx->ntiles = ctx->nt[v];
   }
@@ -758,6 +880,15 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   if (!rc) rc = dalloc(ctx, (void**)&ctx->red, sizeof(RedDev) * nm);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->time_d, sizeof(double) * nm);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->dt_rec, sizeof(double) * nm * ctx->cfg.max_steps);
+  if (leaves) {
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->leaf_nbr, sizeof(int) * 8 * (size_t)n);
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->leaf_meta, sizeof(uint32_t) * (size_t)n);
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->leaf_cdx, sizeof(double) * 2 * (size_t)n);
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->leaf_xc, sizeof(double) * 2 * (size_t)n);
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->lb.kd, sizeof(double) * (size_t)n);
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->lb.dkd, sizeof(double) * (size_t)n);
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->lb.jac, sizeof(double) * 81 * (size_t)n);
+  }
   if (rc) {
     free_fields(ctx);
     return rc;
@@ -765,7 +896,7 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   // static coefficients: field k of member b at statics[k*n + off_b]
   for (int b = 0; b < nm && !rc; b++) {
     const tg_mesh* m = &meshes[b];
-    const long long nb = (long long)m->nx * m->ny, o = ctx->inst_h[b].off;
+    const long long nb = cells_of(m), o = ctx->inst_h[b].off;
     const double* src[7] = {m->grav, m->grav + nb, m->grav + 2 * nb, m->dgrav, m->dgrav + nb,
                             m->curv, m->curv + nb};
     for (int k = 0; k < 7 && !rc; k++) {
@@ -776,6 +907,27 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     }
   }
   if (rc) return rc;
+  if (leaves) {
+    // SoA copies of the leaf geometry: x at [c], y at [n + c]
+    const tg_mesh* m = &meshes[0];
+    std::vector<double> cdx(2 * (size_t)n), xc(2 * (size_t)n);
+    for (long long c = 0; c < n; c++) {
+      cdx[c] = m->cdx[2 * c];
+      cdx[n + c] = m->cdx[2 * c + 1];
+      xc[c] = m->xc[2 * c];
+      xc[n + c] = m->xc[2 * c + 1];
+    }
+    HIPCHK(ctx, hipMemcpy(ctx->leaf_nbr, nbrT.data(), sizeof(int) * nbrT.size(),
+                          hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->leaf_meta, meta.data(), sizeof(uint32_t) * meta.size(),
+                          hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->leaf_cdx, cdx.data(), sizeof(double) * cdx.size(),
+                          hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->leaf_xc, xc.data(), sizeof(double) * xc.size(),
+                          hipMemcpyHostToDevice));
+    ctx->leaf = LeafDev{(int)n, ctx->leaf_nbr, ctx->leaf_meta, ctx->leaf_cdx, ctx->leaf_xc};
+    ctx->leaves = true;
+  }
   for (int v = 0; v < NSHAPE; v++)
     HIPCHK(ctx, hipMemcpyAsync(ctx->tiles_d[v], ctx->tiles_h[v].data(),
                                sizeof(TileDev) * ctx->nt[v], hipMemcpyHostToDevice, ctx->stream));
@@ -1135,8 +1287,14 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
       a.ft = frict_tiny(ctx, 0); a.tiny = ctx->cfg.tiny; a.gamma = ctx->cfg.gamma;
       a.h_out = (double*)ctx->Ut[0]; a.hu_out = (double*)ctx->Ut[1]; a.hv_out = (double*)ctx->Ut[2];
       a.err = ctx->err_d;
-      if (w == 0) launch_coef(a, false, ctx->stream);
-      else launch_update(a, ctx->stream);
+      if (ctx->leaves) {
+        if (w == 0) launch_leaf_coef(a, ctx->leaf, false, ctx->stream);
+        else launch_leaf_update(a, ctx->leaf, ctx->stream);
+      } else if (w == 0) {
+        launch_coef(a, false, ctx->stream);
+      } else {
+        launch_update(a, ctx->stream);
+      }
     } else if (ctx->es == 8) {
       StepArgsM<double> a = step_args_m<double>(ctx, ctx->U, ctx->Ut, 0);
       use_tiles(ctx, w, a);
@@ -1180,6 +1338,15 @@ int tg_autotune(tg_ctx* ctx, const tg_params* prm, int reps, int* variant_out, d
     return set_err(ctx, TG_ERR_STATE, "needs a completed forward run");
   int rc = upload_params(ctx, prm);
   if (rc) return rc;
+  if (ctx->leaves) {  // one thread per leaf: no tile variants to choose from
+    for (int w = 0; w < 3; w++) {
+      ctx->sel[w] = 0;
+      if (variant_out) variant_out[w] = 0;
+      if (ms_out)
+        for (int v = 0; v < NSHAPE; v++) ms_out[w * NSHAPE + v] = 0.;
+    }
+    return TG_OK;
+  }
   // the adjoint variant reads lam[0]: give it defined values (zeros)
   const size_t fb = (size_t)ctx->es * ctx->ncell;
   for (int c = 0; c < 3; c++) HIPCHK(ctx, hipMemsetAsync(ctx->lam[0][c], 0, fb, ctx->stream));

@@ -174,6 +174,37 @@ void launch_adjoint_m(const AdjArgsM<S>& a, hipStream_t s);
 void launch_stats32(const InstDev* inst, const TileDev* tiles, int ntiles, FieldsT<float> U,
                     double* part_mass, double* part_hmax, double tiny, hipStream_t s);
 
+// Adaptive mesh (flattened leaf tables), one thread per leaf.  Sides 0 S,
+// 1 E, 2 N, 3 W; slot k < 4 the neighbour on side k, slot k + 4 the second
+// neighbour of side k.
+struct LeafDev {
+  int n;
+  const int* nbr;        // [8n] slot-major: nbr[k * n + c]
+  // per leaf: bits 2s..2s+1 the kind of side s (LEAF_*); bits 8+3k..10+3k
+  // the slot of this leaf in the table of its slot-k neighbour
+  // (Element::which_neighbor, element2.C:574-583)
+  const uint32_t* meta;
+  const double* cdx;     // [2n] cell size: dx at [c], dy at [n + c]
+  const double* xc;      // [2n] centre: x at [c], y at [n + c]
+};
+constexpr uint32_t LEAF_BOUNDARY = 0;  // domain boundary (neigh_proc INIT)
+constexpr uint32_t LEAF_SAME = 1;      // one neighbour of the same generation
+constexpr uint32_t LEAF_COARSER = 2;   // one coarser neighbour (node S_S_CON)
+constexpr uint32_t LEAF_FINER = 3;     // two finer neighbours (node S_C_CON)
+
+// Reverse-step buffers of the leaf path.
+struct LeafBufs {
+  double* kd;    // [n] kact from the dual slopes
+  double* dkd;   // [n] d kact / d phi_bed
+  double* jac;   // [81n]: the 3x3 block of slot m of leaf c at jac + (m * n + c) * 9
+};
+
+void launch_leaf_coef(const StepArgs& a, const LeafDev& L, bool reduce, hipStream_t s);
+void launch_leaf_update(const StepArgs& a, const LeafDev& L, hipStream_t s);
+void launch_leaf_adjoint(const AdjArgs& a, const LeafDev& L, const LeafBufs& B, hipStream_t s);
+void launch_leaf_stats(const LeafDev& L, Fields U, double* part_mass, double* part_hmax,
+                       double tiny, hipStream_t s);
+
 void launch_coef(const StepArgs& a, bool reduce, hipStream_t s);
 void launch_dt(const StepArgs& a, int ninst, hipStream_t s);
 void launch_update(const StepArgs& a, hipStream_t s);

@@ -396,4 +396,61 @@ __device__ __forceinline__ M3 dual_riemann_side(const DualHfv& l, const DualHfv&
   return j;
 }
 
+// ---- bed-friction gradient (no reference counterpart) ----------------------
+
+// d(flux)/dk for the forward physical flux of one side
+__device__ __forceinline__ void dhfv_dk(int dir, double h, double k, double g3, double tiny,
+                                        double dF[3], double dW[3]) {
+  dF[0] = dF[1] = dF[2] = dW[0] = dW[1] = dW[2] = 0.;
+  if (h < tiny) return;
+  double a = sqrt(k * h * g3);
+  double da = (a > 0.) ? (h * g3) / (2. * a) : 0.;
+  if (dir == 0) dF[1] = 0.5 * h * g3 * h;
+  else dF[2] = 0.5 * h * g3 * h;
+  dW[0] = -da;
+  dW[2] = da;
+}
+
+// d(riemannflux)/dk along (dk_l, dk_r) carried in dFl/dWl, dFr/dWr
+__device__ __forceinline__ void driemann(const Hfv& l, const Hfv& r, const double dFl[3],
+                                         const double dWl[3], const double dFr[3],
+                                         const double dWr[3], double out[3]) {
+  out[0] = out[1] = out[2] = 0.;
+  if (l.u[0] == 0. && r.u[0] == 0.) return;
+  double sl, sr, dsl, dsr;
+  if (l.u[0] == 0.) {
+    double p = 2.0 * r.w[0] - r.w[1], q = 2.0 * r.w[2] - r.w[1];
+    sl = umin(0, p); dsl = (0 > p) ? 2.0 * dWr[0] - dWr[1] : 0.;
+    sr = umax(0, q); dsr = (0 > q) ? 0. : 2.0 * dWr[2] - dWr[1];
+    if (!(sr > 0)) dsr = 0.;
+  } else if (r.u[0] == 0.) {
+    double p = 2.0 * l.w[0] - l.w[1], q = 2.0 * l.w[2] - l.w[1];
+    sl = umin(0, p); dsl = (0 > p) ? 2.0 * dWl[0] - dWl[1] : 0.;
+    sr = umax(0, q); dsr = (0 > q) ? 0. : 2.0 * dWl[2] - dWl[1];
+    if (!(sr > 0)) dsr = 0.;
+  } else {
+    double mi = umin(l.w[0], r.w[0]);
+    double dmi = (l.w[0] > r.w[0]) ? dWr[0] : dWl[0];
+    sl = umin(0, mi); dsl = (0 > mi) ? dmi : 0.;
+    double ma = umax(l.w[2], r.w[2]);
+    double dma = (l.w[2] > r.w[2]) ? dWl[2] : dWr[2];
+    sr = umax(0, ma); dsr = (0 > ma) ? 0. : dma;
+    if (!(sr > 0)) dsr = 0.;
+  }
+  if (sl >= 0.0) {
+    for (int v = 0; v < 3; v++) out[v] = dFl[v];
+  } else if (sr <= 0.0) {
+    for (int v = 0; v < 3; v++) out[v] = dFr[v];
+  } else {
+    double inv = 1. / (sr - sl);
+    for (int v = 0; v < 3; v++) {
+      double vd = r.u[v] - l.u[v];
+      double f = (sr * l.f[v] - sl * r.f[v] + sl * sr * vd) * inv;
+      double dfdsr = (l.f[v] + sl * vd - f) * inv;
+      double dfdsl = (-r.f[v] + sr * vd + f) * inv;
+      out[v] = dfdsr * dsr + dfdsl * dsl + sr * inv * dFl[v] - sl * inv * dFr[v];
+    }
+  }
+}
+
 }  // namespace tg

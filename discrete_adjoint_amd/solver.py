@@ -43,7 +43,8 @@ def params_from_angles(phi_int_deg, phi_bed_deg, epsilon=1.0, gravity_scale=1.0)
 
 
 class Simulation:
-    """One uniform-block problem on one GPU."""
+    """One problem (a uniform block, or an adaptive mesh given as leaf
+    tables) on one GPU."""
 
     def __init__(self, problem, precision=_lib.TG_FP64, checkpoint_every=None, max_steps=None,
                  device=0, **cfg_overrides):
@@ -57,9 +58,20 @@ class Simulation:
         self._check(self.lib.tg_create(C.byref(self.cfg), C.byref(self.ctx)), nocontext=True)
         self._arrays = []
         m = _lib.tg_mesh()
-        m.kind = _lib.TG_MESH_UNIFORM
-        m.nx, m.ny, m.dx, m.dy, m.x0, m.y0 = (problem.nx, problem.ny, problem.dx, problem.dy,
-                                              problem.x0, problem.y0)
+        if hasattr(problem, "table"):  # adaptive mesh: the flattened leaf tables
+            t = problem.table
+            tabs = [np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+                    for a in (t.nbr, t.nkind, t.gen)]
+            geo = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (t.dx, t.xc)]
+            self._arrays += tabs + geo
+            m.kind = _lib.TG_MESH_LEAVES
+            m.n = problem.n
+            m.nbr, m.nproc, m.gen = [_lib.iptr(a) for a in tabs]
+            m.cdx, m.xc = [_lib.dptr(a) for a in geo]
+        else:
+            m.kind = _lib.TG_MESH_UNIFORM
+            m.nx, m.ny, m.dx, m.dy, m.x0, m.y0 = (problem.nx, problem.ny, problem.dx, problem.dy,
+                                                  problem.x0, problem.y0)
         g = np.ascontiguousarray(problem.grav.reshape(-1))
         dg = np.ascontiguousarray(problem.dgrav.reshape(-1))
         cv = np.ascontiguousarray(problem.curv.reshape(-1))
