@@ -196,7 +196,9 @@ constexpr uint32_t LEAF_FINER = 3;     // two finer neighbours (node S_C_CON)
 struct LeafBufs {
   double* kd;    // [n] kact from the dual slopes
   double* dkd;   // [n] d kact / d phi_bed
-  double* jac;   // [81n]: the 3x3 block of slot m of leaf c at jac + (m * n + c) * 9
+  // [81n] gather terms: record slot m of leaf c at jac + (m * n + c) * 9 holds
+  // lambda[l] * J[l][k] of the block leaf c adds in its slot m
+  double* jac;
 };
 
 void launch_leaf_coef(const StepArgs& a, const LeafDev& L, bool reduce, hipStream_t s);

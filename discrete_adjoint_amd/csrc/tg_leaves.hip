@@ -13,8 +13,9 @@
 // Forward step: k_leaf_coef (slopes, kact, CFL reduction), k_dt,
 // k_leaf_update (slopes, face fluxes, update_states).  Reverse step:
 // k_leaf_dualk (kact of the dual slopes), k_leaf_jac (the nine Jacobian
-// blocks of calc_jacobian, stored), k_leaf_gather (calc_adjoint's gather
-// through which_neighbor).
+// blocks of calc_jacobian, each times lambda^{s+1} of its leaf, stored where
+// the leaf that gathers it reads), k_leaf_dphi (bed-friction gradient term,
+// when asked for), k_leaf_gather (calc_adjoint's sum, in slot order).
 #include "tg_kernels.h"
 #include "tg_math.h"
 
@@ -311,11 +312,6 @@ __device__ __forceinline__ DualHfv leaf_dual_hfv(const AdjArgs& a, const double*
   return dual_hfv(dir, a.U.h[c], a.U.hu[c], a.U.hv[c], kd[c], a.S.gz[c], a.tiny);
 }
 
-// FluxJac entries of one direction (side x or y): [minus/plus][self/n1/n2]
-struct FluxJacDir {
-  M3 j[2][3];
-};
-
 __device__ __forceinline__ M3 half_sum(const M3& x, const M3& y) {
   M3 r;
 #pragma unroll
@@ -325,52 +321,36 @@ __device__ __forceinline__ M3 half_sum(const M3& x, const M3& y) {
   return r;
 }
 
-__device__ __forceinline__ FluxJacDir flux_jac_dir(const AdjArgs& a, const LeafDev& L,
-                                                   const double* kd, int c, uint32_t meta,
-                                                   int dir) {
-  FluxJacDir F;
-  const DualHfv me = leaf_dual_hfv(a, kd, dir, c);
-  const M3 Z = zero3();
+// The gather's terms of one block, lam[l] * J[l][k] for every (l, k),
+// stored in the record slot of the leaf that adds them.
+__device__ __forceinline__ void store_products(double* rec, int n, int slot, int leaf,
+                                               const M3& J, const double lam[3]) {
+  double* p = rec + ((size_t)slot * n + leaf) * 9;
 #pragma unroll
-  for (int pm = 0; pm < 2; pm++) {
-    const int s = pm ? plus_side(dir) : minus_side(dir);
-    const uint32_t kd_s = kind_of(meta, s);
-    if (kd_s == LEAF_BOUNDARY) continue;
-    const bool plus = pm == 1;
-    const DualHfv o1 = leaf_dual_hfv(a, kd, dir, L.nbr[s * L.n + c]);
-    // d HLL / d U of this leaf and of the neighbour (left = minus side)
-    const M3 self1 = plus ? dual_riemann_side(me, o1, true) : dual_riemann_side(o1, me, false);
-    const M3 nb1 = plus ? dual_riemann_side(me, o1, false) : dual_riemann_side(o1, me, true);
-    if (kd_s == LEAF_FINER) {
-      const DualHfv o2 = leaf_dual_hfv(a, kd, dir, L.nbr[(s + 4) * L.n + c]);
-      const M3 self2 =
-          plus ? dual_riemann_side(me, o2, true) : dual_riemann_side(o2, me, false);
-      const M3 nb2 = plus ? dual_riemann_side(me, o2, false) : dual_riemann_side(o2, me, true);
-      F.j[pm][0] = half_sum(self1, self2);
-      F.j[pm][1] = half_sum(nb1, Z);
-      F.j[pm][2] = half_sum(nb2, Z);
-    } else {
-      F.j[pm][0] = self1;
-      F.j[pm][1] = nb1;
-      F.j[pm][2] = Z;
-    }
-  }
-  // DualElem::boundary_flux (dualmesh.C:2241-2279): copy the other side
-  if (kind_of(meta, plus_side(dir)) == LEAF_BOUNDARY)
+  for (int l = 0; l < 3; l++)
 #pragma unroll
-    for (int i = 0; i < 3; i++) F.j[1][i] = F.j[0][i];
-  if (kind_of(meta, minus_side(dir)) == LEAF_BOUNDARY)
-#pragma unroll
-    for (int i = 0; i < 3; i++) F.j[0][i] = F.j[1][i];
-  return F;
+    for (int k = 0; k < 3; k++) p[l * 3 + k] = lam[l] * J.a[l][k];
 }
 
-__device__ __forceinline__ void store_block(double* jac, int n, int m, int c, const M3& J) {
-  double* p = jac + ((size_t)m * n + c) * 9;
+// Dual slopes of one leaf (d = (hx, hux, hvx, hy, huy, hvy)) and the h
+// slope sensitivities.
+__device__ __forceinline__ void dual_slopes(const AdjArgs& a, const LeafDev& L, int c,
+                                            uint32_t meta, double d[6], double hs[2][5]) {
 #pragma unroll
-  for (int i = 0; i < 3; i++)
+  for (int k = 0; k < 6; k++) d[k] = 0.;
 #pragma unroll
-    for (int k = 0; k < 3; k++) p[i * 3 + k] = J.a[i][k];
+  for (int k = 0; k < 5; k++) hs[0][k] = hs[1][k] = 0.;
+  if (touches_boundary(meta)) return;
+#pragma unroll
+  for (int dir = 0; dir < 2; dir++) {
+    const DirNb q = dir_nb(L, c, meta, dir);
+    const LSlope sh = lslope_dual(a.U.h, c, q, a.gamma, a.tiny, true);
+    d[3 * dir] = sh.d;
+#pragma unroll
+    for (int k = 0; k < 5; k++) hs[dir][k] = sh.s[k];
+    d[3 * dir + 1] = lslope_dual(a.U.hu, c, q, a.gamma, a.tiny, false).d;
+    d[3 * dir + 2] = lslope_dual(a.U.hv, c, q, a.gamma, a.tiny, false).d;
+  }
 }
 
 // Lambda^{s+1}_c . d U^{s+1}_c / d phi_bed at frozen dt (bed-friction
@@ -439,163 +419,206 @@ __device__ double leaf_dphi(const AdjArgs& a, const LeafDev& L, const LeafBufs& 
   return a.lam_next.h[c] * out[0] + a.lam_next.hu[c] * out[1] + a.lam_next.hv[c] * out[2];
 }
 
+
+// What the blocks of one leaf share (calc_jacobian's per-element values).
+struct JacCtx {
+  int c, n;
+  uint32_t meta;
+  double h, dt, dtdx, dtdy, alpha, betta, g2;
+  bool wet;
+  int stop0, stop1;
+  double lam[3];
+};
+
+// One neighbour block (set_fluxes_hsens + calc_jacobian_elem for
+// effelement = k + 1) from its FluxJac entry E: the own side's entry, and
+// the opposite side's -- its copy, so E again -- only when that side is a
+// domain boundary.  Stored as gather terms in the neighbour's record.
+__device__ __forceinline__ void emit_block(const AdjArgs& a, const LeafDev& L, const LeafBufs& B,
+                                           const JacCtx& q, int dir, bool is_plus, bool opp_bnd,
+                                           int k, int nb, const M3& E, double dh) {
+  const double zero = 0.;
+  M3 J;
+  if (a.U.h[nb] == 0.) {  // void_neigh_elem (calc_jacobain.C:155-166)
+    J = zero3();
+  } else {
+    const double dh0 = dir ? 0. : dh, dh1 = dir ? dh : 0.;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        const double e = E.a[i][r];
+        const double pp = is_plus ? e : (opp_bnd ? e : zero);
+        const double pn = is_plus ? (opp_bnd ? e : zero) : e;
+        J.a[i][r] = dir ? (+q.dtdx * (zero - zero) + q.dtdy * (pp - pn))
+                        : (+q.dtdx * (pp - pn) + q.dtdy * (zero - zero));
+      }
+    if (q.wet) {
+      if (!q.stop0) J.a[1][0] -= -q.dt * q.h * q.alpha * q.g2 * dh1;
+      if (!q.stop1) J.a[2][0] -= -q.dt * q.h * q.betta * q.g2 * dh0;
+    }
+  }
+  store_products(B.jac, q.n, back_of(q.meta, k) + 1, nb, J, q.lam);
+}
+
+// One side of one direction: the HLL Jacobians of its face(s) (Case I / II /
+// III of DualElem::calc_flux), its neighbour blocks emitted, and its
+// FluxJac self entry returned.
+__device__ __forceinline__ M3 side_pass(const AdjArgs& a, const LeafDev& L, const LeafBufs& B,
+                                        const JacCtx& q, const DualHfv& me, int dir, bool is_plus,
+                                        bool opp_bnd, const double hs[5]) {
+  const int s = is_plus ? plus_side(dir) : minus_side(dir);
+  const int f1 = L.nbr[s * q.n + q.c];
+  M3 self1, nb1;
+  {
+    const DualHfv o1 = leaf_dual_hfv(a, B.kd, dir, f1);
+    if (is_plus) dual_riemann_both(me, o1, self1, nb1);
+    else dual_riemann_both(o1, me, nb1, self1);
+  }
+  const double dh1 = hs[is_plus ? 2 : 1], dh2 = hs[is_plus ? 4 : 3];
+  if (kind_of(q.meta, s) != LEAF_FINER) {
+    emit_block(a, L, B, q, dir, is_plus, opp_bnd, s, f1, nb1, dh1);
+    return self1;
+  }
+  const M3 Z = zero3();
+  emit_block(a, L, B, q, dir, is_plus, opp_bnd, s, f1, half_sum(nb1, Z), dh1);
+  const int f2 = L.nbr[(s + 4) * q.n + q.c];
+  M3 self2, nb2;
+  {
+    const DualHfv o2 = leaf_dual_hfv(a, B.kd, dir, f2);
+    if (is_plus) dual_riemann_both(me, o2, self2, nb2);
+    else dual_riemann_both(o2, me, nb2, self2);
+  }
+  emit_block(a, L, B, q, dir, is_plus, opp_bnd, s + 4, f2, half_sum(nb2, Z), dh2);
+  return half_sum(self1, self2);
+}
+
 // The nine Jacobian blocks of one leaf (calc_jacobian, calc_jacobain.C:40-153;
-// set_fluxes_hsens :187-330; calc_jacobian_elem, compute_jacobian_elem.C:15-121),
-// stored for the gather; blocks of absent slots are not written (the gather
-// does not read them).
+// set_fluxes_hsens :187-330; calc_jacobian_elem, compute_jacobian_elem.C:15-121).
+// calc_adjoint reads block m of this leaf only from the leaf in slot m - 1,
+// where this leaf sits in slot which_neighbor + 1: each block is multiplied
+// by this leaf's lambda^{s+1} here and its nine products are stored in that
+// reader's record slot, so the gather reads its own records in slot order.
 __global__ __launch_bounds__(BLOCK) void k_leaf_jac(AdjArgs a, LeafDev L, LeafBufs B) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
-  const bool act = c < L.n;
-  double dphi = 0.;
-  if (act) {
-    const InstDev I = a.inst[0];
-    const uint32_t meta = L.meta[c];
-    const int n = L.n;
-    const double h = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
-    double d[6] = {0., 0., 0., 0., 0., 0.};
-    double hs[2][5] = {{0., 0., 0., 0., 0.}, {0., 0., 0., 0., 0.}};
-    if (!touches_boundary(meta)) {
+  if (c >= L.n) return;
+  const InstDev I = a.inst[0];
+  JacCtx q;
+  q.c = c;
+  q.n = L.n;
+  q.meta = L.meta[c];
+  const double h = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
+  q.h = h;
+  q.lam[0] = a.lam_next.h[c];
+  q.lam[1] = a.lam_next.hu[c];
+  q.lam[2] = a.lam_next.hv[c];
+  double d[6], hs[2][5];
+  dual_slopes(a, L, c, q.meta, d, hs);
+  const double kact = B.kd[c];
+  const double g0 = a.S.gx[c], g1 = a.S.gy[c], g2 = a.S.gz[c];
+  const double dg0 = a.S.dgx[c], dg1 = a.S.dgy[c], cv0 = a.S.kx[c], cv1 = a.S.ky[c];
+  // orgSrcSgn and stop of update_states (they do not depend on the fluxes)
+  const SrcInfo si =
+      src_switches(h, hu, hv, d[0], d[2], d[3], d[4], cv0, cv1, I.tan_bed, g2, a.ft, a.tiny);
+  q.g2 = g2;
+  q.dt = a.dt_rec[a.step];
+  q.dtdx = q.dt / L.cdx[c];
+  q.dtdy = q.dt / L.cdx[q.n + c];
+  q.wet = h > a.tiny;
+  q.alpha = I.sin_int * si.sgn[0] * kact;
+  q.betta = I.sin_int * si.sgn[1] * kact;
+  q.stop0 = si.stop[0];
+  q.stop1 = si.stop[1];
+  double tx[3][3];  // dtdx * (P_x - N_x) of the self block
 #pragma unroll
-      for (int dir = 0; dir < 2; dir++) {
-        const DirNb q = dir_nb(L, c, meta, dir);
-        const LSlope sh = lslope_dual(a.U.h, c, q, a.gamma, a.tiny, true);
-        d[3 * dir] = sh.d;
+  for (int dir = 0; dir < 2; dir++) {
+    const bool bp = kind_of(q.meta, plus_side(dir)) == LEAF_BOUNDARY;
+    const bool bm = kind_of(q.meta, minus_side(dir)) == LEAF_BOUNDARY;
+    const DualHfv me = leaf_dual_hfv(a, B.kd, dir, c);
+    M3 SP = zero3(), SM = zero3();
+    if (!bp) SP = side_pass(a, L, B, q, me, dir, true, bm, hs[dir]);
+    if (!bm) SM = side_pass(a, L, B, q, me, dir, false, bp, hs[dir]);
+    // DualElem::boundary_flux (dualmesh.C:2241-2279): copy the other side
+    if (bp) SP = SM;
+    if (bm) SM = SP;
+    if (dir == 0) {
 #pragma unroll
-        for (int k = 0; k < 5; k++) hs[dir][k] = sh.s[k];
-        d[3 * dir + 1] = lslope_dual(a.U.hu, c, q, a.gamma, a.tiny, false).d;
-        d[3 * dir + 2] = lslope_dual(a.U.hv, c, q, a.gamma, a.tiny, false).d;
-      }
-    }
-    const double kact = B.kd[c];
-    const double g0 = a.S.gx[c], g1 = a.S.gy[c], g2 = a.S.gz[c];
-    const double dg0 = a.S.dgx[c], dg1 = a.S.dgy[c], cv0 = a.S.kx[c], cv1 = a.S.ky[c];
-    // orgSrcSgn and stop of update_states (they do not depend on the fluxes)
-    const SrcInfo si = src_switches(h, hu, hv, d[0], d[2], d[3], d[4], cv0, cv1, I.tan_bed, g2,
-                                    a.ft, a.tiny);
-    const double dt = a.dt_rec[a.step];
-    const double dtdx = dt / L.cdx[c], dtdy = dt / L.cdx[n + c];
-    const bool wet = h > a.tiny;
-    const double alpha = I.sin_int * si.sgn[0] * kact;
-    const double betta = I.sin_int * si.sgn[1] * kact;
-    const double zero = 0.;
-    double tx[3][3];  // dtdx * (P_x - N_x) of the self block
+      for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int dir = 0; dir < 2; dir++) {
-      const FluxJacDir FJ = flux_jac_dir(a, L, B.kd, c, meta, dir);
-      if (dir == 0) {
+        for (int k = 0; k < 3; k++) tx[i][k] = q.dtdx * (SP.a[i][k] - SM.a[i][k]);
+    } else {
+      const double dt = q.dt, alpha = q.alpha, betta = q.betta;
+      M3 J;
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+      for (int i = 0; i < 3; i++)
 #pragma unroll
-          for (int k = 0; k < 3; k++) tx[i][k] = dtdx * (FJ.j[1][0].a[i][k] - FJ.j[0][0].a[i][k]);
-      } else {
-        M3 J;
+        for (int k = 0; k < 3; k++) J.a[i][k] = tx[i][k] + q.dtdy * (SP.a[i][k] - SM.a[i][k]);
 #pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-          for (int k = 0; k < 3; k++)
-            J.a[i][k] = tx[i][k] + dtdy * (FJ.j[1][0].a[i][k] - FJ.j[0][0].a[i][k]);
-#pragma unroll
-        for (int i = 0; i < 3; i++) J.a[i][i] -= 1.;
-        if (wet) {
-          const double vx = hu / h, vy = hv / h;
-          const double vx_sq = vx * vx, vy_sq = vy * vy, tb = I.tan_bed;
-          J.a[1][0] -= dt * (g0 - alpha * h * (2 * dg1 + g2 * hs[1][0]) - alpha * g2 * d[3]);
-          if (si.sgn[2] != 0. && !si.stop[0]) {
-            J.a[1][0] -= -dt * si.sgn[2] * tb * (g2 - vx_sq * cv0);
-            J.a[1][1] -= -dt * si.sgn[2] * tb * (2 * vx * cv0);
-            J.a[1][2] -= 0.;
-          }
-          J.a[2][0] -= dt * (g1 - betta * h * (2 * dg0 + g2 * hs[0][0]) - betta * g2 * d[0]);
-          if (si.sgn[3] != 0. && !si.stop[1]) {
-            J.a[2][0] -= -dt * si.sgn[3] * tb * (g2 - vy_sq * cv1);
-            J.a[2][1] -= 0.;
-            J.a[2][2] -= -dt * si.sgn[3] * tb * (2 * vy * cv1);
-          }
+      for (int i = 0; i < 3; i++) J.a[i][i] -= 1.;
+      if (q.wet) {
+        const double vx = hu / h, vy = hv / h;
+        const double vx_sq = vx * vx, vy_sq = vy * vy, tb = I.tan_bed;
+        J.a[1][0] -= dt * (g0 - alpha * h * (2 * dg1 + g2 * hs[1][0]) - alpha * g2 * d[3]);
+        if (si.sgn[2] != 0. && !si.stop[0]) {
+          J.a[1][0] -= -dt * si.sgn[2] * tb * (g2 - vx_sq * cv0);
+          J.a[1][1] -= -dt * si.sgn[2] * tb * (2 * vx * cv0);
+          J.a[1][2] -= 0.;
         }
-        store_block(B.jac, n, 0, c, (h == 0.) ? zero3() : J);
-      }
-      // neighbour slots of this direction: side sp (slot sp+1, sp+5) and sm
-      const int sp = plus_side(dir), sm = minus_side(dir);
-#pragma unroll
-      for (int side2 = 0; side2 < 2; side2++) {
-        const int sd = side2 ? sm : sp;
-        const bool is_plus = side2 == 0;
-        const bool opp_bnd = kind_of(meta, is_plus ? sm : sp) == LEAF_BOUNDARY;
-#pragma unroll
-        for (int idx = 1; idx <= 2; idx++) {
-          const int k = sd + 4 * (idx - 1);  // table slot (block m = k + 1)
-          if (!slot_present(meta, k)) continue;
-          const int nb = L.nbr[k * n + c];
-          M3 J;
-          if (a.U.h[nb] == 0.) {  // void_neigh_elem (calc_jacobain.C:155-166)
-            J = zero3();
-          } else {
-            // set_fluxes_hsens: the own side's entry; the opposite side's
-            // entry only when that side is a boundary (its copy)
-            M3 pm_plus, pm_minus;
-            if (is_plus) {
-              pm_plus = FJ.j[1][idx];
-              pm_minus = opp_bnd ? FJ.j[0][idx] : zero3();
-            } else {
-              pm_minus = FJ.j[0][idx];
-              pm_plus = opp_bnd ? FJ.j[1][idx] : zero3();
-            }
-            const double dh = hs[dir][is_plus ? (idx == 1 ? 2 : 4) : (idx == 1 ? 1 : 3)];
-            const double dh0 = dir ? 0. : dh, dh1 = dir ? dh : 0.;
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-              for (int q = 0; q < 3; q++)
-                J.a[i][q] = dir ? (+dtdx * (zero - zero) +
-                                   dtdy * (pm_plus.a[i][q] - pm_minus.a[i][q]))
-                                : (+dtdx * (pm_plus.a[i][q] - pm_minus.a[i][q]) +
-                                   dtdy * (zero - zero));
-            if (wet) {
-              if (!si.stop[0]) J.a[1][0] -= -dt * h * alpha * g2 * dh1;
-              if (!si.stop[1]) J.a[2][0] -= -dt * h * betta * g2 * dh0;
-            }
-          }
-          store_block(B.jac, n, k + 1, c, J);
+        J.a[2][0] -= dt * (g1 - betta * h * (2 * dg0 + g2 * hs[0][0]) - betta * g2 * d[0]);
+        if (si.sgn[3] != 0. && !si.stop[1]) {
+          J.a[2][0] -= -dt * si.sgn[3] * tb * (g2 - vy_sq * cv1);
+          J.a[2][1] -= 0.;
+          J.a[2][2] -= -dt * si.sgn[3] * tb * (2 * vy * cv1);
         }
       }
-    }
-    if (a.dphi_part) dphi = leaf_dphi(a, L, B, I, c, meta, d, si, dt, dtdx, dtdy);
-  }
-  if (a.dphi_part) {
-    __shared__ double sp[BLOCK / 64];
-    dphi = lwave_sum(dphi);
-    if (threadIdx.x % 64 == 0) sp[threadIdx.x / 64] = dphi;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int k = 1; k < BLOCK / 64; k++) dphi += sp[k];
-      a.dphi_part[blockIdx.x] += dphi;
+      store_products(B.jac, q.n, 0, c, (h == 0.) ? zero3() : J, q.lam);
     }
   }
 }
 
-// calc_adjoint_elem (calc_adjoint.C:76-157): slots 0..8 in order, each
-// neighbour's block that points back at this leaf (which_neighbor + 1)
+// lambda^{s+1}_c . d U^{s+1}_c / d phi_bed at frozen dt, per workgroup
+// partial sums accumulated over the sweep (bed-friction gradient)
+__global__ __launch_bounds__(BLOCK) void k_leaf_dphi(AdjArgs a, LeafDev L, LeafBufs B) {
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  double dphi = 0.;
+  if (c < L.n) {
+    const InstDev I = a.inst[0];
+    const uint32_t meta = L.meta[c];
+    double d[6], hs[2][5];
+    dual_slopes(a, L, c, meta, d, hs);
+    const SrcInfo si = src_switches(a.U.h[c], a.U.hu[c], a.U.hv[c], d[0], d[2], d[3], d[4],
+                                    a.S.kx[c], a.S.ky[c], I.tan_bed, a.S.gz[c], a.ft, a.tiny);
+    const double dt = a.dt_rec[a.step];
+    const double dtdx = dt / L.cdx[c], dtdy = dt / L.cdx[L.n + c];
+    dphi = leaf_dphi(a, L, B, I, c, meta, d, si, dt, dtdx, dtdy);
+  }
+  __shared__ double sp[BLOCK / 64];
+  dphi = lwave_sum(dphi);
+  if (threadIdx.x % 64 == 0) sp[threadIdx.x / 64] = dphi;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < BLOCK / 64; k++) dphi += sp[k];
+    a.dphi_part[blockIdx.x] += dphi;
+  }
+}
+
+// calc_adjoint_elem (calc_adjoint.C:76-157): the terms of slots 0..8 in
+// order, lambda_src[l] * J_src[l][k] added in l order for each k
 __global__ __launch_bounds__(BLOCK) void k_leaf_gather(AdjArgs a, LeafDev L, LeafBufs B) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   if (c >= L.n) return;
   const int n = L.n;
   const uint32_t meta = L.meta[c];
   double adj[3] = {0., 0., 0.};
-#pragma unroll 1
+#pragma unroll
   for (int m = 0; m < 9; m++) {
-    int src = c, blk = 0;
-    if (m > 0) {
-      if (!slot_present(meta, m - 1)) continue;
-      src = L.nbr[(m - 1) * n + c];
-      blk = back_of(meta, m - 1) + 1;
-    }
-    const double* J = B.jac + ((size_t)blk * n + src) * 9;
-    const double l0 = a.lam_next.h[src], l1 = a.lam_next.hu[src], l2 = a.lam_next.hv[src];
+    if (m > 0 && !slot_present(meta, m - 1)) continue;
+    const double* P = B.jac + ((size_t)m * n + c) * 9;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      adj[k] += l0 * J[0 * 3 + k];
-      adj[k] += l1 * J[1 * 3 + k];
-      adj[k] += l2 * J[2 * 3 + k];
+      adj[k] += P[0 * 3 + k];
+      adj[k] += P[1 * 3 + k];
+      adj[k] += P[2 * 3 + k];
     }
   }
   double out[3];
@@ -626,6 +649,7 @@ void launch_leaf_adjoint(const AdjArgs& a, const LeafDev& L, const LeafBufs& B, 
   const dim3 g(leaf_grid(L.n)), b(BLOCK);
   hipLaunchKernelGGL(k_leaf_dualk, g, b, 0, s, a, L, B, a.dphi_part ? 1 : 0);
   hipLaunchKernelGGL(k_leaf_jac, g, b, 0, s, a, L, B);
+  if (a.dphi_part) hipLaunchKernelGGL(k_leaf_dphi, g, b, 0, s, a, L, B);
   hipLaunchKernelGGL(k_leaf_gather, g, b, 0, s, a, L, B);
 }
 

@@ -396,6 +396,78 @@ __device__ __forceinline__ M3 dual_riemann_side(const DualHfv& l, const DualHfv&
   return j;
 }
 
+// Both one-sided Jacobians of one HLL face at once (dual_riemannflux computes
+// them together, dualmesh.C:1628-1843): the same expressions as
+// dual_riemann_side for each side, the shared intermediates evaluated once.
+__device__ __forceinline__ void dual_riemann_both(const DualHfv& l, const DualHfv& r, M3& jl,
+                                                  M3& jr) {
+  jl = zero3();
+  jr = zero3();
+  if (l.u[0] == 0. && r.u[0] == 0.) return;
+  double sl, sr, dsl_l[3], dsl_r[3], dsr_l[3], dsr_r[3];
+  if (l.u[0] == 0.) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) dsr_l[i] = dsl_l[i] = 0.;
+    sl = fmin(0., 2. * r.w[0] - r.w[1]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) dsl_r[i] = (sl == 0) ? 0. : 2 * r.sj.a[0][i] - r.sj.a[1][i];
+    sr = fmax(0., 2. * r.w[2] - r.w[1]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) dsr_r[i] = (sr == 0) ? 0. : 2 * r.sj.a[2][i] - r.sj.a[1][i];
+  } else if (r.u[0] == 0.) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) dsr_r[i] = dsl_r[i] = 0.;
+    sl = fmin(0., 2. * l.w[0] - l.w[1]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) dsl_l[i] = (sl == 0) ? 0. : 2 * l.sj.a[0][i] - l.sj.a[1][i];
+    sr = fmax(0., 2. * l.w[2] - l.w[1]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) dsr_l[i] = (sr == 0) ? 0. : 2 * l.sj.a[2][i] - l.sj.a[1][i];
+  } else {
+    sl = fmin(0., fmin(l.w[0], r.w[0]));
+    bool sl_zero = (sl == 0), sl_left = (sl == l.w[0]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      dsl_l[i] = (!sl_zero && sl_left) ? l.sj.a[0][i] : 0.;
+      dsl_r[i] = (!sl_zero && !sl_left) ? r.sj.a[0][i] : 0.;
+    }
+    sr = fmax(0., fmax(l.w[2], r.w[2]));
+    bool sr_zero = (sr == 0), sr_left = (sr == l.w[2]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      dsr_l[i] = (!sr_zero && sr_left) ? l.sj.a[2][i] : 0.;
+      dsr_r[i] = (!sr_zero && !sr_left) ? r.sj.a[2][i] : 0.;
+    }
+  }
+  if (sl >= 0.) {
+    jl = l.fj;
+  } else if (sr <= 0.) {
+    jr = r.fj;
+  } else {
+    double inv = 1. / (sr - sl), vd[3], flux[3], dfdsr[3], dfdsl[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) vd[i] = r.u[i] - l.u[i];
+#pragma unroll
+    for (int i = 0; i < 3; i++) flux[i] = (sr * l.f[i] - sl * r.f[i] + sl * sr * vd[i]) * inv;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      dfdsr[i] = (l.f[i] + sl * vd[i] - flux[i]) * inv;
+      dfdsl[i] = (-r.f[i] + sr * vd[i] + flux[i]) * inv;
+    }
+    const double jvl = -sr * sl * inv, dFl = sr * inv;
+    const double jvr = sr * sl * inv, dFr = -sl * inv;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        jl.a[i][k] = dfdsr[i] * dsr_l[k] + dfdsl[i] * dsl_l[k] + dFl * l.fj.a[i][k] +
+                     ((i == k) ? jvl : 0.);
+        jr.a[i][k] = dfdsr[i] * dsr_r[k] + dfdsl[i] * dsl_r[k] + dFr * r.fj.a[i][k] +
+                     ((i == k) ? jvr : 0.);
+      }
+  }
+}
+
 // ---- bed-friction gradient (no reference counterpart) ----------------------
 
 // d(flux)/dk for the forward physical flux of one side

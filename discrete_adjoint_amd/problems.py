@@ -300,3 +300,64 @@ def adaptive_problem(nx0=32, ny0=32, dx0=40.0, max_level=3, nsteps=200, refine_r
                      box=box)
     pb.meta["levels"] = np.bincount(lev, minlength=max_level + 1).tolist()
     return pb
+
+
+def adaptive_fractal_problem(nx0=128, dx0=40.0, max_level=3, radius=2600.0, npiles=2, seed=11,
+                             dem_seed=1, nsteps=200, phi_int=32.0, phi_bed=22.0,
+                             checkpoint_every=25):
+    """A fixed adaptive mesh on the seeded 1025^2 diamond-square DEM (5 m,
+    800 m relief, H = 0.8): base nx0 x nx0 cells of dx0 over the middle of
+    the raster, refined to max_level (dx0 / 2^max_level) within `radius` of
+    any pile centre, 2:1-balanced and flattened into the element-table
+    order; npiles paraboloid piles of log-uniform volume in [1e5, 1e6] m^3
+    at seeded positions.  The defaults give 0.88 M leaves (0.878 M of them
+    at 5 m around the piles, hanging sides on every level).  Static coefficients of a leaf are those of its own
+    level's uniform grid; the functional box is the 400 m square centred
+    on the first pile."""
+    from . import mesh as _mesh
+    dem_n, dem_res = 1025, 5.0
+    elev = fractal_dem(dem_n, 800.0, 0.8, dem_seed)
+    span = (dem_n - 1) * dem_res
+    L = nx0 * dx0
+    if L > span:
+        raise ValueError("domain larger than the DEM")
+    off = 0.5 * (span - L)
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(npiles):
+        v = float(np.exp(rng.uniform(math.log(1e5), math.log(1e6))))
+        r = paraboloid_radius_for_volume(v)
+        specs.append((rng.uniform(0.3, 0.7) * L, rng.uniform(0.3, 0.7) * L, r, r, 0.0, 0.1 * r))
+
+    def want(Lv, I, J):
+        s = dx0 / (1 << Lv)
+        X, Y = (I + 0.5) * s, (J + 0.5) * s
+        near = np.zeros(X.shape, bool)
+        for (cx, cy, *_r) in specs:
+            near |= np.hypot(X - cx, Y - cy) < radius + s
+        return near
+
+    lev, ix, iy = _mesh.refine_tree(nx0, nx0, max_level, want)
+    t = _mesh.flatten(nx0, nx0, 0.0, 0.0, dx0, dx0, lev, ix, iy)
+    n = len(lev)
+    grav, dgrav, curv = np.zeros((3, n)), np.zeros((2, n)), np.zeros((2, n))
+    h, hu, hv = np.zeros(n), np.zeros(n), np.zeros(n)
+    o = t.order
+    dem = (dem_n, dem_n, -off, span - off, dem_res, elev)  # raster in domain coordinates
+    for Lv in range(max_level + 1):
+        sel = np.nonzero(lev[o] == Lv)[0]
+        if len(sel) == 0:
+            continue
+        nl, s = nx0 << Lv, dx0 / (1 << Lv)
+        g, dg, cv = static_coefs(dem, nl, nl, s, s, 0.5 * s, 0.5 * s)
+        hh, hhu, hhv = piles(nl, nl, s, s, 0.5 * s, 0.5 * s, specs)
+        cell = iy[o[sel]].astype(np.int64) * nl + ix[o[sel]]
+        grav[:, sel], dgrav[:, sel], curv[:, sel] = g[:, cell], dg[:, cell], cv[:, cell]
+        h[sel], hu[sel], hv[sel] = hh[cell], hhu[cell], hhv[cell]
+    cx, cy = specs[0][0], specs[0][1]
+    box = (cx - 200.0, cx + 200.0, cy - 200.0, cy + 200.0)
+    pb = LeafProblem(f"adaptive_fractal_{n}", nx0, nx0, dx0, dx0, 0.0, 0.0, t, grav, dgrav, curv,
+                     h, hu, hv, phi_int, phi_bed, nsteps, checkpoint_every, box=box)
+    pb.meta["levels"] = np.bincount(lev, minlength=max_level + 1).tolist()
+    pb.meta["piles"] = specs
+    return pb
