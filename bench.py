@@ -1,7 +1,8 @@
 """Benchmark of the hot path: forward finite-volume steps + the discrete
 adjoint sweep (checkpointed, recomputed in reverse) on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--members 256] [--workload ensemble|grid]
+    python bench.py [--steps K] [--warmup W] [--members 256]
+                    [--workload ensemble|grid|adaptive]
 
 Default workload ("ensemble"): 256 independent members advanced together in
 every launch.  Member b is a uniform side x side block (side drawn from
@@ -24,6 +25,11 @@ reverse sweep (synchronised on both sides, inputs resident on the GPU).
 "grid" workload: one 2048 x 2048 block (2.5 m) with three piles, 1500 steps,
 mixed precision against FP64 (L2 / Linf errors of h, hu, hv and the
 functional).
+
+"adaptive" workload: the FP64 leaf-table path on a fixed adaptive mesh of
+0.88 M leaves (40 m base cells refined to 5 m around two piles on the
+fractal DEM), 200 steps + adjoint.  The ensemble run also reports this leg
+(config.adaptive_leaves_fp64) unless --no-adaptive.
 
 Prints one JSON line.  Diagnostics go to stderr.
 """
@@ -318,6 +324,10 @@ def bench_ensemble(args):
                          f"cell-steps/s"}
         log(f"cpu baseline: fwd {tf:.2f}s adj {ta:.2f}s over {cs} cell-steps")
 
+    adaptive = None
+    if not args.product_only and not args.no_adaptive:
+        adaptive = adaptive_leg(graph=bool(graph))[0]
+
     return {
         "metric": METRIC,
         "value": value,
@@ -357,6 +367,7 @@ def bench_ensemble(args):
             "stored_trajectory_bytes": store,
             "tile_variants": {"coef_update_adjoint": m["tiles"],
                               "shapes": [tile_shape(v) for v in m["tiles"]]},
+            "adaptive_leaves_fp64": adaptive,
         },
         "roofline": {
             "bound": "hbm",
@@ -369,6 +380,102 @@ def bench_ensemble(args):
             "algorithmic_bytes_per_cell": ab[dom],
             "avg_launch_ms": kt[dom],
         },
+        "cpu_baseline": cpu,
+    }
+
+
+LEAF_TABLE_BYTES = 40  # per leaf and kernel: nbr 8 x 4 + side kinds 4 + generation 4
+
+
+def adaptive_leg(K=200, k=25, reps=20, graph=True, cpu_steps=0):
+    """The adaptive-mesh (leaf table) FP64 path on the C1-size mesh: about
+    0.88 M leaves (5 m around two piles, 40 m base cells, three levels of
+    2:1-balanced refinement) on the seeded fractal DEM; K forward steps +
+    adjoint with dJ/dphi_bed, checkpoints every k.  Returns the figures
+    (wall-clock cell-steps/s, per-kernel launch times and algorithmic GB/s:
+    the uniform FP64 bytes plus the leaf tables)."""
+    import numpy as np
+
+    from discrete_adjoint_amd import _lib, problems
+    from discrete_adjoint_amd.solver import Simulation
+
+    t0 = time.perf_counter()
+    pb = problems.adaptive_fractal_problem(nsteps=K, checkpoint_every=k)
+    log(f"adaptive setup {time.perf_counter() - t0:.1f}s: {pb.n} leaves, levels "
+        f"{pb.meta['levels']}")
+    w = pb.box_weights()
+    h0 = (pb.h, pb.hu, pb.hv)
+    with Simulation(pb, checkpoint_every=k, max_steps=K, use_graph=int(graph)) as sim:
+        sim.run(K)  # untimed: capture of the sweeps, first launches
+        sim.dual_solver(w, want_lambda=False)
+        r = timed_run(sim, K, w, h0)
+        kt = {name: sim.time_kernel(i, reps) for i, name in enumerate(KERNELS)}
+    ab = algo_bytes(_lib.TG_FP64)
+    ab = {n: v + LEAF_TABLE_BYTES for n, v in ab.items()}
+    gbs = {n: ab[n] * pb.n / (kt[n] * 1e-3) / 1e9 for n in KERNELS}
+    cs = pb.n * K
+    fwd_step_ms = r["fwd_ms"] / K
+    out = {
+        "leaves": pb.n, "levels": pb.meta["levels"], "steps": K, "checkpoint_every": k,
+        "cell_steps_per_s": cs / (r["wall_f"] + r["wall_a"]),
+        "forward_cell_steps_per_s": cs / r["wall_f"],
+        "adjoint_cell_steps_per_s": cs / r["wall_a"],
+        "kernel_ms": {n: kt[n] for n in KERNELS},
+        "kernel_gbs": gbs,
+        "algorithmic_bytes_per_leaf": ab,
+        "forward_step_hbm": {"achieved": ab["forward_step"] * pb.n / (fwd_step_ms * 1e-3) / 1e9,
+                             "frac": ab["forward_step"] * pb.n / (fwd_step_ms * 1e-3) / 1e9 /
+                             HBM_PEAK_GBS},
+        "dJ_dphibed": float(r["dphi"][0]),
+    }
+    log(f"adaptive: {out['cell_steps_per_s']:.3g} cell-steps/s fwd+adj, kernels "
+        + ", ".join(f"{a}={b:.4f} ms ({gbs[a]:.0f} GB/s)" for a, b in kt.items()))
+    if cpu_steps:
+        import oracle
+        orc = oracle.LeafOracle.from_problem(pb)
+        U = [np.array(a, copy=True) for a in h0]
+        traj, dts = [], []
+        t0 = time.perf_counter()
+        for s in range(cpu_steps):
+            traj.append([a.copy() for a in U])
+            dt, _ = orc.step(U, s)
+            dts.append(dt)
+        tf = time.perf_counter() - t0
+        lam = w.reshape(3, -1)
+        t0 = time.perf_counter()
+        for s in range(cpu_steps - 1, -1, -1):
+            lam, _ = orc.adjoint_step(traj[s], s, dts[s], lam, want_dphi=True)
+        ta = time.perf_counter() - t0
+        ccs = pb.n * cpu_steps
+        out["cpu"] = {"value": ccs / (tf + ta), "unit": "cell-steps/s", "cores": 1, "kind": "port",
+                      "sample": f"first {cpu_steps} forward steps of the {pb.n}-leaf mesh and "
+                                f"their adjoint steps (oracle: FP64 C leaf-table restatement, "
+                                f"-O3, one thread); forward {ccs / tf:.3g}, adjoint "
+                                f"{ccs / ta:.3g} cell-steps/s"}
+    return out, kt, gbs, ab, r, pb
+
+
+def bench_adaptive(args):
+    K, W, k = args.steps, args.warmup, args.checkpoint_every
+    out, kt, gbs, ab, r, pb = adaptive_leg(K, k, args.kernel_reps, not args.no_graph,
+                                           0 if (args.no_cpu or args.product_only)
+                                           else args.cpu_steps)
+    dom = max(("coef", "update", "adjoint"),
+              key=lambda n: kt[n] * ((2 - 1.0 / k) if n != "adjoint" else 1))
+    wall = r["wall_f"] + r["wall_a"]
+    cpu = out.pop("cpu", None)
+    return {
+        "metric": METRIC, "value": pb.n * K / wall, "unit": "cell-steps/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": wall * 1e3 / K, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(workload=f"adaptive mesh of {pb.n} leaves (fractal DEM, two piles, "
+                                f"levels {pb.meta['levels']}), {K} forward steps + discrete "
+                                f"adjoint with dJ/dphi_bed, checkpoints every {k}",
+                       **out),
+        "roofline": {"bound": "hbm", "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs[dom] / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(dom, "adaptive882k"), "kernel": dom,
+                     "algorithmic_bytes_per_cell": ab[dom], "avg_launch_ms": kt[dom]},
         "cpu_baseline": cpu,
     }
 
@@ -488,7 +595,7 @@ def probe(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("ensemble", "grid"), default="ensemble")
+    ap.add_argument("--workload", choices=("ensemble", "grid", "adaptive"), default="ensemble")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--members", type=int, default=256)
@@ -512,6 +619,8 @@ def main():
                     help="short run for hardware-counter profiles (see probe())")
     ap.add_argument("--tiles", default="",
                     help="with --probe: tile variants coef,update,adjoint")
+    ap.add_argument("--no-adaptive", action="store_true",
+                    help="skip the adaptive-mesh leg of the ensemble run")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernel by kernel instead of replaying captured sweeps")
     args = ap.parse_args()
@@ -529,6 +638,10 @@ def main():
         args.steps = args.steps or 500
         args.checkpoint_every = args.checkpoint_every or 25
         out = bench_ensemble(args)
+    elif args.workload == "adaptive":
+        args.steps = args.steps or 200
+        args.checkpoint_every = args.checkpoint_every or 25
+        out = bench_adaptive(args)
     else:
         args.steps = args.steps or 1500
         args.checkpoint_every = args.checkpoint_every or 50
