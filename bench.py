@@ -147,22 +147,46 @@ def timed_run(sim, K, weights, h0, want_lambda=False):
                 wall_f=wall_f, wall_a=wall_a, fwd_ms=fwd_ms, adj_ms=adj_ms, rec_ms=rec_ms)
 
 
-def kernel_roofline(sim, prec, ncells, k, reps, recompute=None):
-    """Times each kernel live (HIP events on the library's stream) and returns
-    the roofline entry of the dominant one plus per-kernel figures.
-    recompute: share of steps whose forward kernels run again in the reverse
-    sweep (default: every step outside the checkpoints)."""
-    ab = algo_bytes(prec)
-    kt = {name: sim.time_kernel(i, reps) for i, name in enumerate(KERNELS)}
-    r = (1 - 1.0 / k) if recompute is None else recompute
-    per_step = {"coef": kt["coef"] * (1 + r), "update": kt["update"] * (1 + r),
-                "adjoint": kt["adjoint"]}
-    dom = max(per_step, key=per_step.get)
-    gbs = {n: ab[n] * ncells / (kt[n] * 1e-3) / 1e9 for n in KERNELS}
-    log("kernel avg ms: " + ", ".join(f"{a}={b:.4f}" for a, b in kt.items()))
-    for n in KERNELS:
-        log(f"  {n}: {gbs[n]:.0f} GB/s algorithmic ({ab[n]} B/cell)")
-    return dom, kt, gbs
+def short_bytes(prec):
+    """Bytes per cell of each kernel's short path: a cell below tiny in the
+    coefficient pass reads h only (its kact is never read); a cell of an
+    entirely dry update tile reads U and writes it unchanged; a cell of a
+    void adjoint tile reads h and lambda^{s+1} and writes lambda^s = 0."""
+    from discrete_adjoint_amd import _lib
+    es = 4 if prec == _lib.TG_FP32 else 8
+    return {"coef": es, "update": 6 * es, "adjoint": 7 * es}
+
+
+def instrumented_roofline(sim, prec, ncells, K, weights, h0, ab=None, sb=None):
+    """One more forward run + reverse sweep, launched kernel by kernel with a
+    HIP event pair around every launch (on the library's stream) and the
+    kernels counting their full-path cells.  Per kernel: summed launch time,
+    launches, and algorithmic bytes = full-path cells x the full per-cell
+    bytes + the other cells x the short-path bytes.  Returns the dominant
+    kernel (largest summed time) and per-kernel figures."""
+    ab = ab or algo_bytes(prec)
+    sb = sb or short_bytes(prec)
+    sim.instrument(True)
+    timed_run(sim, K, weights, h0)
+    ms, launches, full = sim.instrumentation()
+    sim.instrument(False)
+    out = {}
+    for i, name in enumerate(KERNELS):
+        cells = launches[i] * ncells
+        fc = cells if full[i] < 0 else full[i]
+        nbytes = fc * ab[name] + (cells - fc) * sb[name]
+        out[name] = {"ms_total": ms[i], "launches": launches[i],
+                     "avg_launch_ms": ms[i] / max(launches[i], 1),
+                     "full_path_share": fc / max(cells, 1),
+                     "bytes_per_launch": nbytes / max(launches[i], 1),
+                     "gbs": nbytes / (ms[i] * 1e-3) / 1e9 if ms[i] > 0 else None}
+        log(f"  {name}: {launches[i]} launches, avg {out[name]['avg_launch_ms']:.4f} ms, "
+            f"full path {out[name]['full_path_share']:.3f}, {out[name]['gbs']:.0f} GB/s")
+    upd = out["update"]
+    fwd_ms = ms[0] + ms[1]
+    fwd_gbs = upd["bytes_per_launch"] * upd["launches"] / (fwd_ms * 1e-3) / 1e9
+    dom = max(KERNELS, key=lambda n: out[n]["ms_total"])
+    return dom, out, fwd_gbs
 
 
 def tile_shape(v):
@@ -255,10 +279,7 @@ def bench_ensemble(args):
                 f"({r['fwd_ms']:.1f} ms kernels), reverse {r['wall_a'] * 1e3:.1f} ms wall "
                 f"({r['adj_ms']:.1f} ms kernels, recompute {r['rec_ms']:.1f})")
             if mode == prec:
-                es = 4 if prec == _lib.TG_FP32 else 8
-                kept = min(K, store // (es * 3 * ncells))
-                roof = kernel_roofline(ens, prec, ncells, k, args.kernel_reps,
-                                       recompute=(K - kept) / K * (1 - 1.0 / k))
+                roof = instrumented_roofline(ens, prec, ncells, K, weights, h0)
     # plain FP64 GPU path: one member at a time
     plain_s = 0.0
     for b, p in enumerate([] if args.product_only else E):
@@ -308,11 +329,9 @@ def bench_ensemble(args):
                         "errors_vs_fp64": errors(r)}
         log(f"{name}: {others[name]}")
 
-    dom, kt, gbs = roof
+    dom, kin, fwd_gbs = roof
     ab = algo_bytes(prec)
-    fwd_step_ms = m["fwd_ms"] / K
-    fwd_gbs = ab["forward_step"] * ncells / (fwd_step_ms * 1e-3) / 1e9
-    workload = "ensemble256"
+    workload = f"ensemble256_{args.precision}"
     cpu = None
     if not (args.no_cpu or args.product_only):
         warm_oracle(E[0])
@@ -361,8 +380,9 @@ def bench_ensemble(args):
             "errors_vs_fp64": errs,
             "other_precisions": others,
             "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS,
-                                 "algorithmic_bytes_per_cell": ab["forward_step"]},
-            "kernel_gbs": gbs,
+                                 "algorithmic_bytes_per_cell": ab["forward_step"],
+                                 "short_path_bytes_per_cell": short_bytes(prec)["update"]},
+            "kernels": kin,
             "graph_replay": bool(graph),
             "stored_trajectory_bytes": store,
             "tile_variants": {"coef_update_adjoint": m["tiles"],
@@ -371,14 +391,17 @@ def bench_ensemble(args):
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": gbs[dom],
+            "achieved": kin[dom]["gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": gbs[dom] / HBM_PEAK_GBS,
+            "frac": kin[dom]["gbs"] / HBM_PEAK_GBS,
             "traffic": traffic_from_profile(dom, workload),
             "kernel": dom,
             "algorithmic_bytes_per_cell": ab[dom],
-            "avg_launch_ms": kt[dom],
+            "short_path_bytes_per_cell": short_bytes(prec)[dom],
+            "algorithmic_bytes_per_launch": kin[dom]["bytes_per_launch"],
+            "full_path_share": kin[dom]["full_path_share"],
+            "avg_launch_ms": kin[dom]["avg_launch_ms"],
         },
         "cpu_baseline": cpu,
     }
@@ -387,7 +410,7 @@ def bench_ensemble(args):
 LEAF_TABLE_BYTES = 40  # per leaf and kernel: nbr 8 x 4 + side kinds 4 + generation 4
 
 
-def adaptive_leg(K=200, k=25, reps=20, graph=True, cpu_steps=0):
+def adaptive_leg(K=200, k=25, graph=True, cpu_steps=0):
     """The adaptive-mesh (leaf table) FP64 path on the C1-size mesh: about
     0.88 M leaves (5 m around two piles, 40 m base cells, three levels of
     2:1-balanced refinement) on the seeded fractal DEM; K forward steps +
@@ -409,12 +432,11 @@ def adaptive_leg(K=200, k=25, reps=20, graph=True, cpu_steps=0):
         sim.run(K)  # untimed: capture of the sweeps, first launches
         sim.dual_solver(w, want_lambda=False)
         r = timed_run(sim, K, w, h0)
-        kt = {name: sim.time_kernel(i, reps) for i, name in enumerate(KERNELS)}
-    ab = algo_bytes(_lib.TG_FP64)
-    ab = {n: v + LEAF_TABLE_BYTES for n, v in ab.items()}
-    gbs = {n: ab[n] * pb.n / (kt[n] * 1e-3) / 1e9 for n in KERNELS}
+        ab = {n: v + LEAF_TABLE_BYTES for n, v in algo_bytes(_lib.TG_FP64).items()}
+        dom, kin, fwd_gbs = instrumented_roofline(sim, _lib.TG_FP64, pb.n, K, w, h0, ab=ab)
+    kt = {n: kin[n]["avg_launch_ms"] for n in KERNELS}
+    gbs = {n: kin[n]["gbs"] for n in KERNELS}
     cs = pb.n * K
-    fwd_step_ms = r["fwd_ms"] / K
     out = {
         "leaves": pb.n, "levels": pb.meta["levels"], "steps": K, "checkpoint_every": k,
         "cell_steps_per_s": cs / (r["wall_f"] + r["wall_a"]),
@@ -423,9 +445,7 @@ def adaptive_leg(K=200, k=25, reps=20, graph=True, cpu_steps=0):
         "kernel_ms": {n: kt[n] for n in KERNELS},
         "kernel_gbs": gbs,
         "algorithmic_bytes_per_leaf": ab,
-        "forward_step_hbm": {"achieved": ab["forward_step"] * pb.n / (fwd_step_ms * 1e-3) / 1e9,
-                             "frac": ab["forward_step"] * pb.n / (fwd_step_ms * 1e-3) / 1e9 /
-                             HBM_PEAK_GBS},
+        "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS},
         "dJ_dphibed": float(r["dphi"][0]),
     }
     log(f"adaptive: {out['cell_steps_per_s']:.3g} cell-steps/s fwd+adj, kernels "
@@ -452,16 +472,14 @@ def adaptive_leg(K=200, k=25, reps=20, graph=True, cpu_steps=0):
                                 f"their adjoint steps (oracle: FP64 C leaf-table restatement, "
                                 f"-O3, one thread); forward {ccs / tf:.3g}, adjoint "
                                 f"{ccs / ta:.3g} cell-steps/s"}
-    return out, kt, gbs, ab, r, pb
+    return out, kt, gbs, ab, r, pb, dom
 
 
 def bench_adaptive(args):
     K, W, k = args.steps, args.warmup, args.checkpoint_every
-    out, kt, gbs, ab, r, pb = adaptive_leg(K, k, args.kernel_reps, not args.no_graph,
-                                           0 if (args.no_cpu or args.product_only)
-                                           else args.cpu_steps)
-    dom = max(("coef", "update", "adjoint"),
-              key=lambda n: kt[n] * ((2 - 1.0 / k) if n != "adjoint" else 1))
+    out, kt, gbs, ab, r, pb, dom = adaptive_leg(K, k, not args.no_graph,
+                                                0 if (args.no_cpu or args.product_only)
+                                                else args.cpu_steps)
     wall = r["wall_f"] + r["wall_a"]
     cpu = out.pop("cpu", None)
     return {
@@ -514,7 +532,7 @@ def bench_grid(args):
             log(f"precision {mode}: forward {r['wall_f'] * 1e3:.1f} ms, reverse "
                 f"{r['wall_a'] * 1e3:.1f} ms wall")
             if mode == prec:
-                roof = kernel_roofline(sim, prec, pb.n, k, args.kernel_reps)
+                roof = instrumented_roofline(sim, prec, pb.n, K, w, h0)
     m, d = res[prec], res[_lib.TG_FP64]
     cell_steps = pb.n * K
     wall = m["wall_f"] + m["wall_a"]
@@ -530,9 +548,8 @@ def bench_grid(args):
         errs["lambda0_l2_rel"] = float(np.linalg.norm(m["lam"] - d["lam"]) /
                                        np.linalg.norm(d["lam"]))
         log(f"errors vs fp64: {errs}")
-    dom, kt, gbs = roof
+    dom, kin, fwd_gbs = roof
     ab = algo_bytes(prec)
-    fwd_gbs = ab["forward_step"] * pb.n / (m["fwd_ms"] / K * 1e-3) / 1e9
     return {
         "metric": METRIC, "value": cell_steps / wall, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": wall * 1e3 / K, "higher_is_better": True,
@@ -546,26 +563,31 @@ def bench_grid(args):
             "fp64_cell_steps_per_s": cell_steps / (d["wall_f"] + d["wall_a"]),
             "errors_vs_fp64": errs,
             "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS},
-            "kernel_gbs": gbs,
+            "kernels": kin,
         },
-        "roofline": {"bound": "hbm", "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs[dom] / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "achieved": kin[dom]["gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": kin[dom]["gbs"] / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(dom, f"grid{n}"), "kernel": dom,
-                     "algorithmic_bytes_per_cell": ab[dom], "avg_launch_ms": kt[dom]},
+                     "algorithmic_bytes_per_cell": ab[dom],
+                     "short_path_bytes_per_cell": short_bytes(prec)[dom],
+                     "algorithmic_bytes_per_launch": kin[dom]["bytes_per_launch"],
+                     "full_path_share": kin[dom]["full_path_share"],
+                     "avg_launch_ms": kin[dom]["avg_launch_ms"]},
         "cpu_baseline": None,
     }
 
 
 def probe(args):
-    """Short run for hardware-counter profiles (rocprofv3 --pmc): the same
-    ensemble, W forward steps and their adjoint, then `kernel_reps` launches
-    of each kernel on the resulting state (the launches the counters are read
-    from), then a 1 GiB device copy whose counters calibrate FETCH_SIZE /
-    WRITE_SIZE against a known byte count."""
-    import numpy as np
+    """Run for hardware-counter profiles (rocprofv3 --pmc): the ensemble's
+    K-step forward run and reverse sweep (graph replay, so the counter
+    collection sees few dispatches of the sweep itself), then `kernel_reps`
+    instrumented launches of each kernel on the final state -- the launches
+    the counters are read from; their full-path share and algorithmic bytes
+    per launch are logged -- then a 1 GiB device copy whose counters
+    calibrate FETCH_SIZE / WRITE_SIZE against a known byte count."""
     import torch
 
-    from discrete_adjoint_amd import _lib, problems
+    from discrete_adjoint_amd import problems
     from discrete_adjoint_amd.solver import Ensemble
 
     K, k = args.steps, args.checkpoint_every
@@ -573,14 +595,24 @@ def probe(args):
     weights = [p.box_weights().reshape(3, -1) for p in E]
     prec = prec_of(args.precision)
     tiles = [int(v) for v in args.tiles.split(",")] if args.tiles else None
-    with Ensemble(E, precision=prec, checkpoint_every=k, max_steps=max(K, 1)) as ens:
+    ab, sb = algo_bytes(prec), short_bytes(prec)
+    with Ensemble(E, precision=prec, checkpoint_every=k, max_steps=max(K, 1), use_graph=1) as ens:
         ens.run(K)
         ens.dual_solver(weights, want_lambda=False)
         if tiles:  # reproduce a tile selection (e.g. the bench's autotuned one)
             ens.select_tiles(tiles)
         for i, name in enumerate(KERNELS):
-            ms = ens.time_kernel(i, args.kernel_reps)
-            log(f"probe {name}: {ms:.4f} ms/launch over {args.kernel_reps} launches")
+            ens.time_kernel(i, 1)  # first launch of this variant, uncounted
+            ens.instrument(True)
+            ens.time_kernel(i, args.kernel_reps)
+            ms, launches, full = ens.instrumentation()
+            ens.instrument(False)
+            cells = launches[i] * ens.ncell
+            fc = cells if full[i] < 0 else full[i]
+            per = (fc * ab[name] + (cells - fc) * sb[name]) / max(launches[i], 1)
+            log(f"probe {name}: {ms[i] / max(launches[i], 1):.4f} ms/launch over {launches[i]} "
+                f"launches, full path {fc / max(cells, 1):.4f}, algorithmic bytes per launch "
+                f"{per:.0f}")
         log(f"probe cells {ens.ncell} tiles {ens.tile_selection()}")
     n = 1 << 28  # 2^28 FP32 = 1 GiB read + 1 GiB written
     x = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -592,6 +624,61 @@ def probe(args):
     assert float(y[-1]) == 1.0
 
 
+def traffic_summary(fetch_csv, write_csv, probe_log, kernel, workload, out_path, commands):
+    """HBM bytes per launch of `kernel` from two rocprofv3 --pmc runs of the
+    probe (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), each scaled by what
+    the counter reads for the probe's 1 GiB copy; written to out_path with
+    the algorithmic bytes the probe logged for the same launches."""
+    import csv
+
+    role, symbols = kernel.split(":")
+    symbols = symbols.split("+")  # the GPU kernels one launch of `role` consists of
+
+    def per_dispatch(path, counter):
+        rows = list(csv.DictReader(open(path)))
+        name_key = "Kernel_Name" if "Kernel_Name" in rows[0] else "Kernel-Name"
+        val_key = "Counter_Value" if "Counter_Value" in rows[0] else "Counter-Value"
+        ker, copy = [], []
+        for r in rows:
+            if r.get("Counter_Name", r.get("Counter-Name")) != counter:
+                continue
+            nm = r[name_key]
+            if any(f"{sym}<" in nm or f"{sym}(" in nm for sym in symbols):
+                ker.append((nm, float(r[val_key])))
+            elif "copy" in nm.lower() or "elementwise" in nm.lower():
+                copy.append(float(r[val_key]))
+        return ker, copy
+
+    fk, fc = per_dispatch(fetch_csv, "FETCH_SIZE")
+    wk, wc = per_dispatch(write_csv, "WRITE_SIZE")
+    n_reps = None
+    algo = None
+    for line in open(probe_log):
+        if line.startswith(f"probe {role}:"):
+            parts = line.split()
+            n_reps = int(parts[parts.index("over") + 1])
+            algo = float(parts[-1])
+    # the instrumented launches on the final state: the last n_reps launches,
+    # each one dispatch of every symbol
+    nd = n_reps * len(symbols)
+    fk, wk = fk[-nd:], wk[-nd:]
+    copy_bytes = 2.0 ** 30
+    fscale = copy_bytes / (max(fc) * 1024.0)
+    wscale = copy_bytes / (max(wc) * 1024.0)
+    f_kib = sum(v for _, v in fk) / n_reps
+    w_kib = sum(v for _, v in wk) / n_reps
+    out = {"kernel": role, "dispatched_kernels": sorted({nm for nm, _ in fk}),
+           "fetch_kib_per_launch": f_kib, "write_kib_per_launch": w_kib,
+           "fetch_scale": fscale, "write_scale": wscale,
+           "hbm_bytes_per_launch": f_kib * 1024.0 * fscale + w_kib * 1024.0 * wscale,
+           "algorithmic_bytes_per_launch": algo, "launches": n_reps, "dispatches": len(fk),
+           "state": "final state of the ensemble's 500-step run and reverse sweep",
+           "command_fetch": commands[0], "command_write": commands[1], "workload": workload}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -601,8 +688,8 @@ def main():
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--precision", choices=tuple(PRECISIONS), default="adj32")
-    ap.add_argument("--compare", default="mixed,fp32",
+    ap.add_argument("--precision", choices=tuple(PRECISIONS), default="fp64")
+    ap.add_argument("--compare", default="adj32,mixed,fp32",
                     help="further precisions run on the same ensemble (errors and speed)")
     ap.add_argument("--checkpoint-every", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -619,6 +706,10 @@ def main():
                     help="short run for hardware-counter profiles (see probe())")
     ap.add_argument("--tiles", default="",
                     help="with --probe: tile variants coef,update,adjoint")
+    ap.add_argument("--traffic-summary", default="",
+                    help="fetch.csv,write.csv,probe.log,ROLE:SYM+SYM,out.json: HBM traffic "
+                         "of one kernel role (coef, update, adjoint) made of the GPU kernels "
+                         "SYM, from two --pmc runs of --probe")
     ap.add_argument("--no-adaptive", action="store_true",
                     help="skip the adaptive-mesh leg of the ensemble run")
     ap.add_argument("--no-graph", action="store_true",
@@ -629,8 +720,18 @@ def main():
               "(ensembles would shard by member, one large block by SFC range; neither is "
               "built)", file=sys.stderr)
         sys.exit(2)
+    if args.traffic_summary:
+        fetch_csv, write_csv, probe_log, kernel, out_path = args.traffic_summary.split(",")
+        traffic_summary(fetch_csv, write_csv, probe_log, kernel,
+                        f"ensemble of {args.members} members, precision {args.precision}",
+                        out_path,
+                        [f"cd /tmp && TMPDIR=/tmp timeout -s KILL 300 rocprofv3 --pmc {c} -d "
+                         f"/tmp/pmc_{d} -o probe --output-format csv -- python3 "
+                         f"$REPO/bench.py --probe --tiles {args.tiles}"
+                         for c, d in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write"))])
+        return
     if args.probe:
-        args.steps = args.steps or 5
+        args.steps = args.steps or 500
         args.checkpoint_every = args.checkpoint_every or 25
         probe(args)
         return

@@ -63,6 +63,10 @@ struct tg_ctx {
   double* dphi_part = nullptr;
   int2* thin_list = nullptr;  // FP32 adjoint modes: cells gathered in FP64
   int* thin_count = nullptr;
+  // tiles taking the full path of the update / adjoint (compacted by their
+  // classifying pass) and the list length
+  int* act_list = nullptr;
+  int* act_count = nullptr;
 
   // adaptive mesh (TG_MESH_LEAVES): leaf tables and reverse-step buffers
   bool leaves = false;
@@ -92,6 +96,16 @@ struct tg_ctx {
   // timing
   hipEvent_t ev[6] = {};
   double fwd_ms = 0, adj_ms = 0, rec_ms = 0;
+
+  // instrumentation (tg_set_instrumentation): an event pair around every
+  // coefficient / update / adjoint launch, full-path cell counters
+  bool instrument = false;
+  unsigned long long* active_d = nullptr;  // [3]
+  std::vector<hipEvent_t> iev;             // pool, two per recorded launch
+  std::vector<int> iev_kind;               // kernel of each recorded pair
+  double inst_ms[3] = {0, 0, 0};
+  long long inst_launches[3] = {0, 0, 0};
+  bool inst_counted[3] = {true, true, true};
 };
 
 namespace {
@@ -150,6 +164,8 @@ void free_fields(tg_ctx* ctx) {
   dfree(ctx->dphi_part);
   dfree(ctx->thin_list);
   dfree(ctx->thin_count);
+  dfree(ctx->act_list);
+  dfree(ctx->act_count);
   dfree(ctx->leaf_nbr);
   dfree(ctx->leaf_meta);
   dfree(ctx->leaf_cdx);
@@ -223,6 +239,51 @@ void use_tiles(const tg_ctx* ctx, int kernel, A& a) {
   a.shape = v;
 }
 
+// Instrumentation: resolves the recorded event pairs into the per-kernel
+// totals and frees the pool slots.
+int inst_flush(tg_ctx* ctx) {
+  if (ctx->iev_kind.empty()) return TG_OK;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (size_t i = 0; i < ctx->iev_kind.size(); i++) {
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->iev[2 * i], ctx->iev[2 * i + 1]));
+    ctx->inst_ms[ctx->iev_kind[i]] += ms;
+    ctx->inst_launches[ctx->iev_kind[i]] += 1;
+  }
+  ctx->iev_kind.clear();
+  return TG_OK;
+}
+
+// Event recorded before (begin) or after a launch of kernel `kind`; false
+// when instrumentation is off.  A failed pool refill turns it off.
+void inst_mark(tg_ctx* ctx, int kind, bool begin) {
+  if (!ctx->instrument) return;
+  if (begin) {
+    const size_t need = 2 * (ctx->iev_kind.size() + 1);
+    if (need > ctx->iev.size()) {
+      if (ctx->iev.size() >= 8192 && inst_flush(ctx) == TG_OK) {
+        // pool reused after the flush
+      } else {
+        for (int k = 0; k < 256; k++) {
+          hipEvent_t e = nullptr;
+          if (hipEventCreate(&e) != hipSuccess) { ctx->instrument = false; return; }
+          ctx->iev.push_back(e);
+        }
+      }
+    }
+    ctx->iev_kind.push_back(kind);
+    (void)hipEventRecord(ctx->iev[2 * (ctx->iev_kind.size() - 1)], ctx->stream);
+  } else {
+    (void)hipEventRecord(ctx->iev[2 * (ctx->iev_kind.size() - 1) + 1], ctx->stream);
+  }
+}
+
+unsigned long long* inst_counter(tg_ctx* ctx, int kind, bool counts) {
+  if (!ctx->instrument) return nullptr;
+  if (!counts) ctx->inst_counted[kind] = false;
+  return counts ? ctx->active_d + kind : nullptr;
+}
+
 template <class S>
 StepArgsM<S> step_args_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s) {
   StepArgsM<S> a{};
@@ -241,7 +302,11 @@ template <class S>
 void enqueue_step_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s, bool compute_dt) {
   StepArgsM<S> a = step_args_m<S>(ctx, in, out, s);
   use_tiles(ctx, 0, a);
+  inst_counter(ctx, 0, false);
+  inst_counter(ctx, 1, false);
+  inst_mark(ctx, 0, true);
   launch_coef_m<S>(a, compute_dt, ctx->stream);
+  inst_mark(ctx, 0, false);
   if (compute_dt) {
     StepArgs d{};
     d.inst = ctx->inst_d; d.red = ctx->red; d.dt_rec = ctx->dt_rec; d.time = ctx->time_d;
@@ -250,7 +315,9 @@ void enqueue_step_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s, b
     launch_dt(d, ctx->ninst, ctx->stream);
   }
   use_tiles(ctx, 1, a);
+  inst_mark(ctx, 1, true);
   launch_update_m<S>(a, ctx->stream);
+  inst_mark(ctx, 1, false);
 }
 
 template <class S>
@@ -271,7 +338,12 @@ void enqueue_adjoint_m(tg_ctx* ctx, void* const U[3], void* const lnext[3], void
   a.thin_count = ctx->thin_count;
   a.s64 = (ctx->ses == 8) ? (const double*)ctx->statics : nullptr;
   a.ncell = ctx->ncell;
+  a.tile_list = ctx->act_list;
+  a.tile_count = ctx->act_count;
+  a.active = inst_counter(ctx, 2, true);
+  inst_mark(ctx, 2, true);
   launch_adjoint_m<S>(a, ctx->stream);
+  inst_mark(ctx, 2, false);
 }
 
 void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, bool compute_dt) {
@@ -290,17 +362,31 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
     a.cfl_inner = ctx->cfg.cfl_inner; a.cfl_outer = ctx->cfg.cfl_outer;
     a.h_out = (double*)out[0]; a.hu_out = (double*)out[1]; a.hv_out = (double*)out[2];
     a.err = ctx->err_d;
+    a.tile_list = ctx->act_list;
+    a.tile_count = ctx->act_count;
     if (ctx->leaves) {
+      inst_counter(ctx, 0, false);
+      inst_counter(ctx, 1, false);
+      inst_mark(ctx, 0, true);
       launch_leaf_coef(a, ctx->leaf, compute_dt, ctx->stream);
+      inst_mark(ctx, 0, false);
       if (compute_dt) launch_dt(a, 1, ctx->stream);
+      inst_mark(ctx, 1, true);
       launch_leaf_update(a, ctx->leaf, ctx->stream);
+      inst_mark(ctx, 1, false);
       return;
     }
     use_tiles(ctx, 0, a);
+    a.active = inst_counter(ctx, 0, true);
+    inst_mark(ctx, 0, true);
     launch_coef(a, compute_dt, ctx->stream);
+    inst_mark(ctx, 0, false);
     if (compute_dt) launch_dt(a, ctx->ninst, ctx->stream);
     use_tiles(ctx, 1, a);
+    a.active = inst_counter(ctx, 1, true);
+    inst_mark(ctx, 1, true);
     launch_update(a, ctx->stream);
+    inst_mark(ctx, 1, false);
   } else if (ctx->es == 8) {
     enqueue_step_m<double>(ctx, in, out, s, compute_dt);
   } else {
@@ -325,8 +411,13 @@ void enqueue_adjoint(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* 
     a.ft = frict_tiny(ctx, s); a.tiny = ctx->cfg.tiny; a.gamma = ctx->cfg.gamma;
     a.lam_h = (double*)lout[0]; a.lam_hu = (double*)lout[1]; a.lam_hv = (double*)lout[2];
     a.dphi_part = dphi ? ctx->dphi_part : nullptr;
+    a.tile_list = ctx->act_list;
+    a.tile_count = ctx->act_count;
+    a.active = inst_counter(ctx, 2, !ctx->leaves);
+    inst_mark(ctx, 2, true);
     if (ctx->leaves) launch_leaf_adjoint(a, ctx->leaf, ctx->lb, ctx->stream);
     else launch_adjoint(a, ctx->stream);
+    inst_mark(ctx, 2, false);
   } else if (ctx->es == 8) {
     enqueue_adjoint_m<double>(ctx, U, lnext, lout, s, dphi);
   } else {
@@ -542,7 +633,7 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
     return cur;  // A or B (U^nsteps never goes to the stored trajectory)
   };
   void** endU = nullptr;
-  if (ctx->cfg.use_graph) {
+  if (ctx->cfg.use_graph && !ctx->instrument) {
     const long long key[8] = {nsteps, dt_seq != nullptr, (long long)(intptr_t)ctx->U[0],
                               ctx->sel[0], ctx->sel[1], k, M, 0};
     if (!same_key(ctx->gf, key)) {
@@ -632,7 +723,7 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
   };
   int cur = 0;
   float rec_ms = 0, ms = 0;
-  if (ctx->cfg.use_graph) {
+  if (ctx->cfg.use_graph && !ctx->instrument) {
     const long long key[8] = {N, k, dphi, ctx->sel[0], ctx->sel[1], ctx->sel[2], s0, 0};
     if (!same_key(ctx->ga, key)) {
       rc = capture(ctx, ctx->ga, key, [&]() {
@@ -874,6 +965,8 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   if (!rc && ctx->mode != TG_FP64)
     rc = dalloc(ctx, (void**)&ctx->thin_list, sizeof(int2) * (size_t)n);
   if (!rc && ctx->mode != TG_FP64) rc = dalloc(ctx, (void**)&ctx->thin_count, sizeof(int));
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->act_list, sizeof(int) * (size_t)ctx->ntiles);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->act_count, sizeof(int));
   if (!rc) rc = dalloc(ctx, (void**)&ctx->inst_d, sizeof(InstDev) * nm);
   for (int v = 0; v < NSHAPE && !rc; v++)
     rc = dalloc(ctx, (void**)&ctx->tiles_d[v], sizeof(TileDev) * ctx->nt[v]);
@@ -1061,6 +1154,8 @@ void tg_destroy(tg_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_fields(ctx);
   dfree(ctx->err_d);
+  dfree(ctx->active_d);
+  for (hipEvent_t e : ctx->iev) (void)hipEventDestroy(e);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1287,6 +1382,10 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
       a.ft = frict_tiny(ctx, 0); a.tiny = ctx->cfg.tiny; a.gamma = ctx->cfg.gamma;
       a.h_out = (double*)ctx->Ut[0]; a.hu_out = (double*)ctx->Ut[1]; a.hv_out = (double*)ctx->Ut[2];
       a.err = ctx->err_d;
+      a.tile_list = ctx->act_list;
+      a.tile_count = ctx->act_count;
+      a.active = inst_counter(ctx, w, !ctx->leaves);
+      inst_mark(ctx, w, true);
       if (ctx->leaves) {
         if (w == 0) launch_leaf_coef(a, ctx->leaf, false, ctx->stream);
         else launch_leaf_update(a, ctx->leaf, ctx->stream);
@@ -1295,6 +1394,7 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
       } else {
         launch_update(a, ctx->stream);
       }
+      inst_mark(ctx, w, false);
     } else if (ctx->es == 8) {
       StepArgsM<double> a = step_args_m<double>(ctx, ctx->U, ctx->Ut, 0);
       use_tiles(ctx, w, a);
@@ -1401,5 +1501,43 @@ int tg_get_tile_selection(const tg_ctx* ctx, int* variant_out) {
 }
 
 int64_t tg_num_cells(const tg_ctx* ctx) { return ctx ? (int64_t)ctx->ncell : 0; }
+
+int tg_set_instrumentation(tg_ctx* ctx, int on) {
+  if (!ctx) return TG_ERR_ARG;
+  int rc = inst_flush(ctx);
+  if (rc) return rc;
+  if (on && !ctx->active_d) {
+    rc = dalloc(ctx, (void**)&ctx->active_d, 3 * sizeof(unsigned long long));
+    if (rc) return rc;
+  }
+  if (on) {
+    HIPCHK(ctx, hipMemsetAsync(ctx->active_d, 0, 3 * sizeof(unsigned long long), ctx->stream));
+    for (int k = 0; k < 3; k++) {
+      ctx->inst_ms[k] = 0;
+      ctx->inst_launches[k] = 0;
+      ctx->inst_counted[k] = true;
+    }
+  }
+  ctx->instrument = on != 0;
+  return TG_OK;
+}
+
+int tg_get_instrumentation(tg_ctx* ctx, double* kernel_ms, int64_t* launches,
+                           int64_t* full_cells) {
+  if (!ctx) return TG_ERR_ARG;
+  if (!ctx->active_d) return set_err(ctx, TG_ERR_STATE, "instrumentation was never on");
+  int rc = inst_flush(ctx);
+  if (rc) return rc;
+  unsigned long long cnt[3] = {0, 0, 0};
+  HIPCHK(ctx, hipMemcpyAsync(cnt, ctx->active_d, sizeof(cnt), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < 3; k++) {
+    if (kernel_ms) kernel_ms[k] = ctx->inst_ms[k];
+    if (launches) launches[k] = ctx->inst_launches[k];
+    if (full_cells) full_cells[k] = ctx->inst_counted[k] ? (int64_t)cnt[k] : -1;
+  }
+  return TG_OK;
+}
 
 }  // extern "C"

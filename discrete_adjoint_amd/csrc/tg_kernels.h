@@ -79,6 +79,14 @@ struct StepArgs {
   double* hu_out;
   double* hv_out;
   int* err;              // sticky error flag (non-finite state)
+  // null, or a counter the launch adds its full-path cells to: cells the
+  // coefficient pass evaluates above tiny, cells of update tiles that are
+  // not entirely dry
+  unsigned long long* active;
+  // compacted list of the tiles that take the update's full path (written
+  // by the classifying pass, walked by the update) and its length
+  int* tile_list;
+  int* tile_count;
 };
 
 struct AdjArgs {
@@ -98,6 +106,9 @@ struct AdjArgs {
   double* lam_hu;
   double* lam_hv;
   double* dphi_part;     // [ntiles] running sum of lambda^{s+1}.dU/dphi, or null
+  unsigned long long* active;  // null, or counter of cells in non-void tiles
+  int* tile_list;        // compacted non-void tiles (classifying pass) and count
+  int* tile_count;
 };
 
 // Mixed-precision counterparts: FP32 arithmetic, FP32 static coefficients
@@ -138,6 +149,7 @@ struct StepArgsM {
   S* hu_out;
   S* hv_out;
   int* err;
+  unsigned long long* active;  // as in StepArgs (not counted by these kernels)
 };
 template <class S>
 struct AdjArgsM {
@@ -161,9 +173,15 @@ struct AdjArgsM {
   int* thin_count;
   const double* s64;     // FP64 static coefficients [7 * ncell] or null
   long long ncell;
+  unsigned long long* active;  // null, or counter of cells in non-void tiles
+  int* tile_list;        // compacted non-void tiles (classifying pass) and count
+  int* tile_count;
 };
 // workgroups of the FP64 thin-layer adjoint pass (strides over its list)
 constexpr int THIN_GRID = 1024;
+// workgroups of the update / adjoint passes that stride over the compacted
+// list of tiles taking the full path (a fixed grid, for graph capture)
+constexpr int LIST_GRID = 2048;
 
 template <class S>
 void launch_coef_m(const StepArgsM<S>& a, bool reduce, hipStream_t s);

@@ -86,19 +86,21 @@ __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
   double minev = HUGE_VAL, hmax = 0.;
+  bool wet = false;
   if (q.act) {
     const long long c = q.c, nx = I.nx;
-    const double h = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
-    double hx = 0., hux = 0., hy = 0., hvy = 0.;
-    if (q.hasE && q.hasW && q.hasN && q.hasS) {  // boundary cells: zero slopes
-      hx = slope_fwd(a.U.h[c + 1], h, a.U.h[c - 1], I.dx, a.gamma, a.tiny);
-      hux = slope_fwd(a.U.hu[c + 1], hu, a.U.hu[c - 1], I.dx, a.gamma, a.tiny);
-      hy = slope_fwd(a.U.h[c + nx], h, a.U.h[c - nx], I.dy, a.gamma, a.tiny);
-      hvy = slope_fwd(a.U.hv[c + nx], hv, a.U.hv[c - nx], I.dy, a.gamma, a.tiny);
-    }
-    double k;
-    if (h > a.tiny) {
-      k = getcoef(h, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.eps, a.tiny);
+    const double h = a.U.h[c];
+    wet = h > a.tiny;
+    if (wet) {
+      const double hu = a.U.hu[c], hv = a.U.hv[c];
+      double hx = 0., hux = 0., hy = 0., hvy = 0.;
+      if (q.hasE && q.hasW && q.hasN && q.hasS) {  // boundary cells: zero slopes
+        hx = slope_fwd(a.U.h[c + 1], h, a.U.h[c - 1], I.dx, a.gamma, a.tiny);
+        hux = slope_fwd(a.U.hu[c + 1], hu, a.U.hu[c - 1], I.dx, a.gamma, a.tiny);
+        hy = slope_fwd(a.U.h[c + nx], h, a.U.h[c - nx], I.dy, a.gamma, a.tiny);
+        hvy = slope_fwd(a.U.hv[c + nx], hv, a.U.hv[c - nx], I.dy, a.gamma, a.tiny);
+      }
+      const double k = getcoef(h, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.eps, a.tiny);
       if (reduce) {
         double ev = eigen(h, hu, hv, k, a.S.gz[c], a.tiny);
         if (ev > 1000000000.) atomicOr(a.err, 2);
@@ -106,12 +108,19 @@ __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
         minev = mdx / ev;
         hmax = h;
       }
-    } else {
-      k = I.eps;
+      a.kact[c] = k;
+    } else if (!(h < a.tiny)) {
+      // kact is read only where h >= tiny (the physical flux is zero below
+      // tiny, update_states uses kact only above it): at or below tiny the
+      // value is epsilon, and below tiny it is not written
+      a.kact[c] = I.eps;
     }
-    a.kact[c] = k;
   }
-  if (reduce) {
+  if (a.active) {
+    const int cnt = __syncthreads_count(wet);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
+  }
+  if (reduce && __syncthreads_or(wet)) {
     __shared__ double smin[BLOCK / 64], smax[BLOCK / 64];
     minev = wave_min(minev);
     hmax = wave_max(hmax);
@@ -187,14 +196,59 @@ struct FaceLds {
   double dyh[TY + 1][TX], dyhu[TY + 1][TX];  // (U_top - U_bottom) / dy of h, hu
 };
 
+// Update pass 1 (one workgroup per tile): a tile whose cells and side
+// neighbours all hold h < tiny has every face flux zero (the physical flux
+// is zero below tiny and riemannflux of two such sides is zero) and no
+// source terms (update_states adds them only above tiny); its cells get
+// update_states' own arithmetic with zero fluxes, bit for bit, here.  Every
+// other tile is appended to the list pass 2 walks.
 template <int TX>
-__global__ __launch_bounds__(BLOCK) void k_update(StepArgs a) {
+__global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
   constexpr int TY = BLOCK / TX;
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
   const long long nx = I.nx;
-  __shared__ FaceLds<TX> L;
+  bool wet = false;
+  if (q.act) {
+    const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
+    wet = !(a.U.h[q.c] < a.tiny);
+    if (tx == 0 && q.hasW) wet = wet || !(a.U.h[q.c - 1] < a.tiny);
+    if (tx == TX - 1 && q.hasE) wet = wet || !(a.U.h[q.c + 1] < a.tiny);
+    if (ty == 0 && q.hasS) wet = wet || !(a.U.h[q.c - nx] < a.tiny);
+    if (ty == TY - 1 && q.hasN) wet = wet || !(a.U.h[q.c + nx] < a.tiny);
+  }
+  if (__syncthreads_or(wet)) {
+    if (threadIdx.x == 0) a.tile_list[atomicAdd(a.tile_count, 1)] = (int)blockIdx.x;
+    return;
+  }
+  if (!q.act) return;
+  const long long c = q.c;
+  const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
+  const double dtdx = dt / I.dx, dtdy = dt / I.dy, z = 0.;
+  double out[3] = {a.U.h[c], a.U.hu[c], a.U.hv[c]};
+#pragma unroll
+  for (int v = 0; v < 3; v++) out[v] = out[v] - dtdx * (z - z) - dtdy * (z - z);
+  if (out[0] < 0.) out[0] = 0;
+  if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
+  a.h_out[c] = out[0];
+  a.hu_out[c] = out[1];
+  a.hv_out[c] = out[2];
+}
+
+// Update pass 2, one listed tile: face fluxes staged in LDS, slopes,
+// update_states.
+template <int TX>
+__device__ __forceinline__ void update_tile(const StepArgs& a, int tile, FaceLds<TX>& L) {
+  constexpr int TY = BLOCK / TX;
+  const TileDev t = a.tiles[tile];
+  const InstDev I = a.inst[t.inst];
+  const Cell q = cell_of<TX>(t, I);
+  const long long nx = I.nx;
+  if (a.active) {
+    const int cnt = __syncthreads_count(q.act);
+    if (threadIdx.x == 0) atomicAdd(a.active, (unsigned long long)cnt);
+  }
   // phase 1: faces with a cell of the member on both sides
   // (calc_edge_states, element2.C:2744-3072)
   for (int f = (int)threadIdx.x; f < (TX + 1) * TY; f += BLOCK) {
@@ -256,6 +310,16 @@ __global__ __launch_bounds__(BLOCK) void k_update(StepArgs a) {
   a.h_out[c] = out[0];
   a.hu_out[c] = out[1];
   a.hv_out[c] = out[2];
+}
+
+template <int TX>
+__global__ __launch_bounds__(BLOCK) void k_update(StepArgs a) {
+  __shared__ FaceLds<TX> L;
+  const int n = *a.tile_count;
+  for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x) {
+    update_tile<TX>(a, a.tile_list[i], L);
+    __syncthreads();  // LDS reused by the next tile
+  }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_stats(const InstDev* inst, const TileDev* tiles,
@@ -455,16 +519,55 @@ __device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
   return d;
 }
 
-template <int TX>
-__global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
+// Adjoint pass 1 (one workgroup per tile): a tile of void cells (h == 0)
+// gets lambda^s = 0 -- every block that multiplies a void cell's state is
+// zero (calc_jacobain.C:117-122), as in the per-cell path of pass 2 -- and,
+// with lambda^{s+1} == 0 on the tile, zero bed-friction terms; it is
+// written here.  Every other tile is appended to the list pass 2 walks.
+// Shared by the FP64 and the mixed-precision adjoints (storage type S).
+template <class S, int TX, class A>
+__device__ __forceinline__ void adjoint_classify(const A& a) {
   const TileDev t = a.tiles[blockIdx.x];
+  const InstDev I = a.inst[t.inst];
+  const Cell q = cell_of<TX>(t, I);
+  bool busy = false;
+  if (q.act) {
+    busy = a.U.h[q.c] != (S)0;
+    if (a.dphi_part)
+      busy = busy || !(a.lam_next.h[q.c] == (S)0 && a.lam_next.hu[q.c] == (S)0 &&
+                       a.lam_next.hv[q.c] == (S)0);
+  }
+  if (__syncthreads_or(busy)) {
+    if (threadIdx.x == 0) a.tile_list[atomicAdd(a.tile_count, 1)] = (int)blockIdx.x;
+    return;
+  }
+  if (q.act) {
+    a.lam_h[q.c] = (S)0;
+    a.lam_hu[q.c] = (S)0;
+    a.lam_hv[q.c] = (S)0;
+  }
+}
+
+template <int TX>
+__global__ __launch_bounds__(BLOCK) void k_adjoint_classify(AdjArgs a) {
+  adjoint_classify<double, TX>(a);
+}
+
+// Adjoint pass 2, one listed tile.
+template <int TX>
+__device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds& L,
+                                             double* sp) {
+  const TileDev t = a.tiles[tile];
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
   double dphi = 0.;
   const bool want_dphi = a.dphi_part != nullptr;
   constexpr int TY = BLOCK / TX, RX = TX + 2, RY = TY + 2, NHALO = RX * RY - BLOCK;
   static_assert(RX * RY <= RN && NHALO <= BLOCK, "tile does not fit the LDS records");
-  __shared__ DualLds L;
+  if (a.active) {
+    const int cnt = __syncthreads_count(q.act);
+    if (threadIdx.x == 0) atomicAdd(a.active, (unsigned long long)cnt);
+  }
   const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
   const int rself = (ty + 1) * RX + tx + 1;
   // phase 1: dual records of the tile (kept in registers) and of the halo
@@ -605,14 +708,25 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
     }
   }
   if (want_dphi) {
-    __shared__ double sp[BLOCK / 64];
     dphi = wave_sum(dphi);
     if (threadIdx.x % 64 == 0) sp[threadIdx.x / 64] = dphi;
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int k = 1; k < BLOCK / 64; k++) dphi += sp[k];
-      a.dphi_part[blockIdx.x] += dphi;
+      a.dphi_part[tile] += dphi;
     }
+  }
+}
+
+
+template <int TX>
+__global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
+  __shared__ DualLds L;
+  __shared__ double sp[BLOCK / 64];
+  const int n = *a.tile_count;
+  for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x) {
+    adjoint_tile<TX>(a, a.tile_list[i], L, sp);
+    __syncthreads();  // LDS reused by the next tile
   }
 }
 
@@ -634,11 +748,17 @@ void launch_coef(const StepArgs& a, bool reduce, hipStream_t s) {
 void launch_dt(const StepArgs& a, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_dt, dim3((ninst + 63) / 64), dim3(64), 0, s, a, ninst);
 }
+// classify every tile (short path written, other tiles listed), then a
+// fixed grid walks the list with the full path
 void launch_update(const StepArgs& a, hipStream_t s) {
-  TG_LAUNCH_TX(a.shape, k_update, dim3(a.ntiles), s, a);
+  launch_zero(a.tile_count, sizeof(int), s);
+  TG_LAUNCH_TX(a.shape, k_update_classify, dim3(a.ntiles), s, a);
+  TG_LAUNCH_TX(a.shape, k_update, dim3(a.ntiles < LIST_GRID ? a.ntiles : LIST_GRID), s, a);
 }
 void launch_adjoint(const AdjArgs& a, hipStream_t s) {
-  TG_LAUNCH_TX(a.shape, k_adjoint, dim3(a.ntiles), s, a);
+  launch_zero(a.tile_count, sizeof(int), s);
+  TG_LAUNCH_TX(a.shape, k_adjoint_classify, dim3(a.ntiles), s, a);
+  TG_LAUNCH_TX(a.shape, k_adjoint, dim3(a.ntiles < LIST_GRID ? a.ntiles : LIST_GRID), s, a);
 }
 static unsigned grid_for(long long n) {
   const long long b = (n + 255) / 256;

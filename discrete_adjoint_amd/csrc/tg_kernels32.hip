@@ -661,19 +661,48 @@ __device__ __forceinline__ double gather(const AdjArgsM<S>& a, const InstDev& I,
                                  self.lam);
 }
 
-// Five waves per SIMD: the register cap costs a few spills to scratch, but
-// the gather is latency-bound and the extra wave in flight hides more of it
-// (about 6% faster than the unconstrained 4-wave allocation, measured).
+// Adjoint pass 1 (one workgroup per tile): a tile of void cells (h == 0)
+// gets lambda^s = 0 -- every block that multiplies a void cell's state is
+// zero (calc_jacobain.C:117-122), as in pass 2 -- and, with lambda^{s+1} == 0
+// on the tile, zero bed-friction terms; it is written here.  Every other
+// tile is appended to the list pass 2 walks.
 template <class S, int TX>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
-void k32_adjoint(AdjArgsM<S> a) {
+__global__ __launch_bounds__(BLOCK) void k32_adjoint_classify(AdjArgsM<S> a) {
   const TileDev t = a.tiles[blockIdx.x];
+  const InstDev I = a.inst[t.inst];
+  const Cell q = cell_of<TX>(t, I);
+  bool busy = false;
+  if (q.act) {
+    busy = a.U.h[q.c] != (S)0;
+    if (a.dphi_part)
+      busy = busy || !(a.lam_next.h[q.c] == (S)0 && a.lam_next.hu[q.c] == (S)0 &&
+                       a.lam_next.hv[q.c] == (S)0);
+  }
+  if (__syncthreads_or(busy)) {
+    if (threadIdx.x == 0) a.tile_list[atomicAdd(a.tile_count, 1)] = (int)blockIdx.x;
+    return;
+  }
+  if (q.act) {
+    a.lam_h[q.c] = (S)0;
+    a.lam_hu[q.c] = (S)0;
+    a.lam_hv[q.c] = (S)0;
+  }
+}
+
+// Adjoint pass 2, one listed tile.
+template <class S, int TX>
+__device__ __forceinline__ void adjoint32_tile(const AdjArgsM<S>& a, int tile, DualLds& L,
+                                               double* sp) {
+  const TileDev t = a.tiles[tile];
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
   const bool want_dphi = a.dphi_part != nullptr;
   constexpr int TY = BLOCK / TX, RX = TX + 2, RY = TY + 2, NHALO = RX * RY - BLOCK;
   static_assert(RX * RY <= RN && NHALO <= BLOCK, "tile does not fit the LDS records");
-  __shared__ DualLds L;
+  if (a.active) {
+    const int cnt = __syncthreads_count(q.act);
+    if (threadIdx.x == 0) atomicAdd(a.active, (unsigned long long)cnt);
+  }
   const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
   const int rself = (ty + 1) * RX + tx + 1;
   DualInfoT<float> self;
@@ -711,7 +740,7 @@ void k32_adjoint(AdjArgsM<S> a) {
     if (thin) {
       // handed to k32_adjoint_thin (FP64), which writes this cell's lambda
       const int slot = atomicAdd(a.thin_count, 1);
-      a.thin_list[slot] = make_int2((int)blockIdx.x, (int)threadIdx.x);
+      a.thin_list[slot] = make_int2(tile, (int)threadIdx.x);
     } else {
       const S lc[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
       float adj[3];
@@ -730,14 +759,29 @@ void k32_adjoint(AdjArgsM<S> a) {
     }
   }
   if (want_dphi) {
-    __shared__ double sp[BLOCK / 64];
     dphi = wave_sum(dphi);
     if (threadIdx.x % 64 == 0) sp[threadIdx.x / 64] = dphi;
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int k = 1; k < BLOCK / 64; k++) dphi += sp[k];
-      a.dphi_part[blockIdx.x] += dphi;
+      a.dphi_part[tile] += dphi;
     }
+  }
+}
+
+
+// Five waves per SIMD: the register cap costs a few spills to scratch, but
+// the gather is latency-bound and the extra wave in flight hides more of it
+// (about 6% faster than the unconstrained 4-wave allocation, measured).
+template <class S, int TX>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
+void k32_adjoint(AdjArgsM<S> a) {
+  __shared__ DualLds L;
+  __shared__ double sp[BLOCK / 64];
+  const int n = *a.tile_count;
+  for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x) {
+    adjoint32_tile<S, TX>(a, a.tile_list[i], L, sp);
+    __syncthreads();  // LDS reused by the next tile
   }
 }
 
@@ -811,7 +855,10 @@ void launch_update_m(const StepArgsM<S>& a, hipStream_t s) {
 template <class S>
 void launch_adjoint_m(const AdjArgsM<S>& a, hipStream_t s) {
   launch_zero(a.thin_count, sizeof(int), s);
-  TG_LAUNCH_TX(a.shape, k32_adjoint, S, dim3(a.ntiles), s, a);
+  launch_zero(a.tile_count, sizeof(int), s);
+  TG_LAUNCH_TX(a.shape, k32_adjoint_classify, S, dim3(a.ntiles), s, a);
+  TG_LAUNCH_TX(a.shape, k32_adjoint, S, dim3(a.ntiles < LIST_GRID ? a.ntiles : LIST_GRID), s,
+               a);
   const int g = a.ntiles < THIN_GRID ? a.ntiles : THIN_GRID;
   TG_LAUNCH_TX(a.shape, k32_adjoint_thin, S, dim3(g), s, a);
 }

@@ -174,6 +174,20 @@ class Simulation:
                                                   C.cast(C.byref(g), _lib._dp)))
         return out.reshape(3, -1), g.value
 
+    def instrument(self, on=True):
+        """Per-launch kernel timing and full-path cell counts (no graph
+        replay while on); turning it on resets the totals."""
+        self._check(self.lib.tg_set_instrumentation(self.ctx, int(on)))
+
+    def instrumentation(self):
+        """(kernel_ms[3], launches[3], full_cells[3]) since instrument(True);
+        kernels 0 coef, 1 update, 2 adjoint; full_cells -1 where not counted."""
+        ms = (C.c_double * 3)()
+        ln = (C.c_int64 * 3)()
+        fc = (C.c_int64 * 3)()
+        self._check(self.lib.tg_get_instrumentation(self.ctx, C.cast(ms, _lib._dp), ln, fc))
+        return list(ms), list(ln), list(fc)
+
     def time_kernel(self, which, reps=20):
         """Average launch time (ms) of one kernel: 0 coef, 1 update, 2 adjoint."""
         ms = C.c_double()

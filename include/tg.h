@@ -248,6 +248,23 @@ int tg_last_timing(tg_ctx* ctx, double* forward_ms, double* adjoint_ms,
 typedef enum tg_kernel_id { TG_K_COEF = 0, TG_K_UPDATE = 1, TG_K_ADJOINT = 2 } tg_kernel_id;
 int tg_time_kernel(tg_ctx* ctx, const tg_params* prm, int which, int reps, double* avg_ms);
 
+/* Instrumentation (no reference counterpart: measurement only).  While on,
+ * forward runs and reverse sweeps launch kernel by kernel (no graph replay),
+ * every coefficient / update / adjoint launch is timed with a HIP event pair,
+ * and the uniform-block kernels count the cells that take their full path:
+ * the coefficient pass counts cells above tiny (below it a cell only reads h
+ * and writes kact = epsilon), the update and adjoint count cells of tiles
+ * that are not entirely dry (update: h < tiny on the tile and its side
+ * neighbours, state copied) or void (adjoint: h == 0 and lambda^{s+1} == 0
+ * on the tile, lambda^s = 0 written).  Turning it on resets the totals. */
+int tg_set_instrumentation(tg_ctx* ctx, int on);
+/* Totals since instrumentation was turned on, per kernel id: summed launch
+ * time (ms), launches, full-path cells (-1 for kernels that do not count:
+ * the mixed-precision forward kernels and the adaptive-mesh kernels, which
+ * always take their full path). */
+int tg_get_instrumentation(tg_ctx* ctx, double* kernel_ms, int64_t* launches,
+                           int64_t* full_cells);
+
 /* ---- tile shapes (no reference counterpart: launch configuration) ----------
  * Every kernel runs one 256-thread workgroup per tile of cells.  Built tile
  * variants v = 0..TG_NUM_TILE_VARIANTS-1: a tile shape (tile_x x tile_y
