@@ -407,7 +407,13 @@ def bench_ensemble(args):
     }
 
 
-LEAF_TABLE_BYTES = 40  # per leaf and kernel: nbr 8 x 4 + side kinds 4 + generation 4
+# Leaf-table bytes per leaf a full-path kernel reads: nbr 8 x 4, the packed
+# side-kind / back-slot word 4, cell size 16, centre 16.
+LEAF_TABLE_BYTES = 68
+# ... and what the short paths add to the uniform short-path bytes: the dry
+# update reads the packed word, nbr and cell size; the void adjoint reads the
+# packed word and nbr and writes kact and its phi_bed derivative.
+LEAF_SHORT_EXTRA = {"coef": 0, "update": 52, "adjoint": 52}
 
 
 def adaptive_leg(K=200, k=25, graph=True, cpu_steps=0):
@@ -433,7 +439,9 @@ def adaptive_leg(K=200, k=25, graph=True, cpu_steps=0):
         sim.dual_solver(w, want_lambda=False)
         r = timed_run(sim, K, w, h0)
         ab = {n: v + LEAF_TABLE_BYTES for n, v in algo_bytes(_lib.TG_FP64).items()}
-        dom, kin, fwd_gbs = instrumented_roofline(sim, _lib.TG_FP64, pb.n, K, w, h0, ab=ab)
+        sb = {n: v + LEAF_SHORT_EXTRA[n] for n, v in short_bytes(_lib.TG_FP64).items()}
+        dom, kin, fwd_gbs = instrumented_roofline(sim, _lib.TG_FP64, pb.n, K, w, h0, ab=ab,
+                                                  sb=sb)
     kt = {n: kin[n]["avg_launch_ms"] for n in KERNELS}
     gbs = {n: kin[n]["gbs"] for n in KERNELS}
     cs = pb.n * K
@@ -444,7 +452,8 @@ def adaptive_leg(K=200, k=25, graph=True, cpu_steps=0):
         "adjoint_cell_steps_per_s": cs / r["wall_a"],
         "kernel_ms": {n: kt[n] for n in KERNELS},
         "kernel_gbs": gbs,
-        "algorithmic_bytes_per_leaf": ab,
+        "algorithmic_bytes_per_leaf": ab, "short_path_bytes_per_leaf": sb,
+        "full_path_share": {n: kin[n]["full_path_share"] for n in KERNELS},
         "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS},
         "dJ_dphibed": float(r["dphi"][0]),
     }
@@ -493,7 +502,9 @@ def bench_adaptive(args):
         "roofline": {"bound": "hbm", "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs[dom] / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(dom, "adaptive882k"), "kernel": dom,
-                     "algorithmic_bytes_per_cell": ab[dom], "avg_launch_ms": kt[dom]},
+                     "algorithmic_bytes_per_cell": ab[dom],
+                     "short_path_bytes_per_cell": out["short_path_bytes_per_leaf"][dom],
+                     "full_path_share": out["full_path_share"][dom], "avg_launch_ms": kt[dom]},
         "cpu_baseline": cpu,
     }
 

@@ -365,12 +365,12 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
     a.tile_list = ctx->act_list;
     a.tile_count = ctx->act_count;
     if (ctx->leaves) {
-      inst_counter(ctx, 0, false);
-      inst_counter(ctx, 1, false);
+      a.active = inst_counter(ctx, 0, true);
       inst_mark(ctx, 0, true);
       launch_leaf_coef(a, ctx->leaf, compute_dt, ctx->stream);
       inst_mark(ctx, 0, false);
       if (compute_dt) launch_dt(a, 1, ctx->stream);
+      a.active = inst_counter(ctx, 1, true);
       inst_mark(ctx, 1, true);
       launch_leaf_update(a, ctx->leaf, ctx->stream);
       inst_mark(ctx, 1, false);
@@ -413,7 +413,7 @@ void enqueue_adjoint(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* 
     a.dphi_part = dphi ? ctx->dphi_part : nullptr;
     a.tile_list = ctx->act_list;
     a.tile_count = ctx->act_count;
-    a.active = inst_counter(ctx, 2, !ctx->leaves);
+    a.active = inst_counter(ctx, 2, true);
     inst_mark(ctx, 2, true);
     if (ctx->leaves) launch_leaf_adjoint(a, ctx->leaf, ctx->lb, ctx->stream);
     else launch_adjoint(a, ctx->stream);
@@ -1384,7 +1384,7 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
       a.err = ctx->err_d;
       a.tile_list = ctx->act_list;
       a.tile_count = ctx->act_count;
-      a.active = inst_counter(ctx, w, !ctx->leaves);
+      a.active = inst_counter(ctx, w, true);
       inst_mark(ctx, w, true);
       if (ctx->leaves) {
         if (w == 0) launch_leaf_coef(a, ctx->leaf, false, ctx->stream);

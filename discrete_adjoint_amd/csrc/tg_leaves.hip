@@ -175,20 +175,22 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_coef(StepArgs a, LeafDev L, int 
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   const InstDev I = a.inst[0];
   double minev = HUGE_VAL, hmax = 0.;
+  bool wet = false;
   if (c < L.n) {
-    const uint32_t meta = L.meta[c];
-    const double h = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
-    double hx = 0., hux = 0., hy = 0., hvy = 0.;
-    if (!touches_boundary(meta)) {
-      const DirNb qx = dir_nb(L, c, meta, 0), qy = dir_nb(L, c, meta, 1);
-      hx = lslope_fwd(a.U.h, c, qx, a.gamma, a.tiny);
-      hux = lslope_fwd(a.U.hu, c, qx, a.gamma, a.tiny);
-      hy = lslope_fwd(a.U.h, c, qy, a.gamma, a.tiny);
-      hvy = lslope_fwd(a.U.hv, c, qy, a.gamma, a.tiny);
-    }
-    double k;
-    if (h > a.tiny) {
-      k = getcoef(h, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.eps, a.tiny);
+    const double h = a.U.h[c];
+    wet = h > a.tiny;
+    if (wet) {
+      const uint32_t meta = L.meta[c];
+      const double hu = a.U.hu[c], hv = a.U.hv[c];
+      double hx = 0., hux = 0., hy = 0., hvy = 0.;
+      if (!touches_boundary(meta)) {
+        const DirNb qx = dir_nb(L, c, meta, 0), qy = dir_nb(L, c, meta, 1);
+        hx = lslope_fwd(a.U.h, c, qx, a.gamma, a.tiny);
+        hux = lslope_fwd(a.U.hu, c, qx, a.gamma, a.tiny);
+        hy = lslope_fwd(a.U.h, c, qy, a.gamma, a.tiny);
+        hvy = lslope_fwd(a.U.hv, c, qy, a.gamma, a.tiny);
+      }
+      const double k = getcoef(h, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.eps, a.tiny);
       if (reduce) {
         double ev = eigen(h, hu, hv, k, a.S.gz[c], a.tiny);
         if (ev > 1000000000.) atomicOr(a.err, 2);
@@ -197,12 +199,18 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_coef(StepArgs a, LeafDev L, int 
         minev = mdx / ev;
         hmax = h;
       }
-    } else {
-      k = I.eps;
+      a.kact[c] = k;
+    } else if (!(h < a.tiny)) {
+      // kact is read only where h >= tiny (zero physical flux below tiny,
+      // update_states reads it only above): epsilon at tiny, unwritten below
+      a.kact[c] = I.eps;
     }
-    a.kact[c] = k;
   }
-  if (reduce) {
+  if (a.active) {
+    const int cnt = __syncthreads_count(wet);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
+  }
+  if (reduce && __syncthreads_or(wet)) {
     __shared__ double smin[BLOCK / 64], smax[BLOCK / 64];
     minev = lwave_min(minev);
     hmax = lwave_max(hmax);
@@ -220,11 +228,40 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_coef(StepArgs a, LeafDev L, int 
   }
 }
 
+// A leaf whose own h and its neighbours' are all below tiny has zero face
+// fluxes and no sources: update_states' own arithmetic with zero fluxes.
+__device__ __forceinline__ bool leaf_dry(const StepArgs& a, const LeafDev& L, int c,
+                                        uint32_t meta) {
+  if (!(a.U.h[c] < a.tiny)) return false;
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (slot_present(meta, k) && !(a.U.h[L.nbr[k * L.n + c]] < a.tiny)) return false;
+  return true;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_leaf_update(StepArgs a, LeafDev L) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
+  const uint32_t meta = (c < L.n) ? L.meta[c] : 0u;
+  const bool dry = (c < L.n) && leaf_dry(a, L, c, meta);
+  if (a.active) {
+    const int cnt = __syncthreads_count(c < L.n && !dry);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
+  }
   if (c >= L.n) return;
+  if (dry) {
+    const double dt = a.dt_rec[a.step];
+    const double dtdx = dt / L.cdx[c], dtdy = dt / L.cdx[L.n + c], z = 0.;
+    double out[3] = {a.U.h[c], a.U.hu[c], a.U.hv[c]};
+#pragma unroll
+    for (int v = 0; v < 3; v++) out[v] = out[v] - dtdx * (z - z) - dtdy * (z - z);
+    if (out[0] < 0.) out[0] = 0;
+    if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
+    a.h_out[c] = out[0];
+    a.hu_out[c] = out[1];
+    a.hv_out[c] = out[2];
+    return;
+  }
   const InstDev I = a.inst[0];
-  const uint32_t meta = L.meta[c];
   const double h = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
   double hx = 0., hvx = 0., hy = 0., huy = 0.;
   if (!touches_boundary(meta)) {
@@ -291,8 +328,14 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_dualk(AdjArgs a, LeafDev L, Leaf
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   if (c >= L.n) return;
   const InstDev I = a.inst[0];
+  const double h = a.U.h[c];
+  if (!(h > a.tiny)) {  // getcoef's dry branch (1 * epsilon); no phi_bed dependence
+    B.kd[c] = 1. * I.eps;
+    if (want_dk) B.dkd[c] = 0.;
+    return;
+  }
   const uint32_t meta = L.meta[c];
-  const double h = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
+  const double hu = a.U.hu[c], hv = a.U.hv[c];
   double hx = 0., hux = 0., hy = 0., hvy = 0.;
   if (!touches_boundary(meta)) {
     const DirNb qx = dir_nb(L, c, meta, 0), qy = dir_nb(L, c, meta, 1);
@@ -437,11 +480,12 @@ struct JacCtx {
 __device__ __forceinline__ void emit_block(const AdjArgs& a, const LeafDev& L, const LeafBufs& B,
                                            const JacCtx& q, int dir, bool is_plus, bool opp_bnd,
                                            int k, int nb, const M3& E, double dh) {
+  // void_neigh_elem (calc_jacobain.C:155-166): a zero block, and a leaf
+  // with h == 0 gathers nothing (its lambda is zero), so nothing is stored
+  if (a.U.h[nb] == 0.) return;
   const double zero = 0.;
   M3 J;
-  if (a.U.h[nb] == 0.) {  // void_neigh_elem (calc_jacobain.C:155-166)
-    J = zero3();
-  } else {
+  {
     const double dh0 = dir ? 0. : dh, dh1 = dir ? dh : 0.;
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -507,19 +551,35 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_jac(AdjArgs a, LeafDev L, LeafBu
   q.c = c;
   q.n = L.n;
   q.meta = L.meta[c];
-  const double h = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
+  const double h = a.U.h[c];
+  // A void leaf among void neighbours stores no block (every block it would
+  // store lands in the record of a leaf with h == 0).
+  if (h == 0.) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (slot_present(q.meta, k) && a.U.h[L.nbr[k * L.n + c]] != 0.) any = true;
+    if (!any) return;
+  }
+  const double hu = a.U.hu[c], hv = a.U.hv[c];
   q.h = h;
   q.lam[0] = a.lam_next.h[c];
   q.lam[1] = a.lam_next.hu[c];
   q.lam[2] = a.lam_next.hv[c];
-  double d[6], hs[2][5];
-  dual_slopes(a, L, c, q.meta, d, hs);
   const double kact = B.kd[c];
   const double g0 = a.S.gx[c], g1 = a.S.gy[c], g2 = a.S.gz[c];
   const double dg0 = a.S.dgx[c], dg1 = a.S.dgy[c], cv0 = a.S.kx[c], cv1 = a.S.ky[c];
-  // orgSrcSgn and stop of update_states (they do not depend on the fluxes)
-  const SrcInfo si =
-      src_switches(h, hu, hv, d[0], d[2], d[3], d[4], cv0, cv1, I.tan_bed, g2, a.ft, a.tiny);
+  // The slopes and source switches enter only the wet self and neighbour
+  // terms (h > tiny); a dry leaf's blocks are flux Jacobians alone.
+  double d[6] = {0., 0., 0., 0., 0., 0.}, hs[2][5] = {{0., 0., 0., 0., 0.}, {0., 0., 0., 0., 0.}};
+  SrcInfo si;
+  si.stop[0] = si.stop[1] = 0;
+  si.sgn[0] = si.sgn[1] = si.sgn[2] = si.sgn[3] = 0.;
+  if (h > a.tiny) {
+    dual_slopes(a, L, c, q.meta, d, hs);
+    // orgSrcSgn and stop of update_states (they do not depend on the fluxes)
+    si = src_switches(h, hu, hv, d[0], d[2], d[3], d[4], cv0, cv1, I.tan_bed, g2, a.ft, a.tiny);
+  }
   q.g2 = g2;
   q.dt = a.dt_rec[a.step];
   q.dtdx = q.dt / L.cdx[c];
@@ -571,7 +631,7 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_jac(AdjArgs a, LeafDev L, LeafBu
           J.a[2][2] -= -dt * si.sgn[3] * tb * (2 * vy * cv1);
         }
       }
-      store_products(B.jac, q.n, 0, c, (h == 0.) ? zero3() : J, q.lam);
+      if (h != 0.) store_products(B.jac, q.n, 0, c, J, q.lam);
     }
   }
 }
@@ -581,7 +641,8 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_jac(AdjArgs a, LeafDev L, LeafBu
 __global__ __launch_bounds__(BLOCK) void k_leaf_dphi(AdjArgs a, LeafDev L, LeafBufs B) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   double dphi = 0.;
-  if (c < L.n) {
+  // lambda^{s+1}_c == 0 contributes a zero term
+  if (c < L.n && (a.lam_next.h[c] != 0. || a.lam_next.hu[c] != 0. || a.lam_next.hv[c] != 0.)) {
     const InstDev I = a.inst[0];
     const uint32_t meta = L.meta[c];
     double d[6], hs[2][5];
@@ -606,7 +667,19 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_dphi(AdjArgs a, LeafDev L, LeafB
 // order, lambda_src[l] * J_src[l][k] added in l order for each k
 __global__ __launch_bounds__(BLOCK) void k_leaf_gather(AdjArgs a, LeafDev L, LeafBufs B) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
+  if (a.active) {  // leaves on the full path: h != 0
+    const int cnt = __syncthreads_count(c < L.n && a.U.h[c] != 0.);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
+  }
   if (c >= L.n) return;
+  // h == 0: every block of this leaf is zero (void_neigh_elem) and so is
+  // lambda; none of its record slots is written
+  if (a.U.h[c] == 0.) {
+    a.lam_h[c] = 0.;
+    a.lam_hu[c] = 0.;
+    a.lam_hv[c] = 0.;
+    return;
+  }
   const int n = L.n;
   const uint32_t meta = L.meta[c];
   double adj[3] = {0., 0., 0.};

@@ -163,3 +163,25 @@ def test_leaf_tables_are_checked():
     with pytest.raises(TgError) as err:
         Simulation(pb, precision=_lib.TG_MIXED)
     assert err.value.code == _lib.TG_ERR_UNSUPPORTED
+
+
+def test_leaf_short_paths_are_taken_and_exact(oracle_lib):
+    """Dry leaves (update), void leaves (reverse step) and the dry kact pass
+    run their short paths on this mesh -- the full-path counters stay below
+    the leaf count -- and the instrumented run is bit-identical to the
+    restatement, which has no short paths."""
+    pb = small_leaf_flow_problem(nsteps=40)
+    _say(f"case leaves short paths n={pb.n} steps 40")
+    w = pb.box_weights((130.0, 260.0, 60.0, 180.0))
+    orc = oracle_lib.LeafOracle.from_problem(pb)
+    _, _, lam_ref, dphi_ref = orc.forward_adjoint(_U(pb), pb.nsteps, w, want_dphi=True)
+    with Simulation(pb, checkpoint_every=10) as sim:
+        sim.instrument(True)
+        sim.run()
+        lam, dphi = sim.dual_solver(w, want_dphi=True)
+        ms, launches, full = sim.instrumentation()
+    assert np.array_equal(lam, lam_ref)
+    assert dphi == pytest.approx(dphi_ref, rel=1e-12)
+    for k in range(3):
+        assert launches[k] > 0
+        assert 0 < full[k] < launches[k] * pb.n
