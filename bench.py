@@ -131,8 +131,10 @@ def warm_oracle(pb):
 
 def timed_run(sim, K, weights, h0, want_lambda=False):
     """Forward K steps + reverse sweep, each timed by wall clock around a
-    synchronous library call.  Returns a dict."""
+    synchronous library call, inputs resident on the device before the
+    clock starts (initial state, terminal weights).  Returns a dict."""
     import numpy as np
+    sim.set_functional(weights)
     sim.set_state(*h0)
     t0 = time.perf_counter()
     st = sim.run(K)
@@ -140,11 +142,61 @@ def timed_run(sim, K, weights, h0, want_lambda=False):
     fwd_ms, _, _ = sim.timing()
     H = sim.get_state()
     t0 = time.perf_counter()
-    lam, dphi = sim.dual_solver(weights, want_dphi=True, want_lambda=want_lambda)
+    lam, dphi = sim.dual_solver(None, want_dphi=True, want_lambda=want_lambda)
     wall_a = time.perf_counter() - t0
     _, adj_ms, rec_ms = sim.timing()
+    stored_ms, stored_steps = sim.stored_sweep_timing()
     return dict(st=st, H=H, lam=lam, dphi=np.atleast_1d(np.asarray(dphi, dtype=float)),
-                wall_f=wall_f, wall_a=wall_a, fwd_ms=fwd_ms, adj_ms=adj_ms, rec_ms=rec_ms)
+                wall_f=wall_f, wall_a=wall_a, fwd_ms=fwd_ms, adj_ms=adj_ms, rec_ms=rec_ms,
+                stored_ms=stored_ms, stored_steps=stored_steps)
+
+
+def forward_step_json(rl, prec, ab=None, sb=None):
+    ab = ab or algo_bytes(prec)
+    sb = sb or short_bytes(prec)
+    f = rl["forward_step"]
+    return {"achieved": f["gbs"], "frac": f["gbs"] / HBM_PEAK_GBS,
+            "avg_step_ms": f["avg_launch_ms"],
+            "algorithmic_bytes_per_cell": ab["forward_step"],
+            "short_path_bytes_per_cell": sb["update"]}
+
+
+def roofline_json(rl, dom, kin, prec, workload, ab=None, sb=None):
+    """The roofline object of the JSON line for the dominant role `dom`."""
+    ab = ab or algo_bytes(prec)
+    sb = sb or short_bytes(prec)
+    k = kin["update" if dom == "forward_step" else dom]
+    return {"bound": "hbm", "achieved": rl[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": rl[dom]["gbs"] / HBM_PEAK_GBS,
+            "traffic": traffic_from_profile(dom, workload), "kernel": dom,
+            "algorithmic_bytes_per_cell": ab[dom],
+            "short_path_bytes_per_cell": sb["update" if dom == "forward_step" else dom],
+            "algorithmic_bytes_per_launch": rl[dom]["bytes_per_launch"],
+            "full_path_share": k["full_path_share"], "short_path_share": k["short_path_share"],
+            "avg_launch_ms": rl[dom]["avg_launch_ms"],
+            "launches_timed": rl[dom]["launches_timed"]}
+
+
+def replay_roofline(r, kin, K):
+    """Launch durations from the timed (graph-replayed) run itself, HIP
+    events on the library's stream: the reverse step = the stored part of
+    the reverse sweep (reverse steps back to back) / its steps; the forward
+    step (coefficients + dt + update, checkpoint copies included) = the
+    forward sweep / K.  Bytes per launch from the instrumented run's
+    counters (the same inputs give the same counts).  Returns per-role
+    figures and the dominant role (largest time in the run)."""
+    out = {}
+    if r["stored_steps"] > 0:
+        ms = r["stored_ms"] / r["stored_steps"]
+        b = kin["adjoint"]["bytes_per_launch"]
+        out["adjoint"] = {"avg_launch_ms": ms, "launches_timed": r["stored_steps"],
+                          "bytes_per_launch": b, "gbs": b / (ms * 1e-3) / 1e9}
+    ms = r["fwd_ms"] / K
+    b = kin["update"]["bytes_per_launch"]
+    out["forward_step"] = {"avg_launch_ms": ms, "launches_timed": K, "bytes_per_launch": b,
+                           "gbs": b / (ms * 1e-3) / 1e9}
+    dom = max(out, key=lambda n: out[n]["avg_launch_ms"] * K)
+    return dom, out
 
 
 def short_bytes(prec):
@@ -157,31 +209,45 @@ def short_bytes(prec):
     return {"coef": es, "update": 6 * es, "adjoint": 7 * es}
 
 
+def path_cells(cells, full, short):
+    """(full-path cells, short-path cells) out of `cells` visited
+    cell-launches, from the kernels' counters: an uncounted full figure (-1)
+    means every cell took the full path, an uncounted short figure every
+    other cell the short path.  Cells in neither were skipped (known +0.0
+    from the zero marks: nothing read or written)."""
+    fc = cells if full < 0 else full
+    sc = cells - fc if short < 0 else short
+    return fc, sc
+
+
 def instrumented_roofline(sim, prec, ncells, K, weights, h0, ab=None, sb=None):
     """One more forward run + reverse sweep, launched kernel by kernel with a
     HIP event pair around every launch (on the library's stream) and the
-    kernels counting their full-path cells.  Per kernel: summed launch time,
-    launches, and algorithmic bytes = full-path cells x the full per-cell
-    bytes + the other cells x the short-path bytes.  Returns the dominant
-    kernel (largest summed time) and per-kernel figures."""
+    kernels counting their full-path and short-path cells.  Per kernel:
+    summed launch time, launches, and algorithmic bytes = full-path cells x
+    the full per-cell bytes + short-path cells x the short-path bytes
+    (skipped cells move none).  Returns the dominant kernel (largest summed
+    time) and per-kernel figures."""
     ab = ab or algo_bytes(prec)
     sb = sb or short_bytes(prec)
     sim.instrument(True)
     timed_run(sim, K, weights, h0)
-    ms, launches, full = sim.instrumentation()
+    ms, launches, full, short = sim.instrumentation()
     sim.instrument(False)
     out = {}
     for i, name in enumerate(KERNELS):
         cells = launches[i] * ncells
-        fc = cells if full[i] < 0 else full[i]
-        nbytes = fc * ab[name] + (cells - fc) * sb[name]
+        fc, sc = path_cells(cells, full[i], short[i])
+        nbytes = fc * ab[name] + sc * sb[name]
         out[name] = {"ms_total": ms[i], "launches": launches[i],
                      "avg_launch_ms": ms[i] / max(launches[i], 1),
                      "full_path_share": fc / max(cells, 1),
+                     "short_path_share": sc / max(cells, 1),
                      "bytes_per_launch": nbytes / max(launches[i], 1),
                      "gbs": nbytes / (ms[i] * 1e-3) / 1e9 if ms[i] > 0 else None}
         log(f"  {name}: {launches[i]} launches, avg {out[name]['avg_launch_ms']:.4f} ms, "
-            f"full path {out[name]['full_path_share']:.3f}, {out[name]['gbs']:.0f} GB/s")
+            f"full path {out[name]['full_path_share']:.3f}, short path "
+            f"{out[name]['short_path_share']:.3f}, {out[name]['gbs']:.0f} GB/s")
     upd = out["update"]
     fwd_ms = ms[0] + ms[1]
     fwd_gbs = upd["bytes_per_launch"] * upd["launches"] / (fwd_ms * 1e-3) / 1e9
@@ -329,7 +395,8 @@ def bench_ensemble(args):
                         "errors_vs_fp64": errors(r)}
         log(f"{name}: {others[name]}")
 
-    dom, kin, fwd_gbs = roof
+    _, kin, _ = roof
+    dom, rl = replay_roofline(m, kin, K)
     ab = algo_bytes(prec)
     workload = f"ensemble256_{args.precision}"
     cpu = None
@@ -379,30 +446,16 @@ def bench_ensemble(args):
             "speedup_vs_fp64_plain": value / plain if plain else None,
             "errors_vs_fp64": errs,
             "other_precisions": others,
-            "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS,
-                                 "algorithmic_bytes_per_cell": ab["forward_step"],
-                                 "short_path_bytes_per_cell": short_bytes(prec)["update"]},
-            "kernels": kin,
+            "forward_step_hbm": forward_step_json(rl, prec),
+            "timed_run_launches": rl,
+            "kernels_event_pair_per_launch": kin,
             "graph_replay": bool(graph),
             "stored_trajectory_bytes": store,
             "tile_variants": {"coef_update_adjoint": m["tiles"],
                               "shapes": [tile_shape(v) for v in m["tiles"]]},
             "adaptive_leaves_fp64": adaptive,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": kin[dom]["gbs"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": kin[dom]["gbs"] / HBM_PEAK_GBS,
-            "traffic": traffic_from_profile(dom, workload),
-            "kernel": dom,
-            "algorithmic_bytes_per_cell": ab[dom],
-            "short_path_bytes_per_cell": short_bytes(prec)[dom],
-            "algorithmic_bytes_per_launch": kin[dom]["bytes_per_launch"],
-            "full_path_share": kin[dom]["full_path_share"],
-            "avg_launch_ms": kin[dom]["avg_launch_ms"],
-        },
+        "roofline": roofline_json(rl, dom, kin, prec, workload),
         "cpu_baseline": cpu,
     }
 
@@ -434,31 +487,32 @@ def adaptive_leg(K=200, k=25, graph=True, cpu_steps=0):
         f"{pb.meta['levels']}")
     w = pb.box_weights()
     h0 = (pb.h, pb.hu, pb.hv)
-    with Simulation(pb, checkpoint_every=k, max_steps=K, use_graph=int(graph)) as sim:
+    # every state of the run kept whole (a few GB): the reverse sweep is
+    # reverse steps back to back, timed as such
+    with Simulation(pb, checkpoint_every=k, max_steps=K, use_graph=int(graph),
+                    store_bytes=32 * pb.n * (K + 1)) as sim:
         sim.run(K)  # untimed: capture of the sweeps, first launches
         sim.dual_solver(w, want_lambda=False)
         r = timed_run(sim, K, w, h0)
         ab = {n: v + LEAF_TABLE_BYTES for n, v in algo_bytes(_lib.TG_FP64).items()}
         sb = {n: v + LEAF_SHORT_EXTRA[n] for n, v in short_bytes(_lib.TG_FP64).items()}
-        dom, kin, fwd_gbs = instrumented_roofline(sim, _lib.TG_FP64, pb.n, K, w, h0, ab=ab,
-                                                  sb=sb)
-    kt = {n: kin[n]["avg_launch_ms"] for n in KERNELS}
-    gbs = {n: kin[n]["gbs"] for n in KERNELS}
+        _, kin, _ = instrumented_roofline(sim, _lib.TG_FP64, pb.n, K, w, h0, ab=ab, sb=sb)
+    dom, rl = replay_roofline(r, kin, K)
     cs = pb.n * K
     out = {
         "leaves": pb.n, "levels": pb.meta["levels"], "steps": K, "checkpoint_every": k,
         "cell_steps_per_s": cs / (r["wall_f"] + r["wall_a"]),
         "forward_cell_steps_per_s": cs / r["wall_f"],
         "adjoint_cell_steps_per_s": cs / r["wall_a"],
-        "kernel_ms": {n: kt[n] for n in KERNELS},
-        "kernel_gbs": gbs,
+        "timed_run_launches": rl,
+        "kernels_event_pair_per_launch": kin,
         "algorithmic_bytes_per_leaf": ab, "short_path_bytes_per_leaf": sb,
-        "full_path_share": {n: kin[n]["full_path_share"] for n in KERNELS},
-        "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS},
+        "forward_step_hbm": forward_step_json(rl, _lib.TG_FP64, ab, sb),
         "dJ_dphibed": float(r["dphi"][0]),
     }
-    log(f"adaptive: {out['cell_steps_per_s']:.3g} cell-steps/s fwd+adj, kernels "
-        + ", ".join(f"{a}={b:.4f} ms ({gbs[a]:.0f} GB/s)" for a, b in kt.items()))
+    log(f"adaptive: {out['cell_steps_per_s']:.3g} cell-steps/s fwd+adj, timed run "
+        + ", ".join(f"{a} {v['avg_launch_ms']:.4f} ms/launch ({v['gbs']:.0f} GB/s)"
+                    for a, v in rl.items()))
     if cpu_steps:
         import oracle
         orc = oracle.LeafOracle.from_problem(pb)
@@ -481,14 +535,14 @@ def adaptive_leg(K=200, k=25, graph=True, cpu_steps=0):
                                 f"their adjoint steps (oracle: FP64 C leaf-table restatement, "
                                 f"-O3, one thread); forward {ccs / tf:.3g}, adjoint "
                                 f"{ccs / ta:.3g} cell-steps/s"}
-    return out, kt, gbs, ab, r, pb, dom
+    return out, (rl, dom, kin, ab, sb), r, pb
 
 
 def bench_adaptive(args):
     K, W, k = args.steps, args.warmup, args.checkpoint_every
-    out, kt, gbs, ab, r, pb, dom = adaptive_leg(K, k, not args.no_graph,
-                                                0 if (args.no_cpu or args.product_only)
-                                                else args.cpu_steps)
+    out, (rl, dom, kin, ab, sb), r, pb = adaptive_leg(K, k, not args.no_graph,
+                                                      0 if (args.no_cpu or args.product_only)
+                                                      else args.cpu_steps)
     wall = r["wall_f"] + r["wall_a"]
     cpu = out.pop("cpu", None)
     return {
@@ -499,12 +553,7 @@ def bench_adaptive(args):
                                 f"levels {pb.meta['levels']}), {K} forward steps + discrete "
                                 f"adjoint with dJ/dphi_bed, checkpoints every {k}",
                        **out),
-        "roofline": {"bound": "hbm", "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs[dom] / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(dom, "adaptive882k"), "kernel": dom,
-                     "algorithmic_bytes_per_cell": ab[dom],
-                     "short_path_bytes_per_cell": out["short_path_bytes_per_leaf"][dom],
-                     "full_path_share": out["full_path_share"][dom], "avg_launch_ms": kt[dom]},
+        "roofline": roofline_json(rl, dom, kin, None, "adaptive882k", ab, sb),
         "cpu_baseline": cpu,
     }
 
@@ -559,8 +608,8 @@ def bench_grid(args):
         errs["lambda0_l2_rel"] = float(np.linalg.norm(m["lam"] - d["lam"]) /
                                        np.linalg.norm(d["lam"]))
         log(f"errors vs fp64: {errs}")
-    dom, kin, fwd_gbs = roof
-    ab = algo_bytes(prec)
+    _, kin, _ = roof
+    dom, rl = replay_roofline(m, kin, K)
     return {
         "metric": METRIC, "value": cell_steps / wall, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": wall * 1e3 / K, "higher_is_better": True,
@@ -573,17 +622,11 @@ def bench_grid(args):
             "cells": pb.n, "precision": args.precision,
             "fp64_cell_steps_per_s": cell_steps / (d["wall_f"] + d["wall_a"]),
             "errors_vs_fp64": errs,
-            "forward_step_hbm": {"achieved": fwd_gbs, "frac": fwd_gbs / HBM_PEAK_GBS},
-            "kernels": kin,
+            "forward_step_hbm": forward_step_json(rl, prec),
+            "timed_run_launches": rl,
+            "kernels_event_pair_per_launch": kin,
         },
-        "roofline": {"bound": "hbm", "achieved": kin[dom]["gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": kin[dom]["gbs"] / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(dom, f"grid{n}"), "kernel": dom,
-                     "algorithmic_bytes_per_cell": ab[dom],
-                     "short_path_bytes_per_cell": short_bytes(prec)[dom],
-                     "algorithmic_bytes_per_launch": kin[dom]["bytes_per_launch"],
-                     "full_path_share": kin[dom]["full_path_share"],
-                     "avg_launch_ms": kin[dom]["avg_launch_ms"]},
+        "roofline": roofline_json(rl, dom, kin, prec, f"grid{n}"),
         "cpu_baseline": None,
     }
 
@@ -616,14 +659,14 @@ def probe(args):
             ens.time_kernel(i, 1)  # first launch of this variant, uncounted
             ens.instrument(True)
             ens.time_kernel(i, args.kernel_reps)
-            ms, launches, full = ens.instrumentation()
+            ms, launches, full, short = ens.instrumentation()
             ens.instrument(False)
             cells = launches[i] * ens.ncell
-            fc = cells if full[i] < 0 else full[i]
-            per = (fc * ab[name] + (cells - fc) * sb[name]) / max(launches[i], 1)
+            fc, sc = path_cells(cells, full[i], short[i])
+            per = (fc * ab[name] + sc * sb[name]) / max(launches[i], 1)
             log(f"probe {name}: {ms[i] / max(launches[i], 1):.4f} ms/launch over {launches[i]} "
-                f"launches, full path {fc / max(cells, 1):.4f}, algorithmic bytes per launch "
-                f"{per:.0f}")
+                f"launches, full path {fc / max(cells, 1):.4f}, short path "
+                f"{sc / max(cells, 1):.4f}, algorithmic bytes per launch {per:.0f}")
         log(f"probe cells {ens.ncell} tiles {ens.tile_selection()}")
     n = 1 << 28  # 2^28 FP32 = 1 GiB read + 1 GiB written
     x = torch.ones(n, dtype=torch.float32, device="cuda")

@@ -161,16 +161,18 @@ SIGNATURES = {
     "tg_get_dt_record": (C.c_int, [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)]),
     "tg_adjoint_run": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.POINTER(tg_functional),
                                  _dp, _dp]),
+    "tg_set_functional": (C.c_int, [C.c_void_p, C.POINTER(tg_functional)]),
     "tg_adjoint_step": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_double,
                                   _dp, _dp]),
     "tg_adjoint_step_grad": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_double,
                                        _dp, _dp, _dp]),
     "tg_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "tg_last_stored_sweep": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int)]),
     "tg_time_kernel": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_int, _dp]),
     "tg_num_cells": (C.c_int64, [C.c_void_p]),
     "tg_set_instrumentation": (C.c_int, [C.c_void_p, C.c_int]),
     "tg_get_instrumentation": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int64),
-                                         C.POINTER(C.c_int64)]),
+                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "tg_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "tg_tile_variant": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int)]),

@@ -46,7 +46,13 @@ struct tg_ctx {
   int ninst = 0, ntiles = 0;
   long long ncell = 0;
 
-  // device fields (element type double or float by es)
+  // device fields (element type double or float by es).  Every state and
+  // adjoint buffer is one allocation of state_bytes: h, hu, hv (es * ncell
+  // bytes each, h first) and, at mk_off, the zero marks (nsuper words) that
+  // the FP64 uniform kernels keep when `marks` is on.
+  size_t state_bytes = 0, mk_off = 0;
+  long long nsuper = 0;
+  bool marks = false;
   void* U[3] = {nullptr, nullptr, nullptr};
   void* Ut[3] = {nullptr, nullptr, nullptr};
   void* U_home = nullptr;  // U[0] as allocated
@@ -54,6 +60,8 @@ struct tg_ctx {
   void* statics_f = nullptr;  // 7 * ncell FP32 copy for the FP32 adjoint of TG_FWD64_ADJ32
   void* kact = nullptr;
   void* lam[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  void* fw = nullptr;  // terminal weights kept on the device (tg_set_functional)
+  bool have_fw = false;
   RedDev* red = nullptr;
   double* dt_rec = nullptr;  // ninst * max_steps
   double* time_d = nullptr;  // ninst
@@ -90,22 +98,25 @@ struct tg_ctx {
 
   // captured sweeps (cfg.use_graph): one forward and one reverse graph,
   // re-captured when their key changes
-  SweepGraph gf, ga;
+  SweepGraph gf, ga, gb;  // forward; reverse stored part; reverse segments
   bool gf_end_is_b = false;  // the captured forward sweep leaves U^N in Ut
 
   // timing
-  hipEvent_t ev[6] = {};
+  hipEvent_t ev[7] = {};
   double fwd_ms = 0, adj_ms = 0, rec_ms = 0;
+  double stored_ms = 0;  // reverse sweep over the stored trajectory only
+  int stored_steps = 0;
 
   // instrumentation (tg_set_instrumentation): an event pair around every
   // coefficient / update / adjoint launch, full-path cell counters
   bool instrument = false;
-  unsigned long long* active_d = nullptr;  // [3]
+  unsigned long long* active_d = nullptr;  // [6]: full-path, then short-path cells
   std::vector<hipEvent_t> iev;             // pool, two per recorded launch
   std::vector<int> iev_kind;               // kernel of each recorded pair
   double inst_ms[3] = {0, 0, 0};
   long long inst_launches[3] = {0, 0, 0};
   bool inst_counted[3] = {true, true, true};
+  bool inst_short_counted[3] = {true, true, true};
 };
 
 namespace {
@@ -150,12 +161,14 @@ void drop_graph(SweepGraph& G) {
 void free_fields(tg_ctx* ctx) {
   drop_graph(ctx->gf);
   drop_graph(ctx->ga);
-  for (int k = 0; k < 3; k++) {
-    dfree(ctx->U[k]);
-    dfree(ctx->Ut[k]);
-    dfree(ctx->lam[0][k]);
-    dfree(ctx->lam[1][k]);
-  }
+  drop_graph(ctx->gb);
+  dfree(ctx->U[0]);  // one allocation per buffer, h first
+  dfree(ctx->Ut[0]);
+  dfree(ctx->lam[0][0]);
+  dfree(ctx->lam[1][0]);
+  for (int k = 0; k < 3; k++) ctx->U[k] = ctx->Ut[k] = ctx->lam[0][k] = ctx->lam[1][k] = nullptr;
+  dfree(ctx->fw);
+  ctx->have_fw = false;
   dfree(ctx->statics);
   dfree(ctx->statics_f);
   dfree(ctx->kact);
@@ -239,6 +252,18 @@ void use_tiles(const tg_ctx* ctx, int kernel, A& a) {
   a.shape = v;
 }
 
+// Zero marks of a buffer given by its field pointers (h first), or null
+// when the kernels keep none.
+uint32_t* marks_of(const tg_ctx* ctx, void* const f[3]) {
+  return ctx->marks ? (uint32_t*)((char*)f[0] + ctx->mk_off) : nullptr;
+}
+// Marks of a buffer just written other than by the marking kernels:
+// nothing known.
+void clear_marks(tg_ctx* ctx, void* base) {
+  if (ctx->nsuper > 0)
+    launch_zero((char*)base + ctx->mk_off, sizeof(uint32_t) * (size_t)ctx->nsuper, ctx->stream);
+}
+
 // Instrumentation: resolves the recorded event pairs into the per-kernel
 // totals and frees the pool slots.
 int inst_flush(tg_ctx* ctx) {
@@ -283,6 +308,13 @@ unsigned long long* inst_counter(tg_ctx* ctx, int kind, bool counts) {
   if (!counts) ctx->inst_counted[kind] = false;
   return counts ? ctx->active_d + kind : nullptr;
 }
+// counter of the cells a launch writes on a short path (null when not
+// instrumenting; kernels that do not count leave the figure unknown)
+unsigned long long* inst_short(tg_ctx* ctx, int kind, bool counts) {
+  if (!ctx->instrument) return nullptr;
+  if (!counts) ctx->inst_short_counted[kind] = false;
+  return counts ? ctx->active_d + 3 + kind : nullptr;
+}
 
 template <class S>
 StepArgsM<S> step_args_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s) {
@@ -304,6 +336,8 @@ void enqueue_step_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s, b
   use_tiles(ctx, 0, a);
   inst_counter(ctx, 0, false);
   inst_counter(ctx, 1, false);
+  inst_short(ctx, 0, false);
+  inst_short(ctx, 1, false);
   inst_mark(ctx, 0, true);
   launch_coef_m<S>(a, compute_dt, ctx->stream);
   inst_mark(ctx, 0, false);
@@ -341,6 +375,7 @@ void enqueue_adjoint_m(tg_ctx* ctx, void* const U[3], void* const lnext[3], void
   a.tile_list = ctx->act_list;
   a.tile_count = ctx->act_count;
   a.active = inst_counter(ctx, 2, true);
+  inst_short(ctx, 2, false);
   inst_mark(ctx, 2, true);
   launch_adjoint_m<S>(a, ctx->stream);
   inst_mark(ctx, 2, false);
@@ -365,6 +400,8 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
     a.tile_list = ctx->act_list;
     a.tile_count = ctx->act_count;
     if (ctx->leaves) {
+      inst_short(ctx, 0, false);
+      inst_short(ctx, 1, false);
       a.active = inst_counter(ctx, 0, true);
       inst_mark(ctx, 0, true);
       launch_leaf_coef(a, ctx->leaf, compute_dt, ctx->stream);
@@ -376,14 +413,18 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
       inst_mark(ctx, 1, false);
       return;
     }
+    a.mk_in = marks_of(ctx, in);
+    a.mk_out = marks_of(ctx, out);
     use_tiles(ctx, 0, a);
     a.active = inst_counter(ctx, 0, true);
+    a.short_cells = inst_short(ctx, 0, true);
     inst_mark(ctx, 0, true);
     launch_coef(a, compute_dt, ctx->stream);
     inst_mark(ctx, 0, false);
     if (compute_dt) launch_dt(a, ctx->ninst, ctx->stream);
     use_tiles(ctx, 1, a);
     a.active = inst_counter(ctx, 1, true);
+    a.short_cells = inst_short(ctx, 1, true);
     inst_mark(ctx, 1, true);
     launch_update(a, ctx->stream);
     inst_mark(ctx, 1, false);
@@ -413,7 +454,11 @@ void enqueue_adjoint(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* 
     a.dphi_part = dphi ? ctx->dphi_part : nullptr;
     a.tile_list = ctx->act_list;
     a.tile_count = ctx->act_count;
+    a.mk_U = marks_of(ctx, U);
+    a.mk_lnext = marks_of(ctx, lnext);
+    a.mk_out = marks_of(ctx, lout);
     a.active = inst_counter(ctx, 2, true);
+    a.short_cells = inst_short(ctx, 2, !ctx->leaves);
     inst_mark(ctx, 2, true);
     if (ctx->leaves) launch_leaf_adjoint(a, ctx->leaf, ctx->lb, ctx->stream);
     else launch_adjoint(a, ctx->stream);
@@ -512,11 +557,11 @@ int fill_stats(tg_ctx* ctx, tg_stats* st, int nst) {
 }
 
 int ensure_store(tg_ctx* ctx, std::vector<void*>& v, size_t count) {
-  const size_t bytes = (size_t)ctx->es * 3 * (size_t)ctx->ncell;
   while (v.size() < count) {
     void* p = nullptr;
-    int rc = dalloc(ctx, &p, bytes);
+    int rc = dalloc(ctx, &p, ctx->state_bytes);
     if (rc) return rc;
+    clear_marks(ctx, p);
     v.push_back(p);
   }
   return TG_OK;
@@ -529,26 +574,32 @@ int ensure_traj(tg_ctx* ctx, size_t count) {
   if (count <= ctx->traj_cap) return TG_OK;
   drop_graph(ctx->gf);
   drop_graph(ctx->ga);
+  drop_graph(ctx->gb);
   dfree(ctx->traj_block);
   ctx->traj_cap = 0;
-  const size_t bytes = (size_t)ctx->es * 3 * (size_t)ctx->ncell * count;
-  int rc = dalloc(ctx, &ctx->traj_block, bytes);
+  int rc = dalloc(ctx, &ctx->traj_block, ctx->state_bytes * count);
   if (rc) return rc;
   ctx->traj_cap = count;
+  for (size_t i = 0; i < count; i++)
+    clear_marks(ctx, (char*)ctx->traj_block + i * ctx->state_bytes);
   return TG_OK;
 }
 
 void* traj_state(tg_ctx* ctx, int i) {
-  return (char*)ctx->traj_block + (size_t)i * ctx->es * 3 * (size_t)ctx->ncell;
+  return (char*)ctx->traj_block + (size_t)i * ctx->state_bytes;
 }
 
 void split(tg_ctx* ctx, void* base, void* out[3]) {
   for (int k = 0; k < 3; k++) out[k] = (char*)base + (size_t)k * ctx->es * ctx->ncell;
 }
 
+// copies the three fields and the zero marks
 void copy_state(tg_ctx* ctx, void* dst, void* const src[3]) {
   const size_t b = (size_t)ctx->es * ctx->ncell;
   for (int k = 0; k < 3; k++) launch_copy(src[k], (char*)dst + k * b, b, ctx->stream);
+  if (ctx->nsuper > 0)
+    launch_copy((char*)src[0] + ctx->mk_off, (char*)dst + ctx->mk_off,
+                sizeof(uint32_t) * (size_t)ctx->nsuper, ctx->stream);
 }
 
 bool same_key(const SweepGraph& G, const long long key[8]) {
@@ -581,7 +632,7 @@ int capture(tg_ctx* ctx, SweepGraph& G, const long long key[8], F&& body) {
 // below s0 the states alternate between the two working buffers and U^s is
 // checkpointed every k steps.  U^N ends in a working buffer.
 int stored_steps(const tg_ctx* ctx, int nsteps) {
-  const double state = (double)ctx->es * 3 * (double)ctx->ncell;
+  const double state = (double)ctx->state_bytes;
   const long long m = (long long)((double)ctx->cfg.store_bytes / state);
   return (int)(m < nsteps ? (m < 0 ? 0 : m) : nsteps);
 }
@@ -682,10 +733,18 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
   rc = ensure_store(ctx, ctx->seg, (size_t)(s0 > 0 ? k : 1));
   if (rc) return rc;
   const bool dphi = dJ != nullptr;
-  // lambda^N = func_sens at the final step (calc_adjoint.C:93-97)
-  for (int c = 0; c < 3; c++) {
-    rc = h2d_field(ctx, ctx->lam[0][c], 0, n, w + c * n);
-    if (rc) return rc;
+  // lambda^N = func_sens at the final step (calc_adjoint.C:93-97): from the
+  // host, or (w null) the weights kept on the device
+  if (w) {
+    for (int c = 0; c < 3; c++) {
+      rc = h2d_field(ctx, ctx->lam[0][c], 0, n, w + c * n);
+      if (rc) return rc;
+    }
+    clear_marks(ctx, ctx->lam[0][0]);
+  } else {
+    void* f[3];
+    split(ctx, ctx->fw, f);
+    copy_state(ctx, ctx->lam[0][0], f);
   }
   // steps N-1 .. s0 read the stored trajectory; below s0 the sweep works in
   // segments of k: restore U^{a} from its checkpoint, recompute U^{a+1..b-1}
@@ -723,28 +782,41 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
   };
   int cur = 0;
   float rec_ms = 0, ms = 0;
+  // the stored part (adjoint launches only) and the segments are timed
+  // apart: ev[2] .. ev[6] .. ev[3]
   if (ctx->cfg.use_graph && !ctx->instrument) {
     const long long key[8] = {N, k, dphi, ctx->sel[0], ctx->sel[1], ctx->sel[2], s0, 0};
     if (!same_key(ctx->ga, key)) {
+      drop_graph(ctx->gb);
       rc = capture(ctx, ctx->ga, key, [&]() {
         int c = 0;
         if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
+        else launch_zero(ctx->act_count, sizeof(int), ctx->stream);  // the graph is never empty
         stored(c);
-        for (int sg = nseg - 1; sg >= 0; sg--) {
-          recompute(sg);
-          reverse(sg, c);
-        }
       });
       if (rc) return rc;
+      if (nseg > 0) {
+        rc = capture(ctx, ctx->gb, key, [&]() {
+          int c = (N - s0) % 2;
+          for (int sg = nseg - 1; sg >= 0; sg--) {
+            recompute(sg);
+            reverse(sg, c);
+          }
+        });
+        if (rc) return rc;
+      }
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     HIPCHK(ctx, hipGraphLaunch(ctx->ga.x, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+    if (nseg > 0) HIPCHK(ctx, hipGraphLaunch(ctx->gb.x, ctx->stream));
     cur = N % 2;
     rec_ms = -1.f;  // not separable inside one launch
   } else {
     if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     stored(cur);
+    HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
     for (int sg = nseg - 1; sg >= 0; sg--) {
       HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
       recompute(sg);
@@ -762,6 +834,9 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
   ctx->adj_ms = ms;
   ctx->rec_ms = rec_ms;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[6]));
+  ctx->stored_ms = ms;
+  ctx->stored_steps = N - s0;
   if (lambda0)
     for (int c = 0; c < 3; c++) {
       rc = d2h_field(ctx, ctx->lam[cur][c], 0, n, lambda0 + c * n);
@@ -911,6 +986,7 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   ctx->ninst = nm;
   ctx->inst_h.assign(nm, InstDev{});
   for (auto& t : ctx->tiles_h) t.clear();
+  ctx->nsuper = 0;
   long long off = 0;
   for (int b = 0; b < nm && leaves; b++) {  // one member, one thread per leaf
     InstDev& I = ctx->inst_h[0];
@@ -918,13 +994,18 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     I.maxtime = I.nextoutput = I.nextsave = HUGE_VAL;
     if (params) params_into(I, &params[0]);
     for (int v = 0; v < NSHAPE; v++)
-      for (int x = 0; x < meshes[0].n; x += BLOCK) ctx->tiles_h[v].push_back(TileDev{0, x, 0, 0});
+      for (int x = 0; x < meshes[0].n; x += BLOCK)
+        ctx->tiles_h[v].push_back(TileDev{0, x, 0, 0, -1, -1, -1, -1});
     off = meshes[0].n;
   }
   for (int b = 0; b < nm && !leaves; b++) {
     const tg_mesh* m = &meshes[b];
     InstDev& I = ctx->inst_h[b];
     I.nx = m->nx; I.ny = m->ny; I.off = off;
+    I.nsx = (m->nx + MARK_W - 1) / MARK_W;
+    I.nsy = (m->ny + MARK_H - 1) / MARK_H;
+    I.soff = ctx->nsuper;
+    ctx->nsuper += (long long)I.nsx * I.nsy;
     I.dx = m->dx; I.dy = m->dy;
     I.inv_dx = 1 / m->dx; I.inv_sx = 1 / (m->dx + m->dx);
     I.inv_dy = 1 / m->dy; I.inv_sy = 1 / (m->dy + m->dy);
@@ -933,7 +1014,14 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     for (int v = 0; v < NSHAPE; v++) {
       const int TX = SHAPE_TX[v], TY = BLOCK / SHAPE_TX[v];
       for (int ty = 0; ty < m->ny; ty += TY)
-        for (int tx = 0; tx < m->nx; tx += TX) ctx->tiles_h[v].push_back(TileDev{b, tx, ty, 0});
+        for (int tx = 0; tx < m->nx; tx += TX) {
+          const int bx = tx / MARK_W, by = ty / MARK_H;
+          const int mw = (int)I.soff + by * I.nsx + bx;
+          ctx->tiles_h[v].push_back(TileDev{b, tx, ty, mw, bx > 0 ? mw - 1 : -1,
+                                            bx + 1 < I.nsx ? mw + 1 : -1,
+                                            by > 0 ? mw - I.nsx : -1,
+                                            by + 1 < I.nsy ? mw + I.nsx : -1});
+        }
     }
     off += (long long)m->nx * m->ny;
   }
@@ -946,12 +1034,21 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   }
   const long long n = ctx->ncell;
   const size_t fb = (size_t)ctx->es * n;
+  auto align256 = [](size_t b) { return (b + 255) / 256 * 256; };
+  ctx->mk_off = align256(3 * fb);
+  ctx->state_bytes = ctx->mk_off + align256(sizeof(uint32_t) * (size_t)ctx->nsuper);
+  // the FP64 uniform kernels keep zero marks; the leaf and the
+  // mixed-precision kernels do not
+  ctx->marks = ctx->mode == TG_FP64 && !leaves && ctx->nsuper > 0;
   int rc = 0;
-  for (int k = 0; k < 3 && !rc; k++) {
-    rc = dalloc(ctx, &ctx->U[k], fb);
-    if (!rc) rc = dalloc(ctx, &ctx->Ut[k], fb);
-    if (!rc) rc = dalloc(ctx, &ctx->lam[0][k], fb);
-    if (!rc) rc = dalloc(ctx, &ctx->lam[1][k], fb);
+  void** bufs[4] = {ctx->U, ctx->Ut, ctx->lam[0], ctx->lam[1]};
+  for (int i = 0; i < 4 && !rc; i++) {
+    rc = dalloc(ctx, &bufs[i][0], ctx->state_bytes);
+    if (!rc) {
+      bufs[i][1] = (char*)bufs[i][0] + fb;
+      bufs[i][2] = (char*)bufs[i][0] + 2 * fb;
+      clear_marks(ctx, bufs[i][0]);
+    }
   }
   const size_t sb = (size_t)ctx->ses * n;
   ctx->U_home = ctx->U[0];
@@ -1196,6 +1293,7 @@ int tg_set_state(tg_ctx* ctx, const double* h, const double* hu, const double* h
     int rc = h2d_field(ctx, ctx->U[k], 0, ctx->ncell, src[k]);
     if (rc) return rc;
   }
+  clear_marks(ctx, ctx->U[0]);
   int rc = reset_time(ctx);
   if (rc) return rc;
   ctx->rec_steps = ctx->run_steps = 0;
@@ -1213,6 +1311,7 @@ int tg_set_member_state(tg_ctx* ctx, int member, const double* h, const double* 
     int rc = h2d_field(ctx, ctx->U[k], I.off, (long long)I.nx * I.ny, src[k]);
     if (rc) return rc;
   }
+  clear_marks(ctx, ctx->U[0]);
   int rc = reset_time(ctx);
   if (rc) return rc;
   ctx->rec_steps = ctx->run_steps = 0;
@@ -1289,8 +1388,29 @@ int tg_get_member_dt_record(tg_ctx* ctx, int member, double* dt_out, int capacit
 
 int tg_adjoint_run(tg_ctx* ctx, const tg_params* prm, const tg_functional* fn,
                    double* lambda0_out, double* dJ_dphibed_out) {
+  if (!ctx || (fn && !fn->terminal_weight)) return TG_ERR_ARG;
+  if (!fn && !ctx->have_fw) return set_err(ctx, TG_ERR_STATE, "no functional set");
+  return adjoint_loop(ctx, prm, fn ? fn->terminal_weight : nullptr, lambda0_out,
+                      dJ_dphibed_out);
+}
+
+int tg_set_functional(tg_ctx* ctx, const tg_functional* fn) {
   if (!ctx || !fn || !fn->terminal_weight) return TG_ERR_ARG;
-  return adjoint_loop(ctx, prm, fn->terminal_weight, lambda0_out, dJ_dphibed_out);
+  if (!ctx->have_mesh) return set_err(ctx, TG_ERR_STATE, "no mesh");
+  if (!ctx->fw) {
+    int rc = dalloc(ctx, &ctx->fw, ctx->state_bytes);
+    if (rc) return rc;
+  }
+  void* f[3];
+  split(ctx, ctx->fw, f);
+  for (int c = 0; c < 3; c++) {
+    int rc = h2d_field(ctx, f[c], 0, ctx->ncell, fn->terminal_weight + c * ctx->ncell);
+    if (rc) return rc;
+  }
+  clear_marks(ctx, ctx->fw);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->have_fw = true;
+  return TG_OK;
 }
 
 int tg_get_member_dphi(tg_ctx* ctx, double* dJ_dphibed) {
@@ -1316,6 +1436,7 @@ int adjoint_step_impl(tg_ctx* ctx, const tg_params* prm, int s, double dt,
     rc = h2d_field(ctx, ctx->lam[0][c], 0, n, lambda_next + c * n);
     if (rc) return rc;
   }
+  clear_marks(ctx, ctx->lam[0][0]);
   const bool dphi = dphi_member != nullptr;
   if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
   enqueue_adjoint(ctx, ctx->U, ctx->lam[0], ctx->lam[1], s, dphi);
@@ -1356,6 +1477,13 @@ int tg_last_timing(tg_ctx* ctx, double* forward_ms, double* adjoint_ms, double* 
   return TG_OK;
 }
 
+int tg_last_stored_sweep(tg_ctx* ctx, double* stored_ms, int* stored_steps) {
+  if (!ctx) return TG_ERR_ARG;
+  if (stored_ms) *stored_ms = ctx->stored_ms;
+  if (stored_steps) *stored_steps = ctx->stored_steps;
+  return TG_OK;
+}
+
 // Average launch time of kernel `which` with tile variant v (-1: the
 // selected one), over reps back-to-back launches on the current state.
 int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms) {
@@ -1384,7 +1512,10 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
       a.err = ctx->err_d;
       a.tile_list = ctx->act_list;
       a.tile_count = ctx->act_count;
+      a.mk_in = marks_of(ctx, ctx->U);
+      a.mk_out = marks_of(ctx, ctx->Ut);
       a.active = inst_counter(ctx, w, true);
+      a.short_cells = inst_short(ctx, w, !ctx->leaves);
       inst_mark(ctx, w, true);
       if (ctx->leaves) {
         if (w == 0) launch_leaf_coef(a, ctx->leaf, false, ctx->stream);
@@ -1450,6 +1581,7 @@ int tg_autotune(tg_ctx* ctx, const tg_params* prm, int reps, int* variant_out, d
   // the adjoint variant reads lam[0]: give it defined values (zeros)
   const size_t fb = (size_t)ctx->es * ctx->ncell;
   for (int c = 0; c < 3; c++) HIPCHK(ctx, hipMemsetAsync(ctx->lam[0][c], 0, fb, ctx->stream));
+  clear_marks(ctx, ctx->lam[0][0]);
   for (int w = 0; w < 3; w++) {
     double best = HUGE_VAL;
     int bv = ctx->sel[w];
@@ -1507,15 +1639,16 @@ int tg_set_instrumentation(tg_ctx* ctx, int on) {
   int rc = inst_flush(ctx);
   if (rc) return rc;
   if (on && !ctx->active_d) {
-    rc = dalloc(ctx, (void**)&ctx->active_d, 3 * sizeof(unsigned long long));
+    rc = dalloc(ctx, (void**)&ctx->active_d, 6 * sizeof(unsigned long long));
     if (rc) return rc;
   }
   if (on) {
-    HIPCHK(ctx, hipMemsetAsync(ctx->active_d, 0, 3 * sizeof(unsigned long long), ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->active_d, 0, 6 * sizeof(unsigned long long), ctx->stream));
     for (int k = 0; k < 3; k++) {
       ctx->inst_ms[k] = 0;
       ctx->inst_launches[k] = 0;
       ctx->inst_counted[k] = true;
+      ctx->inst_short_counted[k] = true;
     }
   }
   ctx->instrument = on != 0;
@@ -1523,12 +1656,12 @@ int tg_set_instrumentation(tg_ctx* ctx, int on) {
 }
 
 int tg_get_instrumentation(tg_ctx* ctx, double* kernel_ms, int64_t* launches,
-                           int64_t* full_cells) {
+                           int64_t* full_cells, int64_t* short_cells) {
   if (!ctx) return TG_ERR_ARG;
   if (!ctx->active_d) return set_err(ctx, TG_ERR_STATE, "instrumentation was never on");
   int rc = inst_flush(ctx);
   if (rc) return rc;
-  unsigned long long cnt[3] = {0, 0, 0};
+  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
   HIPCHK(ctx, hipMemcpyAsync(cnt, ctx->active_d, sizeof(cnt), hipMemcpyDeviceToHost,
                              ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1536,6 +1669,7 @@ int tg_get_instrumentation(tg_ctx* ctx, double* kernel_ms, int64_t* launches,
     if (kernel_ms) kernel_ms[k] = ctx->inst_ms[k];
     if (launches) launches[k] = ctx->inst_launches[k];
     if (full_cells) full_cells[k] = ctx->inst_counted[k] ? (int64_t)cnt[k] : -1;
+    if (short_cells) short_cells[k] = ctx->inst_short_counted[k] ? (int64_t)cnt[3 + k] : -1;
   }
   return TG_OK;
 }

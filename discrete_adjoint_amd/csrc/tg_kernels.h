@@ -11,6 +11,10 @@ namespace tg {
 struct InstDev {
   int nx, ny;
   long long off;
+  // zero-mark blocks of the member: nsx x nsy blocks of MARK_W x MARK_H
+  // cells, words [soff, soff + nsx * nsy) of a mark array
+  int nsx, nsy;
+  long long soff;
   double dx, dy;
   // 1/dx, 1/(dx+dx), 1/dy, 1/(dy+dy): the dual slopes' reciprocals
   // (dualmesh.C:1459), evaluated once on the host (same IEEE result)
@@ -20,9 +24,12 @@ struct InstDev {
 };
 
 // A launch tile: TX x (BLOCK / TX) cells of instance `inst` starting at
-// (x0, y0).  One workgroup per tile.
+// (x0, y0).  One workgroup per tile.  mw: the zero-mark word of the tile's
+// block; mwW, mwE, mwS, mwN: those of the four side blocks (-1 outside the
+// member).
 struct TileDev {
-  int inst, x0, y0, pad;
+  int inst, x0, y0, mw;
+  int mwW, mwE, mwS, mwN;
 };
 
 // Per-instance CFL reduction cells: bit patterns of non-negative doubles
@@ -44,6 +51,21 @@ constexpr int SHAPE_TX[NSHAPE] = {64, 64, 32, 16};
 constexpr bool SHAPE_XCD[NSHAPE] = {false, true, true, true};
 constexpr int NXCD = 8;
 constexpr int DEFAULT_VARIANT = 0;
+
+// Zero marks.  A state (h, hu, hv) or adjoint buffer can carry one 32-bit
+// word per MARK_W x MARK_H block of a member's cells.  Every tile shape cuts
+// a block into four tiles; byte b of the word is 1 when the b-th tile held
+// +0.0 in all three fields the last time it was written, and 0 when it was
+// written with anything else (each tile stores its own byte: no atomics).
+// A word equal to MARK_FULL says the whole block is +0.0 (a block cut off
+// by the member's edge, with fewer than four tiles of some shape, may never
+// reach MARK_FULL: that only costs a skip).  Every kernel that writes a
+// marked buffer keeps its words; a buffer written any other way has its
+// words cleared (nothing known).
+constexpr int MARK_W = 64, MARK_H = 16;
+constexpr uint32_t MARK_FULL = 0x01010101u;
+static_assert(MARK_W % 64 == 0 && MARK_H % (BLOCK / 16) == 0 && MARK_W * MARK_H == 4 * BLOCK,
+              "every tile shape must cut a mark block into four tiles");
 
 struct Fields {
   const double* h;
@@ -87,6 +109,13 @@ struct StepArgs {
   // by the classifying pass, walked by the update) and its length
   int* tile_list;
   int* tile_count;
+  // zero marks of U and of the output state (both null: no marks); a tile
+  // the marks show to need nothing is skipped by the coefficient and the
+  // classifying pass
+  const uint32_t* mk_in;
+  uint32_t* mk_out;
+  // null, or a counter the launch adds the cells it writes on a short path to
+  unsigned long long* short_cells;
 };
 
 struct AdjArgs {
@@ -109,6 +138,12 @@ struct AdjArgs {
   unsigned long long* active;  // null, or counter of cells in non-void tiles
   int* tile_list;        // compacted non-void tiles (classifying pass) and count
   int* tile_count;
+  // zero marks of U^s, lambda^{s+1} and lambda^s (all null: no marks) and
+  // the short-path cell counter, as in StepArgs
+  const uint32_t* mk_U;
+  const uint32_t* mk_lnext;
+  uint32_t* mk_out;
+  unsigned long long* short_cells;
 };
 
 // Mixed-precision counterparts: FP32 arithmetic, FP32 static coefficients

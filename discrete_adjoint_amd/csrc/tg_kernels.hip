@@ -67,6 +67,50 @@ __device__ __forceinline__ Cell cell_of(const TileDev& t, const InstDev& I) {
   return q;
 }
 
+// ------------------------------------------------------------- zero marks --
+
+// the tile's byte within its block's word
+template <int TX>
+__device__ __forceinline__ int mark_byte(const TileDev& t) {
+  constexpr int TY = BLOCK / TX;
+  return ((t.y0 % MARK_H) / TY) * (MARK_W / TX) + (t.x0 % MARK_W) / TX;
+}
+__device__ __forceinline__ bool bitzero3(double a, double b, double c) {
+  return (dbits(a) | dbits(b) | dbits(c)) == 0ull;
+}
+// Records in the output marks whether the tile holds +0.0 only (all: the
+// workgroup's reduction of "this cell is +0.0 in all three fields or
+// outside the member").  Thread 0 stores the tile's byte.
+template <int TX>
+__device__ __forceinline__ void mark_store(uint32_t* mk, const TileDev& t, bool all) {
+  if (threadIdx.x == 0) ((unsigned char*)(mk + t.mw))[mark_byte<TX>(t)] = all ? 1 : 0;
+}
+
+// Skips decided from the marks alone (uniform per workgroup: scalar loads).
+// Coefficient pass: a tile whose block of U is +0.0 -- h = +0.0 < tiny: no
+// kact written, nothing added to the CFL reduction.
+__device__ __forceinline__ bool coef_skip(const uint32_t* mk, const TileDev& t) {
+  return mk[t.mw] == MARK_FULL;
+}
+// Update: a tile whose block and the four side blocks of U are +0.0 -- every
+// cell and side neighbour has h < tiny, so the fluxes are zero and there
+// are no sources, and U^{s+1} = U^s - dtdx*(0 - 0) - dtdy*(0 - 0) = +0.0
+// for a finite dt -- and whose block of the output already holds +0.0.
+__device__ __forceinline__ bool update_skip(const uint32_t* in, const uint32_t* out,
+                                            const TileDev& t, double dt) {
+  return in[t.mw] == MARK_FULL && out[t.mw] == MARK_FULL &&
+         (t.mwW < 0 || in[t.mwW] == MARK_FULL) && (t.mwE < 0 || in[t.mwE] == MARK_FULL) &&
+         (t.mwS < 0 || in[t.mwS] == MARK_FULL) && (t.mwN < 0 || in[t.mwN] == MARK_FULL) &&
+         isfinite(dt);
+}
+// Adjoint: a tile whose block of U^s is +0.0 (void cells: lambda^s = +0.0),
+// of lambda^{s+1} too when the gradient is wanted (no bed-friction terms),
+// and whose block of lambda^s already holds +0.0.
+__device__ __forceinline__ bool adjoint_skip(const uint32_t* mu, const uint32_t* ml,
+                                             const uint32_t* out, const TileDev& t) {
+  return mu[t.mw] == MARK_FULL && out[t.mw] == MARK_FULL && (!ml || ml[t.mw] == MARK_FULL);
+}
+
 // ---------------------------------------------------------------- forward --
 
 // Limited slope from its two one-sided differences dp = (u+ - u)/dx and
@@ -83,6 +127,7 @@ __device__ __forceinline__ double slope_from(double dp, double dm, double dx, do
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
   const TileDev t = a.tiles[blockIdx.x];
+  if (a.mk_in && coef_skip(a.mk_in, t)) return;
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
   double minev = HUGE_VAL, hmax = 0.;
@@ -119,6 +164,10 @@ __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
   if (a.active) {
     const int cnt = __syncthreads_count(wet);
     if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
+  }
+  if (a.short_cells) {
+    const int cnt = __syncthreads_count(q.act && !wet);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
   }
   if (reduce && __syncthreads_or(wet)) {
     __shared__ double smin[BLOCK / 64], smax[BLOCK / 64];
@@ -205,7 +254,11 @@ struct FaceLds {
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
   constexpr int TY = BLOCK / TX;
-  const TileDev t = a.tiles[blockIdx.x];
+  const int tile = (int)blockIdx.x;
+  const TileDev t = a.tiles[tile];
+  if (a.mk_in &&
+      update_skip(a.mk_in, a.mk_out, t, a.dt_rec[(long long)t.inst * a.max_steps + a.step]))
+    return;
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
   const long long nx = I.nx;
@@ -219,21 +272,76 @@ __global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
     if (ty == TY - 1 && q.hasN) wet = wet || !(a.U.h[q.c + nx] < a.tiny);
   }
   if (__syncthreads_or(wet)) {
-    if (threadIdx.x == 0) a.tile_list[atomicAdd(a.tile_count, 1)] = (int)blockIdx.x;
+    if (threadIdx.x == 0) a.tile_list[atomicAdd(a.tile_count, 1)] = tile;
     return;
   }
-  if (!q.act) return;
+  if (a.short_cells) {
+    const int cnt = __syncthreads_count(q.act);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
+  }
+  double out[3] = {0., 0., 0.};
+  if (q.act) {
+    const long long c = q.c;
+    const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
+    const double dtdx = dt / I.dx, dtdy = dt / I.dy, zero = 0.;
+    out[0] = a.U.h[c];
+    out[1] = a.U.hu[c];
+    out[2] = a.U.hv[c];
+#pragma unroll
+    for (int v = 0; v < 3; v++) out[v] = out[v] - dtdx * (zero - zero) - dtdy * (zero - zero);
+    if (out[0] < 0.) out[0] = 0;
+    if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
+  }
+  // the marks' reduction before the stores: no barrier waits on them
+  const bool all = a.mk_out ? __syncthreads_and(bitzero3(out[0], out[1], out[2])) : false;
+  if (q.act) {
+    a.h_out[q.c] = out[0];
+    a.hu_out[q.c] = out[1];
+    a.hv_out[q.c] = out[2];
+  }
+  if (a.mk_out) mark_store<TX>(a.mk_out, t, all);
+}
+
+// update_tile's per-cell part: slopes from the staged differences, the four
+// face fluxes with outflow copies, update_states; z: the new state is +0.0.
+template <int TX>
+__device__ __forceinline__ void update_cell(const StepArgs& a, const TileDev& t,
+                                            const InstDev& I, const Cell& q, FaceLds<TX>& L,
+                                            bool& z) {
+  const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
   const long long c = q.c;
   const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
-  const double dtdx = dt / I.dx, dtdy = dt / I.dy, z = 0.;
-  double out[3] = {a.U.h[c], a.U.hu[c], a.U.hv[c]};
-#pragma unroll
-  for (int v = 0; v < 3; v++) out[v] = out[v] - dtdx * (z - z) - dtdy * (z - z);
-  if (out[0] < 0.) out[0] = 0;
+  const double prev[3] = {a.U.h[c], a.U.hu[c], a.U.hv[c]};
+  // Element::get_slopes, element2.C:830-939 (only the components the
+  // corrector reads: hx, hvx, hy, huy); boundary cells keep zero slopes
+  double hx = 0., hvx = 0., hy = 0., huy = 0.;
+  if (q.hasE && q.hasW && q.hasN && q.hasS) {
+    hx = slope_from(L.dxh[ty][tx + 1], L.dxh[ty][tx], I.dx, a.gamma, a.tiny);
+    hvx = slope_from(L.dxhv[ty][tx + 1], L.dxhv[ty][tx], I.dx, a.gamma, a.tiny);
+    hy = slope_from(L.dyh[ty + 1][tx], L.dyh[ty][tx], I.dy, a.gamma, a.tiny);
+    huy = slope_from(L.dyhu[ty + 1][tx], L.dyhu[ty][tx], I.dy, a.gamma, a.tiny);
+  }
+  // a face on the domain boundary takes the flux of the opposite face
+  // (boundary_flux, element2.C:3374-3411)
+  double fE[3], fW[3], fN[3], fS[3];
+  for (int v = 0; v < 3; v++) {
+    fE[v] = q.hasE ? L.fx[v][ty][tx + 1] : (q.hasW ? L.fx[v][ty][tx] : 0.);
+    fW[v] = q.hasW ? L.fx[v][ty][tx] : fE[v];
+    fN[v] = q.hasN ? L.fy[v][ty + 1][tx] : (q.hasS ? L.fy[v][ty][tx] : 0.);
+    fS[v] = q.hasS ? L.fy[v][ty][tx] : fN[v];
+  }
+  // correct() -> update_states (corrector.C:36-140, residual.C:162-275)
+  const double kc = (prev[0] > a.tiny) ? a.kact[c] : I.eps;
+  double out[3];
+  SrcInfo si;
+  update_states(prev, out, fE, fN, fW, fS, dt / I.dx, dt / I.dy, dt, hx, hvx, hy, huy,
+                a.S.kx[c], a.S.ky[c], I.sin_int, I.tan_bed, a.S.gx[c], a.S.gy[c], a.S.gz[c],
+                a.S.dgx[c], a.S.dgy[c], kc, a.ft, a.tiny, &si);
   if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
   a.h_out[c] = out[0];
   a.hu_out[c] = out[1];
   a.hv_out[c] = out[2];
+  z = bitzero3(out[0], out[1], out[2]);
 }
 
 // Update pass 2, one listed tile: face fluxes staged in LDS, slopes,
@@ -276,41 +384,11 @@ __device__ __forceinline__ void update_tile(const StepArgs& a, int tile, FaceLds
     }
   }
   __syncthreads();
-  if (!q.act) return;
-  const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
-  const long long c = q.c;
-  const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
-  const double prev[3] = {a.U.h[c], a.U.hu[c], a.U.hv[c]};
-  // Element::get_slopes, element2.C:830-939 (only the components the
-  // corrector reads: hx, hvx, hy, huy); boundary cells keep zero slopes
-  double hx = 0., hvx = 0., hy = 0., huy = 0.;
-  if (q.hasE && q.hasW && q.hasN && q.hasS) {
-    hx = slope_from(L.dxh[ty][tx + 1], L.dxh[ty][tx], I.dx, a.gamma, a.tiny);
-    hvx = slope_from(L.dxhv[ty][tx + 1], L.dxhv[ty][tx], I.dx, a.gamma, a.tiny);
-    hy = slope_from(L.dyh[ty + 1][tx], L.dyh[ty][tx], I.dy, a.gamma, a.tiny);
-    huy = slope_from(L.dyhu[ty + 1][tx], L.dyhu[ty][tx], I.dy, a.gamma, a.tiny);
-  }
-  // a face on the domain boundary takes the flux of the opposite face
-  // (boundary_flux, element2.C:3374-3411)
-  double fE[3], fW[3], fN[3], fS[3];
-  for (int v = 0; v < 3; v++) {
-    fE[v] = q.hasE ? L.fx[v][ty][tx + 1] : (q.hasW ? L.fx[v][ty][tx] : 0.);
-    fW[v] = q.hasW ? L.fx[v][ty][tx] : fE[v];
-    fN[v] = q.hasN ? L.fy[v][ty + 1][tx] : (q.hasS ? L.fy[v][ty][tx] : 0.);
-    fS[v] = q.hasS ? L.fy[v][ty][tx] : fN[v];
-  }
-  // correct() -> update_states (corrector.C:36-140, residual.C:162-275)
-  const double kc = (prev[0] > a.tiny) ? a.kact[c] : I.eps;
-  double out[3];
-  SrcInfo si;
-  update_states(prev, out, fE, fN, fW, fS, dt / I.dx, dt / I.dy, dt, hx, hvx, hy, huy,
-                a.S.kx[c], a.S.ky[c], I.sin_int, I.tan_bed, a.S.gx[c], a.S.gy[c], a.S.gz[c],
-                a.S.dgx[c], a.S.dgy[c], kc, a.ft, a.tiny, &si);
-  if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
-  a.h_out[c] = out[0];
-  a.hu_out[c] = out[1];
-  a.hv_out[c] = out[2];
+  bool z = true;
+  if (q.act) update_cell<TX>(a, t, I, q, L, z);
+  if (a.mk_out) mark_store<TX>(a.mk_out, t, __syncthreads_and(z));
 }
+
 
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_update(StepArgs a) {
@@ -519,39 +597,43 @@ __device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
   return d;
 }
 
-// Adjoint pass 1 (one workgroup per tile): a tile of void cells (h == 0)
-// gets lambda^s = 0 -- every block that multiplies a void cell's state is
-// zero (calc_jacobain.C:117-122), as in the per-cell path of pass 2 -- and,
-// with lambda^{s+1} == 0 on the tile, zero bed-friction terms; it is
-// written here.  Every other tile is appended to the list pass 2 walks.
-// Shared by the FP64 and the mixed-precision adjoints (storage type S).
-template <class S, int TX, class A>
-__device__ __forceinline__ void adjoint_classify(const A& a) {
-  const TileDev t = a.tiles[blockIdx.x];
+// Adjoint pass 1 (one tile per workgroup pass): a tile of void cells
+// (h == 0) gets lambda^s = 0 -- every block that multiplies a void cell's
+// state is zero (calc_jacobain.C:117-122), as in the per-cell path of
+// pass 2 -- and, with lambda^{s+1} == 0 on the tile, zero bed-friction
+// terms; it is written here.  Every other tile is appended to the list
+// pass 2 walks.
+template <int TX>
+__global__ __launch_bounds__(BLOCK) void k_adjoint_classify(AdjArgs a) {
+  const int tile = (int)blockIdx.x;
+  const TileDev t = a.tiles[tile];
+  if (a.mk_out && adjoint_skip(a.mk_U, a.dphi_part ? a.mk_lnext : nullptr, a.mk_out, t))
+    return;
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
   bool busy = false;
   if (q.act) {
-    busy = a.U.h[q.c] != (S)0;
+    busy = a.U.h[q.c] != 0.;
     if (a.dphi_part)
-      busy = busy || !(a.lam_next.h[q.c] == (S)0 && a.lam_next.hu[q.c] == (S)0 &&
-                       a.lam_next.hv[q.c] == (S)0);
+      busy = busy || !(a.lam_next.h[q.c] == 0. && a.lam_next.hu[q.c] == 0. &&
+                       a.lam_next.hv[q.c] == 0.);
   }
   if (__syncthreads_or(busy)) {
-    if (threadIdx.x == 0) a.tile_list[atomicAdd(a.tile_count, 1)] = (int)blockIdx.x;
+    if (threadIdx.x == 0) a.tile_list[atomicAdd(a.tile_count, 1)] = tile;
     return;
   }
-  if (q.act) {
-    a.lam_h[q.c] = (S)0;
-    a.lam_hu[q.c] = (S)0;
-    a.lam_hv[q.c] = (S)0;
+  if (a.short_cells) {
+    const int cnt = __syncthreads_count(q.act);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
   }
+  if (q.act) {
+    a.lam_h[q.c] = 0.;
+    a.lam_hu[q.c] = 0.;
+    a.lam_hv[q.c] = 0.;
+  }
+  if (a.mk_out) mark_store<TX>(a.mk_out, t, true);
 }
 
-template <int TX>
-__global__ __launch_bounds__(BLOCK) void k_adjoint_classify(AdjArgs a) {
-  adjoint_classify<double, TX>(a);
-}
 
 // Adjoint pass 2, one listed tile.
 template <int TX>
@@ -561,6 +643,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
   double dphi = 0.;
+  bool zero_out = true;
   const bool want_dphi = a.dphi_part != nullptr;
   constexpr int TY = BLOCK / TX, RX = TX + 2, RY = TY + 2, NHALO = RX * RY - BLOCK;
   static_assert(RX * RY <= RN && NHALO <= BLOCK, "tile does not fit the LDS records");
@@ -698,6 +781,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
     a.lam_h[c] = out[0];
     a.lam_hu[c] = out[1];
     a.lam_hv[c] = out[2];
+    zero_out = bitzero3(out[0], out[1], out[2]);
     if (want_dphi) {
       const double lam[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
       asm volatile("" ::: "memory");
@@ -716,6 +800,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
       a.dphi_part[tile] += dphi;
     }
   }
+  if (a.mk_out) mark_store<TX>(a.mk_out, t, __syncthreads_and(zero_out));
 }
 
 
@@ -748,8 +833,8 @@ void launch_coef(const StepArgs& a, bool reduce, hipStream_t s) {
 void launch_dt(const StepArgs& a, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_dt, dim3((ninst + 63) / 64), dim3(64), 0, s, a, ninst);
 }
-// classify every tile (short path written, other tiles listed), then a
-// fixed grid walks the list with the full path
+// classify every tile (skipped by the marks, short path written, or
+// listed), then a fixed grid walks the list with the full path
 void launch_update(const StepArgs& a, hipStream_t s) {
   launch_zero(a.tile_count, sizeof(int), s);
   TG_LAUNCH_TX(a.shape, k_update_classify, dim3(a.ntiles), s, a);

@@ -143,15 +143,26 @@ class Simulation:
         return out[: cnt.value]
 
     # -- adjoint ----------------------------------------------------------------
-    def dual_solver(self, terminal_weight, want_dphi=False, want_lambda=True):
-        """Reverse sweep over the last run: returns (lambda0 (3,n) or None,
-        dJ/dphi_bed or None)."""
+    def set_functional(self, terminal_weight):
+        """Keeps the terminal weights on the device for dual_solver(None)."""
         w = np.ascontiguousarray(terminal_weight, dtype=np.float64).reshape(-1)
         fn = _lib.tg_functional()
         fn.terminal_weight = _lib.dptr(w)
+        self._check(self.lib.tg_set_functional(self.ctx, C.byref(fn)))
+
+    def dual_solver(self, terminal_weight, want_dphi=False, want_lambda=True):
+        """Reverse sweep over the last run: returns (lambda0 (3,n) or None,
+        dJ/dphi_bed or None).  terminal_weight None: the weights given to
+        set_functional."""
+        fnp = None
+        if terminal_weight is not None:
+            w = np.ascontiguousarray(terminal_weight, dtype=np.float64).reshape(-1)
+            fn = _lib.tg_functional()
+            fn.terminal_weight = _lib.dptr(w)
+            fnp = C.byref(fn)
         lam = np.empty(3 * self.problem.n) if want_lambda else None
         dphi = C.c_double(0.0)
-        self._check(self.lib.tg_adjoint_run(self.ctx, C.byref(self.params), C.byref(fn),
+        self._check(self.lib.tg_adjoint_run(self.ctx, C.byref(self.params), fnp,
                                             _lib.dptr(lam),
                                             C.cast(C.byref(dphi), _lib._dp) if want_dphi
                                             else None))
@@ -180,13 +191,15 @@ class Simulation:
         self._check(self.lib.tg_set_instrumentation(self.ctx, int(on)))
 
     def instrumentation(self):
-        """(kernel_ms[3], launches[3], full_cells[3]) since instrument(True);
-        kernels 0 coef, 1 update, 2 adjoint; full_cells -1 where not counted."""
+        """(kernel_ms[3], launches[3], full_cells[3], short_cells[3]) since
+        instrument(True); kernels 0 coef, 1 update, 2 adjoint; -1 where a
+        figure is not counted."""
         ms = (C.c_double * 3)()
         ln = (C.c_int64 * 3)()
         fc = (C.c_int64 * 3)()
-        self._check(self.lib.tg_get_instrumentation(self.ctx, C.cast(ms, _lib._dp), ln, fc))
-        return list(ms), list(ln), list(fc)
+        sc = (C.c_int64 * 3)()
+        self._check(self.lib.tg_get_instrumentation(self.ctx, C.cast(ms, _lib._dp), ln, fc, sc))
+        return list(ms), list(ln), list(fc), list(sc)
 
     def time_kernel(self, which, reps=20):
         """Average launch time (ms) of one kernel: 0 coef, 1 update, 2 adjoint."""
@@ -224,6 +237,14 @@ class Simulation:
                                             C.cast(C.byref(a), _lib._dp),
                                             C.cast(C.byref(r), _lib._dp)))
         return f.value, a.value, r.value
+
+    def stored_sweep_timing(self):
+        """(kernel ms, steps) of the last reverse sweep's stored-trajectory
+        part: reverse steps back to back, no recomputation."""
+        ms, n = C.c_double(), C.c_int()
+        self._check(self.lib.tg_last_stored_sweep(self.ctx, C.cast(C.byref(ms), _lib._dp),
+                                                  C.byref(n)))
+        return ms.value, n.value
 
 
 def _mesh_of(problem, keep):
@@ -306,15 +327,24 @@ class Ensemble(Simulation):
         return np.concatenate([np.asarray(a, dtype=np.float64).reshape(3, -1)
                                for a in per_member], axis=1)
 
+    def set_functional(self, weights):
+        """weights: list of (3, n_b) terminal weights, kept on the device for
+        dual_solver(None)."""
+        super().set_functional(np.ascontiguousarray(self.concat(weights)).reshape(-1))
+
     def dual_solver(self, weights, want_dphi=False, want_lambda=True):
-        """weights: list of (3, n_b) terminal weights.  Returns (list of
-        lambda0 (3, n_b) or None, per-member dJ/dphi_bed or None)."""
-        w = np.ascontiguousarray(self.concat(weights)).reshape(-1)
-        fn = _lib.tg_functional()
-        fn.terminal_weight = _lib.dptr(w)
+        """weights: list of (3, n_b) terminal weights, or None for those given
+        to set_functional.  Returns (list of lambda0 (3, n_b) or None,
+        per-member dJ/dphi_bed or None)."""
+        fnp = None
+        if weights is not None:
+            w = np.ascontiguousarray(self.concat(weights)).reshape(-1)
+            fn = _lib.tg_functional()
+            fn.terminal_weight = _lib.dptr(w)
+            fnp = C.byref(fn)
         lam = np.empty(3 * self.ncell) if want_lambda else None
         dphi = C.c_double(0.0)
-        self._check(self.lib.tg_adjoint_run(self.ctx, None, C.byref(fn), _lib.dptr(lam),
+        self._check(self.lib.tg_adjoint_run(self.ctx, None, fnp, _lib.dptr(lam),
                                             C.cast(C.byref(dphi), _lib._dp) if want_dphi
                                             else None))
         out = None

@@ -215,9 +215,15 @@ int tg_get_dt_record(tg_ctx* ctx, double* dt_out, int capacity, int* count);
  * with calc_jacobian + calc_adjoint at every step (not only the MaxH step).
  * Forward states are recomputed segment by segment from the checkpoints.
  * lambda0_out: FP64 [3*n] adjoint at step 0 (= dJ/dU^0), may be NULL.
- * dJ_dphibed_out: dJ/d phi_bed at frozen dt (may be NULL). */
+ * dJ_dphibed_out: dJ/d phi_bed at frozen dt (may be NULL).
+ * fn NULL: the weights last given to tg_set_functional. */
 int tg_adjoint_run(tg_ctx* ctx, const tg_params* prm, const tg_functional* fn,
                    double* lambda0_out, double* dJ_dphibed_out);
+
+/* Keeps a copy of the functional's terminal weights on the device, for
+ * tg_adjoint_run with fn NULL (no host upload per sweep).  No reference
+ * counterpart (the reference evaluates func_sens on its own mesh). */
+int tg_set_functional(tg_ctx* ctx, const tg_functional* fn);
 
 /* One reverse step on the current state: given U^s (the context state) and
  * lambda^{s+1}, writes lambda^s (FP64 SoA [3*n]); step index s selects
@@ -237,6 +243,11 @@ int tg_adjoint_step_grad(tg_ctx* ctx, const tg_params* prm, int s, double dt,
 /* Kernel-only time of the last forward/adjoint run in milliseconds. */
 int tg_last_timing(tg_ctx* ctx, double* forward_ms, double* adjoint_ms,
                    double* recompute_ms);
+/* The part of the last reverse sweep that read the stored trajectory (no
+ * recomputation: reverse steps back to back, and the clearing of the
+ * gradient sums): its kernel time in milliseconds and its number of steps.
+ * No reference counterpart (measurement). */
+int tg_last_stored_sweep(tg_ctx* ctx, double* stored_ms, int* stored_steps);
 
 /* Average duration of one launch of a single kernel, timed with HIP events
  * on the context's stream over `reps` back-to-back launches on the current
@@ -251,19 +262,24 @@ int tg_time_kernel(tg_ctx* ctx, const tg_params* prm, int which, int reps, doubl
 /* Instrumentation (no reference counterpart: measurement only).  While on,
  * forward runs and reverse sweeps launch kernel by kernel (no graph replay),
  * every coefficient / update / adjoint launch is timed with a HIP event pair,
- * and the uniform-block kernels count the cells that take their full path:
- * the coefficient pass counts cells above tiny (below it a cell only reads h
- * and writes kact = epsilon), the update and adjoint count cells of tiles
- * that are not entirely dry (update: h < tiny on the tile and its side
+ * and the kernels count the cells that take their full path and the cells
+ * they write on a short path.  Uniform blocks: the coefficient pass's full
+ * path is a cell above tiny, its short path a cell of a visited tile at or
+ * below tiny (reads h); the update's and adjoint's full path is a cell of a
+ * tile that is not entirely dry (update: h < tiny on the tile and its side
  * neighbours, state copied) or void (adjoint: h == 0 and lambda^{s+1} == 0
- * on the tile, lambda^s = 0 written).  Turning it on resets the totals. */
+ * on the tile, lambda^s = 0 written), which are the short path.  Cells of
+ * TG_FP64 tiles known to be +0.0 from the zero marks are neither (nothing
+ * read or written).  Leaves: full path = h above tiny (coefficients), not
+ * dry (update), h != 0 (adjoint).  Turning it on resets the totals. */
 int tg_set_instrumentation(tg_ctx* ctx, int on);
 /* Totals since instrumentation was turned on, per kernel id: summed launch
- * time (ms), launches, full-path cells (-1 for kernels that do not count:
- * the mixed-precision forward kernels and the adaptive-mesh kernels, which
- * always take their full path). */
+ * time (ms), launches, full-path cells and short-path cells (-1 for figures
+ * a kernel does not count: the mixed-precision forward kernels count
+ * neither, the mixed-precision adjoint and the leaf kernels no short-path
+ * cells). */
 int tg_get_instrumentation(tg_ctx* ctx, double* kernel_ms, int64_t* launches,
-                           int64_t* full_cells);
+                           int64_t* full_cells, int64_t* short_cells);
 
 /* ---- tile shapes (no reference counterpart: launch configuration) ----------
  * Every kernel runs one 256-thread workgroup per tile of cells.  Built tile

@@ -179,7 +179,7 @@ def test_leaf_short_paths_are_taken_and_exact(oracle_lib):
         sim.instrument(True)
         sim.run()
         lam, dphi = sim.dual_solver(w, want_dphi=True)
-        ms, launches, full = sim.instrumentation()
+        ms, launches, full, short = sim.instrumentation()
     assert np.array_equal(lam, lam_ref)
     assert dphi == pytest.approx(dphi_ref, rel=1e-12)
     for k in range(3):
