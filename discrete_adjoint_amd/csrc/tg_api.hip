@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 using namespace tg;
@@ -42,6 +43,10 @@ struct tg_ctx {
   InstDev* inst_d = nullptr;
   TileDev* tiles_d[NSHAPE] = {};
   int nt[NSHAPE] = {};
+  // mark blocks of each variant's tile table (uniform blocks)
+  std::vector<BlockDev> blocks_h[NSHAPE];
+  BlockDev* blocks_d[NSHAPE] = {};
+  int nb[NSHAPE] = {};
   int sel[3] = {0, 0, 0};
   int ninst = 0, ntiles = 0;
   long long ncell = 0;
@@ -190,6 +195,7 @@ void free_fields(tg_ctx* ctx) {
   ctx->leaves = false;
   dfree(ctx->inst_d);
   for (auto& t : ctx->tiles_d) dfree(t);
+  for (auto& t : ctx->blocks_d) dfree(t);
   dfree(ctx->red);
   dfree(ctx->time_d);
   dfree(ctx->dt_rec);
@@ -249,6 +255,8 @@ void use_tiles(const tg_ctx* ctx, int kernel, A& a) {
   const int v = ctx->sel[kernel];
   a.tiles = ctx->tiles_d[v];
   a.ntiles = ctx->nt[v];
+  a.blocks = ctx->blocks_d[v];
+  a.nblocks = ctx->nb[v];
   a.shape = v;
 }
 
@@ -1031,6 +1039,26 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     if (SHAPE_XCD[v] && !leaves) band_by_xcd(ctx->tiles_h[v]);
     ctx->nt[v] = (int)ctx->tiles_h[v].size();
     if (ctx->nt[v] > ctx->ntiles) ctx->ntiles = ctx->nt[v];
+    // mark blocks in the order their first tile appears in the table (a
+    // banded table stays roughly banded); tile byte as in the kernels
+    ctx->blocks_h[v].clear();
+    if (leaves) continue;
+    const int TX = SHAPE_TX[v], TY = BLOCK / SHAPE_TX[v];
+    std::unordered_map<int, int> at;
+    for (int i = 0; i < ctx->nt[v]; i++) {
+      const TileDev& t = ctx->tiles_h[v][i];
+      auto it = at.find(t.mw);
+      if (it == at.end()) {
+        it = at.emplace(t.mw, (int)ctx->blocks_h[v].size()).first;
+        ctx->blocks_h[v].push_back(
+            BlockDev{t.inst, t.mw, t.mwW, t.mwE, t.mwS, t.mwN, {-1, -1, -1, -1}});
+      }
+      const int byte = ((t.y0 % MARK_H) / TY) * (MARK_W / TX) + (t.x0 % MARK_W) / TX;
+      ctx->blocks_h[v][it->second].tile[byte] = i;
+    }
+  }
+  for (int v = 0; v < NSHAPE; v++) {
+    ctx->nb[v] = (int)ctx->blocks_h[v].size();
   }
   const long long n = ctx->ncell;
   const size_t fb = (size_t)ctx->es * n;
@@ -1067,6 +1095,9 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   if (!rc) rc = dalloc(ctx, (void**)&ctx->inst_d, sizeof(InstDev) * nm);
   for (int v = 0; v < NSHAPE && !rc; v++)
     rc = dalloc(ctx, (void**)&ctx->tiles_d[v], sizeof(TileDev) * ctx->nt[v]);
+  for (int v = 0; v < NSHAPE && !rc; v++)
+    if (ctx->nb[v] > 0)
+      rc = dalloc(ctx, (void**)&ctx->blocks_d[v], sizeof(BlockDev) * ctx->nb[v]);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->red, sizeof(RedDev) * nm);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->time_d, sizeof(double) * nm);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->dt_rec, sizeof(double) * nm * ctx->cfg.max_steps);
@@ -1121,6 +1152,11 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   for (int v = 0; v < NSHAPE; v++)
     HIPCHK(ctx, hipMemcpyAsync(ctx->tiles_d[v], ctx->tiles_h[v].data(),
                                sizeof(TileDev) * ctx->nt[v], hipMemcpyHostToDevice, ctx->stream));
+  for (int v = 0; v < NSHAPE; v++)
+    if (ctx->nb[v] > 0)
+      HIPCHK(ctx, hipMemcpyAsync(ctx->blocks_d[v], ctx->blocks_h[v].data(),
+                                 sizeof(BlockDev) * ctx->nb[v], hipMemcpyHostToDevice,
+                                 ctx->stream));
   rc = upload_params(ctx, nullptr);
   if (rc) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -32,6 +32,15 @@ struct TileDev {
   int mwW, mwE, mwS, mwN;
 };
 
+// A mark block of one tile shape: its zero-mark words (as in TileDev) and
+// the indices in that shape's tile table of its four tiles, by byte of the
+// mark word (-1 where the member edge leaves no tile).  The coefficient and
+// classifying passes run one workgroup per block.
+struct BlockDev {
+  int inst, mw, mwW, mwE, mwS, mwN;
+  int tile[4];
+};
+
 // Per-instance CFL reduction cells: bit patterns of non-negative doubles
 // (IEEE order of non-negative doubles equals unsigned-integer order, so
 // integer atomic min/max is an exact FP64 min/max).
@@ -87,6 +96,8 @@ struct StepArgs {
   const TileDev* tiles;
   int ntiles;
   int shape;             // tile variant of `tiles`
+  const BlockDev* blocks;  // mark blocks of that variant
+  int nblocks;
   Fields U;
   Statics S;
   double* kact;          // [n] forward kact of U
@@ -123,6 +134,8 @@ struct AdjArgs {
   const TileDev* tiles;
   int ntiles;
   int shape;             // tile variant of `tiles`
+  const BlockDev* blocks;  // mark blocks of that variant
+  int nblocks;
   Fields U;              // U^s
   Statics S;
   Fields lam_next;       // lambda^{s+1}
@@ -170,6 +183,8 @@ struct StepArgsM {
   const TileDev* tiles;
   int ntiles;
   int shape;             // tile variant of `tiles`
+  const BlockDev* blocks;  // (not used by these kernels)
+  int nblocks;
   FieldsT<S> U;
   Statics32 S32;
   float* kact;
@@ -192,6 +207,8 @@ struct AdjArgsM {
   const TileDev* tiles;
   int ntiles;
   int shape;             // tile variant of `tiles`
+  const BlockDev* blocks;  // (not used by these kernels)
+  int nblocks;
   FieldsT<S> U;
   Statics32 S32;
   FieldsT<S> lam_next;

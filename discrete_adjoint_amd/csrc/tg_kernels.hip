@@ -86,29 +86,43 @@ __device__ __forceinline__ void mark_store(uint32_t* mk, const TileDev& t, bool 
   if (threadIdx.x == 0) ((unsigned char*)(mk + t.mw))[mark_byte<TX>(t)] = all ? 1 : 0;
 }
 
-// Skips decided from the marks alone (uniform per workgroup: scalar loads).
-// Coefficient pass: a tile whose block of U is +0.0 -- h = +0.0 < tiny: no
-// kact written, nothing added to the CFL reduction.
-__device__ __forceinline__ bool coef_skip(const uint32_t* mk, const TileDev& t) {
-  return mk[t.mw] == MARK_FULL;
+// Skips decided from the marks alone, for a whole mark block.
+// Coefficient pass: the block of U is +0.0 -- h = +0.0 < tiny: no kact
+// written, nothing added to the CFL reduction.
+__device__ __forceinline__ bool coef_skip(const uint32_t* mk, const BlockDev& B) {
+  return mk[B.mw] == MARK_FULL;
 }
-// Update: a tile whose block and the four side blocks of U are +0.0 -- every
-// cell and side neighbour has h < tiny, so the fluxes are zero and there
-// are no sources, and U^{s+1} = U^s - dtdx*(0 - 0) - dtdy*(0 - 0) = +0.0
-// for a finite dt -- and whose block of the output already holds +0.0.
+// Update: the block and the four side blocks of U are +0.0 -- every cell
+// and side neighbour has h < tiny, so the fluxes are zero and there are no
+// sources, and U^{s+1} = U^s - dtdx*(0 - 0) - dtdy*(0 - 0) = +0.0 for a
+// finite dt -- and the block of the output already holds +0.0.
 __device__ __forceinline__ bool update_skip(const uint32_t* in, const uint32_t* out,
-                                            const TileDev& t, double dt) {
-  return in[t.mw] == MARK_FULL && out[t.mw] == MARK_FULL &&
-         (t.mwW < 0 || in[t.mwW] == MARK_FULL) && (t.mwE < 0 || in[t.mwE] == MARK_FULL) &&
-         (t.mwS < 0 || in[t.mwS] == MARK_FULL) && (t.mwN < 0 || in[t.mwN] == MARK_FULL) &&
+                                            const BlockDev& B, double dt) {
+  return in[B.mw] == MARK_FULL && out[B.mw] == MARK_FULL &&
+         (B.mwW < 0 || in[B.mwW] == MARK_FULL) && (B.mwE < 0 || in[B.mwE] == MARK_FULL) &&
+         (B.mwS < 0 || in[B.mwS] == MARK_FULL) && (B.mwN < 0 || in[B.mwN] == MARK_FULL) &&
          isfinite(dt);
 }
-// Adjoint: a tile whose block of U^s is +0.0 (void cells: lambda^s = +0.0),
-// of lambda^{s+1} too when the gradient is wanted (no bed-friction terms),
-// and whose block of lambda^s already holds +0.0.
+// Adjoint: the block of U^s is +0.0 (void cells: lambda^s = +0.0), that of
+// lambda^{s+1} too when the gradient is wanted (no bed-friction terms), and
+// the block of lambda^s already holds +0.0.
 __device__ __forceinline__ bool adjoint_skip(const uint32_t* mu, const uint32_t* ml,
-                                             const uint32_t* out, const TileDev& t) {
-  return mu[t.mw] == MARK_FULL && out[t.mw] == MARK_FULL && (!ml || ml[t.mw] == MARK_FULL);
+                                             const uint32_t* out, const BlockDev& B) {
+  return mu[B.mw] == MARK_FULL && out[B.mw] == MARK_FULL && (!ml || ml[B.mw] == MARK_FULL);
+}
+// the byte of block word mw that tile byte b owns
+__device__ __forceinline__ void mark_byte_store(uint32_t* mk, int mw, int b, bool zero) {
+  ((unsigned char*)(mk + mw))[b] = zero ? 1 : 0;
+}
+// The cells of a block's four tiles this thread handles (one per tile).
+template <int TX>
+__device__ __forceinline__ void block_cells(const StepArgs& a, const BlockDev& B,
+                                            const InstDev& I, Cell q[4]) {
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    q[b].act = false;
+    if (B.tile[b] >= 0) q[b] = cell_of<TX>(a.tiles[B.tile[b]], I);
+  }
 }
 
 // ---------------------------------------------------------------- forward --
@@ -124,52 +138,62 @@ __device__ __forceinline__ double slope_from(double dp, double dm, double dx, do
 }
 
 
+// Coefficient pass, one workgroup per mark block (one cell of each of its
+// four tiles per thread): forward slopes -> kact (getcoef.f) and the
+// per-instance CFL min / hmax reduction (get_coef_and_eigen.C:29-180).
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
-  const TileDev t = a.tiles[blockIdx.x];
-  if (a.mk_in && coef_skip(a.mk_in, t)) return;
-  const InstDev I = a.inst[t.inst];
-  const Cell q = cell_of<TX>(t, I);
+  const BlockDev B = a.blocks[blockIdx.x];
+  if (a.mk_in && coef_skip(a.mk_in, B)) return;
+  const InstDev I = a.inst[B.inst];
+  Cell q[4];
+  block_cells<TX>(a, B, I, q);
+  double h[4];
+#pragma unroll
+  for (int b = 0; b < 4; b++) h[b] = q[b].act ? a.U.h[q[b].c] : 0.;
   double minev = HUGE_VAL, hmax = 0.;
-  bool wet = false;
-  if (q.act) {
-    const long long c = q.c, nx = I.nx;
-    const double h = a.U.h[c];
-    wet = h > a.tiny;
-    if (wet) {
-      const double hu = a.U.hu[c], hv = a.U.hv[c];
+  int nwet = 0, nshort = 0;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    if (!q[b].act) continue;
+    const long long c = q[b].c, nx = I.nx;
+    if (h[b] > a.tiny) {
+      nwet++;
+      const double hc = h[b], hu = a.U.hu[c], hv = a.U.hv[c];
       double hx = 0., hux = 0., hy = 0., hvy = 0.;
-      if (q.hasE && q.hasW && q.hasN && q.hasS) {  // boundary cells: zero slopes
-        hx = slope_fwd(a.U.h[c + 1], h, a.U.h[c - 1], I.dx, a.gamma, a.tiny);
+      if (q[b].hasE && q[b].hasW && q[b].hasN && q[b].hasS) {  // boundary cells: zero slopes
+        hx = slope_fwd(a.U.h[c + 1], hc, a.U.h[c - 1], I.dx, a.gamma, a.tiny);
         hux = slope_fwd(a.U.hu[c + 1], hu, a.U.hu[c - 1], I.dx, a.gamma, a.tiny);
-        hy = slope_fwd(a.U.h[c + nx], h, a.U.h[c - nx], I.dy, a.gamma, a.tiny);
+        hy = slope_fwd(a.U.h[c + nx], hc, a.U.h[c - nx], I.dy, a.gamma, a.tiny);
         hvy = slope_fwd(a.U.hv[c + nx], hv, a.U.hv[c - nx], I.dy, a.gamma, a.tiny);
       }
-      const double k = getcoef(h, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.eps, a.tiny);
+      const double k = getcoef(hc, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.eps, a.tiny);
       if (reduce) {
-        double ev = eigen(h, hu, hv, k, a.S.gz[c], a.tiny);
+        double ev = eigen(hc, hu, hv, k, a.S.gz[c], a.tiny);
         if (ev > 1000000000.) atomicOr(a.err, 2);
         double mdx = (I.dx > I.dy) ? I.dy : I.dx;
-        minev = mdx / ev;
-        hmax = h;
+        const double m = mdx / ev;
+        minev = (m < minev) ? m : minev;
+        hmax = (hc > hmax) ? hc : hmax;
       }
       a.kact[c] = k;
-    } else if (!(h < a.tiny)) {
+    } else {
+      nshort++;
       // kact is read only where h >= tiny (the physical flux is zero below
       // tiny, update_states uses kact only above it): at or below tiny the
       // value is epsilon, and below tiny it is not written
-      a.kact[c] = I.eps;
+      if (!(h[b] < a.tiny)) a.kact[c] = I.eps;
     }
   }
   if (a.active) {
-    const int cnt = __syncthreads_count(wet);
-    if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
+    const int cnt = (int)wave_sum((double)nwet);
+    if (threadIdx.x % 64 == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
   }
   if (a.short_cells) {
-    const int cnt = __syncthreads_count(q.act && !wet);
-    if (threadIdx.x == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
+    const int cnt = (int)wave_sum((double)nshort);
+    if (threadIdx.x % 64 == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
   }
-  if (reduce && __syncthreads_or(wet)) {
+  if (reduce && __syncthreads_or(nwet > 0)) {
     __shared__ double smin[BLOCK / 64], smax[BLOCK / 64];
     minev = wave_min(minev);
     hmax = wave_max(hmax);
@@ -181,8 +205,8 @@ __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
         minev = (smin[k] < minev) ? smin[k] : minev;
         hmax = (smax[k] > hmax) ? smax[k] : hmax;
       }
-      if (minev < HUGE_VAL) atomicMin(&a.red[t.inst].min_ev, dbits(minev));
-      if (hmax > 0.) atomicMax(&a.red[t.inst].hmax, dbits(hmax));
+      if (minev < HUGE_VAL) atomicMin(&a.red[B.inst].min_ev, dbits(minev));
+      if (hmax > 0.) atomicMax(&a.red[B.inst].hmax, dbits(hmax));
     }
   }
 }
@@ -245,61 +269,89 @@ struct FaceLds {
   double dyh[TY + 1][TX], dyhu[TY + 1][TX];  // (U_top - U_bottom) / dy of h, hu
 };
 
-// Update pass 1 (one workgroup per tile): a tile whose cells and side
-// neighbours all hold h < tiny has every face flux zero (the physical flux
-// is zero below tiny and riemannflux of two such sides is zero) and no
-// source terms (update_states adds them only above tiny); its cells get
-// update_states' own arithmetic with zero fluxes, bit for bit, here.  Every
-// other tile is appended to the list pass 2 walks.
+// Update pass 1 (one workgroup per mark block, one cell of each of its four
+// tiles per thread): a tile whose cells and side neighbours all hold
+// h < tiny has every face flux zero (the physical flux is zero below tiny
+// and riemannflux of two such sides is zero) and no source terms
+// (update_states adds them only above tiny); its cells get update_states'
+// own arithmetic with zero fluxes, bit for bit, here.  Every other tile is
+// appended to the list pass 2 walks.
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
   constexpr int TY = BLOCK / TX;
-  const int tile = (int)blockIdx.x;
-  const TileDev t = a.tiles[tile];
-  if (a.mk_in &&
-      update_skip(a.mk_in, a.mk_out, t, a.dt_rec[(long long)t.inst * a.max_steps + a.step]))
-    return;
-  const InstDev I = a.inst[t.inst];
-  const Cell q = cell_of<TX>(t, I);
+  const BlockDev B = a.blocks[blockIdx.x];
+  const double dt = a.dt_rec[(long long)B.inst * a.max_steps + a.step];
+  if (a.mk_in && update_skip(a.mk_in, a.mk_out, B, dt)) return;
+  const InstDev I = a.inst[B.inst];
+  __shared__ unsigned s_wet, s_nz;
+  if (threadIdx.x == 0) { s_wet = 0u; s_nz = 0u; }
+  Cell q[4];
+  block_cells<TX>(a, B, I, q);
   const long long nx = I.nx;
-  bool wet = false;
-  if (q.act) {
-    const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
-    wet = !(a.U.h[q.c] < a.tiny);
-    if (tx == 0 && q.hasW) wet = wet || !(a.U.h[q.c - 1] < a.tiny);
-    if (tx == TX - 1 && q.hasE) wet = wet || !(a.U.h[q.c + 1] < a.tiny);
-    if (ty == 0 && q.hasS) wet = wet || !(a.U.h[q.c - nx] < a.tiny);
-    if (ty == TY - 1 && q.hasN) wet = wet || !(a.U.h[q.c + nx] < a.tiny);
+  const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
+  unsigned wet = 0u;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    if (!q[b].act) continue;
+    const long long c = q[b].c;
+    bool w = !(a.U.h[c] < a.tiny);
+    if (tx == 0 && q[b].hasW) w = w || !(a.U.h[c - 1] < a.tiny);
+    if (tx == TX - 1 && q[b].hasE) w = w || !(a.U.h[c + 1] < a.tiny);
+    if (ty == 0 && q[b].hasS) w = w || !(a.U.h[c - nx] < a.tiny);
+    if (ty == TY - 1 && q[b].hasN) w = w || !(a.U.h[c + nx] < a.tiny);
+    if (w) wet |= 1u << b;
   }
-  if (__syncthreads_or(wet)) {
-    if (threadIdx.x == 0) a.tile_list[atomicAdd(a.tile_count, 1)] = tile;
-    return;
+  __syncthreads();  // s_wet, s_nz set
+  if (wet) atomicOr(&s_wet, wet);
+  __syncthreads();
+  const unsigned W = s_wet;
+  if (threadIdx.x == 0 && W) {
+    int k = atomicAdd(a.tile_count, __popc(W));
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      if ((W >> b) & 1u) a.tile_list[k++] = B.tile[b];
+  }
+  const double dtdx = dt / I.dx, dtdy = dt / I.dy, zero = 0.;
+  double out[4][3];
+  unsigned nz = 0u;
+  int nshort = 0;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    if (!q[b].act || ((W >> b) & 1u)) continue;
+    const long long c = q[b].c;
+    nshort++;
+    out[b][0] = a.U.h[c];
+    out[b][1] = a.U.hu[c];
+    out[b][2] = a.U.hv[c];
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+      out[b][v] = out[b][v] - dtdx * (zero - zero) - dtdy * (zero - zero);
+    if (out[b][0] < 0.) out[b][0] = 0;
+    if (!isfinite(out[b][0]) || !isfinite(out[b][1]) || !isfinite(out[b][2])) atomicOr(a.err, 1);
+    if (!bitzero3(out[b][0], out[b][1], out[b][2])) nz |= 1u << b;
   }
   if (a.short_cells) {
-    const int cnt = __syncthreads_count(q.act);
-    if (threadIdx.x == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
-  }
-  double out[3] = {0., 0., 0.};
-  if (q.act) {
-    const long long c = q.c;
-    const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
-    const double dtdx = dt / I.dx, dtdy = dt / I.dy, zero = 0.;
-    out[0] = a.U.h[c];
-    out[1] = a.U.hu[c];
-    out[2] = a.U.hv[c];
-#pragma unroll
-    for (int v = 0; v < 3; v++) out[v] = out[v] - dtdx * (zero - zero) - dtdy * (zero - zero);
-    if (out[0] < 0.) out[0] = 0;
-    if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
+    const int cnt = (int)wave_sum((double)nshort);
+    if (threadIdx.x % 64 == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
   }
   // the marks' reduction before the stores: no barrier waits on them
-  const bool all = a.mk_out ? __syncthreads_and(bitzero3(out[0], out[1], out[2])) : false;
-  if (q.act) {
-    a.h_out[q.c] = out[0];
-    a.hu_out[q.c] = out[1];
-    a.hv_out[q.c] = out[2];
+  if (a.mk_out) {
+    if (nz) atomicOr(&s_nz, nz);
+    __syncthreads();
   }
-  if (a.mk_out) mark_store<TX>(a.mk_out, t, all);
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    if (!q[b].act || ((W >> b) & 1u)) continue;
+    a.h_out[q[b].c] = out[b][0];
+    a.hu_out[q[b].c] = out[b][1];
+    a.hv_out[q[b].c] = out[b][2];
+  }
+  if (a.mk_out && threadIdx.x == 0) {
+    const unsigned NZ = s_nz;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      if (B.tile[b] >= 0 && !((W >> b) & 1u)) mark_byte_store(a.mk_out, B.mw, b, !((NZ >> b) & 1u));
+  }
 }
 
 // update_tile's per-cell part: slopes from the staged differences, the four
@@ -597,41 +649,64 @@ __device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
   return d;
 }
 
-// Adjoint pass 1 (one tile per workgroup pass): a tile of void cells
-// (h == 0) gets lambda^s = 0 -- every block that multiplies a void cell's
-// state is zero (calc_jacobain.C:117-122), as in the per-cell path of
-// pass 2 -- and, with lambda^{s+1} == 0 on the tile, zero bed-friction
-// terms; it is written here.  Every other tile is appended to the list
-// pass 2 walks.
+// Adjoint pass 1 (one workgroup per mark block, one cell of each of its four
+// tiles per thread): a tile of void cells (h == 0) gets lambda^s = 0 --
+// every block that multiplies a void cell's state is zero
+// (calc_jacobain.C:117-122), as in the per-cell path of pass 2 -- and, with
+// lambda^{s+1} == 0 on the tile, zero bed-friction terms; it is written
+// here.  Every other tile is appended to the list pass 2 walks.
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_classify(AdjArgs a) {
-  const int tile = (int)blockIdx.x;
-  const TileDev t = a.tiles[tile];
-  if (a.mk_out && adjoint_skip(a.mk_U, a.dphi_part ? a.mk_lnext : nullptr, a.mk_out, t))
+  const BlockDev B = a.blocks[blockIdx.x];
+  if (a.mk_out && adjoint_skip(a.mk_U, a.dphi_part ? a.mk_lnext : nullptr, a.mk_out, B))
     return;
-  const InstDev I = a.inst[t.inst];
-  const Cell q = cell_of<TX>(t, I);
-  bool busy = false;
-  if (q.act) {
-    busy = a.U.h[q.c] != 0.;
-    if (a.dphi_part)
-      busy = busy || !(a.lam_next.h[q.c] == 0. && a.lam_next.hu[q.c] == 0. &&
-                       a.lam_next.hv[q.c] == 0.);
+  const InstDev I = a.inst[B.inst];
+  __shared__ unsigned s_busy;
+  if (threadIdx.x == 0) s_busy = 0u;
+  Cell q[4];
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    q[b].act = false;
+    if (B.tile[b] >= 0) q[b] = cell_of<TX>(a.tiles[B.tile[b]], I);
   }
-  if (__syncthreads_or(busy)) {
-    if (threadIdx.x == 0) a.tile_list[atomicAdd(a.tile_count, 1)] = tile;
-    return;
+  unsigned busy = 0u;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    if (!q[b].act) continue;
+    const long long c = q[b].c;
+    bool u = a.U.h[c] != 0.;
+    if (a.dphi_part)
+      u = u || !(a.lam_next.h[c] == 0. && a.lam_next.hu[c] == 0. && a.lam_next.hv[c] == 0.);
+    if (u) busy |= 1u << b;
+  }
+  __syncthreads();  // s_busy set
+  if (busy) atomicOr(&s_busy, busy);
+  __syncthreads();
+  const unsigned W = s_busy;
+  if (threadIdx.x == 0 && W) {
+    int k = atomicAdd(a.tile_count, __popc(W));
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      if ((W >> b) & 1u) a.tile_list[k++] = B.tile[b];
+  }
+  int nshort = 0;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    if (!q[b].act || ((W >> b) & 1u)) continue;
+    nshort++;
+    a.lam_h[q[b].c] = 0.;
+    a.lam_hu[q[b].c] = 0.;
+    a.lam_hv[q[b].c] = 0.;
   }
   if (a.short_cells) {
-    const int cnt = __syncthreads_count(q.act);
-    if (threadIdx.x == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
+    const int cnt = (int)wave_sum((double)nshort);
+    if (threadIdx.x % 64 == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
   }
-  if (q.act) {
-    a.lam_h[q.c] = 0.;
-    a.lam_hu[q.c] = 0.;
-    a.lam_hv[q.c] = 0.;
+  if (a.mk_out && threadIdx.x == 0) {
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      if (B.tile[b] >= 0 && !((W >> b) & 1u)) mark_byte_store(a.mk_out, B.mw, b, true);
   }
-  if (a.mk_out) mark_store<TX>(a.mk_out, t, true);
 }
 
 
@@ -828,7 +903,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
   } while (0)
 
 void launch_coef(const StepArgs& a, bool reduce, hipStream_t s) {
-  TG_LAUNCH_TX(a.shape, k_coef, dim3(a.ntiles), s, a, reduce ? 1 : 0);
+  TG_LAUNCH_TX(a.shape, k_coef, dim3(a.nblocks), s, a, reduce ? 1 : 0);
 }
 void launch_dt(const StepArgs& a, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_dt, dim3((ninst + 63) / 64), dim3(64), 0, s, a, ninst);
@@ -837,12 +912,12 @@ void launch_dt(const StepArgs& a, int ninst, hipStream_t s) {
 // listed), then a fixed grid walks the list with the full path
 void launch_update(const StepArgs& a, hipStream_t s) {
   launch_zero(a.tile_count, sizeof(int), s);
-  TG_LAUNCH_TX(a.shape, k_update_classify, dim3(a.ntiles), s, a);
+  TG_LAUNCH_TX(a.shape, k_update_classify, dim3(a.nblocks), s, a);
   TG_LAUNCH_TX(a.shape, k_update, dim3(a.ntiles < LIST_GRID ? a.ntiles : LIST_GRID), s, a);
 }
 void launch_adjoint(const AdjArgs& a, hipStream_t s) {
   launch_zero(a.tile_count, sizeof(int), s);
-  TG_LAUNCH_TX(a.shape, k_adjoint_classify, dim3(a.ntiles), s, a);
+  TG_LAUNCH_TX(a.shape, k_adjoint_classify, dim3(a.nblocks), s, a);
   TG_LAUNCH_TX(a.shape, k_adjoint, dim3(a.ntiles < LIST_GRID ? a.ntiles : LIST_GRID), s, a);
 }
 static unsigned grid_for(long long n) {
