@@ -631,6 +631,73 @@ def bench_grid(args):
     }
 
 
+def sweeps(args):
+    """Run for kernel traces (rocprofv3 --kernel-trace): `--sweeps R`
+    repetitions of the measured sweep (initial state, K forward steps,
+    reverse sweep with dJ/dphi_bed; the bench's stored-trajectory budget and
+    tile variants), nothing else.  Repetitions after the first see every
+    stored slot's zero marks, as the bench's timed run does."""
+    import numpy as np
+
+    from discrete_adjoint_amd import problems
+    from discrete_adjoint_amd.solver import Ensemble
+
+    K, k = args.steps, args.checkpoint_every
+    E = problems.ensemble_problems(args.members, seed=args.seed, nsteps=K, checkpoint_every=k)
+    h0 = [np.concatenate([getattr(p, f) for p in E]) for f in ("h", "hu", "hv")]
+    prec = prec_of(args.precision)
+    with Ensemble(E, precision=prec, checkpoint_every=k, max_steps=max(K, 1),
+                  use_graph=0 if args.no_graph else 1, store_bytes=store_budget(args)) as ens:
+        ens.select_tiles([int(v) for v in (args.tiles or "3,3,3").split(",")])
+        ens.set_functional([p.box_weights().reshape(3, -1) for p in E])
+        for r in range(args.sweeps):
+            ens.set_state(*h0)
+            t0 = time.perf_counter()
+            ens.run(K)
+            t1 = time.perf_counter()
+            ens.dual_solver(None, want_dphi=True, want_lambda=False)
+            t2 = time.perf_counter()
+            log(f"sweep {r}: forward {1e3 * (t1 - t0):.1f} ms, reverse {1e3 * (t2 - t1):.1f} ms")
+
+
+def trace_summary(trace_csv, out_path, command):
+    """Per-kernel dispatch counts and durations of the last forward run and
+    the last reverse sweep of a `--sweeps` kernel trace (the forward run ends
+    with its statistics kernel k_stats), as a markdown table."""
+    import csv
+    import re
+
+    rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
+
+    def name(r):
+        m = re.search(r"(k_\w+(?:<\d+>)?)", r["Kernel_Name"])
+        return m.group(1) if m else r["Kernel_Name"][:40]
+
+    ks = [i for i, r in enumerate(rows) if name(r) == "k_stats"]
+    # the last forward run starts after the previous reverse sweep's last
+    # adjoint dispatch
+    start = 1 + max([i for i in range(ks[-2] + 1, ks[-1]) if name(rows[i]).startswith(
+        "k_adjoint")] or [ks[-2]])
+    parts = [("last forward run", start, ks[-1] + 1), ("last reverse sweep", ks[-1] + 1,
+                                                       len(rows))]
+    lines = [f"Command: `{command}`", ""]
+    for label, lo, hi in parts:
+        per = {}
+        for r in rows[lo:hi]:
+            per.setdefault(name(r), []).append(
+                (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        span = (int(rows[hi - 1]["End_Timestamp"]) - int(rows[lo]["Start_Timestamp"])) / 1e6
+        busy = sum(sum(v) for v in per.values()) / 1e3
+        lines += [f"### {label}: {hi - lo} dispatches, {span:.1f} ms span, {busy:.1f} ms in "
+                  f"kernels", "", "| kernel | dispatches | average (us) | total (ms) |",
+                  "|---|---:|---:|---:|"]
+        for k, v in sorted(per.items(), key=lambda x: -sum(x[1])):
+            lines.append(f"| `{k}` | {len(v)} | {sum(v) / len(v):.1f} | {sum(v) / 1e3:.2f} |")
+        lines.append("")
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines))
+
+
 def probe(args):
     """Run for hardware-counter profiles (rocprofv3 --pmc): the ensemble's
     K-step forward run and reverse sweep (graph replay, so the counter
@@ -768,6 +835,11 @@ def main():
                     help="skip the adaptive-mesh leg of the ensemble run")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernel by kernel instead of replaying captured sweeps")
+    ap.add_argument("--sweeps", type=int, default=0,
+                    help="R repetitions of the measured sweep only, for kernel traces")
+    ap.add_argument("--trace-summary", default="",
+                    help="trace.csv,out.md: per-kernel table of the last forward run and "
+                         "reverse sweep of a --sweeps kernel trace")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this hot path has no multi-GPU mode "
@@ -783,6 +855,17 @@ def main():
                          f"/tmp/pmc_{d} -o probe --output-format csv -- python3 "
                          f"$REPO/bench.py --probe --tiles {args.tiles}"
                          for c, d in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write"))])
+        return
+    if args.trace_summary:
+        trace_csv, out_path = args.trace_summary.split(",")
+        trace_summary(trace_csv, out_path,
+                      "cd /tmp && TMPDIR=/tmp rocprofv3 --kernel-trace -d /tmp/prof_sw -o sw "
+                      "--output-format csv -- python3 $REPO/bench.py --sweeps 2 --no-graph")
+        return
+    if args.sweeps:
+        args.steps = args.steps or 500
+        args.checkpoint_every = args.checkpoint_every or 25
+        sweeps(args)
         return
     if args.probe:
         args.steps = args.steps or 500
