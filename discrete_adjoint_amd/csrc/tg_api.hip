@@ -1056,6 +1056,11 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
       const int byte = ((t.y0 % MARK_H) / TY) * (MARK_W / TX) + (t.x0 % MARK_W) / TX;
       ctx->blocks_h[v][it->second].tile[byte] = i;
     }
+    // every mark word has one block: index the table by it, so a workgroup
+    // reads its block's marks without waiting for the table entry
+    std::vector<BlockDev> by_word(ctx->blocks_h[v].size());
+    for (const BlockDev& B : ctx->blocks_h[v]) by_word[B.mw] = B;
+    ctx->blocks_h[v].swap(by_word);
   }
   for (int v = 0; v < NSHAPE; v++) {
     ctx->nb[v] = (int)ctx->blocks_h[v].size();

@@ -35,7 +35,8 @@ struct TileDev {
 // A mark block of one tile shape: its zero-mark words (as in TileDev) and
 // the indices in that shape's tile table of its four tiles, by byte of the
 // mark word (-1 where the member edge leaves no tile).  The coefficient and
-// classifying passes run one workgroup per block.
+// classifying passes run one workgroup per block; a shape's block table is
+// indexed by mark word (entry b has mw == b).
 struct BlockDev {
   int inst, mw, mwW, mwE, mwS, mwN;
   int tile[4];

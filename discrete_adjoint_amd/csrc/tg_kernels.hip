@@ -89,8 +89,8 @@ __device__ __forceinline__ void mark_store(uint32_t* mk, const TileDev& t, bool 
 // Skips decided from the marks alone, for a whole mark block.
 // Coefficient pass: the block of U is +0.0 -- h = +0.0 < tiny: no kact
 // written, nothing added to the CFL reduction.
-__device__ __forceinline__ bool coef_skip(const uint32_t* mk, const BlockDev& B) {
-  return mk[B.mw] == MARK_FULL;
+__device__ __forceinline__ bool coef_skip(const uint32_t* mk, int mw) {
+  return mk[mw] == MARK_FULL;
 }
 // Update: the block and the four side blocks of U are +0.0 -- every cell
 // and side neighbour has h < tiny, so the fluxes are zero and there are no
@@ -107,8 +107,8 @@ __device__ __forceinline__ bool update_skip(const uint32_t* in, const uint32_t* 
 // lambda^{s+1} too when the gradient is wanted (no bed-friction terms), and
 // the block of lambda^s already holds +0.0.
 __device__ __forceinline__ bool adjoint_skip(const uint32_t* mu, const uint32_t* ml,
-                                             const uint32_t* out, const BlockDev& B) {
-  return mu[B.mw] == MARK_FULL && out[B.mw] == MARK_FULL && (!ml || ml[B.mw] == MARK_FULL);
+                                             const uint32_t* out, int mw) {
+  return mu[mw] == MARK_FULL && out[mw] == MARK_FULL && (!ml || ml[mw] == MARK_FULL);
 }
 // the byte of block word mw that tile byte b owns
 __device__ __forceinline__ void mark_byte_store(uint32_t* mk, int mw, int b, bool zero) {
@@ -143,8 +143,9 @@ __device__ __forceinline__ double slope_from(double dp, double dm, double dx, do
 // per-instance CFL min / hmax reduction (get_coef_and_eigen.C:29-180).
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
+  // (the block table is indexed by mark word)
+  if (a.mk_in && coef_skip(a.mk_in, (int)blockIdx.x)) return;
   const BlockDev B = a.blocks[blockIdx.x];
-  if (a.mk_in && coef_skip(a.mk_in, B)) return;
   const InstDev I = a.inst[B.inst];
   Cell q[4];
   block_cells<TX>(a, B, I, q);
@@ -657,9 +658,10 @@ __device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
 // here.  Every other tile is appended to the list pass 2 walks.
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_classify(AdjArgs a) {
-  const BlockDev B = a.blocks[blockIdx.x];
-  if (a.mk_out && adjoint_skip(a.mk_U, a.dphi_part ? a.mk_lnext : nullptr, a.mk_out, B))
+  if (a.mk_out &&
+      adjoint_skip(a.mk_U, a.dphi_part ? a.mk_lnext : nullptr, a.mk_out, (int)blockIdx.x))
     return;
+  const BlockDev B = a.blocks[blockIdx.x];
   const InstDev I = a.inst[B.inst];
   __shared__ unsigned s_busy;
   if (threadIdx.x == 0) s_busy = 0u;
