@@ -1,0 +1,86 @@
+"""CPU checks of bench.py's contract and byte accounting (no GPU): the
+--gpus guard, the path-cell split, the roofline assembled from the timed
+run, and the kernel-trace summary on a synthetic trace."""
+import csv
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_path_cells_split():
+    # counted: full and short as given, the rest skipped
+    assert bench.path_cells(1000, 60, 100) == (60, 100)
+    # short not counted: every other cell took the short path
+    assert bench.path_cells(1000, 60, -1) == (60, 940)
+    # full not counted: every cell took the full path
+    assert bench.path_cells(1000, -1, -1) == (1000, 0)
+
+
+def _kin(full, short, bpl):
+    return {"full_path_share": full, "short_path_share": short, "bytes_per_launch": bpl}
+
+
+def test_replay_roofline_uses_the_timed_run():
+    kin = {"coef": _kin(0.03, 0.06, 1e8), "update": _kin(0.06, 0.13, 3e8),
+           "adjoint": _kin(0.064, 0.041, 2.6e8)}
+    r = {"stored_ms": 36.3, "stored_steps": 100, "fwd_ms": 114.0}
+    dom, rl = bench.replay_roofline(r, kin, 500)
+    assert rl["adjoint"]["avg_launch_ms"] == pytest.approx(0.363)
+    assert rl["adjoint"]["gbs"] == pytest.approx(2.6e8 / 0.363e-3 / 1e9)
+    assert rl["forward_step"]["avg_launch_ms"] == pytest.approx(0.228)
+    assert rl["forward_step"]["bytes_per_launch"] == 3e8
+    assert dom == "adjoint"  # 0.363 ms x 500 > 114 ms
+    # no stored part: only the forward step can be timed from the run
+    dom, rl = bench.replay_roofline({"stored_ms": 0., "stored_steps": 0, "fwd_ms": 114.0},
+                                    kin, 500)
+    assert dom == "forward_step" and "adjoint" not in rl
+
+
+def test_roofline_json_keys_and_fraction():
+    from discrete_adjoint_amd import _lib
+    kin = {"coef": _kin(0.03, 0.06, 1e8), "update": _kin(0.06, 0.13, 3e8),
+           "adjoint": _kin(0.064, 0.041, 2.6e8)}
+    r = {"stored_ms": 34.5, "stored_steps": 100, "fwd_ms": 114.0}
+    dom, rl = bench.replay_roofline(r, kin, 500)
+    j = bench.roofline_json(rl, dom, kin, _lib.TG_FP64, "no_such_workload")
+    for key in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
+        assert key in j
+    assert j["bound"] == "hbm" and j["unit"] == "GB/s" and j["peak"] == 8000.0
+    assert j["frac"] == pytest.approx(j["achieved"] / 8000.0)
+    assert j["traffic"] is None  # no committed summary for this workload
+    assert j["algorithmic_bytes_per_cell"] == bench.algo_bytes(_lib.TG_FP64)["adjoint"]
+
+
+def test_trace_summary_splits_the_last_sweep(tmp_path):
+    # forward run 0, reverse 0, forward run 1, reverse 1: k_stats ends each forward run
+    seq = (["k_coef<16>", "k_update<16>"] * 3 + ["k_stats"] + ["k_adjoint<16>"] * 2 +
+           ["k_coef<16>", "k_update<16>"] * 2 + ["k_stats"] + ["k_adjoint<16>"] * 4)
+    path = tmp_path / "trace.csv"
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        w.writeheader()
+        t = 1000
+        for name in seq:
+            w.writerow({"Kernel_Name": f"void tg::(anonymous namespace)::{name}(tg::Args)",
+                        "Start_Timestamp": t, "End_Timestamp": t + 2000})
+            t += 3000
+    out = tmp_path / "summary.md"
+    bench.trace_summary(str(path), str(out), "cmd")
+    text = out.read_text()
+    fwd, rev = text.split("### last reverse sweep")
+    assert "5 dispatches" in fwd  # two coef + two update + k_stats
+    assert "| `k_coef<16>` | 2 |" in fwd and "k_adjoint" not in fwd
+    assert "| `k_adjoint<16>` | 4 | 2.0 |" in rev
