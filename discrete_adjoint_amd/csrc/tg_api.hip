@@ -86,6 +86,8 @@ struct tg_ctx {
   LeafDev leaf{};
   int* leaf_nbr = nullptr;
   uint32_t* leaf_meta = nullptr;
+  int* leaf_cnb_off = nullptr;  // chunks holding neighbours of each chunk (CSR)
+  int* leaf_cnb = nullptr;
   double* leaf_cdx = nullptr;
   double* leaf_xc = nullptr;
   LeafBufs lb{};
@@ -186,6 +188,9 @@ void free_fields(tg_ctx* ctx) {
   dfree(ctx->act_count);
   dfree(ctx->leaf_nbr);
   dfree(ctx->leaf_meta);
+  dfree(ctx->leaf_cnb_off);
+  dfree(ctx->leaf_cnb);
+  dfree(ctx->lb.kdmk);
   dfree(ctx->leaf_cdx);
   dfree(ctx->leaf_xc);
   dfree(ctx->lb.kd);
@@ -407,22 +412,22 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
     a.err = ctx->err_d;
     a.tile_list = ctx->act_list;
     a.tile_count = ctx->act_count;
+    a.mk_in = marks_of(ctx, in);
+    a.mk_out = marks_of(ctx, out);
     if (ctx->leaves) {
-      inst_short(ctx, 0, false);
-      inst_short(ctx, 1, false);
       a.active = inst_counter(ctx, 0, true);
+      a.short_cells = inst_short(ctx, 0, true);
       inst_mark(ctx, 0, true);
       launch_leaf_coef(a, ctx->leaf, compute_dt, ctx->stream);
       inst_mark(ctx, 0, false);
       if (compute_dt) launch_dt(a, 1, ctx->stream);
       a.active = inst_counter(ctx, 1, true);
+      a.short_cells = inst_short(ctx, 1, true);
       inst_mark(ctx, 1, true);
       launch_leaf_update(a, ctx->leaf, ctx->stream);
       inst_mark(ctx, 1, false);
       return;
     }
-    a.mk_in = marks_of(ctx, in);
-    a.mk_out = marks_of(ctx, out);
     use_tiles(ctx, 0, a);
     a.active = inst_counter(ctx, 0, true);
     a.short_cells = inst_short(ctx, 0, true);
@@ -466,7 +471,7 @@ void enqueue_adjoint(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* 
     a.mk_lnext = marks_of(ctx, lnext);
     a.mk_out = marks_of(ctx, lout);
     a.active = inst_counter(ctx, 2, true);
-    a.short_cells = inst_short(ctx, 2, !ctx->leaves);
+    a.short_cells = inst_short(ctx, 2, true);
     inst_mark(ctx, 2, true);
     if (ctx->leaves) launch_leaf_adjoint(a, ctx->leaf, ctx->lb, ctx->stream);
     else launch_adjoint(a, ctx->stream);
@@ -1006,6 +1011,28 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
         ctx->tiles_h[v].push_back(TileDev{0, x, 0, 0, -1, -1, -1, -1});
     off = meshes[0].n;
   }
+  // leaf chunks (the BLOCK leaves of one workgroup) carry one zero-mark word
+  // each; a chunk's neighbour chunks are those holding its leaves' neighbours
+  std::vector<int> cnb_off(1, 0), cnb;
+  if (leaves) {
+    const int n = meshes[0].n, nq = (n + BLOCK - 1) / BLOCK;
+    ctx->nsuper = nq;
+    std::vector<int> seen(nq, -1);
+    for (int q = 0; q < nq; q++) {
+      for (int e = q * BLOCK; e < n && e < (q + 1) * BLOCK; e++)
+        for (int k = 0; k < 8; k++) {
+          const uint32_t ks = (meta[e] >> (2 * (k & 3))) & 3u;
+          const bool present = (k < 4) ? ks != LEAF_BOUNDARY : ks == LEAF_FINER;
+          if (!present) continue;
+          const int r = nbrT[(size_t)k * n + e] / BLOCK;
+          if (r != q && seen[r] != q) {
+            seen[r] = q;
+            cnb.push_back(r);
+          }
+        }
+      cnb_off.push_back((int)cnb.size());
+    }
+  }
   for (int b = 0; b < nm && !leaves; b++) {
     const tg_mesh* m = &meshes[b];
     InstDev& I = ctx->inst_h[b];
@@ -1072,7 +1099,7 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   ctx->state_bytes = ctx->mk_off + align256(sizeof(uint32_t) * (size_t)ctx->nsuper);
   // the FP64 uniform kernels keep zero marks; the leaf and the
   // mixed-precision kernels do not
-  ctx->marks = ctx->mode == TG_FP64 && !leaves && ctx->nsuper > 0;
+  ctx->marks = ctx->mode == TG_FP64 && ctx->nsuper > 0;
   int rc = 0;
   void** bufs[4] = {ctx->U, ctx->Ut, ctx->lam[0], ctx->lam[1]};
   for (int i = 0; i < 4 && !rc; i++) {
@@ -1114,6 +1141,9 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     if (!rc) rc = dalloc(ctx, (void**)&ctx->lb.kd, sizeof(double) * (size_t)n);
     if (!rc) rc = dalloc(ctx, (void**)&ctx->lb.dkd, sizeof(double) * (size_t)n);
     if (!rc) rc = dalloc(ctx, (void**)&ctx->lb.jac, sizeof(double) * 81 * (size_t)n);
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->lb.kdmk, sizeof(uint32_t) * (size_t)ctx->nsuper);
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->leaf_cnb_off, sizeof(int) * cnb_off.size());
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->leaf_cnb, sizeof(int) * (cnb.empty() ? 1 : cnb.size()));
   }
   if (rc) {
     free_fields(ctx);
@@ -1151,7 +1181,14 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
                           hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->leaf_xc, xc.data(), sizeof(double) * xc.size(),
                           hipMemcpyHostToDevice));
-    ctx->leaf = LeafDev{(int)n, ctx->leaf_nbr, ctx->leaf_meta, ctx->leaf_cdx, ctx->leaf_xc};
+    HIPCHK(ctx, hipMemcpy(ctx->leaf_cnb_off, cnb_off.data(), sizeof(int) * cnb_off.size(),
+                          hipMemcpyHostToDevice));
+    if (!cnb.empty())
+      HIPCHK(ctx, hipMemcpy(ctx->leaf_cnb, cnb.data(), sizeof(int) * cnb.size(),
+                            hipMemcpyHostToDevice));
+    launch_zero(ctx->lb.kdmk, sizeof(uint32_t) * (size_t)ctx->nsuper, ctx->stream);
+    ctx->leaf = LeafDev{(int)n,        ctx->leaf_nbr, ctx->leaf_meta, ctx->leaf_cdx,
+                        ctx->leaf_xc,  ctx->leaf_cnb_off, ctx->leaf_cnb};
     ctx->leaves = true;
   }
   for (int v = 0; v < NSHAPE; v++)
@@ -1556,7 +1593,7 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
       a.mk_in = marks_of(ctx, ctx->U);
       a.mk_out = marks_of(ctx, ctx->Ut);
       a.active = inst_counter(ctx, w, true);
-      a.short_cells = inst_short(ctx, w, !ctx->leaves);
+      a.short_cells = inst_short(ctx, w, true);
       inst_mark(ctx, w, true);
       if (ctx->leaves) {
         if (w == 0) launch_leaf_coef(a, ctx->leaf, false, ctx->stream);

@@ -257,6 +257,12 @@ struct LeafDev {
   const uint32_t* meta;
   const double* cdx;     // [2n] cell size: dx at [c], dy at [n + c]
   const double* xc;      // [2n] centre: x at [c], y at [n + c]
+  // zero marks: one word per chunk of BLOCK leaves (the kernels' workgroup),
+  // MARK_FULL when the chunk held +0.0 in all three fields when last
+  // written; the chunks holding neighbours of chunk q's leaves (q itself
+  // excluded) are cnb[cnb_off[q] .. cnb_off[q + 1])
+  const int* cnb_off;
+  const int* cnb;
 };
 constexpr uint32_t LEAF_BOUNDARY = 0;  // domain boundary (neigh_proc INIT)
 constexpr uint32_t LEAF_SAME = 1;      // one neighbour of the same generation
@@ -270,6 +276,9 @@ struct LeafBufs {
   // [81n] gather terms: record slot m of leaf c at jac + (m * n + c) * 9 holds
   // lambda[l] * J[l][k] of the block leaf c adds in its slot m
   double* jac;
+  // per chunk: bit 0 set when kd holds getcoef's dry value (epsilon) for
+  // every leaf of the chunk, bit 1 when dkd holds 0 for every leaf
+  uint32_t* kdmk;
 };
 
 void launch_leaf_coef(const StepArgs& a, const LeafDev& L, bool reduce, hipStream_t s);

@@ -48,6 +48,23 @@ __device__ __forceinline__ double lwave_sum(double v) {
   return v;
 }
 
+__device__ __forceinline__ bool lzero3(double a, double b, double c) {
+  return (lbits(a) | lbits(b) | lbits(c)) == 0ull;
+}
+// chunk q and every chunk holding a neighbour of its leaves are +0.0 in mk
+__device__ __forceinline__ bool chunk_nbhd_zero(const uint32_t* mk, const LeafDev& L, int q) {
+  if (mk[q] != MARK_FULL) return false;
+  for (int i = L.cnb_off[q]; i < L.cnb_off[q + 1]; i++)
+    if (mk[L.cnb[i]] != MARK_FULL) return false;
+  return true;
+}
+// the writer of chunk q records whether it now holds +0.0 only (every
+// thread calls it; all: this thread's leaf is +0.0 or beyond n)
+__device__ __forceinline__ void chunk_mark(uint32_t* mk, bool z) {
+  const bool all = __syncthreads_and(z);
+  if (threadIdx.x == 0) mk[blockIdx.x] = all ? MARK_FULL : 0u;
+}
+
 __device__ __forceinline__ uint32_t kind_of(uint32_t meta, int s) { return (meta >> (2 * s)) & 3u; }
 __device__ __forceinline__ int back_of(uint32_t meta, int k) { return (int)((meta >> (8 + 3 * k)) & 7u); }
 __device__ __forceinline__ bool slot_present(uint32_t meta, int k) {
@@ -172,6 +189,8 @@ __device__ __forceinline__ void side_flux(const LeafDev& L, const Fields& U, con
 // ---------------------------------------------------------------- forward --
 
 __global__ __launch_bounds__(BLOCK) void k_leaf_coef(StepArgs a, LeafDev L, int reduce) {
+  // a chunk of +0.0 leaves (h = 0 < tiny) writes no kact and adds nothing
+  if (a.mk_in && a.mk_in[blockIdx.x] == MARK_FULL) return;
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   const InstDev I = a.inst[0];
   double minev = HUGE_VAL, hmax = 0.;
@@ -210,6 +229,10 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_coef(StepArgs a, LeafDev L, int 
     const int cnt = __syncthreads_count(wet);
     if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
   }
+  if (a.short_cells) {
+    const int cnt = __syncthreads_count(c < L.n && !wet);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
+  }
   if (reduce && __syncthreads_or(wet)) {
     __shared__ double smin[BLOCK / 64], smax[BLOCK / 64];
     minev = lwave_min(minev);
@@ -239,28 +262,9 @@ __device__ __forceinline__ bool leaf_dry(const StepArgs& a, const LeafDev& L, in
   return true;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_leaf_update(StepArgs a, LeafDev L) {
-  const int c = blockIdx.x * BLOCK + threadIdx.x;
-  const uint32_t meta = (c < L.n) ? L.meta[c] : 0u;
-  const bool dry = (c < L.n) && leaf_dry(a, L, c, meta);
-  if (a.active) {
-    const int cnt = __syncthreads_count(c < L.n && !dry);
-    if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
-  }
-  if (c >= L.n) return;
-  if (dry) {
-    const double dt = a.dt_rec[a.step];
-    const double dtdx = dt / L.cdx[c], dtdy = dt / L.cdx[L.n + c], z = 0.;
-    double out[3] = {a.U.h[c], a.U.hu[c], a.U.hv[c]};
-#pragma unroll
-    for (int v = 0; v < 3; v++) out[v] = out[v] - dtdx * (z - z) - dtdy * (z - z);
-    if (out[0] < 0.) out[0] = 0;
-    if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
-    a.h_out[c] = out[0];
-    a.hu_out[c] = out[1];
-    a.hv_out[c] = out[2];
-    return;
-  }
+// One leaf of the update: update_states with its face fluxes.
+__device__ __forceinline__ void leaf_update_full(const StepArgs& a, const LeafDev& L, int c,
+                                                 uint32_t meta, double out[3]) {
   const InstDev I = a.inst[0];
   const double h = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
   double hx = 0., hvx = 0., hy = 0., huy = 0.;
@@ -287,15 +291,50 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_update(StepArgs a, LeafDev L) {
   const double dt = a.dt_rec[a.step];
   const double dtdx = dt / L.cdx[c], dtdy = dt / L.cdx[L.n + c];
   const double prev[3] = {h, hu, hv};
-  double out[3];
   SrcInfo si;
   update_states(prev, out, F[1], F[2], F[3], F[0], dtdx, dtdy, dt, hx, hvx, hy, huy,
                 a.S.kx[c], a.S.ky[c], I.sin_int, I.tan_bed, a.S.gx[c], a.S.gy[c], g2,
                 a.S.dgx[c], a.S.dgy[c], k, a.ft, a.tiny, &si);
-  if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
-  a.h_out[c] = out[0];
-  a.hu_out[c] = out[1];
-  a.hv_out[c] = out[2];
+}
+
+// A chunk whose leaves and neighbour leaves are all +0.0 has U^{s+1} =
+// U^s - dtdx*(0 - 0) - dtdy*(0 - 0) = +0.0 (finite dt): skipped when its
+// output chunk already holds +0.0.
+__global__ __launch_bounds__(BLOCK) void k_leaf_update(StepArgs a, LeafDev L) {
+  const double dt = a.dt_rec[a.step];
+  if (a.mk_in && a.mk_out[blockIdx.x] == MARK_FULL && isfinite(dt) &&
+      chunk_nbhd_zero(a.mk_in, L, (int)blockIdx.x))
+    return;
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  const uint32_t meta = (c < L.n) ? L.meta[c] : 0u;
+  const bool dry = (c < L.n) && leaf_dry(a, L, c, meta);
+  if (a.active) {
+    const int cnt = __syncthreads_count(c < L.n && !dry);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
+  }
+  if (a.short_cells) {
+    const int cnt = __syncthreads_count(dry);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
+  }
+  double out[3] = {0., 0., 0.};
+  if (dry) {
+    const double dtdx = dt / L.cdx[c], dtdy = dt / L.cdx[L.n + c], z = 0.;
+    out[0] = a.U.h[c];
+    out[1] = a.U.hu[c];
+    out[2] = a.U.hv[c];
+#pragma unroll
+    for (int v = 0; v < 3; v++) out[v] = out[v] - dtdx * (z - z) - dtdy * (z - z);
+    if (out[0] < 0.) out[0] = 0;
+  } else if (c < L.n) {
+    leaf_update_full(a, L, c, meta, out);
+  }
+  if (c < L.n) {
+    if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
+    a.h_out[c] = out[0];
+    a.hu_out[c] = out[1];
+    a.hv_out[c] = out[2];
+  }
+  if (a.mk_out) chunk_mark(a.mk_out, lzero3(out[0], out[1], out[2]));
 }
 
 __global__ __launch_bounds__(BLOCK) void k_leaf_stats(LeafDev L, Fields U, double* part_mass,
@@ -323,17 +362,10 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_stats(LeafDev L, Fields U, doubl
 
 // kact of the dual slopes (update_dual_grid, dual_grid_routines.C:68-84) and
 // its phi_bed derivative
-__global__ __launch_bounds__(BLOCK) void k_leaf_dualk(AdjArgs a, LeafDev L, LeafBufs B,
-                                                      int want_dk) {
-  const int c = blockIdx.x * BLOCK + threadIdx.x;
-  if (c >= L.n) return;
+__device__ __forceinline__ void leaf_dualk_wet(const AdjArgs& a, const LeafDev& L,
+                                               const LeafBufs& B, int c, bool want_dk) {
   const InstDev I = a.inst[0];
   const double h = a.U.h[c];
-  if (!(h > a.tiny)) {  // getcoef's dry branch (1 * epsilon); no phi_bed dependence
-    B.kd[c] = 1. * I.eps;
-    if (want_dk) B.dkd[c] = 0.;
-    return;
-  }
   const uint32_t meta = L.meta[c];
   const double hu = a.U.hu[c], hv = a.U.hv[c];
   double hx = 0., hux = 0., hy = 0., hvy = 0.;
@@ -348,6 +380,33 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_dualk(AdjArgs a, LeafDev L, Leaf
   if (want_dk)
     B.dkd[c] = dgetcoef_dphi(h, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.dkroot, I.eps,
                              a.tiny);
+}
+
+// A chunk of +0.0 leaves whose kd (and dkd, when wanted) already hold the
+// dry values is skipped.
+__global__ __launch_bounds__(BLOCK) void k_leaf_dualk(AdjArgs a, LeafDev L, LeafBufs B,
+                                                      int want_dk) {
+  const uint32_t need = want_dk ? 3u : 1u;
+  if (a.mk_U && a.mk_U[blockIdx.x] == MARK_FULL && (B.kdmk[blockIdx.x] & need) == need) return;
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  bool dry = true;
+  if (c < L.n) {
+    const double h = a.U.h[c];
+    if (!(h > a.tiny)) {  // getcoef's dry branch (1 * epsilon); no phi_bed dependence
+      B.kd[c] = 1. * a.inst[0].eps;
+      if (want_dk) B.dkd[c] = 0.;
+    } else {
+      dry = false;
+      leaf_dualk_wet(a, L, B, c, want_dk != 0);
+    }
+  }
+  if (a.mk_U) {
+    const bool all = __syncthreads_and(dry);
+    if (threadIdx.x == 0) {
+      const uint32_t old = B.kdmk[blockIdx.x];
+      B.kdmk[blockIdx.x] = all ? (want_dk ? 3u : (1u | (old & 2u))) : 0u;
+    }
+  }
 }
 
 __device__ __forceinline__ DualHfv leaf_dual_hfv(const AdjArgs& a, const double* kd, int dir,
@@ -544,6 +603,8 @@ __device__ __forceinline__ M3 side_pass(const AdjArgs& a, const LeafDev& L, cons
 // by this leaf's lambda^{s+1} here and its nine products are stored in that
 // reader's record slot, so the gather reads its own records in slot order.
 __global__ __launch_bounds__(BLOCK) void k_leaf_jac(AdjArgs a, LeafDev L, LeafBufs B) {
+  // void leaves among void neighbours only: no block to store
+  if (a.mk_U && chunk_nbhd_zero(a.mk_U, L, (int)blockIdx.x)) return;
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   if (c >= L.n) return;
   const InstDev I = a.inst[0];
@@ -639,6 +700,8 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_jac(AdjArgs a, LeafDev L, LeafBu
 // lambda^{s+1}_c . d U^{s+1}_c / d phi_bed at frozen dt, per workgroup
 // partial sums accumulated over the sweep (bed-friction gradient)
 __global__ __launch_bounds__(BLOCK) void k_leaf_dphi(AdjArgs a, LeafDev L, LeafBufs B) {
+  // lambda^{s+1} = +0.0 on the whole chunk: zero terms
+  if (a.mk_lnext && a.mk_lnext[blockIdx.x] == MARK_FULL) return;
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   double dphi = 0.;
   // lambda^{s+1}_c == 0 contributes a zero term
@@ -665,21 +728,8 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_dphi(AdjArgs a, LeafDev L, LeafB
 
 // calc_adjoint_elem (calc_adjoint.C:76-157): the terms of slots 0..8 in
 // order, lambda_src[l] * J_src[l][k] added in l order for each k
-__global__ __launch_bounds__(BLOCK) void k_leaf_gather(AdjArgs a, LeafDev L, LeafBufs B) {
-  const int c = blockIdx.x * BLOCK + threadIdx.x;
-  if (a.active) {  // leaves on the full path: h != 0
-    const int cnt = __syncthreads_count(c < L.n && a.U.h[c] != 0.);
-    if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
-  }
-  if (c >= L.n) return;
-  // h == 0: every block of this leaf is zero (void_neigh_elem) and so is
-  // lambda; none of its record slots is written
-  if (a.U.h[c] == 0.) {
-    a.lam_h[c] = 0.;
-    a.lam_hu[c] = 0.;
-    a.lam_hv[c] = 0.;
-    return;
-  }
+__device__ __forceinline__ void leaf_gather_full(const AdjArgs& a, const LeafDev& L,
+                                                 const LeafBufs& B, int c, double out[3]) {
   const int n = L.n;
   const uint32_t meta = L.meta[c];
   double adj[3] = {0., 0., 0.};
@@ -694,16 +744,38 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_gather(AdjArgs a, LeafDev L, Lea
       adj[k] += P[2 * 3 + k];
     }
   }
-  double out[3];
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     double v = 0. - adj[j];
     if (fabs(v) < 1e-16) v = 0.;
     out[j] = v;
   }
-  a.lam_h[c] = out[0];
-  a.lam_hu[c] = out[1];
-  a.lam_hv[c] = out[2];
+}
+
+// calc_adjoint_elem (calc_adjoint.C:76-157) per leaf; a leaf with h == 0
+// has every block zero (void_neigh_elem) and lambda = 0, none of its record
+// slots written; a chunk of +0.0 leaves whose output already holds +0.0 is
+// skipped.
+__global__ __launch_bounds__(BLOCK) void k_leaf_gather(AdjArgs a, LeafDev L, LeafBufs B) {
+  if (a.mk_U && a.mk_U[blockIdx.x] == MARK_FULL && a.mk_out[blockIdx.x] == MARK_FULL) return;
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  const bool full = c < L.n && a.U.h[c] != 0.;
+  if (a.active) {  // leaves on the full path: h != 0
+    const int cnt = __syncthreads_count(full);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
+  }
+  if (a.short_cells) {
+    const int cnt = __syncthreads_count(c < L.n && !full);
+    if (threadIdx.x == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
+  }
+  double out[3] = {0., 0., 0.};
+  if (full) leaf_gather_full(a, L, B, c, out);
+  if (c < L.n) {
+    a.lam_h[c] = out[0];
+    a.lam_hu[c] = out[1];
+    a.lam_hv[c] = out[2];
+  }
+  if (a.mk_out) chunk_mark(a.mk_out, lzero3(out[0], out[1], out[2]));
 }
 
 unsigned leaf_grid(int n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }

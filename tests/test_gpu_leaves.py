@@ -185,3 +185,30 @@ def test_leaf_short_paths_are_taken_and_exact(oracle_lib):
     for k in range(3):
         assert launches[k] > 0
         assert 0 < full[k] < launches[k] * pb.n
+
+
+@pytest.mark.parametrize("graph,store", [(0, 0), (1, 1)])
+def test_leaf_marks_repeated_sweeps_bit_exact(oracle_lib, graph, store):
+    """Chunks of +0.0 leaves are skipped by every leaf kernel once the zero
+    marks know them; a second sweep on the same context starts from the
+    marks the first left in every buffer.  Both stay bit-identical to the
+    restatement."""
+    pb = problems.adaptive_problem(nx0=24, ny0=24, dx0=40.0, max_level=3, nsteps=60)
+    _say(f"case leaves marks n={pb.n} graph={graph} store={store}")
+    w = pb.box_weights((250.0, 450.0, 380.0, 580.0))
+    orc = oracle_lib.LeafOracle.from_problem(pb)
+    UN, dts, lam_ref, dphi_ref = orc.forward_adjoint(_U(pb), pb.nsteps, w, want_dphi=True)
+    extra = {"use_graph": graph}
+    if store:
+        extra["store_bytes"] = 32 * pb.n * 30
+    with Simulation(pb, checkpoint_every=9, **extra) as sim:
+        for rep in range(2):
+            sim.set_state(*_U(pb))
+            sim.run()
+            h, hu, hv = sim.get_state()
+            lam, dphi = sim.dual_solver(w, want_dphi=True)
+            assert np.array_equal(sim.dt_record(), dts), rep
+            assert np.array_equal(h, UN[0]) and np.array_equal(hu, UN[1]), rep
+            assert np.array_equal(hv, UN[2]), rep
+            assert np.array_equal(lam, lam_ref), rep
+            assert dphi == pytest.approx(dphi_ref, rel=1e-12), rep
