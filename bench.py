@@ -819,7 +819,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-store", action="store_true",
                     help="checkpoints only: recompute every reverse segment")
-    ap.add_argument("--store-fraction", type=float, default=0.7,
+    ap.add_argument("--store-fraction", type=float, default=0.85,
                     help="share of free HBM for the stored trajectory")
     ap.add_argument("--product-only", action="store_true",
                     help="only the measured path (no FP64 / plain / CPU legs): for profiling")
