@@ -557,35 +557,159 @@ __device__ __forceinline__ void accum_T(double adj[3], const M3& J, const double
     for (int l = 0; l < 3; l++) adj[k] += lam[l] * J.a[l][k];
 }
 
-// face flux and its derivative along d kact/d phi_bed (both sides)
-__device__ __forceinline__ void face_dk(const AdjArgs& a, int side, const DualInfo& L,
-                                        const DualInfo& R, double flux[3], double dflux[3]) {
-  Hfv hl = hfv_fwd(side, L.h, L.hu, L.hv, L.k, L.g2, a.tiny);
-  Hfv hr = hfv_fwd(side, R.h, R.hu, R.hv, R.k, R.g2, a.tiny);
+
+// The part of a cell's dual record its neighbours read, staged in LDS for the
+// tile plus a one-cell halo: each record is computed once per workgroup
+// instead of once per reading cell.
+// LDS record slots: the largest tile plus halo over the built tile widths
+// ((TX + 2) * (BLOCK / TX + 2) is largest at TX = 64)
+constexpr int RN = (64 + 2) * (BLOCK / 64 + 2);
+struct DualLds {
+  double h[RN], hu[RN], hv[RN], g2[RN], k[RN], dk[RN];
+  double hsx1[RN], hsx2[RN], hsy1[RN], hsy2[RN];
+  int flags[RN];  // sgn0+1 | (sgn1+1)<<2 | stop0<<4 | stop1<<5
+  // per record, for h >= tiny: 1/h and sqrt(k h g2) (dual_hfv, hfv_fwd,
+  // dhfv_dk), hu/h and hv/h (hfv_fwd), dhfv_dk's da -- each evaluated once
+  // with the expression the flux functions use, instead of once per face
+  double inv[RN], sq[RN], vx[RN], vy[RN], da[RN];
+};
+
+__device__ __forceinline__ void lds_put(DualLds& L, int r, const DualInfo& d, double tiny) {
+  L.h[r] = d.h; L.hu[r] = d.hu; L.hv[r] = d.hv; L.g2[r] = d.g2; L.k[r] = d.k; L.dk[r] = d.dk;
+  double inv = 0., sq = 0., vx = 0., vy = 0., da = 0.;
+  if (!(d.h < tiny)) {
+    inv = 1. / d.h;
+    sq = sqrt(d.k * d.h * d.g2);
+    vx = d.hu / d.h;
+    vy = d.hv / d.h;
+    da = (sq > 0.) ? (d.h * d.g2) / (2. * sq) : 0.;
+  }
+  L.inv[r] = inv; L.sq[r] = sq; L.vx[r] = vx; L.vy[r] = vy; L.da[r] = da;
+  L.hsx1[r] = d.hsx[1]; L.hsx2[r] = d.hsx[2]; L.hsy1[r] = d.hsy[1]; L.hsy2[r] = d.hsy[2];
+  L.flags[r] = (int)(d.si.sgn[0] + 1.) | ((int)(d.si.sgn[1] + 1.) << 2) | (d.si.stop[0] << 4) |
+               (d.si.stop[1] << 5);
+}
+
+// Rebuilds the fields a neighbour block and the face Jacobians use; the
+// other fields (own slopes, sgn[2..3]) belong to the owning cell only.
+__device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
+  DualInfo d;
+  d.h = L.h[r]; d.hu = L.hu[r]; d.hv = L.hv[r]; d.g2 = L.g2[r]; d.k = L.k[r]; d.dk = L.dk[r];
+  d.hsx[0] = d.hsy[0] = 0.;
+  d.hsx[1] = L.hsx1[r]; d.hsx[2] = L.hsx2[r]; d.hsy[1] = L.hsy1[r]; d.hsy[2] = L.hsy2[r];
+  const int f = L.flags[r];
+  d.si.sgn[0] = (double)((f & 3) - 1);
+  d.si.sgn[1] = (double)(((f >> 2) & 3) - 1);
+  d.si.sgn[2] = d.si.sgn[3] = 0.;
+  d.si.stop[0] = (f >> 4) & 1;
+  d.si.stop[1] = (f >> 5) & 1;
+#pragma unroll
+  for (int k = 0; k < 6; k++) d.d[k] = 0.;
+  return d;
+}
+
+// dual_hfv of record r (tg_math.h), with its staged 1/h and sqrt(k h g2)
+__device__ __forceinline__ DualHfv lds_dual_hfv(const DualLds& L, int dir, int r, double tiny) {
+  DualHfv o;
+  const double h = L.h[r], hu = L.hu[r], hv = L.hv[r];
+  if (h < tiny) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) o.u[i] = o.f[i] = o.w[i] = 0.;
+    o.fj = zero3(); o.sj = zero3();
+    return o;
+  }
+  const double h_inv = L.inv[r], a = L.sq[r];
+  o.u[0] = h; o.u[1] = hu; o.u[2] = hv;
+  if (dir == 0) {
+    double vel = hu * h_inv;
+    o.f[0] = hu; o.f[1] = hu * vel + 0.5 * a * a * h; o.f[2] = hv * vel;
+    o.w[0] = vel - a; o.w[1] = vel; o.w[2] = vel + a;
+    o.fj.a[0][0] = 0.; o.fj.a[0][1] = 1.; o.fj.a[0][2] = 0.;
+    o.fj.a[1][0] = a * a - vel * vel; o.fj.a[1][1] = 2 * vel; o.fj.a[1][2] = 0.;
+    o.fj.a[2][0] = -vel * hv * h_inv; o.fj.a[2][1] = hv * h_inv; o.fj.a[2][2] = vel;
+    o.sj.a[0][0] = -.5 * a * h_inv - vel * h_inv; o.sj.a[0][1] = h_inv; o.sj.a[0][2] = 0.;
+    o.sj.a[1][0] = -vel * h_inv; o.sj.a[1][1] = h_inv; o.sj.a[1][2] = 0.;
+    o.sj.a[2][0] = .5 * a * h_inv - vel * h_inv; o.sj.a[2][1] = h_inv; o.sj.a[2][2] = 0.;
+  } else {
+    double vel = hv * h_inv;
+    o.f[0] = hv; o.f[1] = hu * vel; o.f[2] = hv * vel + 0.5 * a * a * h;
+    o.w[0] = vel - a; o.w[1] = vel; o.w[2] = vel + a;
+    o.fj.a[0][0] = 0.; o.fj.a[0][1] = 0.; o.fj.a[0][2] = 1.;
+    o.fj.a[1][0] = -vel * hu * h_inv; o.fj.a[1][1] = vel; o.fj.a[1][2] = hu * h_inv;
+    o.fj.a[2][0] = a * a - vel * vel; o.fj.a[2][1] = 0.; o.fj.a[2][2] = 2 * vel;
+    o.sj.a[0][0] = -.5 * a * h_inv - vel * h_inv; o.sj.a[0][1] = 0.; o.sj.a[0][2] = h_inv;
+    o.sj.a[1][0] = -vel * h_inv; o.sj.a[1][1] = 0.; o.sj.a[1][2] = h_inv;
+    o.sj.a[2][0] = .5 * a * h_inv - vel * h_inv; o.sj.a[2][1] = 0.; o.sj.a[2][2] = h_inv;
+  }
+  return o;
+}
+
+// hfv_fwd and dhfv_dk (tg_math.h) of record r, with its staged values
+__device__ __forceinline__ Hfv lds_hfv(const DualLds& L, int dir, int r, double tiny) {
+  Hfv o;
+  const double h = L.h[r], hu = L.hu[r], hv = L.hv[r];
+  if (h < tiny) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) o.u[i] = o.f[i] = o.w[i] = 0.;
+    return o;
+  }
+  const double a = L.sq[r];
+  o.u[0] = h; o.u[1] = hu; o.u[2] = hv;
+  if (dir == 0) {
+    const double vel = L.vx[r];
+    o.f[0] = hu; o.f[1] = hu * vel + 0.5 * a * a * h; o.f[2] = hv * vel;
+    o.w[0] = vel - a; o.w[1] = vel; o.w[2] = vel + a;
+  } else {
+    const double vel = L.vy[r];
+    o.f[0] = hv; o.f[1] = hu * vel; o.f[2] = hv * vel + 0.5 * a * a * h;
+    o.w[0] = vel - a; o.w[1] = vel; o.w[2] = vel + a;
+  }
+  return o;
+}
+__device__ __forceinline__ void lds_dhfv_dk(const DualLds& L, int dir, int r, double tiny,
+                                            double dF[3], double dW[3]) {
+  dF[0] = dF[1] = dF[2] = dW[0] = dW[1] = dW[2] = 0.;
+  const double h = L.h[r];
+  if (h < tiny) return;
+  const double g3 = L.g2[r], da = L.da[r];
+  if (dir == 0) dF[1] = 0.5 * h * g3 * h;
+  else dF[2] = 0.5 * h * g3 * h;
+  dW[0] = -da;
+  dW[2] = da;
+}
+
+// face flux and its derivative along d kact/d phi_bed (both sides) between
+// records rl and rr
+__device__ __forceinline__ void face_dk(const AdjArgs& a, int side, const DualLds& L, int rl,
+                                        int rr, double flux[3], double dflux[3]) {
+  Hfv hl = lds_hfv(L, side, rl, a.tiny);
+  Hfv hr = lds_hfv(L, side, rr, a.tiny);
   riemann(hl, hr, flux);
   double dFl[3], dWl[3], dFr[3], dWr[3];
-  dhfv_dk(side, L.h, L.k, L.g2, a.tiny, dFl, dWl);
-  dhfv_dk(side, R.h, R.k, R.g2, a.tiny, dFr, dWr);
+  lds_dhfv_dk(L, side, rl, a.tiny, dFl, dWl);
+  lds_dhfv_dk(L, side, rr, a.tiny, dFr, dWr);
+  const double dkl = L.dk[rl], dkr = L.dk[rr];
 #pragma unroll
   for (int v = 0; v < 3; v++) {
-    dFl[v] *= L.dk; dWl[v] *= L.dk; dFr[v] *= R.dk; dWr[v] *= R.dk;
+    dFl[v] *= dkl; dWl[v] *= dkl; dFr[v] *= dkr; dWr[v] *= dkr;
   }
   driemann(hl, hr, dFl, dWl, dFr, dWr, dflux);
 }
 
 // lambda^{s+1}_c . d U^{s+1}_c / d phi_bed at frozen dt (bed-friction
-// gradient; no reference counterpart).  Faces with no neighbour copy the
-// opposite face (outflow), as in the forward step.
+// gradient; no reference counterpart).  C: the cell's own record; rc, rE,
+// rN, rW, rS: the LDS records of the cell and its neighbours (absent
+// neighbours' slots are passed but not read).  Faces with no neighbour copy
+// the opposite face (outflow), as in the forward step.
 __device__ double dphi_term(const AdjArgs& a, const InstDev& I, const Cell& q,
-                            const DualInfo& C, const DualInfo& E, const DualInfo& N,
-                            const DualInfo& Wc, const DualInfo& Sc, double dt,
-                            const double lam[3]) {
+                            const DualInfo& C, const DualLds& L, int rc, int rE, int rN, int rW,
+                            int rS, double dt, const double lam[3]) {
   double fE[3] = {0., 0., 0.}, fN[3] = {0., 0., 0.}, fW[3] = {0., 0., 0.}, fS[3] = {0., 0., 0.};
   double dE[3] = {0., 0., 0.}, dN[3] = {0., 0., 0.}, dW[3] = {0., 0., 0.}, dS[3] = {0., 0., 0.};
-  if (q.hasE) face_dk(a, 0, C, E, fE, dE);
-  if (q.hasW) face_dk(a, 0, Wc, C, fW, dW);
-  if (q.hasN) face_dk(a, 1, C, N, fN, dN);
-  if (q.hasS) face_dk(a, 1, Sc, C, fS, dS);
+  if (q.hasE) face_dk(a, 0, L, rc, rE, fE, dE);
+  if (q.hasW) face_dk(a, 0, L, rW, rc, fW, dW);
+  if (q.hasN) face_dk(a, 1, L, rc, rN, fN, dN);
+  if (q.hasS) face_dk(a, 1, L, rS, rc, fS, dS);
 #pragma unroll
   for (int v = 0; v < 3; v++) {
     if (!q.hasE) { fE[v] = fW[v]; dE[v] = dW[v]; }
@@ -611,43 +735,6 @@ __device__ double dphi_term(const AdjArgs& a, const InstDev& I, const Cell& q,
     out[2] += dt * (-ds2y - ds3y);
   }
   return lam[0] * out[0] + lam[1] * out[1] + lam[2] * out[2];
-}
-
-// The part of a cell's dual record its neighbours read, staged in LDS for the
-// tile plus a one-cell halo: each record is computed once per workgroup
-// instead of once per reading cell.
-// LDS record slots: the largest tile plus halo over the built tile widths
-// ((TX + 2) * (BLOCK / TX + 2) is largest at TX = 64)
-constexpr int RN = (64 + 2) * (BLOCK / 64 + 2);
-struct DualLds {
-  double h[RN], hu[RN], hv[RN], g2[RN], k[RN], dk[RN];
-  double hsx1[RN], hsx2[RN], hsy1[RN], hsy2[RN];
-  int flags[RN];  // sgn0+1 | (sgn1+1)<<2 | stop0<<4 | stop1<<5
-};
-
-__device__ __forceinline__ void lds_put(DualLds& L, int r, const DualInfo& d) {
-  L.h[r] = d.h; L.hu[r] = d.hu; L.hv[r] = d.hv; L.g2[r] = d.g2; L.k[r] = d.k; L.dk[r] = d.dk;
-  L.hsx1[r] = d.hsx[1]; L.hsx2[r] = d.hsx[2]; L.hsy1[r] = d.hsy[1]; L.hsy2[r] = d.hsy[2];
-  L.flags[r] = (int)(d.si.sgn[0] + 1.) | ((int)(d.si.sgn[1] + 1.) << 2) | (d.si.stop[0] << 4) |
-               (d.si.stop[1] << 5);
-}
-
-// Rebuilds the fields a neighbour block and the face Jacobians use; the
-// other fields (own slopes, sgn[2..3]) belong to the owning cell only.
-__device__ __forceinline__ DualInfo lds_get(const DualLds& L, int r) {
-  DualInfo d;
-  d.h = L.h[r]; d.hu = L.hu[r]; d.hv = L.hv[r]; d.g2 = L.g2[r]; d.k = L.k[r]; d.dk = L.dk[r];
-  d.hsx[0] = d.hsy[0] = 0.;
-  d.hsx[1] = L.hsx1[r]; d.hsx[2] = L.hsx2[r]; d.hsy[1] = L.hsy1[r]; d.hsy[2] = L.hsy2[r];
-  const int f = L.flags[r];
-  d.si.sgn[0] = (double)((f & 3) - 1);
-  d.si.sgn[1] = (double)(((f >> 2) & 3) - 1);
-  d.si.sgn[2] = d.si.sgn[3] = 0.;
-  d.si.stop[0] = (f >> 4) & 1;
-  d.si.stop[1] = (f >> 5) & 1;
-#pragma unroll
-  for (int k = 0; k < 6; k++) d.d[k] = 0.;
-  return d;
 }
 
 // Adjoint pass 1 (one workgroup per mark block, one cell of each of its four
@@ -734,7 +821,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
   DualInfo self;
   if (q.act) {
     self = dual_info(a, I, q.i, q.j, want_dphi);
-    lds_put(L, rself, self);
+    lds_put(L, rself, self, a.tiny);
   }
   if ((int)threadIdx.x < NHALO) {
     const int h = (int)threadIdx.x;
@@ -745,7 +832,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
     else { rx = RX - 1; ry = h - 2 * RX - TY + 1; }
     const int gi = t.x0 - 1 + rx, gj = t.y0 - 1 + ry;
     if (gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny)
-      lds_put(L, ry * RX + rx, dual_info(a, I, gi, gj, want_dphi));
+      lds_put(L, ry * RX + rx, dual_info(a, I, gi, gj, want_dphi), a.tiny);
   }
   __syncthreads();
   if (q.act) {
@@ -761,19 +848,17 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
     // Neighbour records are re-read from LDS at each use (not held in
     // registers across the phases); the empty asm statements keep the
     // compiler from merging the re-reads into long-lived registers.
-    auto nbr_hfv = [&](int dir, int r) {
-      return dual_hfv(dir, L.h[r], L.hu[r], L.hv[r], L.k[r], L.g2[r], a.tiny);
-    };
+    auto nbr_hfv = [&](int dir, int r) { return lds_dual_hfv(L, dir, r, a.tiny); };
     if (hc != 0.) {  // a void cell receives nothing (calc_jacobain.C:117-122)
       // c-side Jacobians of the four faces (dual_riemannflux)
       M3 JE = zero3(), JN = zero3(), JW = zero3(), JS = zero3();
       {
-        const DualHfv cx = dual_hfv(0, self.h, self.hu, self.hv, self.k, self.g2, a.tiny);
+        const DualHfv cx = lds_dual_hfv(L, 0, rself, a.tiny);
         if (has[1]) JE = dual_riemann_side(cx, nbr_hfv(0, rn[1]), true);
         if (has[3]) JW = dual_riemann_side(nbr_hfv(0, rn[3]), cx, false);
       }
       {
-        const DualHfv cy = dual_hfv(1, self.h, self.hu, self.hv, self.k, self.g2, a.tiny);
+        const DualHfv cy = lds_dual_hfv(L, 1, rself, a.tiny);
         if (has[2]) JN = dual_riemann_side(cy, nbr_hfv(1, rn[2]), true);
         if (has[4]) JS = dual_riemann_side(nbr_hfv(1, rn[4]), cy, false);
       }
@@ -864,8 +949,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
       asm volatile("" ::: "memory");
       // every rn[] slot lies inside the LDS tile; records of absent
       // neighbours are passed but not read (dphi_term checks q.has*)
-      dphi = dphi_term(a, I, q, self, lds_get(L, rn[1]), lds_get(L, rn[2]), lds_get(L, rn[3]),
-                       lds_get(L, rn[4]), dt, lam);
+      dphi = dphi_term(a, I, q, self, L, rself, rn[1], rn[2], rn[3], rn[4], dt, lam);
     }
   }
   if (want_dphi) {
