@@ -725,7 +725,7 @@ __device__ double dphi_term(const AdjArgs& a, const InstDev& I, const Cell& q,
   if (C.h > a.tiny) {
     const long long c = q.c;
     const double sec2 = 1. + I.tan_bed * I.tan_bed;
-    const double vx = C.hu / C.h, vy = C.hv / C.h;
+    const double vx = L.vx[rc], vy = L.vy[rc];  // hu / h, hv / h (staged)
     const double g2 = C.g2, dg0 = a.S.dgx[c], dg1 = a.S.dgy[c], k0 = a.S.kx[c], k1 = a.S.ky[c];
     double ds2x = C.si.sgn[0] * C.h * C.dk * (g2 * C.d[3] + dg1 * C.h) * I.sin_int;
     double ds3x = C.si.sgn[2] * umax(g2 * C.h + vx * C.hu * k0, 0.0) * sec2;
@@ -882,7 +882,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
         if (C.h > a.tiny) {
           const double alpha = I.sin_int * C.si.sgn[0] * C.k;
           const double betta = I.sin_int * C.si.sgn[1] * C.k;
-          const double vx = C.hu / C.h, vy = C.hv / C.h;
+          const double vx = L.vx[rself], vy = L.vy[rself];  // hu / h, hv / h (staged)
           const double vx_sq = vx * vx, vy_sq = vy * vy, tb = I.tan_bed;
           const double g0 = a.S.gx[c], g1 = a.S.gy[c], g2 = C.g2;
           const double dg0 = a.S.dgx[c], dg1 = a.S.dgy[c], k0 = a.S.kx[c], k1 = a.S.ky[c];
