@@ -264,11 +264,39 @@ __device__ __forceinline__ Hfv hfv_at(const StepArgs& a, int dir, long long c) {
 // staged in LDS.  x-face fi of row fj lies between tile columns fi-1 and fi.
 template <int TX>
 struct FaceLds {
-  static constexpr int TY = BLOCK / TX;
+  static constexpr int TY = BLOCK / TX, RX = TX + 2, RXY = (TX + 2) * (TY + 2);
   double fx[3][TY][TX + 1], fy[3][TY + 1][TX];
   double dxh[TY][TX + 1], dxhv[TY][TX + 1];  // (U_right - U_left) / dx of h, hv
   double dyh[TY + 1][TX], dyhu[TY + 1][TX];  // (U_top - U_bottom) / dy of h, hu
+  // records of the tile and its one-cell frame: the state and, for h >=
+  // tiny, hu/h, hv/h and sqrt(kact h g2) -- hfv_fwd's values, evaluated
+  // once per cell instead of once per face
+  double h[RXY], hu[RXY], hv[RXY], vx[RXY], vy[RXY], sq[RXY];
 };
+
+// hfv_fwd (tg_math.h) of record r
+template <int TX>
+__device__ __forceinline__ Hfv rec_hfv(const FaceLds<TX>& L, int dir, int r, double tiny) {
+  Hfv o;
+  const double h = L.h[r], hu = L.hu[r], hv = L.hv[r];
+  if (h < tiny) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) o.u[i] = o.f[i] = o.w[i] = 0.;
+    return o;
+  }
+  const double a = L.sq[r];
+  o.u[0] = h; o.u[1] = hu; o.u[2] = hv;
+  if (dir == 0) {
+    const double vel = L.vx[r];
+    o.f[0] = hu; o.f[1] = hu * vel + 0.5 * a * a * h; o.f[2] = hv * vel;
+    o.w[0] = vel - a; o.w[1] = vel; o.w[2] = vel + a;
+  } else {
+    const double vel = L.vy[r];
+    o.f[0] = hv; o.f[1] = hu * vel; o.f[2] = hv * vel + 0.5 * a * a * h;
+    o.w[0] = vel - a; o.w[1] = vel; o.w[2] = vel + a;
+  }
+  return o;
+}
 
 // Update pass 1 (one workgroup per mark block, one cell of each of its four
 // tiles per thread): a tile whose cells and side neighbours all hold
@@ -410,30 +438,48 @@ __device__ __forceinline__ void update_tile(const StepArgs& a, int tile, FaceLds
     const int cnt = __syncthreads_count(q.act);
     if (threadIdx.x == 0) atomicAdd(a.active, (unsigned long long)cnt);
   }
+  // phase 0: records of the tile and its frame (cells of the member only)
+  constexpr int RX = FaceLds<TX>::RX, RXY = FaceLds<TX>::RXY;
+  for (int r = (int)threadIdx.x; r < RXY; r += BLOCK) {
+    const int gi = t.x0 - 1 + r % RX, gj = t.y0 - 1 + r / RX;
+    if (gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny) {
+      const long long c = I.off + (long long)gj * nx + gi;
+      const double h = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
+      L.h[r] = h; L.hu[r] = hu; L.hv[r] = hv;
+      double vx = 0., vy = 0., sq = 0.;
+      if (!(h < a.tiny)) {
+        vx = hu / h;
+        vy = hv / h;
+        sq = sqrt(a.kact[c] * h * a.S.gz[c]);
+      }
+      L.vx[r] = vx; L.vy[r] = vy; L.sq[r] = sq;
+    }
+  }
+  __syncthreads();
   // phase 1: faces with a cell of the member on both sides
   // (calc_edge_states, element2.C:2744-3072)
   for (int f = (int)threadIdx.x; f < (TX + 1) * TY; f += BLOCK) {
     const int fi = f % (TX + 1), fj = f / (TX + 1);
     const int gi = t.x0 + fi, gj = t.y0 + fj;
     if (gi > 0 && gi < I.nx && gj < I.ny) {
-      const long long r = I.off + (long long)gj * nx + gi;
+      const int r = (fj + 1) * RX + fi + 1;  // record of the right cell
       double fl[3];
-      riemann(hfv_at(a, 0, r - 1), hfv_at(a, 0, r), fl);
+      riemann(rec_hfv<TX>(L, 0, r - 1, a.tiny), rec_hfv<TX>(L, 0, r, a.tiny), fl);
       for (int v = 0; v < 3; v++) L.fx[v][fj][fi] = fl[v];
-      L.dxh[fj][fi] = (a.U.h[r] - a.U.h[r - 1]) / I.dx;
-      L.dxhv[fj][fi] = (a.U.hv[r] - a.U.hv[r - 1]) / I.dx;
+      L.dxh[fj][fi] = (L.h[r] - L.h[r - 1]) / I.dx;
+      L.dxhv[fj][fi] = (L.hv[r] - L.hv[r - 1]) / I.dx;
     }
   }
   for (int f = (int)threadIdx.x; f < TX * (TY + 1); f += BLOCK) {
     const int fi = f % TX, fj = f / TX;
     const int gi = t.x0 + fi, gj = t.y0 + fj;
     if (gj > 0 && gj < I.ny && gi < I.nx) {
-      const long long r = I.off + (long long)gj * nx + gi;
+      const int r = (fj + 1) * RX + fi + 1;  // record of the upper cell
       double fl[3];
-      riemann(hfv_at(a, 1, r - nx), hfv_at(a, 1, r), fl);
+      riemann(rec_hfv<TX>(L, 1, r - RX, a.tiny), rec_hfv<TX>(L, 1, r, a.tiny), fl);
       for (int v = 0; v < 3; v++) L.fy[v][fj][fi] = fl[v];
-      L.dyh[fj][fi] = (a.U.h[r] - a.U.h[r - nx]) / I.dy;
-      L.dyhu[fj][fi] = (a.U.hu[r] - a.U.hu[r - nx]) / I.dy;
+      L.dyh[fj][fi] = (L.h[r] - L.h[r - RX]) / I.dy;
+      L.dyhu[fj][fi] = (L.hu[r] - L.hu[r - RX]) / I.dy;
     }
   }
   __syncthreads();
