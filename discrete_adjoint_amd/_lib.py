@@ -103,7 +103,7 @@ class tg_stats(C.Structure):
 
 
 class tg_functional(C.Structure):
-    _fields_ = [("terminal_weight", _dp)]
+    _fields_ = [("terminal_weight", _dp), ("step", C.POINTER(C.c_int))]
 
 
 class tg_dem(C.Structure):
@@ -162,6 +162,10 @@ SIGNATURES = {
     "tg_adjoint_run": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.POINTER(tg_functional),
                                  _dp, _dp]),
     "tg_set_functional": (C.c_int, [C.c_void_p, C.POINTER(tg_functional)]),
+    "tg_set_maxh_region": (C.c_int, [C.c_void_p, C.POINTER(C.c_ubyte)]),
+    "tg_get_member_box_record": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int,
+                                           C.POINTER(C.c_int)]),
+    "tg_get_maxh": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int)]),
     "tg_adjoint_step": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_double,
                                   _dp, _dp]),
     "tg_adjoint_step_grad": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_double,

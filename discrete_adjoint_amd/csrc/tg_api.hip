@@ -67,6 +67,16 @@ struct tg_ctx {
   void* lam[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   void* fw = nullptr;  // terminal weights kept on the device (tg_set_functional)
   bool have_fw = false;
+  std::vector<int> fw_steps;  // their per-member steps (empty: the final step)
+  void* wsrc = nullptr;  // host weights of a sweep that places them at several steps
+  // MaxH functional: each member's box cells in chunks, and the per-step
+  // record of the largest h over them (ninst * max_steps)
+  BoxChunk* box_chunks = nullptr;
+  long long* box_cells = nullptr;
+  int nbox_chunks = 0;
+  bool box_on = false;  // a region is set (it may hold no cell)
+  long long box_version = 0;  // changes with every tg_set_maxh_region
+  double* box_rec = nullptr;
   RedDev* red = nullptr;
   double* dt_rec = nullptr;  // ninst * max_steps
   double* time_d = nullptr;  // ninst
@@ -176,6 +186,13 @@ void free_fields(tg_ctx* ctx) {
   for (int k = 0; k < 3; k++) ctx->U[k] = ctx->Ut[k] = ctx->lam[0][k] = ctx->lam[1][k] = nullptr;
   dfree(ctx->fw);
   ctx->have_fw = false;
+  ctx->fw_steps.clear();
+  dfree(ctx->wsrc);
+  dfree(ctx->box_chunks);
+  dfree(ctx->box_cells);
+  dfree(ctx->box_rec);
+  ctx->nbox_chunks = 0;
+  ctx->box_on = false;
   dfree(ctx->statics);
   dfree(ctx->statics_f);
   dfree(ctx->kact);
@@ -615,6 +632,36 @@ void copy_state(tg_ctx* ctx, void* dst, void* const src[3]) {
                 sizeof(uint32_t) * (size_t)ctx->nsuper, ctx->stream);
 }
 
+// The MaxH record of the state U just produced by step s.
+void enqueue_box_max(tg_ctx* ctx, void* const U[3], int s) {
+  if (ctx->nbox_chunks <= 0) return;
+  if (fwd64(ctx) || ctx->es == 8)
+    launch_box_max<double>(ctx->box_chunks, ctx->nbox_chunks, ctx->box_cells,
+                           (const double*)U[0], ctx->box_rec, ctx->cfg.max_steps, s, ctx->stream);
+  else
+    launch_box_max<float>(ctx->box_chunks, ctx->nbox_chunks, ctx->box_cells, (const float*)U[0],
+                          ctx->box_rec, ctx->cfg.max_steps, s, ctx->stream);
+}
+void zero_box_record(tg_ctx* ctx) {
+  if (ctx->nbox_chunks > 0)
+    launch_zero(ctx->box_rec, sizeof(double) * ctx->ninst * (size_t)ctx->cfg.max_steps,
+                ctx->stream);
+}
+
+// Copies (src) or zeroes (src null) member b's cells of a state / adjoint
+// buffer; a copy clears the member's zero marks.
+void member_fill(tg_ctx* ctx, const void* src, void* dst, int b) {
+  const InstDev& I = ctx->inst_h[b];
+  long long mw0 = 0, nmw = 0;
+  if (ctx->marks) {
+    mw0 = ctx->leaves ? 0 : I.soff;
+    nmw = ctx->leaves ? ctx->nsuper : (long long)I.nsx * I.nsy;
+  }
+  launch_member_fill(src, dst, ctx->es, ctx->ncell, I.off, (long long)I.nx * I.ny,
+                     ctx->marks ? (uint32_t*)((char*)dst + ctx->mk_off) : nullptr, mw0, nmw,
+                     ctx->stream);
+}
+
 bool same_key(const SweepGraph& G, const long long key[8]) {
   if (!G.valid) return false;
   for (int i = 0; i < 8; i++)
@@ -673,6 +720,7 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
   void* B[3] = {ctx->Ut[0], ctx->Ut[1], ctx->Ut[2]};
   auto body = [&]() -> void** {
     launch_reset_red(ctx->red, ctx->ninst, ctx->stream);
+    zero_box_record(ctx);
     void** cur = A;
     void* slot[3];
     for (int s = 0; s < nsteps; s++) {
@@ -687,6 +735,7 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
         dstp = (cur == A) ? B : A;
       }
       enqueue_step(ctx, cur, dstp, s, dt_seq == nullptr);
+      enqueue_box_max(ctx, dstp, s);
       if (dstp == dst) {
         for (int c = 0; c < 3; c++) slot[c] = dst[c];
         cur = slot;
@@ -699,7 +748,8 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
   void** endU = nullptr;
   if (ctx->cfg.use_graph && !ctx->instrument) {
     const long long key[8] = {nsteps, dt_seq != nullptr, (long long)(intptr_t)ctx->U[0],
-                              ctx->sel[0], ctx->sel[1], k, M, 0};
+                              ctx->sel[0], ctx->sel[1], k, M,
+                              ctx->nbox_chunks > 0 ? ctx->box_version : 0};
     if (!same_key(ctx->gf, key)) {
       rc = capture(ctx, ctx->gf, key, [&]() { endU = body(); });
       if (rc) return rc;
@@ -736,19 +786,64 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
   return fill_stats(ctx, stats, nstats);
 }
 
-int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lambda0,
-                 double* dJ) {
+// FNV-1a over 64-bit words (graph keys of staged functionals)
+long long fnv(const std::vector<long long>& v) {
+  unsigned long long h = 1469598103934665603ull;
+  for (long long x : v)
+    for (int i = 0; i < 8; i++) {
+      h ^= (unsigned long long)((x >> (8 * i)) & 0xff);
+      h *= 1099511628211ull;
+    }
+  return (long long)(h >> 1);
+}
+
+// Reverse sweep.  steps[b] (null: every member at N) is the state U^{steps[b]}
+// that member b's weights apply to: lambda^{steps[b]} = w_b, lambda^s = 0
+// above it (calc_adjoint.C:49 sets func_sens at the MaxH step only, zero at
+// the others).  The sweep starts at S = max_b steps[b].
+int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* steps,
+                 double* lambda0, double* dJ) {
   if (ctx->run_steps <= 0) return set_err(ctx, TG_ERR_STATE, "no forward run recorded");
+  const int N = ctx->run_steps, k = ctx->cfg.checkpoint_every, s0 = ctx->traj_s0;
+  std::vector<int> st(ctx->ninst, N);
+  int S = 0;
+  for (int b = 0; b < ctx->ninst; b++) {
+    if (steps) {
+      if (steps[b] < 0 || steps[b] > N)
+        return set_err(ctx, TG_ERR_ARG,
+                       "functional step of member " + std::to_string(b) +
+                           " outside [0, steps of the last forward run]");
+      st[b] = steps[b];
+    }
+    S = st[b] > S ? st[b] : S;
+  }
+  bool staged = false;  // some member's weights enter below S
+  for (int b = 0; b < ctx->ninst; b++) staged |= st[b] < S;
   int rc = upload_params(ctx, prm);
   if (rc) return rc;
   const long long n = ctx->ncell;
-  const int N = ctx->run_steps, k = ctx->cfg.checkpoint_every, s0 = ctx->traj_s0;
   rc = ensure_store(ctx, ctx->seg, (size_t)(s0 > 0 ? k : 1));
   if (rc) return rc;
   const bool dphi = dJ != nullptr;
-  // lambda^N = func_sens at the final step (calc_adjoint.C:93-97): from the
-  // host, or (w null) the weights kept on the device
-  if (w) {
+  // lambda^S = func_sens (calc_adjoint.C:93-97): from the host, or (w null)
+  // the weights kept on the device; members whose step lies below S start
+  // from zero and take their weights when the sweep reaches their step
+  const void* src = nullptr;
+  if (w && staged) {
+    if (!ctx->wsrc) {
+      rc = dalloc(ctx, &ctx->wsrc, ctx->state_bytes);
+      if (rc) return rc;
+    }
+    void* f[3];
+    split(ctx, ctx->wsrc, f);
+    for (int c = 0; c < 3; c++) {
+      rc = h2d_field(ctx, f[c], 0, n, w + c * n);
+      if (rc) return rc;
+    }
+    clear_marks(ctx, ctx->wsrc);
+    copy_state(ctx, ctx->lam[0][0], f);
+    src = ctx->wsrc;
+  } else if (w) {
     for (int c = 0; c < 3; c++) {
       rc = h2d_field(ctx, ctx->lam[0][c], 0, n, w + c * n);
       if (rc) return rc;
@@ -758,21 +853,40 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
     void* f[3];
     split(ctx, ctx->fw, f);
     copy_state(ctx, ctx->lam[0][0], f);
+    src = ctx->fw;
   }
-  // steps N-1 .. s0 read the stored trajectory; below s0 the sweep works in
+  std::vector<std::vector<int>> enter(S);  // members whose weights enter at lambda^s
+  for (int b = 0; b < ctx->ninst; b++) {
+    if (st[b] < S) {
+      member_fill(ctx, nullptr, ctx->lam[0][0], b);
+      enter[st[b]].push_back(b);
+    }
+  }
+  auto inject = [&](int s, void* lout) {
+    for (int b : enter[s]) member_fill(ctx, src, lout, b);
+  };
+  // steps S-1 .. s0 read the stored trajectory; below s0 the sweep works in
   // segments of k: restore U^{a} from its checkpoint, recompute U^{a+1..b-1}
-  // with the recorded dt, then the adjoint steps b-1 .. a
+  // with the recorded dt, then the adjoint steps b-1 .. a (segments at or
+  // above S are not needed, the one holding S stops below it)
   const int nseg = (s0 + k - 1) / k;
+  const bool need_gb = nseg > 0 && S > 0;
+  const int nstored = S > s0 ? S - s0 : 0;
+  auto seg_end = [&](int a) {
+    const int b = (a + k < s0) ? a + k : s0;
+    return b < S ? b : S;
+  };
   auto stored = [&](int& cur) {
-    for (int s = N - 1; s >= s0; s--) {
+    for (int s = S - 1; s >= s0; s--) {
       void* Us[3];
       split(ctx, traj_state(ctx, s - s0), Us);
       enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi);
+      inject(s, ctx->lam[1 - cur][0]);
       cur = 1 - cur;
     }
   };
   auto recompute = [&](int sg) {
-    const int a = sg * k, b = (a + k < s0) ? a + k : s0;
+    const int a = sg * k, b = seg_end(a);
     void* base[3];
     split(ctx, ctx->ckpt[sg], base);
     copy_state(ctx, ctx->seg[0], base);
@@ -785,11 +899,12 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
     }
   };
   auto reverse = [&](int sg, int& cur) {
-    const int a = sg * k, b = (a + k < s0) ? a + k : s0;
+    const int a = sg * k, b = seg_end(a);
     for (int s = b - 1; s >= a; s--) {
       void* Us[3];
       split(ctx, ctx->seg[s - a], Us);
       enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi);
+      inject(s, ctx->lam[1 - cur][0]);
       cur = 1 - cur;
     }
   };
@@ -798,7 +913,14 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
   // the stored part (adjoint launches only) and the segments are timed
   // apart: ev[2] .. ev[6] .. ev[3]
   if (ctx->cfg.use_graph && !ctx->instrument) {
-    const long long key[8] = {N, k, dphi, ctx->sel[0], ctx->sel[1], ctx->sel[2], s0, 0};
+    long long stage = 0;  // what the captured launches depend on beyond N
+    if (S < N || staged) {
+      std::vector<long long> v(st.begin(), st.end());
+      v.push_back(S);
+      v.push_back((long long)(intptr_t)src);
+      stage = fnv(v);
+    }
+    const long long key[8] = {N, k, dphi, ctx->sel[0], ctx->sel[1], ctx->sel[2], s0, stage};
     if (!same_key(ctx->ga, key)) {
       drop_graph(ctx->gb);
       rc = capture(ctx, ctx->ga, key, [&]() {
@@ -808,10 +930,11 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
         stored(c);
       });
       if (rc) return rc;
-      if (nseg > 0) {
+      if (need_gb) {
         rc = capture(ctx, ctx->gb, key, [&]() {
-          int c = (N - s0) % 2;
+          int c = nstored % 2;
           for (int sg = nseg - 1; sg >= 0; sg--) {
+            if (sg * k >= S) continue;
             recompute(sg);
             reverse(sg, c);
           }
@@ -822,8 +945,8 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     HIPCHK(ctx, hipGraphLaunch(ctx->ga.x, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
-    if (nseg > 0) HIPCHK(ctx, hipGraphLaunch(ctx->gb.x, ctx->stream));
-    cur = N % 2;
+    if (need_gb) HIPCHK(ctx, hipGraphLaunch(ctx->gb.x, ctx->stream));
+    cur = S % 2;
     rec_ms = -1.f;  // not separable inside one launch
   } else {
     if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
@@ -831,6 +954,7 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
     stored(cur);
     HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
     for (int sg = nseg - 1; sg >= 0; sg--) {
+      if (sg * k >= S) continue;
       HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
       recompute(sg);
       HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
@@ -849,7 +973,7 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, double* lam
   ctx->rec_ms = rec_ms;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[6]));
   ctx->stored_ms = ms;
-  ctx->stored_steps = N - s0;
+  ctx->stored_steps = nstored;
   if (lambda0)
     for (int c = 0; c < 3; c++) {
       rc = d2h_field(ctx, ctx->lam[cur][c], 0, n, lambda0 + c * n);
@@ -1426,8 +1550,12 @@ int tg_forward_step(tg_ctx* ctx, const tg_params* prm, tg_stats* stats) {
   if (ctx->rec_steps >= ctx->cfg.max_steps) return set_err(ctx, TG_ERR_ARG, "dt record full");
   int rc = upload_params(ctx, prm);
   if (rc) return rc;
-  if (ctx->rec_steps == 0) launch_reset_red(ctx->red, ctx->ninst, ctx->stream);
+  if (ctx->rec_steps == 0) {
+    launch_reset_red(ctx->red, ctx->ninst, ctx->stream);
+    zero_box_record(ctx);
+  }
   enqueue_step(ctx, ctx->U, ctx->Ut, ctx->rec_steps, true);
+  enqueue_box_max(ctx, ctx->Ut, ctx->rec_steps);
   for (int c = 0; c < 3; c++) std::swap(ctx->U[c], ctx->Ut[c]);
   HIPCHK(ctx, hipGetLastError());
   rc = check_err(ctx);
@@ -1468,7 +1596,8 @@ int tg_adjoint_run(tg_ctx* ctx, const tg_params* prm, const tg_functional* fn,
                    double* lambda0_out, double* dJ_dphibed_out) {
   if (!ctx || (fn && !fn->terminal_weight)) return TG_ERR_ARG;
   if (!fn && !ctx->have_fw) return set_err(ctx, TG_ERR_STATE, "no functional set");
-  return adjoint_loop(ctx, prm, fn ? fn->terminal_weight : nullptr, lambda0_out,
+  const int* steps = fn ? fn->step : (ctx->fw_steps.empty() ? nullptr : ctx->fw_steps.data());
+  return adjoint_loop(ctx, prm, fn ? fn->terminal_weight : nullptr, steps, lambda0_out,
                       dJ_dphibed_out);
 }
 
@@ -1488,6 +1617,90 @@ int tg_set_functional(tg_ctx* ctx, const tg_functional* fn) {
   clear_marks(ctx, ctx->fw);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->have_fw = true;
+  if (fn->step) ctx->fw_steps.assign(fn->step, fn->step + ctx->ninst);
+  else ctx->fw_steps.clear();
+  return TG_OK;
+}
+
+int tg_set_maxh_region(tg_ctx* ctx, const unsigned char* mask) {
+  if (!ctx) return TG_ERR_ARG;
+  if (!ctx->have_mesh) return set_err(ctx, TG_ERR_STATE, "no mesh");
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // earlier launches may read the lists
+  dfree(ctx->box_chunks);
+  dfree(ctx->box_cells);
+  ctx->nbox_chunks = 0;
+  ctx->box_on = false;
+  ctx->box_version++;
+  if (!mask) return TG_OK;
+  std::vector<long long> cells;
+  std::vector<BoxChunk> ch;
+  for (int b = 0; b < ctx->ninst; b++) {
+    const InstDev& I = ctx->inst_h[b];
+    const long long end = I.off + (long long)I.nx * I.ny;
+    for (long long i = I.off; i < end; i++) {
+      if (!mask[i]) continue;
+      if (ch.empty() || ch.back().inst != b || ch.back().end - ch.back().begin == BOX_CHUNK)
+        ch.push_back(BoxChunk{b, (int)cells.size(), (int)cells.size(), 0});
+      cells.push_back(i);
+      ch.back().end++;
+    }
+  }
+  if (cells.size() > (size_t)0x7fffffff) return set_err(ctx, TG_ERR_ARG, "box too large");
+  if (!ctx->box_rec) {
+    int rc = dalloc(ctx, (void**)&ctx->box_rec,
+                    sizeof(double) * ctx->ninst * (size_t)ctx->cfg.max_steps);
+    if (rc) return rc;
+  }
+  // a record made for another region is not kept
+  launch_zero(ctx->box_rec, sizeof(double) * ctx->ninst * (size_t)ctx->cfg.max_steps,
+              ctx->stream);
+  ctx->box_on = true;
+  if (ch.empty()) return TG_OK;  // no cell in the box: the record stays zero
+  int rc = dalloc(ctx, (void**)&ctx->box_cells, sizeof(long long) * cells.size());
+  if (rc) return rc;
+  rc = dalloc(ctx, (void**)&ctx->box_chunks, sizeof(BoxChunk) * ch.size());
+  if (rc) return rc;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->box_cells, cells.data(), sizeof(long long) * cells.size(),
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->box_chunks, ch.data(), sizeof(BoxChunk) * ch.size(),
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->nbox_chunks = (int)ch.size();
+  return TG_OK;
+}
+
+int tg_get_member_box_record(tg_ctx* ctx, int member, double* hmax_out, int capacity,
+                             int* count) {
+  if (!ctx || !count || member < 0 || member >= ctx->ninst) return TG_ERR_ARG;
+  *count = ctx->box_on ? ctx->rec_steps : 0;
+  if (hmax_out && capacity > 0 && *count > 0) {
+    const int m = capacity < *count ? capacity : *count;
+    HIPCHK(ctx, hipMemcpyAsync(hmax_out, ctx->box_rec + (long long)member * ctx->cfg.max_steps,
+                               sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return TG_OK;
+}
+
+int tg_get_maxh(tg_ctx* ctx, double* hmax, int* step) {
+  if (!ctx || !hmax || !step) return TG_ERR_ARG;
+  if (!ctx->box_on) return set_err(ctx, TG_ERR_STATE, "no MaxH region set");
+  std::vector<double> r(ctx->rec_steps);
+  for (int b = 0; b < ctx->ninst; b++) {
+    int cnt = 0;
+    int rc = tg_get_member_box_record(ctx, b, r.data(), (int)r.size(), &cnt);
+    if (rc) return rc;
+    // check_maxh: a strict '>' over the steps in order, from hmax = 0
+    double m = 0.;
+    int it = 0;
+    for (int s = 0; s < cnt; s++)
+      if (r[s] > m) {
+        m = r[s];
+        it = s + 1;
+      }
+    hmax[b] = m;
+    step[b] = it;
+  }
   return TG_OK;
 }
 

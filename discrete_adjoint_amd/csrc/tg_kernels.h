@@ -32,6 +32,13 @@ struct TileDev {
   int mwW, mwE, mwS, mwN;
 };
 
+// A run of at most BOX_CHUNK listed box cells [begin, end) of member inst
+// (MaxH functional record).
+struct BoxChunk {
+  int inst, begin, end, pad;
+};
+constexpr int BOX_CHUNK = 1024;
+
 // A mark block of one tile shape: its zero-mark words (as in TileDev) and
 // the indices in that shape's tile table of its four tiles, by byte of the
 // mark word (-1 where the member edge leaves no tile).  The coefficient and
@@ -292,6 +299,17 @@ void launch_dt(const StepArgs& a, int ninst, hipStream_t s);
 void launch_update(const StepArgs& a, hipStream_t s);
 void launch_adjoint(const AdjArgs& a, hipStream_t s);
 void launch_reset_red(RedDev* red, int ninst, hipStream_t s);
+// rec[member * max_steps + step] = max(itself, largest h > 0 over the listed
+// box cells); rec holds FP64 values >= +0.0
+template <class S>
+void launch_box_max(const BoxChunk* ch, int nchunks, const long long* cells, const S* h,
+                    double* rec, int max_steps, int step, hipStream_t s);
+// copy (src) or zero (src null) one member's cells [off, off + count) of the
+// three fields of a state / adjoint buffer; a copy clears the member's mark
+// words [mw0, mw0 + nmw) of mk
+void launch_member_fill(const void* src, void* dst, int es, long long ncell, long long off,
+                        long long count, uint32_t* mk, long long mw0, long long nmw,
+                        hipStream_t s);
 // bytes must be a multiple of 4 (fields are FP32 or FP64)
 void launch_zero(void* p, size_t bytes, hipStream_t s);
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -234,6 +234,51 @@ __global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restric
     dst[i] = src[i];
 }
 
+// MaxH box record (check_maxh, calc_adjoint.C:443-465): each workgroup
+// takes one chunk of a member's box cells and raises rec[member][step] to
+// the largest h > 0 among them.  The running value starts at +0.0 and only
+// grows by a strict '>' (NaN never wins), so it is a non-negative double
+// whose bit pattern orders like the integer: an integer atomicMax is exact.
+template <class S>
+__global__ void __launch_bounds__(256) k_box_max(const BoxChunk* __restrict__ ch,
+                                                 const long long* __restrict__ cells,
+                                                 const S* __restrict__ h,
+                                                 unsigned long long* rec, int max_steps,
+                                                 int step) {
+  __shared__ double wm[4];
+  const BoxChunk c = ch[blockIdx.x];
+  double m = 0.;
+  for (int i = c.begin + (int)threadIdx.x; i < c.end; i += 256) {
+    const double v = (double)h[cells[i]];
+    m = (v > m) ? v : m;
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++) m = (wm[w] > m) ? wm[w] : m;
+    if (m > 0.) atomicMax(rec + (long long)c.inst * max_steps + step, dbits(m));
+  }
+}
+
+// The three field ranges [off, off + count) of one member (words of `es`
+// bytes in buffers of `field_bytes` per field): copied from src, or zeroed
+// when src is null; the member's zero-mark words [mw0, mw0 + nmw) of dst are
+// cleared (nothing known) after a copy.
+__global__ void k_member_fill(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                              long long field_words, long long off_words, long long nwords,
+                              uint32_t* mk, long long mw0, long long nmw) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  for (int f = 0; f < 3; f++) {
+    const long long base = f * field_words + off_words;
+    for (long long i = t0; i < nwords; i += stride)
+      dst[base + i] = src ? src[base + i] : 0u;
+  }
+  if (src && mk)
+    for (long long i = t0; i < nmw; i += stride) mk[mw0 + i] = 0u;
+}
+
 __global__ void k_dt(StepArgs a, int ninst) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ninst) return;
@@ -1064,6 +1109,25 @@ void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
   const long long n = (long long)(bytes / 4);
   hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n)), dim3(256), 0, s, (const uint32_t*)src,
                      (uint32_t*)dst, n);
+}
+template <class S>
+void launch_box_max(const BoxChunk* ch, int nchunks, const long long* cells, const S* h,
+                    double* rec, int max_steps, int step, hipStream_t s) {
+  if (nchunks <= 0) return;
+  hipLaunchKernelGGL(k_box_max<S>, dim3(nchunks), dim3(256), 0, s, ch, cells, h,
+                     (unsigned long long*)rec, max_steps, step);
+}
+template void launch_box_max<double>(const BoxChunk*, int, const long long*, const double*,
+                                     double*, int, int, hipStream_t);
+template void launch_box_max<float>(const BoxChunk*, int, const long long*, const float*,
+                                    double*, int, int, hipStream_t);
+void launch_member_fill(const void* src, void* dst, int es, long long ncell, long long off,
+                        long long count, uint32_t* mk, long long mw0, long long nmw,
+                        hipStream_t s) {
+  const long long w = es / 4;  // 32-bit words per value
+  const long long n = count * w > nmw ? count * w : nmw;
+  hipLaunchKernelGGL(k_member_fill, dim3(grid_for(n)), dim3(256), 0, s, (const uint32_t*)src,
+                     (uint32_t*)dst, ncell * w, off * w, count * w, mk, mw0, nmw);
 }
 void launch_reset_red(RedDev* red, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_red, dim3((ninst + 63) / 64), dim3(64), 0, s, red, ninst);

@@ -143,22 +143,62 @@ class Simulation:
         return out[: cnt.value]
 
     # -- adjoint ----------------------------------------------------------------
-    def set_functional(self, terminal_weight):
-        """Keeps the terminal weights on the device for dual_solver(None)."""
-        w = np.ascontiguousarray(terminal_weight, dtype=np.float64).reshape(-1)
+    def _functional(self, w_flat, step):
+        """tg_functional over the flat weights and per-member steps (None:
+        the final step); returns it with the arrays it points into."""
+        w = np.ascontiguousarray(w_flat, dtype=np.float64).reshape(-1)
         fn = _lib.tg_functional()
         fn.terminal_weight = _lib.dptr(w)
+        st = None
+        if step is not None:
+            st = np.ascontiguousarray(np.broadcast_to(np.asarray(step, dtype=np.int32),
+                                                      (self.lib.tg_num_members(self.ctx),)))
+            fn.step = st.ctypes.data_as(C.POINTER(C.c_int))
+        return fn, (w, st)
+
+    def set_functional(self, terminal_weight, step=None):
+        """Keeps the weights (and their step, None: the final one) on the
+        device for dual_solver(None)."""
+        fn, keep = self._functional(terminal_weight, step)
         self._check(self.lib.tg_set_functional(self.ctx, C.byref(fn)))
 
-    def dual_solver(self, terminal_weight, want_dphi=False, want_lambda=True):
+    # -- MaxH functional ----------------------------------------------------------
+    def set_maxh_region(self, mask):
+        """Cells of the MaxH box (n booleans, all members concatenated), or
+        None to stop recording."""
+        if mask is None:
+            self._check(self.lib.tg_set_maxh_region(self.ctx, None))
+            return
+        m = np.ascontiguousarray(np.asarray(mask).reshape(-1) != 0, dtype=np.uint8)
+        self._check(self.lib.tg_set_maxh_region(self.ctx,
+                                                m.ctypes.data_as(C.POINTER(C.c_ubyte))))
+
+    def box_record(self, member=0):
+        """Largest h over the box in U^{s+1}, per recorded step s."""
+        cnt = C.c_int()
+        self._check(self.lib.tg_get_member_box_record(self.ctx, member, None, 0, C.byref(cnt)))
+        out = np.empty(max(cnt.value, 1))
+        self._check(self.lib.tg_get_member_box_record(self.ctx, member, _lib.dptr(out),
+                                                      cnt.value, C.byref(cnt)))
+        return out[: cnt.value]
+
+    def maxh(self):
+        """(hmax, step) per member: the MaxH value and the state U^step that
+        first holds it (0 when the box stayed dry)."""
+        nm = self.lib.tg_num_members(self.ctx)
+        hm = np.empty(nm)
+        st = (C.c_int * nm)()
+        self._check(self.lib.tg_get_maxh(self.ctx, _lib.dptr(hm), st))
+        return hm, np.array(list(st), dtype=np.int64)
+
+    def dual_solver(self, terminal_weight, want_dphi=False, want_lambda=True, step=None):
         """Reverse sweep over the last run: returns (lambda0 (3,n) or None,
-        dJ/dphi_bed or None).  terminal_weight None: the weights given to
-        set_functional."""
+        dJ/dphi_bed or None).  terminal_weight None: the functional given to
+        set_functional.  step: the state U^step the weights apply to (None:
+        the final one)."""
         fnp = None
         if terminal_weight is not None:
-            w = np.ascontiguousarray(terminal_weight, dtype=np.float64).reshape(-1)
-            fn = _lib.tg_functional()
-            fn.terminal_weight = _lib.dptr(w)
+            fn, keep = self._functional(terminal_weight, step)
             fnp = C.byref(fn)
         lam = np.empty(3 * self.problem.n) if want_lambda else None
         dphi = C.c_double(0.0)
@@ -327,20 +367,24 @@ class Ensemble(Simulation):
         return np.concatenate([np.asarray(a, dtype=np.float64).reshape(3, -1)
                                for a in per_member], axis=1)
 
-    def set_functional(self, weights):
-        """weights: list of (3, n_b) terminal weights, kept on the device for
-        dual_solver(None)."""
-        super().set_functional(np.ascontiguousarray(self.concat(weights)).reshape(-1))
+    def set_functional(self, weights, step=None):
+        """weights: list of (3, n_b) weights, kept on the device (with their
+        per-member steps, None: the final one) for dual_solver(None)."""
+        super().set_functional(np.ascontiguousarray(self.concat(weights)).reshape(-1), step)
 
-    def dual_solver(self, weights, want_dphi=False, want_lambda=True):
-        """weights: list of (3, n_b) terminal weights, or None for those given
-        to set_functional.  Returns (list of lambda0 (3, n_b) or None,
+    def set_maxh_region(self, masks):
+        """masks: list of per-member box masks (n_b booleans), or None."""
+        super().set_maxh_region(None if masks is None else
+                                np.concatenate([np.asarray(m).reshape(-1) for m in masks]))
+
+    def dual_solver(self, weights, want_dphi=False, want_lambda=True, step=None):
+        """weights: list of (3, n_b) weights, or None for the functional
+        given to set_functional; step: per-member states U^step they apply
+        to (None: the final one).  Returns (list of lambda0 (3, n_b) or None,
         per-member dJ/dphi_bed or None)."""
         fnp = None
         if weights is not None:
-            w = np.ascontiguousarray(self.concat(weights)).reshape(-1)
-            fn = _lib.tg_functional()
-            fn.terminal_weight = _lib.dptr(w)
+            fn, keep = self._functional(self.concat(weights), step)
             fnp = C.byref(fn)
         lam = np.empty(3 * self.ncell) if want_lambda else None
         dphi = C.c_double(0.0)

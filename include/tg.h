@@ -137,10 +137,14 @@ typedef struct tg_stats {
   int step;        /* number of steps completed                              */
 } tg_stats;
 
-/* Functional J = sum_cells w . U^N (terminal weights, FP64 SoA [3*n]); the
- * reference's per-step func_sens vector at the final step (calc_adjoint.C:93-97). */
+/* Functional J = sum_cells w . U^{step_b} of each member b (weights FP64 SoA
+ * [3*n]): the reference's per-step func_sens vector, nonzero at one step
+ * (calc_adjoint.C:49, 93-97, 150; MaxH_Functional::clac_func_sens
+ * calc_adjoint.C:470-496 sets it at the MaxH step only).  step: per member,
+ * in [0, steps of the last forward run]; NULL = the final step of the run. */
 typedef struct tg_functional {
   const double* terminal_weight;
+  const int* step;
 } tg_functional;
 
 typedef struct tg_ctx tg_ctx;
@@ -216,7 +220,9 @@ int tg_get_dt_record(tg_ctx* ctx, double* dt_out, int capacity, int* count);
  * Forward states are recomputed segment by segment from the checkpoints.
  * lambda0_out: FP64 [3*n] adjoint at step 0 (= dJ/dU^0), may be NULL.
  * dJ_dphibed_out: dJ/d phi_bed at frozen dt (may be NULL).
- * fn NULL: the weights last given to tg_set_functional. */
+ * fn NULL: the functional last given to tg_set_functional (weights and steps).
+ * Members whose step lies below the largest step start from lambda = 0 and
+ * take their weights when the sweep reaches their step. */
 int tg_adjoint_run(tg_ctx* ctx, const tg_params* prm, const tg_functional* fn,
                    double* lambda0_out, double* dJ_dphibed_out);
 
@@ -224,6 +230,25 @@ int tg_adjoint_run(tg_ctx* ctx, const tg_params* prm, const tg_functional* fn,
  * tg_adjoint_run with fn NULL (no host upload per sweep).  No reference
  * counterpart (the reference evaluates func_sens on its own mesh). */
 int tg_set_functional(tg_ctx* ctx, const tg_functional* fn);
+
+/* MaxH functional (MaxH_Functional, calc_adjoint.C:420-548, adjoint.h:270-297):
+ * mask [n] marks the functional's box (nonzero = cell centre strictly inside,
+ * calc_adjoint.C:462-464); NULL switches the record off.  While set, every
+ * step of tg_forward_run / tg_forward_replay / tg_forward_step records the
+ * largest h > 0 over each member's box cells in the state it produced
+ * (check_maxh, calc_adjoint.C:443-465); a run restarts the record. */
+int tg_set_maxh_region(tg_ctx* ctx, const unsigned char* mask);
+/* The record of member b: hmax_out[s] = largest h in the box of U^{s+1}
+ * (0 when none is > 0), s < count (the steps recorded). */
+int tg_get_member_box_record(tg_ctx* ctx, int member, double* hmax_out, int capacity,
+                             int* count);
+/* Per member: the largest recorded h and the first step at which the box
+ * holds it (check_maxh's strict '>' over the steps in order, from 0;
+ * MaxH_Functional::update calc_adjoint.C:498-514 for one process): the
+ * state U^{step}, 0 when the box never held h > 0.  With weights (1, 0, 0)
+ * on the box cells and this step in tg_functional, tg_adjoint_run is the
+ * reverse sweep of the MaxH functional. */
+int tg_get_maxh(tg_ctx* ctx, double* hmax, int* step);
 
 /* One reverse step on the current state: given U^s (the context state) and
  * lambda^{s+1}, writes lambda^s (FP64 SoA [3*n]); step index s selects
