@@ -159,6 +159,7 @@ SIGNATURES = {
     "tg_forward_replay": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, _dp,
                                     C.POINTER(tg_stats)]),
     "tg_get_dt_record": (C.c_int, [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)]),
+    "tg_get_trajectory_state": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
     "tg_adjoint_run": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.POINTER(tg_functional),
                                  _dp, _dp]),
     "tg_set_functional": (C.c_int, [C.c_void_p, C.POINTER(tg_functional)]),
@@ -199,6 +200,9 @@ SIGNATURES = {
     "tg_host_box_weights": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_double, _dp]),
+    "tg_host_write_solution": (C.c_int, [C.c_char_p, C.c_int64, _u32p, _i32p, _dp, _dp, _dp]),
+    "tg_host_read_solution": (C.c_int, [C.c_char_p, C.c_int64, _u32p, _dp, _dp, _dp,
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "tg_host_hsfc2d": (C.c_int, [C.c_int, _dp, _u32p]),
     "tg_host_hash_bucket": (C.c_int, [C.c_int, _u32p, C.c_double, C.c_double, C.c_int,
                                       _i32p]),

@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TG_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["tg_api.hip", "tg_kernels.hip", "tg_kernels32.hip", "tg_leaves.hip", "tg_host.cpp",
-           "tg_flatten.cpp"]
+           "tg_flatten.cpp", "tg_solrec.cpp"]
 HEADERS = ["tg_kernels.h", "tg_math.h", "tg_math32.h"]
 
 FLAGS = [
@@ -37,7 +37,7 @@ FLAGS = [
 
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), *srcs, "-o", LIB + ".tmp"]
+    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), *srcs, "-lz", "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

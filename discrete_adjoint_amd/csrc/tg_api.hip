@@ -1592,6 +1592,44 @@ int tg_get_member_dt_record(tg_ctx* ctx, int member, double* dt_out, int capacit
   return TG_OK;
 }
 
+int tg_get_trajectory_state(tg_ctx* ctx, int s, double* h, double* hu, double* hv) {
+  if (!ctx || !h || !hu || !hv) return TG_ERR_ARG;
+  const int N = ctx->run_steps, k = ctx->cfg.checkpoint_every, s0 = ctx->traj_s0;
+  if (N <= 0) return set_err(ctx, TG_ERR_STATE, "no forward run recorded");
+  if (s < 0 || s > N)
+    return set_err(ctx, TG_ERR_ARG, "step outside [0, steps of the last forward run]");
+  void* f[3];
+  if (s == N) {
+    for (int c = 0; c < 3; c++) f[c] = ctx->U[c];
+  } else if (s >= s0) {
+    split(ctx, traj_state(ctx, s - s0), f);
+  } else {  // U^a from its checkpoint, then steps a .. s-1 with the recorded dt
+    int rc = ensure_store(ctx, ctx->seg, (size_t)k);
+    if (rc) return rc;
+    const int a = (s / k) * k;
+    void* base[3];
+    split(ctx, ctx->ckpt[s / k], base);
+    copy_state(ctx, ctx->seg[0], base);
+    for (int t = a; t < s; t++) {
+      void* in[3];
+      void* out[3];
+      split(ctx, ctx->seg[t - a], in);
+      split(ctx, ctx->seg[t - a + 1], out);
+      enqueue_step(ctx, in, out, t, false);
+    }
+    split(ctx, ctx->seg[s - a], f);
+  }
+  HIPCHK(ctx, hipGetLastError());
+  int rc = check_err(ctx);
+  if (rc) return rc;
+  double* dst[3] = {h, hu, hv};
+  for (int c = 0; c < 3; c++) {
+    rc = d2h_field(ctx, f[c], 0, ctx->ncell, dst[c]);
+    if (rc) return rc;
+  }
+  return TG_OK;
+}
+
 int tg_adjoint_run(tg_ctx* ctx, const tg_params* prm, const tg_functional* fn,
                    double* lambda0_out, double* dJ_dphibed_out) {
   if (!ctx || (fn && !fn->terminal_weight)) return TG_ERR_ARG;

@@ -1,5 +1,7 @@
 """Adaptive-mesh flattening: numpy views of tg_host_hsfc2d,
-tg_host_hash_bucket and tg_host_flatten (include/tg_host.h)."""
+tg_host_hash_bucket and tg_host_flatten (include/tg_host.h), and the
+trajectory spill files (tg_host_write_solution / tg_host_read_solution)."""
+import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
@@ -19,6 +21,37 @@ def hsfc2d(norm_xy):
     if rc:
         raise ValueError(f"tg_host_hsfc2d returned {rc}")
     return keys
+
+
+def write_solution(path, keys, U, order=None):
+    """One step's trajectory spill file in the SolRec layout
+    (dualmesh.C:87-142): keys (n, 2) uint32, U = (h, hu, hv); elements in
+    `order` (None: index order)."""
+    k = np.ascontiguousarray(keys, dtype=np.uint32).reshape(-1, 2)
+    h, hu, hv = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in U)
+    o = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+    rc = _lib.load().tg_host_write_solution(str(path).encode(), len(k), _lib.uptr(k),
+                                            None if o is None else _lib.iptr(o),
+                                            _lib.dptr(h), _lib.dptr(hu), _lib.dptr(hv))
+    if rc:
+        raise OSError(f"tg_host_write_solution returned {rc}")
+
+
+def read_solution(path, keys, U=None):
+    """Reads a spill file into the elements with `keys` (dualmesh.C:234-297).
+    U: initial (h, hu, hv) for elements absent from the file (None: NaN).
+    Returns ((h, hu, hv), found, unknown)."""
+    k = np.ascontiguousarray(keys, dtype=np.uint32).reshape(-1, 2)
+    n = len(k)
+    h, hu, hv = ([np.full(n, np.nan) for _ in range(3)] if U is None else
+                 [np.array(a, dtype=np.float64, copy=True).reshape(-1) for a in U])
+    found, unknown = C.c_int64(), C.c_int64()
+    rc = _lib.load().tg_host_read_solution(str(path).encode(), n, _lib.uptr(k), _lib.dptr(h),
+                                           _lib.dptr(hu), _lib.dptr(hv), C.byref(found),
+                                           C.byref(unknown))
+    if rc:
+        raise OSError(f"tg_host_read_solution returned {rc}")
+    return (h, hu, hv), found.value, unknown.value
 
 
 def hash_bucket(keys, nbuckets=EL_TABLE_SIZE, keyrange=(KEY_RANGE, KEY_RANGE)):
@@ -70,7 +103,6 @@ def flatten(nx0, ny0, xmin, ymin, dx0, dy0, lev, ix, iy, nbuckets=EL_TABLE_SIZE)
     c.order, c.key, c.gen = _lib.iptr(t.order), _lib.uptr(t.key), _lib.iptr(t.gen)
     c.nbr, c.nkind, c.ngen = _lib.iptr(t.nbr), _lib.iptr(t.nkind), _lib.iptr(t.ngen)
     c.dx, c.xc = _lib.dptr(t.dx), _lib.dptr(t.xc)
-    import ctypes as C
     rc = _lib.load().tg_host_flatten(nx0, ny0, xmin, ymin, dx0, dy0, n, _lib.iptr(lev),
                                      _lib.iptr(ix), _lib.iptr(iy), nbuckets, C.byref(c))
     if rc:

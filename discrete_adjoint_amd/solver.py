@@ -135,6 +135,14 @@ class Simulation:
                                                _lib.dptr(dt), C.byref(st)))
         return st
 
+    def trajectory_state(self, s):
+        """U^s of the last run (0 <= s <= its steps), all members."""
+        n = self.lib.tg_num_cells(self.ctx)
+        h, hu, hv = np.empty(n), np.empty(n), np.empty(n)
+        self._check(self.lib.tg_get_trajectory_state(self.ctx, s, _lib.dptr(h), _lib.dptr(hu),
+                                                     _lib.dptr(hv)))
+        return h, hu, hv
+
     def dt_record(self):
         cnt = C.c_int()
         self._check(self.lib.tg_get_dt_record(self.ctx, None, 0, C.byref(cnt)))

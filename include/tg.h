@@ -210,6 +210,12 @@ int tg_forward_run(tg_ctx* ctx, const tg_params* prm, int nsteps, tg_stats* stat
 int tg_forward_replay(tg_ctx* ctx, const tg_params* prm, int nsteps,
                       const double* dt_seq, tg_stats* stats);
 
+/* U^s of the last forward run, 0 <= s <= its steps (the states
+ * SolRec::record_solution keeps, dualmesh.C:35-78): read from the stored
+ * trajectory, or recomputed from the checkpoint below s with the recorded
+ * dt.  FP64 host arrays of n. */
+int tg_get_trajectory_state(tg_ctx* ctx, int s, double* h, double* hu, double* hv);
+
 /* Copies the dt[] record of the last run (TimeProps::dt). */
 int tg_get_dt_record(tg_ctx* ctx, double* dt_out, int capacity, int* count);
 

@@ -114,6 +114,27 @@ int tg_host_flatten(int nx0, int ny0, double xmin, double ymin, double dx0, doub
                     const int32_t* lev, const int32_t* ix, const int32_t* iy, int nbuckets,
                     tg_leaf_table* out);
 
+/* Trajectory spill file of one step (SolRec::wrtie_sol_to_disk,
+ * dualmesh.C:87-142; the reference names it solution_<rank %04d>_<step %08d>):
+ * a gzip stream of int32 nzero, int32 nnz, nzero element keys (2 x uint32),
+ * then nnz records of key (2 x uint32) and h, hu, hv (FP64), host byte
+ * order.  Elements are taken in `order` (positions into the arrays; NULL =
+ * 0..n-1); an element whose h > 0 is false (h <= 0 or NaN) is a zero entry
+ * (SolRec::record_solution, dualmesh.C:53-70).
+ * Returns 0; -1 bad argument; -2 I/O error. */
+int tg_host_write_solution(const char* path, int64_t n, const uint32_t* keys,
+                           const int32_t* order, const double* h, const double* hu,
+                           const double* hv);
+
+/* Reads such a file into the elements with the given keys
+ * (SolRec::read_sol_from_disk, dualmesh.C:234-297): a zero entry gives
+ * (0, 0, 0), an element absent from the file keeps its values.  found: the
+ * file's entries matched to a key; unknown: those whose key is not among
+ * `keys` (the reference adds them to its table; here they are counted).
+ * Returns 0; -1 bad argument; -2 cannot open; -3 truncated or trailing data. */
+int tg_host_read_solution(const char* path, int64_t n, const uint32_t* keys, double* h,
+                          double* hu, double* hv, int64_t* found, int64_t* unknown);
+
 #ifdef __cplusplus
 }
 #endif
