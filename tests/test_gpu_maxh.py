@@ -196,3 +196,31 @@ def test_leaf_maxh(oracle_lib):
             lam, dphi = sim.dual_solver(w, want_dphi=True, step=s)
             assert np.array_equal(lam, lam_ref), s
             assert dphi == pytest.approx(dphi_ref, rel=1e-12), s
+
+
+@pytest.mark.parametrize("prec", [_lib.TG_MIXED, _lib.TG_FP32, _lib.TG_FWD64_ADJ32])
+def test_step_placed_sweep_in_every_precision(prec):
+    """In every precision mode, weights placed at U^s of an N-step run give
+    the terminal sweep of an s-step run bit for bit (same kernels on the
+    same states), member by member of an ensemble."""
+    N = 30
+    mem = _members(N)[:3]
+    masks = [_mask(p, 0.2, 0.4) for p in mem]
+    weights = [_maxh_weights(m) for m in masks]
+    steps = [N, 17, 4]
+    _say(f"case staged sweep precision {prec}")
+
+    def sweep(nrun, step):
+        with Ensemble(mem, precision=prec, checkpoint_every=6, use_graph=1) as ens:
+            ens.set_maxh_region(masks)
+            ens.run(nrun)
+            rec = [ens.box_record(b) for b in range(len(mem))]
+            lam, dphi = ens.dual_solver(weights, want_dphi=True, step=step)
+            return rec, lam, dphi
+
+    rec_all, lam_all, dphi_all = sweep(N, steps)
+    for b, s in enumerate(steps):
+        rec_s, lam_s, dphi_s = sweep(s, None)
+        assert np.array_equal(rec_all[b][:s], rec_s[b]), (prec, b)
+        assert np.array_equal(lam_all[b], lam_s[b]), (prec, b)
+        assert dphi_all[b] == pytest.approx(dphi_s[b], rel=1e-12, abs=0.0), (prec, b)
