@@ -200,6 +200,7 @@ SIGNATURES = {
     "tg_host_box_weights": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_double, _dp]),
+    "tg_host_leaf_d_gravity": (C.c_int, [C.c_int, _i32p, _dp, _dp, _dp]),
     "tg_host_write_solution": (C.c_int, [C.c_char_p, C.c_int64, _u32p, _i32p, _dp, _dp, _dp]),
     "tg_host_read_solution": (C.c_int, [C.c_char_p, C.c_int64, _u32p, _dp, _dp, _dp,
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

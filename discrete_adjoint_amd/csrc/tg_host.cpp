@@ -304,3 +304,40 @@ extern "C" int tg_host_box_weights(int nx, int ny, double dx, double dy, double 
     }
   return 0;
 }
+
+// calc_d_gravity (element2.C:3574-3653) on a flattened leaf table: the
+// neighbour of side xp / xm / yp / ym is the leaf in slot 1 / 3 / 2 / 0
+// (positive_x_side = 1) -- on a side with two finer leaves, the one of
+// slot s (neighbor[s]), never slot s + 4 -- and the spacing is the
+// difference of the centres along the direction; a side without neighbour
+// (-1: domain boundary) makes the difference one-sided, none on either side
+// gives 0.
+extern "C" int tg_host_leaf_d_gravity(int n, const int32_t* nbr, const double* xc,
+                                      const double* gz, double* dgrav) {
+  if (n < 0 || (n > 0 && (!nbr || !xc || !gz || !dgrav))) return -1;
+  for (int e = 0; e < n; e++)
+    for (int s = 0; s < 8; s++)
+      if (nbr[8 * e + s] < -1 || nbr[8 * e + s] >= n) return -1;
+  static const int kP[2] = {1, 2}, kM[2] = {3, 0};  // xp, yp; xm, ym
+  for (int e = 0; e < n; e++)
+    for (int d = 0; d < 2; d++) {
+      const int ep = nbr[8 * e + kP[d]], em = nbr[8 * e + kM[d]];
+      const double c = xc[2 * e + d];
+      double out;
+      if (ep >= 0 && em >= 0) {
+        const double dxp = xc[2 * ep + d] - c, dxm = c - xc[2 * em + d];
+        const double dp = (gz[ep] - gz[e]) / dxp, dm = (gz[e] - gz[em]) / dxm;
+        out = (dp * dxm + dm * dxp) / (dxm + dxp);  // weighted average
+      } else if (em >= 0) {
+        const double dxm = c - xc[2 * em + d];
+        out = (gz[e] - gz[em]) / dxm;
+      } else if (ep >= 0) {
+        const double dxp = xc[2 * ep + d] - c;
+        out = (gz[ep] - gz[e]) / dxp;
+      } else {
+        out = 0;  // no neighbour on either side: flat ground
+      }
+      dgrav[(size_t)d * n + e] = out;
+    }
+  return 0;
+}

@@ -23,6 +23,20 @@ def hsfc2d(norm_xy):
     return keys
 
 
+def leaf_d_gravity(table, gz):
+    """calc_d_gravity over the leaves of a flattened table: (2, n)."""
+    n = len(table.gen)
+    nbr = np.ascontiguousarray(table.nbr, dtype=np.int32).reshape(-1)
+    xc = np.ascontiguousarray(table.xc, dtype=np.float64).reshape(-1)
+    g = np.ascontiguousarray(gz, dtype=np.float64).reshape(-1)
+    out = np.zeros(2 * n)
+    rc = _lib.load().tg_host_leaf_d_gravity(n, _lib.iptr(nbr), _lib.dptr(xc), _lib.dptr(g),
+                                            _lib.dptr(out))
+    if rc:
+        raise ValueError(f"tg_host_leaf_d_gravity returned {rc}")
+    return out.reshape(2, n)
+
+
 def write_solution(path, keys, U, order=None):
     """One step's trajectory spill file in the SolRec layout
     (dualmesh.C:87-142): keys (n, 2) uint32, U = (h, hu, hv); elements in

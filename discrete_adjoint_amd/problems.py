@@ -251,8 +251,9 @@ def adaptive_problem(nx0=32, ny0=32, dx0=40.0, max_level=3, nsteps=200, refine_r
     centre refined down to max_level, then 2:1-balanced and flattened into the
     element-table order.  dem: None for the 35-degree plane / runout DEM of
     plane_runout_problem over the same extent, or a (nrows, ncols, xmin, ymax,
-    res, elev) raster.  Static coefficients of a leaf are those of its own
-    level's uniform grid (tg_host_static_coefs at that level's spacing);
+    res, elev) raster.  Gravity and curvature of a leaf are those of its own
+    level's uniform grid (tg_host_static_coefs at that level's spacing), d
+    gravity_z comes from the leaf's own neighbours (tg_host_leaf_d_gravity);
     the initial state samples the paraboloid at the leaf centres."""
     from . import mesh as _mesh
     Lx, Ly = nx0 * dx0, ny0 * dx0
@@ -288,11 +289,12 @@ def adaptive_problem(nx0=32, ny0=32, dx0=40.0, max_level=3, nsteps=200, refine_r
         if len(sel) == 0:
             continue
         nxl, nyl, s = nx0 << L, ny0 << L, dx0 / (1 << L)
-        g, dg, cv = static_coefs(dem, nxl, nyl, s, s, 0.5 * s, 0.5 * s)
+        g, _, cv = static_coefs(dem, nxl, nyl, s, s, 0.5 * s, 0.5 * s)
         hh, hhu, hhv = piles(nxl, nyl, s, s, 0.5 * s, 0.5 * s, [pile])
         cell = iy[o[sel]].astype(np.int64) * nxl + ix[o[sel]]
-        grav[:, sel], dgrav[:, sel], curv[:, sel] = g[:, cell], dg[:, cell], cv[:, cell]
+        grav[:, sel], curv[:, sel] = g[:, cell], cv[:, cell]
         h[sel], hu[sel], hv[sel] = hh[cell], hhu[cell], hhv[cell]
+    dgrav = _mesh.leaf_d_gravity(t, grav[2])  # across the leaves' own neighbours
     xb = 0.6 * Lx + 100.0
     box = (xb, xb + 200.0, 0.5 * Ly - 100.0, 0.5 * Ly + 100.0)
     pb = LeafProblem(name or f"adaptive_{nx0}x{ny0}_L{max_level}", nx0, ny0, dx0, dx0, 0.0, 0.0,
@@ -349,11 +351,12 @@ def adaptive_fractal_problem(nx0=128, dx0=40.0, max_level=3, radius=2600.0, npil
         if len(sel) == 0:
             continue
         nl, s = nx0 << Lv, dx0 / (1 << Lv)
-        g, dg, cv = static_coefs(dem, nl, nl, s, s, 0.5 * s, 0.5 * s)
+        g, _, cv = static_coefs(dem, nl, nl, s, s, 0.5 * s, 0.5 * s)
         hh, hhu, hhv = piles(nl, nl, s, s, 0.5 * s, 0.5 * s, specs)
         cell = iy[o[sel]].astype(np.int64) * nl + ix[o[sel]]
-        grav[:, sel], dgrav[:, sel], curv[:, sel] = g[:, cell], dg[:, cell], cv[:, cell]
+        grav[:, sel], curv[:, sel] = g[:, cell], cv[:, cell]
         h[sel], hu[sel], hv[sel] = hh[cell], hhu[cell], hhv[cell]
+    dgrav = _mesh.leaf_d_gravity(t, grav[2])
     cx, cy = specs[0][0], specs[0][1]
     box = (cx - 200.0, cx + 200.0, cy - 200.0, cy + 200.0)
     pb = LeafProblem(f"adaptive_fractal_{n}", nx0, nx0, dx0, dx0, 0.0, 0.0, t, grav, dgrav, curv,

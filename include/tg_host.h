@@ -114,6 +114,16 @@ int tg_host_flatten(int nx0, int ny0, double xmin, double ymin, double dx0, doub
                     const int32_t* lev, const int32_t* ix, const int32_t* iy, int nbuckets,
                     tg_leaf_table* out);
 
+/* d gravity_z of the leaves of a flattened table (calc_d_gravity,
+ * element2.C:3574-3653): per direction the weighted average of the one-sided
+ * differences of gz towards the neighbours of slots 1 (x+) and 3 (x-), 2 (y+)
+ * and 0 (y-) -- on a side with two finer leaves the one in slot s -- over the
+ * distances between centres; one-sided where a side has no neighbour (-1),
+ * 0 with none.  nbr [8n] and xc [2n] as tg_leaf_table; gz [n] the leaves'
+ * gravity_z; dgrav FP64 SoA [2n].  Returns 0; -1 bad argument. */
+int tg_host_leaf_d_gravity(int n, const int32_t* nbr, const double* xc, const double* gz,
+                           double* dgrav);
+
 /* Trajectory spill file of one step (SolRec::wrtie_sol_to_disk,
  * dualmesh.C:87-142; the reference names it solution_<rank %04d>_<step %08d>):
  * a gzip stream of int32 nzero, int32 nnz, nzero element keys (2 x uint32),
