@@ -1670,7 +1670,11 @@ int tg_set_maxh_region(tg_ctx* ctx, const unsigned char* mask) {
   ctx->box_on = false;
   ctx->box_version++;
   if (!mask) return TG_OK;
+  long long nmask = 0;
+  for (long long i = 0; i < ctx->ncell; i++) nmask += mask[i] != 0;
+  if (nmask > 0x7fffffffLL) return set_err(ctx, TG_ERR_ARG, "box holds more than 2^31 - 1 cells");
   std::vector<long long> cells;
+  cells.reserve((size_t)nmask);
   std::vector<BoxChunk> ch;
   for (int b = 0; b < ctx->ninst; b++) {
     const InstDev& I = ctx->inst_h[b];
@@ -1683,7 +1687,6 @@ int tg_set_maxh_region(tg_ctx* ctx, const unsigned char* mask) {
       ch.back().end++;
     }
   }
-  if (cells.size() > (size_t)0x7fffffff) return set_err(ctx, TG_ERR_ARG, "box too large");
   if (!ctx->box_rec) {
     int rc = dalloc(ctx, (void**)&ctx->box_rec,
                     sizeof(double) * ctx->ninst * (size_t)ctx->cfg.max_steps);
