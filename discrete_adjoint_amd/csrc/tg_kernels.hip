@@ -364,11 +364,17 @@ __global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
   const long long nx = I.nx;
   const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
   unsigned wet = 0u;
+  // the state of every cell is read here, before the barriers: a dry tile's
+  // update needs it, and one round trip to memory serves both uses
+  double out[4][3];
 #pragma unroll
   for (int b = 0; b < 4; b++) {
     if (!q[b].act) continue;
     const long long c = q[b].c;
-    bool w = !(a.U.h[c] < a.tiny);
+    out[b][0] = a.U.h[c];
+    out[b][1] = a.U.hu[c];
+    out[b][2] = a.U.hv[c];
+    bool w = !(out[b][0] < a.tiny);
     if (tx == 0 && q[b].hasW) w = w || !(a.U.h[c - 1] < a.tiny);
     if (tx == TX - 1 && q[b].hasE) w = w || !(a.U.h[c + 1] < a.tiny);
     if (ty == 0 && q[b].hasS) w = w || !(a.U.h[c - nx] < a.tiny);
@@ -386,17 +392,12 @@ __global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
       if ((W >> b) & 1u) a.tile_list[k++] = B.tile[b];
   }
   const double dtdx = dt / I.dx, dtdy = dt / I.dy, zero = 0.;
-  double out[4][3];
   unsigned nz = 0u;
   int nshort = 0;
 #pragma unroll
   for (int b = 0; b < 4; b++) {
     if (!q[b].act || ((W >> b) & 1u)) continue;
-    const long long c = q[b].c;
     nshort++;
-    out[b][0] = a.U.h[c];
-    out[b][1] = a.U.hu[c];
-    out[b][2] = a.U.hv[c];
 #pragma unroll
     for (int v = 0; v < 3; v++)
       out[b][v] = out[b][v] - dtdx * (zero - zero) - dtdy * (zero - zero);
@@ -836,10 +837,12 @@ __device__ double dphi_term(const AdjArgs& a, const InstDev& I, const Cell& q,
 // here.  Every other tile is appended to the list pass 2 walks.
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_classify(AdjArgs a) {
+  // the block record is read alongside the mark words (a skipped block
+  // wastes its 48 bytes; a listed one saves a round trip)
+  const BlockDev B = a.blocks[blockIdx.x];
   if (a.mk_out &&
       adjoint_skip(a.mk_U, a.dphi_part ? a.mk_lnext : nullptr, a.mk_out, (int)blockIdx.x))
     return;
-  const BlockDev B = a.blocks[blockIdx.x];
   const InstDev I = a.inst[B.inst];
   __shared__ unsigned s_busy;
   if (threadIdx.x == 0) s_busy = 0u;
