@@ -130,6 +130,17 @@ def test_maxh_single_steps_and_errors(oracle_lib):
         sim.set_state(*_U(pb))
         sim.run(30)
         assert sim.box_record().size == 0
+        # an empty box: a zero record and step 0; weights at step 0 are lambda^0
+        sim.set_maxh_region(np.zeros(pb.n, bool))
+        sim.set_state(*_U(pb))
+        sim.run(30)
+        rec = sim.box_record()
+        assert rec.size == 30 and not np.any(rec)
+        hm, st = sim.maxh()
+        assert hm[0] == 0.0 and st[0] == 0
+        w = _maxh_weights(mask)
+        lam, dphi = sim.dual_solver(w, want_dphi=True, step=0)
+        assert np.array_equal(lam, w) and dphi == 0.0
 
 
 def _members(nsteps):
