@@ -451,7 +451,10 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
     inst_mark(ctx, 0, true);
     launch_coef(a, compute_dt, ctx->stream);
     inst_mark(ctx, 0, false);
-    if (compute_dt) launch_dt(a, ctx->ninst, ctx->stream);
+    if (compute_dt) {
+      a.dt_clears_count = 1;
+      launch_dt(a, ctx->ninst, ctx->stream);
+    }
     use_tiles(ctx, 1, a);
     a.active = inst_counter(ctx, 1, true);
     a.short_cells = inst_short(ctx, 1, true);

@@ -128,6 +128,8 @@ struct StepArgs {
   // by the classifying pass, walked by the update) and its length
   int* tile_list;
   int* tile_count;
+  // 1: k_dt clears tile_count (the update pass then launches no fill)
+  int dt_clears_count;
   // zero marks of U and of the output state (both null: no marks); a tile
   // the marks show to need nothing is skipped by the coefficient and the
   // classifying pass

@@ -281,6 +281,7 @@ __global__ void k_member_fill(const uint32_t* __restrict__ src, uint32_t* __rest
 
 __global__ void k_dt(StepArgs a, int ninst) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0 && a.dt_clears_count) *a.tile_count = 0;  // the update's list starts empty
   if (b >= ninst) return;
   const InstDev I = a.inst[b];
   double min_ev = bitsd(a.red[b].min_ev), hmax = bitsd(a.red[b].hmax);
@@ -1091,7 +1092,7 @@ void launch_dt(const StepArgs& a, int ninst, hipStream_t s) {
 // classify every tile (skipped by the marks, short path written, or
 // listed), then a fixed grid walks the list with the full path
 void launch_update(const StepArgs& a, hipStream_t s) {
-  launch_zero(a.tile_count, sizeof(int), s);
+  if (!a.dt_clears_count) launch_zero(a.tile_count, sizeof(int), s);
   TG_LAUNCH_TX(a.shape, k_update_classify, dim3(a.nblocks), s, a);
   TG_LAUNCH_TX(a.shape, k_update, dim3(a.ntiles < LIST_GRID ? a.ntiles : LIST_GRID), s, a);
 }
