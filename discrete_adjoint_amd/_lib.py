@@ -160,6 +160,8 @@ SIGNATURES = {
                                     C.POINTER(tg_stats)]),
     "tg_get_dt_record": (C.c_int, [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)]),
     "tg_get_trajectory_state": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "tg_write_solution_files": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _u32p, _i32p,
+                                          C.c_int, C.c_int]),
     "tg_adjoint_run": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.POINTER(tg_functional),
                                  _dp, _dp]),
     "tg_set_functional": (C.c_int, [C.c_void_p, C.POINTER(tg_functional)]),

@@ -4,6 +4,7 @@
 // launch covers all members (tile table), each member with its own material
 // parameters, CFL dt and time.
 #include "../../include/tg.h"
+#include "../../include/tg_host.h"
 #include "tg_kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -1629,6 +1630,27 @@ int tg_get_trajectory_state(tg_ctx* ctx, int s, double* h, double* hu, double* h
   for (int c = 0; c < 3; c++) {
     rc = d2h_field(ctx, f[c], 0, ctx->ncell, dst[c]);
     if (rc) return rc;
+  }
+  return TG_OK;
+}
+
+int tg_write_solution_files(tg_ctx* ctx, const char* dir, int rank, const uint32_t* keys,
+                            const int32_t* order, int first, int last) {
+  if (!ctx || !dir || !keys || rank < 0 || first < 0 || last < first) return TG_ERR_ARG;
+  if (ctx->run_steps <= 0) return set_err(ctx, TG_ERR_STATE, "no forward run recorded");
+  if (last > ctx->run_steps + 1)
+    return set_err(ctx, TG_ERR_ARG, "last step beyond the last forward run");
+  const size_t n = (size_t)ctx->ncell;
+  std::vector<double> h(n), hu(n), hv(n);
+  for (int s = first; s < last; s++) {
+    int rc = tg_get_trajectory_state(ctx, s, h.data(), hu.data(), hv.data());
+    if (rc) return rc;
+    char name[64];
+    std::snprintf(name, sizeof name, "/solution_%04d_%08d", rank, s);
+    const std::string path = std::string(dir) + name;
+    if (tg_host_write_solution(path.c_str(), (int64_t)n, keys, order, h.data(), hu.data(),
+                               hv.data()) != 0)
+      return set_err(ctx, TG_ERR_ARG, "cannot write " + path);
   }
   return TG_OK;
 }

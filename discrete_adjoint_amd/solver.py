@@ -143,6 +143,17 @@ class Simulation:
                                                      _lib.dptr(hv)))
         return h, hu, hv
 
+    def write_solution_files(self, directory, keys, first=0, last=None, rank=0, order=None):
+        """U^s of the last run for s in [first, last) as the reference's spill
+        files directory/solution_<rank>_<s>; last None: every state."""
+        k = np.ascontiguousarray(keys, dtype=np.uint32).reshape(-1)
+        o = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+        if last is None:
+            last = len(self.dt_record()) + 1
+        self._check(self.lib.tg_write_solution_files(
+            self.ctx, str(directory).encode(), rank, _lib.uptr(k),
+            None if o is None else _lib.iptr(o), first, last))
+
     def dt_record(self):
         cnt = C.c_int()
         self._check(self.lib.tg_get_dt_record(self.ctx, None, 0, C.byref(cnt)))

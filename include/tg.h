@@ -216,6 +216,14 @@ int tg_forward_replay(tg_ctx* ctx, const tg_params* prm, int nsteps,
  * dt.  FP64 host arrays of n. */
 int tg_get_trajectory_state(tg_ctx* ctx, int s, double* h, double* hu, double* hv);
 
+/* Writes U^s of the last forward run for s in [first, last) as the
+ * reference's spill files <dir>/solution_<rank %04d>_<s %08d>
+ * (SolRec::wrtie_sol_to_disk, dualmesh.C:87-142; layout as
+ * tg_host_write_solution): keys [2n] the element keys, order as there
+ * (NULL = array order).  last may be the run's steps + 1 (U^N included). */
+int tg_write_solution_files(tg_ctx* ctx, const char* dir, int rank, const uint32_t* keys,
+                            const int32_t* order, int first, int last);
+
 /* Copies the dt[] record of the last run (TimeProps::dt). */
 int tg_get_dt_record(tg_ctx* ctx, double* dt_out, int capacity, int* count);
 

@@ -78,6 +78,16 @@ def test_leaf_states_and_spill_files(oracle_lib, tmp_path):
             assert (found, unknown) == (pb.n, 0)
             wet = ref[s][0] > 0
             assert np.array_equal(h[wet], ref[s][0][wet]) and not np.any(h[~wet])
+        # the whole run as the reference's per-step files, in one call
+        run = tmp_path / "run"
+        run.mkdir()
+        sim.write_solution_files(run, keys, rank=3)
+        for s in range(21):
+            got = gzip.open(run / f"solution_0003_{s:08d}", "rb").read()
+            mesh.write_solution(tmp_path / "o", keys, ref[s])
+            assert got == gzip.open(tmp_path / "o", "rb").read(), s
+        with pytest.raises(TgError):
+            sim.write_solution_files(run, keys, first=0, last=22)
 
 
 def test_ensemble_states(oracle_lib):
