@@ -47,6 +47,7 @@ constexpr int BOX_CHUNK = 1024;
 struct BlockDev {
   int inst, mw, mwW, mwE, mwS, mwN;
   int tile[4];
+  int x0, y0;  // the block's first cell in the member (multiples of MARK_W, MARK_H)
 };
 
 // Per-instance CFL reduction cells: bit patterns of non-negative doubles

@@ -116,12 +116,18 @@ __device__ __forceinline__ void mark_byte_store(uint32_t* mk, int mw, int b, boo
 }
 // The cells of a block's four tiles this thread handles (one per tile).
 template <int TX>
-__device__ __forceinline__ void block_cells(const StepArgs& a, const BlockDev& B,
-                                            const InstDev& I, Cell q[4]) {
+__device__ __forceinline__ void block_cells(const BlockDev& B, const InstDev& I, Cell q[4]) {
+  constexpr int TY = BLOCK / TX, PER_ROW = MARK_W / TX;
 #pragma unroll
   for (int b = 0; b < 4; b++) {
     q[b].act = false;
-    if (B.tile[b] >= 0) q[b] = cell_of<TX>(a.tiles[B.tile[b]], I);
+    if (B.tile[b] < 0) continue;
+    // tile byte b (mark_byte) from the block origin: no tile record to load
+    TileDev t{};
+    t.inst = B.inst;
+    t.x0 = B.x0 + (b % PER_ROW) * TX;
+    t.y0 = B.y0 + (b / PER_ROW) * TY;
+    q[b] = cell_of<TX>(t, I);
   }
 }
 
@@ -148,7 +154,7 @@ __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
   const BlockDev B = a.blocks[blockIdx.x];
   const InstDev I = a.inst[B.inst];
   Cell q[4];
-  block_cells<TX>(a, B, I, q);
+  block_cells<TX>(B, I, q);
   double h[4];
 #pragma unroll
   for (int b = 0; b < 4; b++) h[b] = q[b].act ? a.U.h[q[b].c] : 0.;
@@ -361,7 +367,7 @@ __global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
   __shared__ unsigned s_wet, s_nz;
   if (threadIdx.x == 0) { s_wet = 0u; s_nz = 0u; }
   Cell q[4];
-  block_cells<TX>(a, B, I, q);
+  block_cells<TX>(B, I, q);
   const long long nx = I.nx;
   const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
   unsigned wet = 0u;
@@ -848,11 +854,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_classify(AdjArgs a) {
   __shared__ unsigned s_busy;
   if (threadIdx.x == 0) s_busy = 0u;
   Cell q[4];
-#pragma unroll
-  for (int b = 0; b < 4; b++) {
-    q[b].act = false;
-    if (B.tile[b] >= 0) q[b] = cell_of<TX>(a.tiles[B.tile[b]], I);
-  }
+  block_cells<TX>(B, I, q);
   unsigned busy = 0u;
 #pragma unroll
   for (int b = 0; b < 4; b++) {
