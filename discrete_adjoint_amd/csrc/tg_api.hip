@@ -1205,9 +1205,10 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
       auto it = at.find(t.mw);
       if (it == at.end()) {
         it = at.emplace(t.mw, (int)ctx->blocks_h[v].size()).first;
+        const InstDev& I = ctx->inst_h[t.inst];
         ctx->blocks_h[v].push_back(BlockDev{t.inst, t.mw, t.mwW, t.mwE, t.mwS, t.mwN,
                                             {-1, -1, -1, -1}, t.x0 - t.x0 % MARK_W,
-                                            t.y0 - t.y0 % MARK_H});
+                                            t.y0 - t.y0 % MARK_H, I.nx, I.ny, I.off});
       }
       const int byte = ((t.y0 % MARK_H) / TY) * (MARK_W / TX) + (t.x0 % MARK_W) / TX;
       ctx->blocks_h[v][it->second].tile[byte] = i;

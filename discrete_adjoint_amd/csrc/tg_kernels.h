@@ -48,6 +48,8 @@ struct BlockDev {
   int inst, mw, mwW, mwE, mwS, mwN;
   int tile[4];
   int x0, y0;  // the block's first cell in the member (multiples of MARK_W, MARK_H)
+  int nx, ny;  // the member's extent and first cell (InstDev nx, ny, off), so
+  long long off;  // the block passes address cells without the member record
 };
 
 // Per-instance CFL reduction cells: bit patterns of non-negative doubles

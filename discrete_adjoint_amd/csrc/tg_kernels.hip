@@ -116,8 +116,10 @@ __device__ __forceinline__ void mark_byte_store(uint32_t* mk, int mw, int b, boo
 }
 // The cells of a block's four tiles this thread handles (one per tile).
 template <int TX>
-__device__ __forceinline__ void block_cells(const BlockDev& B, const InstDev& I, Cell q[4]) {
+__device__ __forceinline__ void block_cells(const BlockDev& B, Cell q[4]) {
   constexpr int TY = BLOCK / TX, PER_ROW = MARK_W / TX;
+  InstDev I;  // the fields cell_of reads, carried by the block record
+  I.nx = B.nx; I.ny = B.ny; I.off = B.off;
 #pragma unroll
   for (int b = 0; b < 4; b++) {
     q[b].act = false;
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
   const BlockDev B = a.blocks[blockIdx.x];
   const InstDev I = a.inst[B.inst];
   Cell q[4];
-  block_cells<TX>(B, I, q);
+  block_cells<TX>(B, q);
   double h[4];
 #pragma unroll
   for (int b = 0; b < 4; b++) h[b] = q[b].act ? a.U.h[q[b].c] : 0.;
@@ -367,7 +369,7 @@ __global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
   __shared__ unsigned s_wet, s_nz;
   if (threadIdx.x == 0) { s_wet = 0u; s_nz = 0u; }
   Cell q[4];
-  block_cells<TX>(B, I, q);
+  block_cells<TX>(B, q);
   const long long nx = I.nx;
   const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
   unsigned wet = 0u;
@@ -845,16 +847,15 @@ __device__ double dphi_term(const AdjArgs& a, const InstDev& I, const Cell& q,
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_classify(AdjArgs a) {
   // the block record is read alongside the mark words (a skipped block
-  // wastes its 48 bytes; a listed one saves a round trip)
+  // wastes its 64 bytes; a listed one saves a round trip)
   const BlockDev B = a.blocks[blockIdx.x];
   if (a.mk_out &&
       adjoint_skip(a.mk_U, a.dphi_part ? a.mk_lnext : nullptr, a.mk_out, (int)blockIdx.x))
     return;
-  const InstDev I = a.inst[B.inst];
   __shared__ unsigned s_busy;
   if (threadIdx.x == 0) s_busy = 0u;
   Cell q[4];
-  block_cells<TX>(B, I, q);
+  block_cells<TX>(B, q);
   unsigned busy = 0u;
 #pragma unroll
   for (int b = 0; b < 4; b++) {
