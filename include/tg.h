@@ -253,7 +253,8 @@ int tg_set_functional(tg_ctx* ctx, const tg_functional* fn);
  * (check_maxh, calc_adjoint.C:443-465); a run restarts the record. */
 int tg_set_maxh_region(tg_ctx* ctx, const unsigned char* mask);
 /* The record of member b: hmax_out[s] = largest h in the box of U^{s+1}
- * (0 when none is > 0), s < count (the steps recorded). */
+ * (0 when none is > 0), s < count (the steps recorded); the per-step values
+ * MaxH_Functional::check_maxh compares (calc_adjoint.C:443-465). */
 int tg_get_member_box_record(tg_ctx* ctx, int member, double* hmax_out, int capacity,
                              int* count);
 /* Per member: the largest recorded h and the first step at which the box
