@@ -660,6 +660,43 @@ def sweeps(args):
             log(f"sweep {r}: forward {1e3 * (t1 - t0):.1f} ms, reverse {1e3 * (t2 - t1):.1f} ms")
 
 
+# the two counter passes of the gather's profile (at most 8 SQ counters each)
+COUNTER_PASSES = [
+    ("SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_INSTS_LDS "
+     "SQ_LDS_BANK_CONFLICT SQ_WAIT_INST_ANY", "sqa"),
+    ("SQ_INSTS_VALU_FLOPS_FP64 SQ_INSTS_VALU_TRANS_F64 SQ_INSTS_VMEM_RD SQ_INSTS_SALU "
+     "SQ_WAIT_ANY SQ_ACTIVE_INST_LDS SQ_ACTIVE_INST_VMEM SQ_LDS_IDX_ACTIVE", "sqb"),
+]
+
+
+def counter_summary(csvs, symbol, last, out_path, commands):
+    """Per-dispatch averages of every counter in rocprofv3 --pmc CSVs over
+    the last `last` dispatches of the GPU kernel `symbol` (the probe's
+    instrumented launches on the final state), as a markdown table."""
+    import csv
+
+    vals = {}
+    for path in csvs:
+        rows = list(csv.DictReader(open(path)))
+        name_key = "Kernel_Name" if "Kernel_Name" in rows[0] else "Kernel-Name"
+        val_key = "Counter_Value" if "Counter_Value" in rows[0] else "Counter-Value"
+        for r in rows:
+            nm = r[name_key]
+            if f"{symbol}<" in nm or f"{symbol}(" in nm:
+                vals.setdefault(r.get("Counter_Name", r.get("Counter-Name")), []).append(
+                    float(r[val_key]))
+    lines = ["Commands:", ""] + [f"    {c}" for c in commands] + [
+        "", f"Averages over the last {last} dispatches of `{symbol}` per counter.", "",
+        "| counter | per dispatch |", "|---|---:|"]
+    avg = {}
+    for c in sorted(vals):
+        v = vals[c][-last:]
+        avg[c] = sum(v) / len(v)
+        lines.append(f"| `{c}` | {avg[c]:.4g} |")
+    open(out_path, "w").write("\n".join(lines) + "\n")
+    return avg
+
+
 def trace_summary(trace_csv, out_path, command):
     """Per-kernel dispatch counts and durations of the last forward run and
     the last reverse sweep of a `--sweeps` kernel trace (the forward run ends
@@ -837,6 +874,9 @@ def main():
                     help="launch kernel by kernel instead of replaying captured sweeps")
     ap.add_argument("--sweeps", type=int, default=0,
                     help="R repetitions of the measured sweep only, for kernel traces")
+    ap.add_argument("--counter-summary", default="",
+                    help="a.csv+b.csv,SYMBOL,N,out.md: per-dispatch counter averages of the "
+                         "last N dispatches of one GPU kernel from --pmc runs of --probe")
     ap.add_argument("--trace-summary", default="",
                     help="trace.csv,out.md: per-kernel table of the last forward run and "
                          "reverse sweep of a --sweeps kernel trace")
@@ -855,6 +895,13 @@ def main():
                          f"/tmp/pmc_{d} -o probe --output-format csv -- python3 "
                          f"$REPO/bench.py --probe --tiles {args.tiles}"
                          for c, d in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write"))])
+        return
+    if args.counter_summary:
+        paths, symbol, last, out_path = args.counter_summary.split(",")
+        counter_summary(paths.split("+"), symbol, int(last), out_path,
+                        ["cd /tmp && TMPDIR=/tmp timeout -s KILL 300 rocprofv3 --pmc " + c +
+                         " -d /tmp/pmc_" + d + " -o probe --output-format csv -- python3 "
+                         "$REPO/bench.py --probe --tiles 3,3,3" for c, d in COUNTER_PASSES])
         return
     if args.trace_summary:
         trace_csv, out_path = args.trace_summary.split(",")
