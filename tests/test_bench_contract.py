@@ -84,3 +84,19 @@ def test_trace_summary_splits_the_last_sweep(tmp_path):
     assert "5 dispatches" in fwd  # two coef + two update + k_stats
     assert "| `k_coef<16>` | 2 |" in fwd and "k_adjoint" not in fwd
     assert "| `k_adjoint<16>` | 4 | 2.0 |" in rev
+
+
+def test_counter_summary_averages_the_last_dispatches(tmp_path):
+    path = tmp_path / "c.csv"
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Counter_Name", "Counter_Value"])
+        w.writeheader()
+        for v in (100, 1, 2, 3):  # the first dispatch is not among the last three
+            w.writerow({"Kernel_Name": "void tg::k_adjoint<16>(tg::AdjArgs)",
+                        "Counter_Name": "SQ_WAVES", "Counter_Value": v})
+            w.writerow({"Kernel_Name": "void tg::k_update<16>(tg::StepArgs)",
+                        "Counter_Name": "SQ_WAVES", "Counter_Value": 1000})
+    out = tmp_path / "s.md"
+    avg = bench.counter_summary([str(path)], "k_adjoint", 3, str(out), ["cmd"])
+    assert avg == {"SQ_WAVES": 2.0}
+    assert "| `SQ_WAVES` | 2 |" in out.read_text()
