@@ -56,6 +56,9 @@ def load():
         lib.or_dstep_dphi.restype = None
         lib.or_dstep_dphi.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), C.c_double, _dp,
                                       _dp, _dp, C.c_double, _dp]
+        lib.or_kact_slopes.restype = None
+        lib.or_kact_slopes.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), _dp, _dp, _dp,
+                                       _dp, _dp]
         _LIB = lib
     return _LIB
 
@@ -132,6 +135,16 @@ class Oracle:
     def compute_dt(self, U):
         return self.lib.or_compute_dt(C.byref(self.mesh), C.byref(self.params), _p(U[0]),
                                       _p(U[1]), _p(U[2]), self.time)
+
+    def kact_slopes(self, U):
+        """(kact [n], slopes [n, 6]) of the state, as a forward step computes
+        them before its fluxes."""
+        U = [np.ascontiguousarray(a, dtype=np.float64) for a in U]
+        k = np.empty(self.n)
+        d = np.empty(6 * self.n)
+        self.lib.or_kact_slopes(C.byref(self.mesh), C.byref(self.params), _p(U[0]), _p(U[1]),
+                                _p(U[2]), _p(k), _p(d))
+        return k, d.reshape(self.n, 6)
 
     def jacobian(self, U, s, dt):
         J = np.empty(self.n * 81)

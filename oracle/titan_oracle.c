@@ -170,6 +170,15 @@ double or_compute_dt(const or_mesh* m, const or_params* p, const double* h,
   return incrtime(p, time, cfl_dt(m, p, U, kact));
 }
 
+/* The forward slopes and kact of every cell exactly as or_step computes them
+ * (slopes() then the get_coef_and_eigen.C:29-115 per-cell coefficient);
+ * kact [n], dU [6n] (per cell dh/dx, dhu/dx, dhv/dx, dh/dy, dhu/dy, dhv/dy). */
+void or_kact_slopes(const or_mesh* m, const or_params* p, const double* h,
+                    const double* hu, const double* hv, double* kact, double* dU) {
+  const double* U[3] = {h, hu, hv};
+  all_kact_fwd(m, p, U, kact, dU);
+}
+
 /* ---- forward step (step.C:29-275, order 1) ------------------------------- */
 
 int or_step(const or_mesh* m, const or_params* p, double frict_tiny, double* h,

@@ -52,6 +52,12 @@ int or_step(const or_mesh* m, const or_params* p, double frict_tiny,
             double* h, double* hu, double* hv,
             double dt_in, double* time, double* dt_out, double* mass_out);
 
+/* Forward slopes (dU [6n]: dh/dx, dhu/dx, dhv/dx, dh/dy, dhu/dy, dhv/dy per
+ * cell) and kact [n] of every cell, as or_step computes them (slopes.C:25-83,
+ * element2.C:830-939, getcoef.f:19-66; kact = epsilon where h <= tiny). */
+void or_kact_slopes(const or_mesh* m, const or_params* p, const double* h,
+                    const double* hu, const double* hv, double* kact, double* dU);
+
 /* CFL dt of the current state (get_coef_and_eigen.C:29-180 + incrtime). */
 double or_compute_dt(const or_mesh* m, const or_params* p,
                      const double* h, const double* hu, const double* hv,
