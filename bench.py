@@ -2,28 +2,35 @@
 adjoint sweep (checkpointed, recomputed in reverse) on one MI355X.
 
     python bench.py [--steps K] [--warmup W] [--members 256]
-                    [--workload ensemble|grid|adaptive]
+                    [--workload ensemble|grid|adaptive] [--precision fp64|adj32|mixed|fp32]
 
 Default workload ("ensemble"): 256 independent members advanced together in
 every launch.  Member b is a uniform side x side block (side drawn from
 {64, 128, 256, 512}, 5 m cells) on its own seeded diamond-square DEM, with
 phi_bed ~ U(15, 30) deg, phi_int ~ U(28, 38) deg and one paraboloid pile of
-volume lognormal(ln 3e5 m^3, 0.7); K = 500 forward steps, then the adjoint of
-"h integrated over the middle half of the block at the last step" back to
-step 0, checkpoints every 25 steps, and dJ/dphi_bed per member.
+volume lognormal(ln 3e5 m^3, 0.7); K = 500 forward steps (the default; any
+--steps), then the adjoint of "h integrated over the middle half of the
+block at the last step" back to step 0, with dJ/dphi_bed per member.
 
-The measured path is the batched mixed-precision one (FP32 flux/Jacobian
-arithmetic, FP64 state, switches, CFL and sums).  The same ensemble is then
-run (a) batched in FP64 and (b) member by member in FP64 (the plain FP64 GPU
-path); (a) is the accuracy reference for the per-member functional and
-gradient errors, (b) the speed reference.
+The measured path (the headline value) is the FP64 batched path (bit-exact
+with the CPU restatement; --precision selects another mode) with checkpoints
+every 25 steps and nothing else kept: the reverse sweep restores each
+checkpoint and recomputes its segment, as the memory-bounded schedule does.
+Sweeps are captured once and replayed (hipGraph), tiles come from the
+project's autotuner.  The same ensemble is also run (a) with the last states
+of the forward run kept whole in HBM (stored trajectory: less recompute),
+(b) member by member in FP64 (the plain FP64 GPU path, warmed, graph
+replayed, with the same tiles), and (c) batched in the other precisions
+(errors against FP64 per member).
 
 A "step" is one forward step plus its reverse step over every cell of every
 member; value = total cell-steps / wall time of the timed forward run plus
-reverse sweep (synchronised on both sides, inputs resident on the GPU).
+reverse sweep (synchronised on both sides, inputs resident on the GPU).  The
+kernels skip blocks known to hold +0.0 and take exact short paths on dry or
+void tiles, so the line also gives the cell-steps that took the full path.
 
 "grid" workload: one 2048 x 2048 block (2.5 m) with three piles, 1500 steps,
-mixed precision against FP64 (L2 / Linf errors of h, hu, hv and the
+a reduced-precision mode against FP64 (L2 / Linf errors of h, hu, hv and the
 functional).
 
 "adaptive" workload: the FP64 leaf-table path on a fixed adaptive mesh of
@@ -87,42 +94,62 @@ def stats(e):
             "p90": float(np.percentile(e, 90))}
 
 
-def cpu_baseline(problems_, nsteps):
-    """The oracle (FP64 C restatement of the reference, one core) on a
-    bounded sample: the first `nsteps` forward steps of each given member and
-    their adjoint steps.  Returns (cell-steps, forward s, adjoint s)."""
+def cpu_info():
+    """Host CPU model and the cores this process may run on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return model, usable, os.cpu_count()
+
+
+def cpu_baseline(problems_, U_mid, s_mid, nsteps, omp):
+    """The oracle (FP64 C restatement of the reference) on a bounded sample
+    of the same workload: `nsteps` forward steps of every member from its
+    state U^{s_mid} of the GPU run (a representative, moving state) and the
+    adjoint steps back over them.  omp: the OpenMP build on all usable
+    cores, else one thread.  Returns (cell-steps, forward s, adjoint s)."""
     import numpy as np
 
     import oracle
     tf = ta = 0.0
     cs = 0
+    o = 0
     for pb in problems_:
-        orc = oracle.Oracle.from_problem(pb)
-        lam = np.zeros((3, pb.n))
-        lam[0] = 1.0
-        U = [pb.h.copy(), pb.hu.copy(), pb.hv.copy()]
+        orc = oracle.Oracle.from_problem(pb, omp=omp)
+        U = [np.array(a[o:o + pb.n], copy=True) for a in U_mid]
+        o += pb.n
+        lam = pb.box_weights().reshape(3, -1).copy()
         traj = [[a.copy() for a in U]]
         dts = []
-        for s in range(nsteps):
+        for s in range(s_mid, s_mid + nsteps):
             t0 = time.perf_counter()
             dt, _ = orc.step(U, s)
             tf += time.perf_counter() - t0
             dts.append(dt)
             traj.append([a.copy() for a in U])   # trajectory store: bookkeeping, not timed
         t0 = time.perf_counter()
-        for s in range(nsteps - 1, -1, -1):
-            lam, _ = orc.adjoint_step(traj[s], s, dts[s], lam)
+        for i in range(nsteps - 1, -1, -1):
+            lam, _ = orc.adjoint_step(traj[i], s_mid + i, dts[i], lam)
         ta += time.perf_counter() - t0
         cs += pb.n * nsteps
     return cs, tf, ta
 
 
-def warm_oracle(pb):
+def warm_oracle(pb, omp=False):
     """Faults in the oracle's reusable scratch buffers (untimed)."""
     import numpy as np
 
     import oracle
-    orc = oracle.Oracle.from_problem(pb)
+    orc = oracle.Oracle.from_problem(pb, omp=omp)
     U = [pb.h.copy(), pb.hu.copy(), pb.hv.copy()]
     dt, _ = orc.step(U, 0)
     lam = np.zeros((3, pb.n))
@@ -161,42 +188,76 @@ def forward_step_json(rl, prec, ab=None, sb=None):
             "short_path_bytes_per_cell": sb["update"]}
 
 
-def roofline_json(rl, dom, kin, prec, workload, ab=None, sb=None):
+def roofline_json(rl, dom, kin, prec, workload, ab=None, sb=None, run_config=None):
     """The roofline object of the JSON line for the dominant role `dom`."""
     ab = ab or algo_bytes(prec)
     sb = sb or short_bytes(prec)
     k = kin["update" if dom == "forward_step" else dom]
-    return {"bound": "hbm", "achieved": rl[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": rl[dom]["gbs"] / HBM_PEAK_GBS,
-            "traffic": traffic_from_profile(dom, workload), "kernel": dom,
-            "algorithmic_bytes_per_cell": ab[dom],
-            "short_path_bytes_per_cell": sb["update" if dom == "forward_step" else dom],
-            "algorithmic_bytes_per_launch": rl[dom]["bytes_per_launch"],
-            "full_path_share": k["full_path_share"], "short_path_share": k["short_path_share"],
-            "avg_launch_ms": rl[dom]["avg_launch_ms"],
-            "launches_timed": rl[dom]["launches_timed"]}
+    traffic, tinfo = (traffic_from_profile(dom, workload, run_config) if run_config
+                      else (None, {"source": None}))
+    out = {"bound": "hbm", "achieved": rl[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": rl[dom]["gbs"] / HBM_PEAK_GBS,
+           "traffic": traffic, "kernel": dom,
+           "algorithmic_bytes_per_cell": ab["forward_step" if dom == "forward_step" else dom],
+           "short_path_bytes_per_cell": sb["update" if dom == "forward_step" else dom],
+           "algorithmic_bytes_per_launch": rl[dom]["bytes_per_launch"],
+           "full_path_share": k["full_path_share"], "short_path_share": k["short_path_share"],
+           "avg_launch_ms": rl[dom]["avg_launch_ms"],
+           "launches_timed": rl[dom]["launches_timed"],
+           "traffic_info": tinfo}
+    if traffic:
+        out["traffic_over_algorithmic"] = traffic / rl[dom]["bytes_per_launch"]
+    return out
 
 
-def replay_roofline(r, kin, K):
+def recomputed_steps(K, k, stored):
+    """Forward steps the reverse sweep recomputes: below the stored part
+    (K - stored states), each checkpoint segment [a, b) recomputes
+    U^{a+1} .. U^{b-1}."""
+    s0 = K - stored
+    return sum(min(a + k, s0) - a - 1 for a in range(0, s0, k))
+
+
+def replay_roofline(r, kin, K, k):
     """Launch durations from the timed (graph-replayed) run itself, HIP
-    events on the library's stream: the reverse step = the stored part of
-    the reverse sweep (reverse steps back to back) / its steps; the forward
+    events on the library's stream around the replayed sweeps: the forward
     step (coefficients + dt + update, checkpoint copies included) = the
-    forward sweep / K.  Bytes per launch from the instrumented run's
-    counters (the same inputs give the same counts).  Returns per-role
-    figures and the dominant role (largest time in the run)."""
+    forward sweep / K; the reverse step = the reverse sweep minus its
+    checkpoint-segment recompute (timed apart by events between the
+    segment graphs) / K; the recompute step = that recompute / its steps.
+    Bytes per launch from the instrumented run's counters (the same inputs
+    give the same counts).  Returns per-role figures and the dominant role
+    (largest time in the run)."""
     out = {}
-    if r["stored_steps"] > 0:
-        ms = r["stored_ms"] / r["stored_steps"]
-        b = kin["adjoint"]["bytes_per_launch"]
-        out["adjoint"] = {"avg_launch_ms": ms, "launches_timed": r["stored_steps"],
-                          "bytes_per_launch": b, "gbs": b / (ms * 1e-3) / 1e9}
+    rec = max(r["rec_ms"], 0.0)
+    ms = (r["adj_ms"] - rec) / K
+    b = kin["adjoint"]["bytes_per_launch"]
+    out["adjoint"] = {"avg_launch_ms": ms, "launches_timed": K, "bytes_per_launch": b,
+                      "gbs": b / (ms * 1e-3) / 1e9}
     ms = r["fwd_ms"] / K
     b = kin["update"]["bytes_per_launch"]
     out["forward_step"] = {"avg_launch_ms": ms, "launches_timed": K, "bytes_per_launch": b,
                            "gbs": b / (ms * 1e-3) / 1e9}
-    dom = max(out, key=lambda n: out[n]["avg_launch_ms"] * K)
+    nrec = recomputed_steps(K, k, r["stored_steps"])
+    if nrec > 0 and rec > 0:
+        ms = rec / nrec
+        out["recompute_step"] = {"avg_launch_ms": ms, "launches_timed": nrec,
+                                 "bytes_per_launch": b, "gbs": b / (ms * 1e-3) / 1e9,
+                                 "note": "checkpoint restores included"}
+    dom = max(("adjoint", "forward_step"), key=lambda n: out[n]["avg_launch_ms"])
     return dom, out
+
+
+def path_rates(kin, K, wall, ncells):
+    """Cell-steps per second that took each kernel's full path, and that
+    were visited at all (full or short path; the rest were skipped by the
+    zero marks), averaged over the update and the adjoint launches."""
+    full = sum(kin[n]["full_path_share"] for n in ("update", "adjoint")) / 2
+    short = sum(kin[n]["short_path_share"] for n in ("update", "adjoint")) / 2
+    cs = ncells * K
+    return {"full_path_cell_steps_per_s": full * cs / wall,
+            "visited_cell_steps_per_s": (full + short) * cs / wall,
+            "full_path_share": full, "short_path_share": short}
 
 
 def short_bytes(prec):
@@ -264,23 +325,30 @@ def tile_shape(v):
     return f"{tx.value}x{ty.value}" + (" xcd-banded" if band.value else " row-major")
 
 
-def traffic_from_profile(kernel_name, workload):
+def traffic_from_profile(kernel_name, workload, run_config):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
-    WRITE_SIZE runs, scaled by a copy kernel's counters), read from the
-    committed summary for this kernel and workload; None when absent."""
+    WRITE_SIZE runs of --probe, scaled by a copy kernel's counters), from the
+    committed summary for this kernel role and workload -- used only when the
+    summary's run configuration (members, steps, checkpoint interval,
+    precision, stored trajectory, tiles) is this run's, so that traffic and
+    algorithmic bytes describe the same launches.  Returns (traffic, info)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel_name}_{workload}.json")
     if not os.path.exists(path):
-        return None
+        return None, {"source": None, "reason": f"no {os.path.relpath(path, ROOT)}"}
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        t = json.load(f)
+    info = {"source": os.path.relpath(path, ROOT), "profile_config": t.get("config"),
+            "profile_algorithmic_bytes_per_launch": t.get("algorithmic_bytes_per_launch")}
+    if t.get("config") != run_config:
+        info["reason"] = "profile taken on another configuration: not used"
+        return None, info
+    return t.get("hbm_bytes_per_launch"), info
 
 
 def store_budget(args):
-    """Bytes of device memory each context may use to keep the last forward
+    """Bytes of device memory a context may use to keep the last forward
     states whole (the reverse sweep recomputes only below them): a fraction
-    of the free HBM, 0 with --no-store."""
-    if args.no_store:
-        return 0
+    of the free HBM."""
     import ctypes as C
 
     from discrete_adjoint_amd import _lib
@@ -291,11 +359,41 @@ def store_budget(args):
     return int(args.store_fraction * free.value)
 
 
+def ensemble_run_config(args, store, tiles):
+    """What a traffic profile must share with this run to describe the same
+    launches."""
+    return {"members": args.members, "seed": args.seed, "steps": args.steps,
+            "checkpoint_every": args.checkpoint_every, "precision": args.precision,
+            "stored_trajectory": bool(store), "tiles": list(tiles)}
+
+
+def plain_fp64(E, weights, K, W, k, graph, tiles):
+    """The plain FP64 GPU path: one member at a time, each warmed untimed
+    (first launches, checkpoint store, graph capture of its K-step sweeps),
+    its functional handed over before the clock, the batched run's tiles,
+    checkpoints only like the measured path.  Returns wall seconds."""
+    from discrete_adjoint_amd.solver import Simulation
+    total = 0.0
+    for b, p in enumerate(E):
+        with Simulation(p, checkpoint_every=k, max_steps=max(K, W, 1), use_graph=graph,
+                        store_bytes=0) as sim:
+            sim.select_tiles(tiles)
+            sim.run(K)
+            sim.dual_solver(weights[b], want_dphi=True, want_lambda=False)
+            sim.set_functional(weights[b])
+            sim.set_state(p.h, p.hu, p.hv)
+            t0 = time.perf_counter()
+            sim.run(K)
+            sim.dual_solver(None, want_dphi=True, want_lambda=False)
+            total += time.perf_counter() - t0
+    return total
+
+
 def bench_ensemble(args):
     import numpy as np
 
     from discrete_adjoint_amd import _lib, problems
-    from discrete_adjoint_amd.solver import Ensemble, Simulation
+    from discrete_adjoint_amd.solver import Ensemble
 
     K, W, k = args.steps, args.warmup, args.checkpoint_every
     t0 = time.perf_counter()
@@ -305,6 +403,7 @@ def bench_ensemble(args):
     h0 = tuple(np.concatenate([getattr(p, f) for p in E]) for f in ("h", "hu", "hv"))
     log(f"setup {time.perf_counter() - t0:.1f}s: {len(E)} members, {ncells} cells")
     prec = prec_of(args.precision)
+    cell_steps = ncells * K
 
     def member_J(H):
         out, o = [], 0
@@ -313,59 +412,62 @@ def bench_ensemble(args):
             o += p.n
         return np.array(out)
 
-    res = {}
-    roof = None
     graph = 0 if args.no_graph else 1
-    store = store_budget(args)
-    log(f"stored-trajectory budget {store / 2**30:.1f} GiB per context")
+    store_head = store_budget(args) if args.store else 0
+
+    def measure(mode, store, roofline=False, mid_state=False):
+        with Ensemble(E, precision=mode, checkpoint_every=k, max_steps=max(K, W, 1),
+                      use_graph=graph, store_bytes=store) as ens:
+            # untimed: first launches, allocation of the checkpoint store
+            ens.run(max(W, 1))
+            ens.dual_solver(weights, want_lambda=False)
+            if not args.no_autotune:  # tile variant per kernel (untimed)
+                var, _ = ens.autotune(reps=5)
+                log(f"precision {mode}: autotuned tile variants {var}")
+            if graph:  # capture the K-step sweeps (untimed)
+                ens.set_state(*h0)
+                ens.run(K)
+                ens.dual_solver(weights, want_dphi=True, want_lambda=False)
+            r = timed_run(ens, K, weights, h0)
+            r["tiles"] = ens.tile_selection()
+            r["J"] = member_J(r["H"])
+            del r["H"]
+            log(f"precision {mode} store {store > 0}: forward {r['wall_f'] * 1e3:.1f} ms wall "
+                f"({r['fwd_ms']:.1f} ms events), reverse {r['wall_a'] * 1e3:.1f} ms wall "
+                f"({r['adj_ms']:.1f} ms events, recompute {r['rec_ms']:.1f}), "
+                f"{cell_steps / (r['wall_f'] + r['wall_a']):.3g} cell-steps/s")
+            if roofline:
+                r["roof"] = instrumented_roofline(ens, mode, ncells, K, weights, h0)
+            if mid_state:
+                r["mid"] = ens.trajectory_state(K // 2)
+            return r
+
+    res = {}
     names = [args.precision]
     if not args.product_only:
         names += ["fp64"] if args.precision != "fp64" else []
         names += [c for c in args.compare.split(",") if c and c not in names]
     for name in names:
         mode = prec_of(name)
-        with Ensemble(E, precision=mode, checkpoint_every=k, max_steps=max(K, W, 1),
-                      use_graph=graph, store_bytes=store) as ens:
-            if W > 0:  # untimed: first launches, allocation of the checkpoint store
-                ens.run(W)
-                ens.dual_solver(weights, want_lambda=False)
-                if not args.no_autotune:  # tile variant per kernel (untimed)
-                    var, _ = ens.autotune(reps=5)
-                    log(f"precision {mode}: autotuned tile variants {var}")
-                if graph:  # capture the K-step sweeps (untimed)
-                    ens.set_state(*h0)
-                    ens.run(K)
-                    ens.dual_solver(weights, want_lambda=False)
-            r = timed_run(ens, K, weights, h0)
-            r["tiles"] = ens.tile_selection()
-            r["J"] = member_J(r["H"])
-            del r["H"]
-            res[mode] = r
-            log(f"precision {mode}: forward {r['wall_f'] * 1e3:.1f} ms wall "
-                f"({r['fwd_ms']:.1f} ms kernels), reverse {r['wall_a'] * 1e3:.1f} ms wall "
-                f"({r['adj_ms']:.1f} ms kernels, recompute {r['rec_ms']:.1f})")
-            if mode == prec:
-                roof = instrumented_roofline(ens, prec, ncells, K, weights, h0)
-    # plain FP64 GPU path: one member at a time
-    plain_s = 0.0
-    for b, p in enumerate([] if args.product_only else E):
-        with Simulation(p, checkpoint_every=k, max_steps=max(K, 1), store_bytes=store) as sim:
-            if b == 0 and W > 0:
-                sim.run(W)
-                sim.dual_solver(weights[b], want_lambda=False)
-                sim.set_state(p.h, p.hu, p.hv)
-            t0 = time.perf_counter()
-            sim.run(K)
-            sim.dual_solver(weights[b], want_dphi=True, want_lambda=False)
-            plain_s += time.perf_counter() - t0
-    cell_steps = ncells * K
-    plain = cell_steps / plain_s if plain_s > 0 else None
-    if plain:
-        log(f"plain FP64 per-member: {plain_s:.2f} s wall, {plain:.3g} cell-steps/s")
-
+        res[mode] = measure(mode, store_head, roofline=(mode == prec),
+                            mid_state=(mode == _lib.TG_FP64 and not args.no_cpu))
     m, d = res[prec], res.get(_lib.TG_FP64)
     wall = m["wall_f"] + m["wall_a"]
     value = cell_steps / wall
+
+    stored = None
+    if not args.product_only and not args.store:
+        r = measure(prec, store_budget(args))
+        stored = {"cell_steps_per_s": cell_steps / (r["wall_f"] + r["wall_a"]),
+                  "stored_states": r["stored_steps"], "bytes": store_budget(args),
+                  "kernel_ms": {"forward": r["fwd_ms"], "reverse": r["adj_ms"],
+                                "recompute": r["rec_ms"]}}
+
+    plain = None
+    if not args.product_only:
+        plain_s = plain_fp64(E, weights, K, W, k, graph, m["tiles"])
+        plain = cell_steps / plain_s
+        log(f"plain FP64 per-member: {plain_s:.2f} s wall, {plain:.3g} cell-steps/s")
 
     def errors(r):
         Jd, Jm = d["J"], r["J"]
@@ -382,9 +484,6 @@ def bench_ensemble(args):
                 "members": len(E)}
 
     errs = errors(m) if (d is not None and prec != _lib.TG_FP64) else None
-    if errs:
-        log(f"per-member functional rel err: {errs['functional_rel']}, "
-            f"dJ/dphi_bed rel err: {errs['dJ_dphibed_rel']}")
     others = {}
     for name in names:
         mode = prec_of(name)
@@ -395,20 +494,32 @@ def bench_ensemble(args):
                         "errors_vs_fp64": errors(r)}
         log(f"{name}: {others[name]}")
 
-    _, kin, _ = roof
-    dom, rl = replay_roofline(m, kin, K)
-    ab = algo_bytes(prec)
-    workload = f"ensemble256_{args.precision}"
+    _, kin, _ = m["roof"]
+    dom, rl = replay_roofline(m, kin, K, k)
+    workload = f"ensemble{args.members}_{args.precision}"
+    run_config = ensemble_run_config(args, store_head, m["tiles"])
+    rates = path_rates(kin, K, wall, ncells)
+
     cpu = None
     if not (args.no_cpu or args.product_only):
+        model, usable, total = cpu_info()
+        s_mid = K // 2
+        warm_oracle(E[0], omp=True)
+        cs, tf, ta = cpu_baseline(E, d["mid"], s_mid, args.cpu_steps, omp=True)
+        threads = int(os.environ.get("OMP_NUM_THREADS", usable))
         warm_oracle(E[0])
-        cs, tf, ta = cpu_baseline(E, args.cpu_steps)
-        cpu = {"value": cs / (tf + ta), "unit": "cell-steps/s", "cores": 1, "kind": "port",
-               "sample": f"first {args.cpu_steps} forward steps of each of the {len(E)} members "
-                         f"and their {args.cpu_steps} adjoint steps (oracle: FP64 C restatement, "
-                         f"-O3, one thread); forward {cs / tf:.3g}, adjoint {cs / ta:.3g} "
-                         f"cell-steps/s"}
-        log(f"cpu baseline: fwd {tf:.2f}s adj {ta:.2f}s over {cs} cell-steps")
+        cs1, tf1, ta1 = cpu_baseline(E, d["mid"], s_mid, args.cpu_steps, omp=False)
+        cpu = {"value": cs / (tf + ta), "unit": "cell-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{args.cpu_steps} forward steps of each of the {len(E)} members from "
+                         f"its state at step {s_mid} of the GPU run, and their adjoint steps "
+                         f"(oracle: FP64 C restatement of the reference, -O3, OpenMP over "
+                         f"cells, {threads} threads); forward {cs / tf:.3g}, adjoint "
+                         f"{cs / ta:.3g} cell-steps/s",
+               "single_thread_value": cs1 / (tf1 + ta1),
+               "cpu_model": model, "usable_cores": usable, "host_cpus": total}
+        log(f"cpu baseline: {threads} threads {cpu['value']:.3g}, one thread "
+            f"{cpu['single_thread_value']:.3g} cell-steps/s over {cs} cell-steps")
 
     adaptive = None
     if not args.product_only and not args.no_adaptive:
@@ -431,15 +542,19 @@ def bench_ensemble(args):
             "workload": f"ensemble of {len(E)} members (uniform blocks, side in "
                         f"{{64,128,256,512}}, 5 m, seeded fractal DEMs, one pile each), "
                         f"{K} forward steps + discrete adjoint with dJ/dphi_bed, checkpoints "
-                        f"every {k}, all members in every launch",
+                        f"every {k}" + (" and the last states kept whole" if store_head else
+                                        " (nothing else kept: segments recomputed)") +
+                        ", all members in every launch",
             "members": len(E),
             "cells": ncells,
             "checkpoint_every": k,
             "precision": args.precision,
             "forward_cell_steps_per_s": cell_steps / m["wall_f"],
             "adjoint_cell_steps_per_s": cell_steps / m["wall_a"],
+            **rates,
             "kernel_ms": {"forward": m["fwd_ms"], "reverse": m["adj_ms"],
                           "recompute": m["rec_ms"]},
+            "stored_trajectory_variant": stored,
             "fp64_batched_cell_steps_per_s":
                 cell_steps / (d["wall_f"] + d["wall_a"]) if d is not None else None,
             "fp64_plain_cell_steps_per_s": plain,
@@ -450,12 +565,12 @@ def bench_ensemble(args):
             "timed_run_launches": rl,
             "kernels_event_pair_per_launch": kin,
             "graph_replay": bool(graph),
-            "stored_trajectory_bytes": store,
+            "stored_trajectory_bytes": store_head,
             "tile_variants": {"coef_update_adjoint": m["tiles"],
                               "shapes": [tile_shape(v) for v in m["tiles"]]},
             "adaptive_leaves_fp64": adaptive,
         },
-        "roofline": roofline_json(rl, dom, kin, prec, workload),
+        "roofline": roofline_json(rl, dom, kin, prec, workload, run_config=run_config),
         "cpu_baseline": cpu,
     }
 
@@ -497,7 +612,7 @@ def adaptive_leg(K=200, k=25, graph=True, cpu_steps=0):
         ab = {n: v + LEAF_TABLE_BYTES for n, v in algo_bytes(_lib.TG_FP64).items()}
         sb = {n: v + LEAF_SHORT_EXTRA[n] for n, v in short_bytes(_lib.TG_FP64).items()}
         _, kin, _ = instrumented_roofline(sim, _lib.TG_FP64, pb.n, K, w, h0, ab=ab, sb=sb)
-    dom, rl = replay_roofline(r, kin, K)
+    dom, rl = replay_roofline(r, kin, K, k)
     cs = pb.n * K
     out = {
         "leaves": pb.n, "levels": pb.meta["levels"], "steps": K, "checkpoint_every": k,
@@ -609,7 +724,7 @@ def bench_grid(args):
                                        np.linalg.norm(d["lam"]))
         log(f"errors vs fp64: {errs}")
     _, kin, _ = roof
-    dom, rl = replay_roofline(m, kin, K)
+    dom, rl = replay_roofline(m, kin, K, k)
     return {
         "metric": METRIC, "value": cell_steps / wall, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": wall * 1e3 / K, "higher_is_better": True,
@@ -647,7 +762,8 @@ def sweeps(args):
     h0 = [np.concatenate([getattr(p, f) for p in E]) for f in ("h", "hu", "hv")]
     prec = prec_of(args.precision)
     with Ensemble(E, precision=prec, checkpoint_every=k, max_steps=max(K, 1),
-                  use_graph=0 if args.no_graph else 1, store_bytes=store_budget(args)) as ens:
+                  use_graph=0 if args.no_graph else 1,
+                  store_bytes=store_budget(args) if args.store else 0) as ens:
         ens.select_tiles([int(v) for v in (args.tiles or "3,3,3").split(",")])
         ens.set_functional([p.box_weights().reshape(3, -1) for p in E])
         for r in range(args.sweeps):
@@ -736,13 +852,16 @@ def trace_summary(trace_csv, out_path, command):
 
 
 def probe(args):
-    """Run for hardware-counter profiles (rocprofv3 --pmc): the ensemble's
-    K-step forward run and reverse sweep (graph replay, so the counter
-    collection sees few dispatches of the sweep itself), then `kernel_reps`
-    instrumented launches of each kernel on the final state -- the launches
-    the counters are read from; their full-path share and algorithmic bytes
-    per launch are logged -- then a 1 GiB device copy whose counters
-    calibrate FETCH_SIZE / WRITE_SIZE against a known byte count."""
+    """Run for hardware-counter profiles (rocprofv3 --pmc): the measured
+    ensemble run -- same members, K, checkpoints, stored trajectory and
+    (--tiles) tile variants as the bench's timed run -- once, launched kernel
+    by kernel with the kernels counting their full-path and short-path
+    cells; per role (forward_step: coefficients + dt + update, forward and
+    recompute launches; adjoint: the reverse steps) it logs the launches and
+    the algorithmic bytes per launch of exactly these launches.  Then a 1 GiB
+    device copy whose counters calibrate FETCH_SIZE / WRITE_SIZE against a
+    known byte count."""
+    import numpy as np
     import torch
 
     from discrete_adjoint_amd import problems
@@ -751,27 +870,31 @@ def probe(args):
     K, k = args.steps, args.checkpoint_every
     E = problems.ensemble_problems(args.members, seed=args.seed, nsteps=K, checkpoint_every=k)
     weights = [p.box_weights().reshape(3, -1) for p in E]
+    h0 = tuple(np.concatenate([getattr(p, f) for p in E]) for f in ("h", "hu", "hv"))
     prec = prec_of(args.precision)
     tiles = [int(v) for v in args.tiles.split(",")] if args.tiles else None
+    store = store_budget(args) if args.store else 0
     ab, sb = algo_bytes(prec), short_bytes(prec)
-    with Ensemble(E, precision=prec, checkpoint_every=k, max_steps=max(K, 1), use_graph=1) as ens:
-        ens.run(K)
-        ens.dual_solver(weights, want_lambda=False)
-        if tiles:  # reproduce a tile selection (e.g. the bench's autotuned one)
+    with Ensemble(E, precision=prec, checkpoint_every=k, max_steps=max(K, 1), use_graph=0,
+                  store_bytes=store) as ens:
+        if tiles:  # reproduce a tile selection (the bench's autotuned one)
             ens.select_tiles(tiles)
-        for i, name in enumerate(KERNELS):
-            ens.time_kernel(i, 1)  # first launch of this variant, uncounted
-            ens.instrument(True)
-            ens.time_kernel(i, args.kernel_reps)
-            ms, launches, full, short = ens.instrumentation()
-            ens.instrument(False)
+        ens.set_functional(weights)
+        ens.set_state(*h0)
+        ens.instrument(True)
+        ens.run(K)
+        ens.dual_solver(None, want_dphi=True, want_lambda=False)
+        ms, launches, full, short = ens.instrumentation()
+        ens.instrument(False)
+        cfg = ensemble_run_config(args, store, ens.tile_selection())
+        log("probe config " + json.dumps(cfg))
+        for role, i, name in (("forward_step", 1, "update"), ("adjoint", 2, "adjoint")):
             cells = launches[i] * ens.ncell
             fc, sc = path_cells(cells, full[i], short[i])
             per = (fc * ab[name] + sc * sb[name]) / max(launches[i], 1)
-            log(f"probe {name}: {ms[i] / max(launches[i], 1):.4f} ms/launch over {launches[i]} "
+            log(f"probe {role}: {ms[i] / max(launches[i], 1):.4f} ms/launch over {launches[i]} "
                 f"launches, full path {fc / max(cells, 1):.4f}, short path "
                 f"{sc / max(cells, 1):.4f}, algorithmic bytes per launch {per:.0f}")
-        log(f"probe cells {ens.ncell} tiles {ens.tile_selection()}")
     n = 1 << 28  # 2^28 FP32 = 1 GiB read + 1 GiB written
     x = torch.ones(n, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
@@ -782,11 +905,13 @@ def probe(args):
     assert float(y[-1]) == 1.0
 
 
-def traffic_summary(fetch_csv, write_csv, probe_log, kernel, workload, out_path, commands):
-    """HBM bytes per launch of `kernel` from two rocprofv3 --pmc runs of the
-    probe (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), each scaled by what
-    the counter reads for the probe's 1 GiB copy; written to out_path with
-    the algorithmic bytes the probe logged for the same launches."""
+def traffic_summary(fetch_csv, write_csv, probe_log, kernel, out_path, commands):
+    """HBM bytes per launch of one role from two rocprofv3 --pmc runs of the
+    probe (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) summed over every
+    dispatch of the role's GPU kernels in the probe's run and divided by the
+    role's launches, each counter scaled by what it reads for the probe's
+    1 GiB copy; written to out_path with the probe's run configuration and
+    the algorithmic bytes per launch of the same launches."""
     import csv
 
     role, symbols = kernel.split(":")
@@ -809,29 +934,27 @@ def traffic_summary(fetch_csv, write_csv, probe_log, kernel, workload, out_path,
 
     fk, fc = per_dispatch(fetch_csv, "FETCH_SIZE")
     wk, wc = per_dispatch(write_csv, "WRITE_SIZE")
-    n_reps = None
-    algo = None
+    launches = algo = cfg = None
     for line in open(probe_log):
+        if line.startswith("probe config "):
+            cfg = json.loads(line[len("probe config "):])
         if line.startswith(f"probe {role}:"):
             parts = line.split()
-            n_reps = int(parts[parts.index("over") + 1])
+            launches = int(parts[parts.index("over") + 1])
             algo = float(parts[-1])
-    # the instrumented launches on the final state: the last n_reps launches,
-    # each one dispatch of every symbol
-    nd = n_reps * len(symbols)
-    fk, wk = fk[-nd:], wk[-nd:]
     copy_bytes = 2.0 ** 30
     fscale = copy_bytes / (max(fc) * 1024.0)
     wscale = copy_bytes / (max(wc) * 1024.0)
-    f_kib = sum(v for _, v in fk) / n_reps
-    w_kib = sum(v for _, v in wk) / n_reps
+    f_kib = sum(v for _, v in fk) / launches
+    w_kib = sum(v for _, v in wk) / launches
+    hbm = f_kib * 1024.0 * fscale + w_kib * 1024.0 * wscale
     out = {"kernel": role, "dispatched_kernels": sorted({nm for nm, _ in fk}),
            "fetch_kib_per_launch": f_kib, "write_kib_per_launch": w_kib,
            "fetch_scale": fscale, "write_scale": wscale,
-           "hbm_bytes_per_launch": f_kib * 1024.0 * fscale + w_kib * 1024.0 * wscale,
-           "algorithmic_bytes_per_launch": algo, "launches": n_reps, "dispatches": len(fk),
-           "state": "final state of the ensemble's 500-step run and reverse sweep",
-           "command_fetch": commands[0], "command_write": commands[1], "workload": workload}
+           "hbm_bytes_per_launch": hbm, "algorithmic_bytes_per_launch": algo,
+           "traffic_over_algorithmic": hbm / algo if algo else None,
+           "launches": launches, "dispatches": len(fk), "config": cfg,
+           "command_fetch": commands[0], "command_write": commands[1]}
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
@@ -850,12 +973,15 @@ def main():
     ap.add_argument("--compare", default="adj32,mixed,fp32",
                     help="further precisions run on the same ensemble (errors and speed)")
     ap.add_argument("--checkpoint-every", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="forward (+ adjoint) steps per member of the CPU baseline's sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--no-store", action="store_true",
-                    help="checkpoints only: recompute every reverse segment")
+    ap.add_argument("--store", action="store_true",
+                    help="measure with the last forward states kept whole in HBM (the "
+                         "reverse sweep recomputes only below them) instead of checkpoints "
+                         "only")
     ap.add_argument("--store-fraction", type=float, default=0.85,
                     help="share of free HBM for the stored trajectory")
     ap.add_argument("--product-only", action="store_true",
@@ -888,12 +1014,12 @@ def main():
         sys.exit(2)
     if args.traffic_summary:
         fetch_csv, write_csv, probe_log, kernel, out_path = args.traffic_summary.split(",")
-        traffic_summary(fetch_csv, write_csv, probe_log, kernel,
-                        f"ensemble of {args.members} members, precision {args.precision}",
-                        out_path,
+        probe_args = (f"--probe --steps {args.steps} --warmup {args.warmup} --tiles {args.tiles}"
+                      + (" --store" if args.store else ""))
+        traffic_summary(fetch_csv, write_csv, probe_log, kernel, out_path,
                         [f"cd /tmp && TMPDIR=/tmp timeout -s KILL 300 rocprofv3 --pmc {c} -d "
                          f"/tmp/pmc_{d} -o probe --output-format csv -- python3 "
-                         f"$REPO/bench.py --probe --tiles {args.tiles}"
+                         f"$REPO/bench.py {probe_args}"
                          for c, d in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write"))])
         return
     if args.counter_summary:

@@ -114,10 +114,15 @@ struct tg_ctx {
   int traj_s0 = 0;
   std::vector<double> member_dphi;
 
-  // captured sweeps (cfg.use_graph): one forward and one reverse graph,
-  // re-captured when their key changes
-  SweepGraph gf, ga, gb;  // forward; reverse stored part; reverse segments
+  // captured sweeps (cfg.use_graph), re-captured when their key changes:
+  // the forward sweep, the reverse sweep's stored part, and per checkpoint
+  // segment its recompute and its adjoint steps (gr[sg], gs[sg]; launched
+  // one after the other with events between them, so the recompute and the
+  // adjoint steps are timed apart inside a replayed sweep)
+  SweepGraph gf, ga;
+  std::vector<SweepGraph> gr, gs;
   bool gf_end_is_b = false;  // the captured forward sweep leaves U^N in Ut
+  std::vector<hipEvent_t> seg_ev;  // two per segment: around its recompute
 
   // timing
   hipEvent_t ev[7] = {};
@@ -176,10 +181,17 @@ void drop_graph(SweepGraph& G) {
   G = SweepGraph{};
 }
 
+void drop_segment_graphs(tg_ctx* ctx) {
+  for (auto& G : ctx->gr) drop_graph(G);
+  for (auto& G : ctx->gs) drop_graph(G);
+  ctx->gr.clear();
+  ctx->gs.clear();
+}
+
 void free_fields(tg_ctx* ctx) {
   drop_graph(ctx->gf);
   drop_graph(ctx->ga);
-  drop_graph(ctx->gb);
+  drop_segment_graphs(ctx);
   dfree(ctx->U[0]);  // one allocation per buffer, h first
   dfree(ctx->Ut[0]);
   dfree(ctx->lam[0][0]);
@@ -608,7 +620,7 @@ int ensure_traj(tg_ctx* ctx, size_t count) {
   if (count <= ctx->traj_cap) return TG_OK;
   drop_graph(ctx->gf);
   drop_graph(ctx->ga);
-  drop_graph(ctx->gb);
+  drop_segment_graphs(ctx);
   dfree(ctx->traj_block);
   ctx->traj_cap = 0;
   int rc = dalloc(ctx, &ctx->traj_block, ctx->state_bytes * count);
@@ -710,7 +722,11 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
   if (rc) return rc;
   const int k = ctx->cfg.checkpoint_every;
   int M = stored_steps(ctx, nsteps);
-  if (ensure_traj(ctx, (size_t)M) != TG_OK) M = 0;  // no room: checkpoints only
+  if (ensure_traj(ctx, (size_t)M) != TG_OK) {  // no room: checkpoints only
+    M = 0;
+    (void)hipGetLastError();  // the handled allocation failure is not the run's error
+    ctx->err.clear();
+  }
   const int s0 = nsteps - M;
   rc = ensure_store(ctx, ctx->ckpt, (size_t)(s0 + k - 1) / k + 1);
   if (rc) return rc;
@@ -914,9 +930,18 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
   };
   int cur = 0;
   float rec_ms = 0, ms = 0;
-  // the stored part (adjoint launches only) and the segments are timed
-  // apart: ev[2] .. ev[6] .. ev[3]
-  if (ctx->cfg.use_graph && !ctx->instrument) {
+  // timing: ev[2] .. stored part .. ev[6] .. segments .. ev[3]; the i-th
+  // visited segment's recompute lies between seg_ev[2i] and seg_ev[2i + 1]
+  int nvisit = 0;
+  for (int sg = nseg - 1; sg >= 0; sg--)
+    if (sg * k < S) nvisit++;
+  while ((int)ctx->seg_ev.size() < 2 * nvisit) {
+    hipEvent_t e = nullptr;
+    HIPCHK(ctx, hipEventCreate(&e));
+    ctx->seg_ev.push_back(e);
+  }
+  const bool graph = ctx->cfg.use_graph && !ctx->instrument;
+  if (graph) {
     long long stage = 0;  // what the captured launches depend on beyond N
     if (S < N || staged) {
       std::vector<long long> v(st.begin(), st.end());
@@ -926,7 +951,7 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
     }
     const long long key[8] = {N, k, dphi, ctx->sel[0], ctx->sel[1], ctx->sel[2], s0, stage};
     if (!same_key(ctx->ga, key)) {
-      drop_graph(ctx->gb);
+      drop_segment_graphs(ctx);
       rc = capture(ctx, ctx->ga, key, [&]() {
         int c = 0;
         if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
@@ -934,46 +959,46 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
         stored(c);
       });
       if (rc) return rc;
-      if (need_gb) {
-        rc = capture(ctx, ctx->gb, key, [&]() {
-          int c = nstored % 2;
-          for (int sg = nseg - 1; sg >= 0; sg--) {
-            if (sg * k >= S) continue;
-            recompute(sg);
-            reverse(sg, c);
-          }
-        });
+      ctx->gr.assign(nseg, SweepGraph{});
+      ctx->gs.assign(nseg, SweepGraph{});
+      int c = nstored % 2;
+      for (int sg = nseg - 1; sg >= 0 && need_gb; sg--) {
+        if (sg * k >= S) continue;
+        rc = capture(ctx, ctx->gr[sg], key, [&]() { recompute(sg); });
+        if (rc) return rc;
+        rc = capture(ctx, ctx->gs[sg], key, [&]() { reverse(sg, c); });
         if (rc) return rc;
       }
     }
-    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    HIPCHK(ctx, hipGraphLaunch(ctx->ga.x, ctx->stream));
-    HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
-    if (need_gb) HIPCHK(ctx, hipGraphLaunch(ctx->gb.x, ctx->stream));
-    cur = S % 2;
-    rec_ms = -1.f;  // not separable inside one launch
-  } else {
-    if (dphi) launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
-    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    stored(cur);
-    HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
-    for (int sg = nseg - 1; sg >= 0; sg--) {
-      if (sg * k >= S) continue;
-      HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-      recompute(sg);
-      HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-      reverse(sg, cur);
-      HIPCHK(ctx, hipEventSynchronize(ctx->ev[5]));
-      HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
-      rec_ms += ms;
-    }
+  } else if (dphi) {
+    launch_zero(ctx->dphi_part, sizeof(double) * ctx->ntiles, ctx->stream);
   }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  if (graph) HIPCHK(ctx, hipGraphLaunch(ctx->ga.x, ctx->stream));
+  else stored(cur);
+  HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+  int iv = 0;
+  for (int sg = nseg - 1; sg >= 0 && need_gb; sg--) {
+    if (sg * k >= S) continue;
+    HIPCHK(ctx, hipEventRecord(ctx->seg_ev[2 * iv], ctx->stream));
+    if (graph) HIPCHK(ctx, hipGraphLaunch(ctx->gr[sg].x, ctx->stream));
+    else recompute(sg);
+    HIPCHK(ctx, hipEventRecord(ctx->seg_ev[2 * iv + 1], ctx->stream));
+    if (graph) HIPCHK(ctx, hipGraphLaunch(ctx->gs[sg].x, ctx->stream));
+    else reverse(sg, cur);
+    iv++;
+  }
+  if (graph) cur = S % 2;
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   HIPCHK(ctx, hipGetLastError());
   rc = check_err(ctx);
   if (rc) return rc;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
   ctx->adj_ms = ms;
+  for (int i = 0; i < iv; i++) {
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->seg_ev[2 * i], ctx->seg_ev[2 * i + 1]));
+    rec_ms += ms;
+  }
   ctx->rec_ms = rec_ms;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[6]));
   ctx->stored_ms = ms;
@@ -1461,6 +1486,7 @@ void tg_destroy(tg_ctx* ctx) {
   dfree(ctx->err_d);
   dfree(ctx->active_d);
   for (hipEvent_t e : ctx->iev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->seg_ev) (void)hipEventDestroy(e);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -11,6 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+OMP_LIB_PATH = os.path.join(HERE, "liboracle_omp.so")
 _dp = C.POINTER(C.c_double)
 
 
@@ -32,15 +33,17 @@ def build():
     return LIB_PATH
 
 
-_LIB = None
+_LIBS = {}
 
 
-def load():
-    global _LIB
-    if _LIB is None:
-        if not os.path.exists(LIB_PATH):
+def load(omp=False):
+    """The checker (omp False) or its OpenMP build (the all-cores CPU
+    baseline: same per-cell results, FP64 sums in another order)."""
+    path = OMP_LIB_PATH if omp else LIB_PATH
+    if path not in _LIBS:
+        if not os.path.exists(path):
             build()
-        lib = C.CDLL(LIB_PATH)
+        lib = C.CDLL(path)
         lib.or_step.restype = C.c_int
         lib.or_step.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), C.c_double, _dp, _dp,
                                 _dp, C.c_double, _dp, _dp, _dp]
@@ -59,8 +62,8 @@ def load():
         lib.or_kact_slopes.restype = None
         lib.or_kact_slopes.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), _dp, _dp, _dp,
                                        _dp, _dp]
-        _LIB = lib
-    return _LIB
+        _LIBS[path] = lib
+    return _LIBS[path]
 
 
 def _p(a):
@@ -76,8 +79,8 @@ class Oracle:
     """FP64 CPU restatement bound to one uniform problem."""
 
     def __init__(self, nx, ny, dx, dy, x0, y0, grav, dgrav, curv, phi_int_deg, phi_bed_deg,
-                 epsilon=1.0, gscale=1.0, tiny=1e-4, gamma=1.0):
-        self.lib = load()
+                 epsilon=1.0, gscale=1.0, tiny=1e-4, gamma=1.0, omp=False):
+        self.lib = load(omp)
         self.n = nx * ny
         self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
                       (grav[0], grav[1], grav[2], dgrav[0], dgrav[1], curv[0], curv[1])]
@@ -90,9 +93,9 @@ class Oracle:
         self.time = 0.0
 
     @classmethod
-    def from_problem(cls, pb):
+    def from_problem(cls, pb, omp=False):
         return cls(pb.nx, pb.ny, pb.dx, pb.dy, pb.x0, pb.y0, pb.grav, pb.dgrav, pb.curv,
-                   pb.phi_int_deg, pb.phi_bed_deg, pb.epsilon, pb.gravity_scale)
+                   pb.phi_int_deg, pb.phi_bed_deg, pb.epsilon, pb.gravity_scale, omp=omp)
 
     def set_angles(self, phi_int_deg, phi_bed_deg):
         """MatProps + getcoef.f:39-43 constants, evaluated with libm."""

@@ -151,6 +151,7 @@ static double cfl_dt(const or_mesh* m, const or_params* p, const double* U[3],
 static void all_kact_fwd(const or_mesh* m, const or_params* p, const double* U[3],
                          double* kact, double* dU) {
   int n = m->nx * m->ny;
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < n; c++) {
     slopes_fwd(m, p->gamma, p->tiny, U, c, dU + 6 * c);
     if (U[0][c] > p->tiny)
@@ -206,25 +207,31 @@ int or_step(const or_mesh* m, const or_params* p, double frict_tiny, double* h,
   *time += dt;
   if (dt_out) *dt_out = dt;
 
-  /* calc_edge_states<Element> (edge_states.C:37-77, element2.C:2744-3072) */
+  /* calc_edge_states<Element> (edge_states.C:37-77, element2.C:2744-3072):
+   * every cell writes the fluxes of its + faces (a face has one writer) */
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < n; c++) {
     for (int side = 0; side < 2; side++) {
-      int zp = side, zm = side + 2;
-      int e = nbr_of(m, c, zp);
-      if (e == INIT_NB) {
-        if (side) ylist[nyl++] = c; else xlist[nxl++] = c;
-      } else {
+      int e = nbr_of(m, c, side);
+      if (e != INIT_NB) {
         double hl[3][3], hr[3][3];
         hfv_fwd(p, side, h[c], hu[c], hv[c], kact[c], m->gz[c], hl);
         hfv_fwd(p, side, h[e], hu[e], hv[e], kact[e], m->gz[e], hr);
         double* dst = side ? fy + NV * yface(m, c, 1) : fx + NV * xface(m, c, 1);
         riemann(hl, hr, dst);
       }
-      if (nbr_of(m, c, zm) == INIT_NB) {
+    }
+  }
+  /* the x / y boundary lists of edge_states.C:37-77, in traversal order */
+  for (int c = 0; c < n; c++)
+    for (int side = 0; side < 2; side++) {
+      if (nbr_of(m, c, side) == INIT_NB) {
+        if (side) ylist[nyl++] = c; else xlist[nxl++] = c;
+      }
+      if (nbr_of(m, c, side + 2) == INIT_NB) {
         if (side) ylist[nyl++] = c; else xlist[nxl++] = c;
       }
     }
-  }
   /* Element::boundary_flux, element2.C:3374-3411 */
   for (int s = 0; s < 2; s++) {
     int cnt = s ? nyl : nxl;
@@ -242,6 +249,7 @@ int or_step(const or_mesh* m, const or_params* p, double frict_tiny, double* h,
 
   /* corrector loop (step.C:182-232) -> correct() -> update_states */
   double dtdx = dt / m->dx, dtdy = dt / m->dy, mass = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : mass) reduction(min : rc)
   for (int c = 0; c < n; c++) {
     double prev[3] = {h[c], hu[c], hv[c]}, out[3];
     double g[3] = {m->gx[c], m->gy[c], m->gz[c]};
@@ -289,18 +297,18 @@ static dual_cell* build_dual(const or_mesh* m, const or_params* p, double ft,
   int* xlist = (int*)ws(WS_XL, sizeof(int) * 2 * n, 0);
   int* ylist = (int*)ws(WS_YL, sizeof(int) * 2 * n, 0);
   int nxl = 0, nyl = 0;
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < n; c++) {
     slopes_dual(m, p->gamma, p->tiny, U, c, dc[c].d, dc[c].hs);
     dc[c].k = getcoef(p, U[0][c], U[1][c], U[2][c], dc[c].d);
   }
-  /* calc_flux (edge_states.C:79-124) -> DualElem::calc_flux (dualmesh.C:1853-2239) */
+  /* calc_flux (edge_states.C:79-124) -> DualElem::calc_flux (dualmesh.C:1853-2239):
+   * the face between c and its + neighbour e is written by c alone */
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < n; c++) {
     for (int side = 0; side < 2; side++) {
-      int zp = side, zm = side + 2;
-      int e = nbr_of(m, c, zp);
-      if (e == INIT_NB) {
-        if (side) ylist[nyl++] = c; else xlist[nxl++] = c;
-      } else {
+      int e = nbr_of(m, c, side);
+      if (e != INIT_NB) {
         M3 hl, hr, fjl, fjr, sjl, sjr, jl, jr;
         dual_hfv(p, side, U[0][c], U[1][c], U[2][c], dc[c].k, m->gz[c], hl, fjl, sjl);
         dual_hfv(p, side, U[0][e], U[1][e], U[2][e], dc[e].k, m->gz[e], hr, fjr, sjr);
@@ -310,11 +318,17 @@ static dual_cell* build_dual(const or_mesh* m, const or_params* p, double ft,
         memcpy(dc[e].fjc[side][0][0], jr, sizeof(M3));
         memcpy(dc[e].fjc[side][0][1], jl, sizeof(M3));
       }
-      if (nbr_of(m, c, zm) == INIT_NB) {
+    }
+  }
+  for (int c = 0; c < n; c++)
+    for (int side = 0; side < 2; side++) {
+      if (nbr_of(m, c, side) == INIT_NB) {
+        if (side) ylist[nyl++] = c; else xlist[nxl++] = c;
+      }
+      if (nbr_of(m, c, side + 2) == INIT_NB) {
         if (side) ylist[nyl++] = c; else xlist[nxl++] = c;
       }
     }
-  }
   /* DualElem::boundary_flux, dualmesh.C:2241-2279 */
   for (int s = 0; s < 2; s++) {
     int cnt = s ? nyl : nxl;
@@ -329,6 +343,7 @@ static dual_cell* build_dual(const or_mesh* m, const or_params* p, double ft,
   }
   /* calc_jacobian */
   double dtdx = dt / m->dx, dtdy = dt / m->dy;
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < n; c++) {
     double prev[3] = {U[0][c], U[1][c], U[2][c]}, out[3];
     double g[3] = {m->gx[c], m->gy[c], m->gz[c]};
@@ -432,13 +447,15 @@ void or_dstep_dphi(const or_mesh* m, const or_params* p, double ft, const double
   double* d = (double*)ws(WS_D, sizeof(double) * 6 * n, 0);
   double* k = (double*)ws(WS_K, sizeof(double) * n, 0);
   double* dk = (double*)ws(WS_DK, sizeof(double) * n, 0);
-  double hs[2][5];
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < n; c++) {
+    double hs[2][5];
     slopes_dual(m, p->gamma, p->tiny, U, c, d + 6 * c, hs);
     k[c] = getcoef(p, h[c], hu[c], hv[c], d + 6 * c);
     dk[c] = dgetcoef_dphi(p, h[c], hu[c], hv[c], d + 6 * c);
   }
   double dtdx = dt / m->dx, dtdy = dt / m->dy;
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < n; c++) {
     double flux[4][3], dflux[4][3]; /* E, N, W, S faces */
     for (int f = 0; f < 4; f++) {
@@ -504,6 +521,7 @@ void or_adjoint_step(const or_mesh* m, const or_params* p, double frict_tiny,
   int n = m->nx * m->ny;
   const double* U[3] = {h, hu, hv};
   dual_cell* dc = build_dual(m, p, frict_tiny, U, dt);
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < n; c++) {
     double adj[3] = {0., 0., 0.};
     /* effelement 1..4 visit neighbor[0..3]; generated grids have
@@ -533,6 +551,7 @@ void or_adjoint_step(const or_mesh* m, const or_params* p, double frict_tiny,
     double* dU = (double*)ws(WS_DPHI, sizeof(double) * 3 * n, 0);
     or_dstep_dphi(m, p, frict_tiny, h, hu, hv, dt, dU);
     double acc = 0.;
+#pragma omp parallel for schedule(static) reduction(+ : acc)
     for (int c = 0; c < n; c++)
       for (int v = 0; v < 3; v++) acc += lam_next[v * n + c] * dU[v * n + c];
     *dJdphi += acc;

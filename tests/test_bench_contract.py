@@ -36,32 +36,65 @@ def _kin(full, short, bpl):
 def test_replay_roofline_uses_the_timed_run():
     kin = {"coef": _kin(0.03, 0.06, 1e8), "update": _kin(0.06, 0.13, 3e8),
            "adjoint": _kin(0.064, 0.041, 2.6e8)}
-    r = {"stored_ms": 36.3, "stored_steps": 100, "fwd_ms": 114.0}
-    dom, rl = bench.replay_roofline(r, kin, 500)
-    assert rl["adjoint"]["avg_launch_ms"] == pytest.approx(0.363)
-    assert rl["adjoint"]["gbs"] == pytest.approx(2.6e8 / 0.363e-3 / 1e9)
+    # checkpoints only, K = 500, k = 25: the reverse sweep recomputes 24
+    # steps of each of its 20 segments; the events time the recompute apart
+    r = {"fwd_ms": 114.0, "adj_ms": 290.0, "rec_ms": 108.5, "stored_steps": 0}
+    dom, rl = bench.replay_roofline(r, kin, 500, 25)
+    assert rl["adjoint"]["avg_launch_ms"] == pytest.approx((290.0 - 108.5) / 500)
+    assert rl["adjoint"]["gbs"] == pytest.approx(2.6e8 / (0.363e-3) / 1e9)
     assert rl["forward_step"]["avg_launch_ms"] == pytest.approx(0.228)
     assert rl["forward_step"]["bytes_per_launch"] == 3e8
-    assert dom == "adjoint"  # 0.363 ms x 500 > 114 ms
-    # no stored part: only the forward step can be timed from the run
-    dom, rl = bench.replay_roofline({"stored_ms": 0., "stored_steps": 0, "fwd_ms": 114.0},
-                                    kin, 500)
-    assert dom == "forward_step" and "adjoint" not in rl
+    assert rl["recompute_step"]["launches_timed"] == 480
+    assert rl["recompute_step"]["avg_launch_ms"] == pytest.approx(108.5 / 480)
+    assert dom == "adjoint"  # 0.363 ms > 0.228 ms
+    # the whole trajectory kept: nothing recomputed
+    r = {"fwd_ms": 114.0, "adj_ms": 100.0, "rec_ms": 0.0, "stored_steps": 500}
+    dom, rl = bench.replay_roofline(r, kin, 500, 25)
+    assert dom == "forward_step" and "recompute_step" not in rl
 
 
-def test_roofline_json_keys_and_fraction():
+def test_recomputed_steps():
+    assert bench.recomputed_steps(20, 25, 0) == 19
+    assert bench.recomputed_steps(500, 25, 0) == 480
+    assert bench.recomputed_steps(500, 25, 441) == (59 - 3)   # segments of 25, 25, 9
+    assert bench.recomputed_steps(500, 25, 500) == 0
+
+
+def test_path_rates():
+    kin = {"update": _kin(0.06, 0.14, 0), "adjoint": _kin(0.04, 0.06, 0)}
+    r = bench.path_rates(kin, 10, 2.0, 1000)
+    assert r["full_path_share"] == pytest.approx(0.05)
+    assert r["full_path_cell_steps_per_s"] == pytest.approx(0.05 * 10000 / 2.0)
+    assert r["visited_cell_steps_per_s"] == pytest.approx(0.15 * 10000 / 2.0)
+
+
+def test_roofline_json_keys_and_fraction(tmp_path, monkeypatch):
     from discrete_adjoint_amd import _lib
     kin = {"coef": _kin(0.03, 0.06, 1e8), "update": _kin(0.06, 0.13, 3e8),
            "adjoint": _kin(0.064, 0.041, 2.6e8)}
-    r = {"stored_ms": 34.5, "stored_steps": 100, "fwd_ms": 114.0}
-    dom, rl = bench.replay_roofline(r, kin, 500)
-    j = bench.roofline_json(rl, dom, kin, _lib.TG_FP64, "no_such_workload")
+    r = {"fwd_ms": 114.0, "adj_ms": 290.0, "rec_ms": 108.5, "stored_steps": 0}
+    dom, rl = bench.replay_roofline(r, kin, 500, 25)
+    j = bench.roofline_json(rl, dom, kin, _lib.TG_FP64, "no_such_workload",
+                            run_config={"steps": 500})
     for key in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
         assert key in j
     assert j["bound"] == "hbm" and j["unit"] == "GB/s" and j["peak"] == 8000.0
     assert j["frac"] == pytest.approx(j["achieved"] / 8000.0)
     assert j["traffic"] is None  # no committed summary for this workload
     assert j["algorithmic_bytes_per_cell"] == bench.algo_bytes(_lib.TG_FP64)["adjoint"]
+    # a committed summary is used only for the run configuration it was taken on
+    import json
+    prof = tmp_path / "profiles"
+    prof.mkdir()
+    (prof / "traffic_adjoint_wl.json").write_text(json.dumps(
+        {"hbm_bytes_per_launch": 3.9e8, "algorithmic_bytes_per_launch": 2.6e8,
+         "config": {"steps": 500}}))
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    j = bench.roofline_json(rl, dom, kin, _lib.TG_FP64, "wl", run_config={"steps": 500})
+    assert j["traffic"] == 3.9e8
+    assert j["traffic_over_algorithmic"] == pytest.approx(1.5)
+    j = bench.roofline_json(rl, dom, kin, _lib.TG_FP64, "wl", run_config={"steps": 20})
+    assert j["traffic"] is None and "another configuration" in j["traffic_info"]["reason"]
 
 
 def test_trace_summary_splits_the_last_sweep(tmp_path):

@@ -103,3 +103,35 @@ def test_ensemble_states(oracle_lib):
                 got = ens.trajectory_state(s)
                 for a, r in zip(got, ref[s]):
                     assert np.array_equal(a[o:o + n], r), (b, s)
+
+
+def test_store_that_does_not_fit_falls_back_to_checkpoints():
+    # a stored trajectory larger than the device (3000 states of a 2048^2
+    # block, ~300 GB) cannot be allocated: the run keeps checkpoints only,
+    # the failed allocation is not reported as the run's error, and results
+    # equal a run that asked for no store
+    from helpers import analytic_problem, paraboloid
+    n, N = 2048, 3000
+    pb = analytic_problem(n, n, 5.0, slope_x_deg=20.0, h=paraboloid(2000.0, 5120.0, 60.0, 50.0,
+                                                                   6.0), nsteps=N)
+    w = np.zeros((3, pb.n))
+    w[0, :] = 1.0
+    out = []
+    for store in (1 << 50, 0):
+        _say(f"case store fallback 2048^2 {N} steps store_bytes={store}")
+        with Simulation(pb, checkpoint_every=100, max_steps=N, store_bytes=store,
+                        use_graph=0) as sim:
+            sim.run(N)
+            U = sim.get_state()
+            lam, _ = sim.dual_solver(w)
+            _, stored = sim.stored_sweep_timing()
+            out.append((U, lam, stored))
+            # the context stays usable
+            sim.set_state(pb.h, pb.hu, pb.hv)
+            sim.run(10)
+    (Ua, la, sa), (Ub, lb, sb) = out
+    assert sa == 0 and sb == 0
+    for a, b in zip(Ua, Ub):
+        assert np.array_equal(a, b)
+    assert np.array_equal(la, lb)
+    assert Ua[0].max() > 0

@@ -295,3 +295,32 @@ def test_planted_sign_error_is_caught(oracle_lib):
     fd = (vals[0] - vals[1]) / (2 * e)
     assert lam0[0, c] == pytest.approx(fd, rel=1e-5, abs=1e-8)
     assert not (-lam0[0, c] == pytest.approx(fd, rel=1e-5, abs=1e-8))
+
+
+def test_openmp_build_matches_the_checker(oracle_lib):
+    # the all-cores CPU baseline runs the OpenMP build of the same source:
+    # per-cell results bit-identical, sums within rounding
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import numpy as np, oracle\n"
+        "from helpers import small_flow_problem\n"
+        "pb = small_flow_problem(40, 30, 30)\n"
+        "w = np.zeros((3, pb.n)); w[0, pb.n // 2:] = 1.0\n"
+        "out = []\n"
+        "for omp in (False, True):\n"
+        "    orc = oracle.Oracle.from_problem(pb, omp=omp)\n"
+        "    UN, dts, lam, dphi = orc.forward_adjoint([pb.h, pb.hu, pb.hv], 30, w, want_dphi=True)\n"
+        "    out.append((UN, dts, lam, dphi))\n"
+        "(a, b) = out\n"
+        "assert all(np.array_equal(x, y) for x, y in zip(a[0], b[0]))\n"
+        "assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])\n"
+        "assert abs(a[3]) > 1e-3 and abs(a[3] - b[3]) <= 1e-12 * abs(a[3])\n"
+        "print('ok')\n")
+    here = os.path.dirname(os.path.abspath(__file__))
+    env = dict(os.environ, OMP_NUM_THREADS="4",
+               PYTHONPATH=os.pathsep.join([os.path.dirname(here), here]))
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stderr
