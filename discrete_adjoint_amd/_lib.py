@@ -74,7 +74,15 @@ class tg_mesh(C.Structure):
         ("gen", _i32p),
         ("cdx", _dp),
         ("xc", _dp),
+        ("mat", C.POINTER(C.c_uint8)),
     ]
+
+
+TG_MAX_MATERIALS = 16
+
+
+class tg_material(C.Structure):
+    _fields_ = [("tan_bed", C.c_double), ("kact_root", C.c_double), ("dkroot_dphi", C.c_double)]
 
 
 class tg_params(C.Structure):
@@ -137,6 +145,8 @@ SIGNATURES = {
     "tg_config_default": (None, [C.POINTER(tg_config)]),
     "tg_params_from_angles": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.POINTER(tg_params)]),
+    "tg_material_from_angles": (C.c_int, [C.c_double, C.c_double, C.POINTER(tg_material)]),
+    "tg_set_materials": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(tg_material)]),
     "tg_create": (C.c_int, [C.POINTER(tg_config), C.POINTER(C.c_void_p)]),
     "tg_destroy": (None, [C.c_void_p]),
     "tg_last_error": (C.c_char_p, [C.c_void_p]),

@@ -103,6 +103,15 @@ struct tg_ctx {
   double* leaf_xc = nullptr;
   LeafBufs lb{};
 
+  // per-cell materials (tg_mesh.mat, tg_set_materials): the cells' material
+  // index (null: one material per member) and the material table, MAX_MAT
+  // entries per member; entries at or above mat_count[b] follow member b's
+  // own parameters
+  uint8_t* mat_d = nullptr;
+  MatDev* mtab_d = nullptr;
+  std::vector<MatDev> mtab_h;
+  std::vector<int> mat_count;
+
   // forward record
   int rec_steps = 0;  // steps in the dt record
   int run_steps = 0;  // steps of the last forward run/replay (0 = none)
@@ -210,6 +219,8 @@ void free_fields(tg_ctx* ctx) {
   dfree(ctx->statics_f);
   dfree(ctx->kact);
   dfree(ctx->part_mass);
+  dfree(ctx->mat_d);
+  dfree(ctx->mtab_d);
   dfree(ctx->part_hmax);
   dfree(ctx->dphi_part);
   dfree(ctx->thin_list);
@@ -265,6 +276,16 @@ int upload_params(tg_ctx* ctx, const tg_params* p) {
     for (auto& I : ctx->inst_h) params_into(I, p);
   HIPCHK(ctx, hipMemcpyAsync(ctx->inst_d, ctx->inst_h.data(), sizeof(InstDev) * ctx->ninst,
                              hipMemcpyHostToDevice, ctx->stream));
+  if (ctx->mat_d) {  // materials no tg_set_materials gave follow the member's parameters
+    for (int b = 0; b < ctx->ninst; b++) {
+      const InstDev& I = ctx->inst_h[b];
+      for (int m = ctx->mat_count[b]; m < MAX_MAT; m++)
+        ctx->mtab_h[(size_t)b * MAX_MAT + m] = MatDev{I.tan_bed, I.kroot, I.dkroot, 0.};
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->mtab_d, ctx->mtab_h.data(),
+                               sizeof(MatDev) * ctx->mtab_h.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+  }
   return TG_OK;
 }
 
@@ -287,6 +308,8 @@ Statics32 statics32(tg_ctx* ctx) {
 // points a kernel's arguments at the tile table of its selected variant
 template <class A>
 void use_tiles(const tg_ctx* ctx, int kernel, A& a) {
+  a.mat = ctx->mat_d;
+  a.mtab = ctx->mtab_d;
   const int v = ctx->sel[kernel];
   a.tiles = ctx->tiles_d[v];
   a.ntiles = ctx->nt[v];
@@ -445,6 +468,8 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
     a.mk_in = marks_of(ctx, in);
     a.mk_out = marks_of(ctx, out);
     if (ctx->leaves) {
+      a.mat = ctx->mat_d;
+      a.mtab = ctx->mtab_d;
       a.active = inst_counter(ctx, 0, true);
       a.short_cells = inst_short(ctx, 0, true);
       inst_mark(ctx, 0, true);
@@ -1144,6 +1169,13 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
       if ((m->y0 + (j + 1) * m->dy) - (m->y0 + j * m->dy) != m->dy)
         return set_err(ctx, TG_ERR_ARG, "y centre spacing is not exactly dy");
   }
+  for (int b = 0; b < nm; b++)
+    if (meshes[b].mat)
+      for (long long c = 0; c < cells_of(&meshes[b]); c++)
+        if (meshes[b].mat[c] >= MAX_MAT)
+          return set_err(ctx, TG_ERR_ARG, "material index of member " + std::to_string(b) +
+                                              " cell " + std::to_string(c) + " >= " +
+                                              std::to_string(MAX_MAT));
   HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   free_fields(ctx);
@@ -1158,6 +1190,7 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     InstDev& I = ctx->inst_h[0];
     I.nx = meshes[0].n; I.ny = 1; I.off = 0;
     I.maxtime = I.nextoutput = I.nextsave = HUGE_VAL;
+    I.mat_base = 0;
     if (params) params_into(I, &params[0]);
     for (int v = 0; v < NSHAPE; v++)
       for (int x = 0; x < meshes[0].n; x += BLOCK)
@@ -1198,6 +1231,7 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     I.inv_dx = 1 / m->dx; I.inv_sx = 1 / (m->dx + m->dx);
     I.inv_dy = 1 / m->dy; I.inv_sy = 1 / (m->dy + m->dy);
     I.maxtime = I.nextoutput = I.nextsave = HUGE_VAL;
+    I.mat_base = b * MAX_MAT;
     if (params) params_into(I, &params[b]);
     for (int v = 0; v < NSHAPE; v++) {
       const int TX = SHAPE_TX[v], TY = BLOCK / SHAPE_TX[v];
@@ -1285,6 +1319,10 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   for (int v = 0; v < NSHAPE && !rc; v++)
     if (ctx->nb[v] > 0)
       rc = dalloc(ctx, (void**)&ctx->blocks_d[v], sizeof(BlockDev) * ctx->nb[v]);
+  bool any_mat = false;
+  for (int b = 0; b < nm; b++) any_mat |= meshes[b].mat != nullptr;
+  if (!rc && any_mat) rc = dalloc(ctx, (void**)&ctx->mat_d, (size_t)n);
+  if (!rc && any_mat) rc = dalloc(ctx, (void**)&ctx->mtab_d, sizeof(MatDev) * MAX_MAT * nm);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->red, sizeof(RedDev) * nm);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->time_d, sizeof(double) * nm);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->dt_rec, sizeof(double) * nm * ctx->cfg.max_steps);
@@ -1318,6 +1356,15 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     }
   }
   if (rc) return rc;
+  ctx->mat_count.assign(nm, 0);
+  ctx->mtab_h.assign(any_mat ? (size_t)MAX_MAT * nm : 0, MatDev{});
+  if (any_mat) {  // member b's cell materials at [off_b, off_b + cells); none given: 0
+    std::vector<uint8_t> mat((size_t)n, 0);
+    for (int b = 0; b < nm; b++)
+      if (meshes[b].mat)
+        std::memcpy(mat.data() + ctx->inst_h[b].off, meshes[b].mat, (size_t)cells_of(&meshes[b]));
+    HIPCHK(ctx, hipMemcpy(ctx->mat_d, mat.data(), (size_t)n, hipMemcpyHostToDevice));
+  }
   if (leaves) {
     // SoA copies of the leaf geometry: x at [c], y at [n + c]
     const tg_mesh* m = &meshes[0];
@@ -1415,6 +1462,32 @@ int tg_params_from_angles(double phi_int, double phi_bed, double epsilon, double
   out->nextoutput = HUGE_VAL;
   out->nextsave = HUGE_VAL;
   return TG_OK;
+}
+
+int tg_material_from_angles(double phi_int, double phi_bed, tg_material* out) {
+  if (!out) return TG_ERR_ARG;
+  tg_params p;
+  const int rc = tg_params_from_angles(phi_int, phi_bed, 1.0, 1.0, &p);
+  if (rc) return rc;
+  out->tan_bed = p.tan_bed;
+  out->kact_root = p.kact_root;
+  out->dkroot_dphi = p.dkroot_dphi;
+  return TG_OK;
+}
+
+int tg_set_materials(tg_ctx* ctx, int member, int nmat, const tg_material* mats) {
+  if (!ctx || nmat < 0 || nmat > TG_MAX_MATERIALS || (nmat > 0 && !mats))
+    return TG_ERR_ARG;
+  if (!ctx->have_mesh) return set_err(ctx, TG_ERR_STATE, "no mesh");
+  if (member < 0 || member >= ctx->ninst) return set_err(ctx, TG_ERR_ARG, "no such member");
+  if (!ctx->mat_d)
+    return set_err(ctx, TG_ERR_STATE, "the mesh gives no per-cell materials (tg_mesh.mat)");
+  for (int m = 0; m < nmat; m++)
+    ctx->mtab_h[(size_t)member * MAX_MAT + m] =
+        MatDev{mats[m].tan_bed, mats[m].kact_root, mats[m].dkroot_dphi, 0.};
+  ctx->mat_count[member] = nmat;
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  return upload_params(ctx, nullptr);
 }
 
 int tg_create(const tg_config* cfg, tg_ctx** out) {

@@ -21,7 +21,25 @@ struct InstDev {
   double inv_dx, inv_sx, inv_dy, inv_sy;
   double sin_int, tan_bed, kc1, kroot, dkroot, eps, gscale;
   double maxtime, nextoutput, nextsave;
+  // first entry of the member's materials in the material table
+  int mat_base;
 };
+
+// Bed friction of one material (MatProps::bedfrict[material],
+// get_coef_and_eigen.C:112-114, residual.C:213): tan(phi_bed), kact's root
+// sqrt|1 - (1 + tan^2 phi_bed) cos^2 phi_int| and its phi_bed derivative.
+struct MatDev {
+  double tan_bed, kroot, dkroot, pad;
+};
+constexpr int MAX_MAT = 16;  // materials per member (TG_MAX_MATERIALS)
+
+// The bed friction of cell c of member I: its material's entry when the
+// mesh gives per-cell materials (mat non-null), else the member's own.
+__device__ __forceinline__ MatDev mat_of(const InstDev& I, const uint8_t* mat, const MatDev* tab,
+                                         long long c) {
+  if (!mat) return MatDev{I.tan_bed, I.kroot, I.dkroot, 0.};
+  return tab[I.mat_base + mat[c]];
+}
 
 // A launch tile: TX x (BLOCK / TX) cells of instance `inst` starting at
 // (x0, y0).  One workgroup per tile.  mw: the zero-mark word of the tile's
@@ -140,6 +158,10 @@ struct StepArgs {
   uint32_t* mk_out;
   // null, or a counter the launch adds the cells it writes on a short path to
   unsigned long long* short_cells;
+  // per-cell material index and the material table (mat null: one material
+  // per member, the InstDev fields)
+  const uint8_t* mat;
+  const MatDev* mtab;
 };
 
 struct AdjArgs {
@@ -170,6 +192,8 @@ struct AdjArgs {
   const uint32_t* mk_lnext;
   uint32_t* mk_out;
   unsigned long long* short_cells;
+  const uint8_t* mat;     // as in StepArgs
+  const MatDev* mtab;
 };
 
 // Mixed-precision counterparts: FP32 arithmetic, FP32 static coefficients
@@ -213,6 +237,8 @@ struct StepArgsM {
   S* hv_out;
   int* err;
   unsigned long long* active;  // as in StepArgs (not counted by these kernels)
+  const uint8_t* mat;          // as in StepArgs
+  const MatDev* mtab;
 };
 template <class S>
 struct AdjArgsM {
@@ -241,6 +267,8 @@ struct AdjArgsM {
   unsigned long long* active;  // null, or counter of cells in non-void tiles
   int* tile_list;        // compacted non-void tiles (classifying pass) and count
   int* tile_count;
+  const uint8_t* mat;    // as in StepArgs
+  const MatDev* mtab;
 };
 // workgroups of the FP64 thin-layer adjoint pass (strides over its list)
 constexpr int THIN_GRID = 1024;

@@ -176,7 +176,8 @@ __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
         hy = slope_fwd(a.U.h[c + nx], hc, a.U.h[c - nx], I.dy, a.gamma, a.tiny);
         hvy = slope_fwd(a.U.hv[c + nx], hv, a.U.hv[c - nx], I.dy, a.gamma, a.tiny);
       }
-      const double k = getcoef(hc, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.eps, a.tiny);
+      const double kroot = mat_of(I, a.mat, a.mtab, c).kroot;
+      const double k = getcoef(hc, hu, hv, hx, hux, hy, hvy, I.kc1, kroot, I.eps, a.tiny);
       if (reduce) {
         double ev = eigen(hc, hu, hv, k, a.S.gz[c], a.tiny);
         if (ev > 1000000000.) atomicOr(a.err, 2);
@@ -471,7 +472,8 @@ __device__ __forceinline__ void update_cell(const StepArgs& a, const TileDev& t,
   double out[3];
   SrcInfo si;
   update_states(prev, out, fE, fN, fW, fS, dt / I.dx, dt / I.dy, dt, hx, hvx, hy, huy,
-                a.S.kx[c], a.S.ky[c], I.sin_int, I.tan_bed, a.S.gx[c], a.S.gy[c], a.S.gz[c],
+                a.S.kx[c], a.S.ky[c], I.sin_int, mat_of(I, a.mat, a.mtab, c).tan_bed,
+                a.S.gx[c], a.S.gy[c], a.S.gz[c],
                 a.S.dgx[c], a.S.dgy[c], kc, a.ft, a.tiny, &si);
   if (!isfinite(out[0]) || !isfinite(out[1]) || !isfinite(out[2])) atomicOr(a.err, 1);
   a.h_out[c] = out[0];
@@ -612,12 +614,13 @@ __device__ __forceinline__ DualInfo dual_info(const AdjArgs& a, const InstDev& I
     r.d[4] = slope_dual(a.U.hu[c + nx], r.hu, a.U.hu[c - nx], I.dy, iy, isy, a.gamma, a.tiny, false).d;
     r.d[5] = slope_dual(a.U.hv[c + nx], r.hv, a.U.hv[c - nx], I.dy, iy, isy, a.gamma, a.tiny, false).d;
   }
-  r.k = getcoef(r.h, r.hu, r.hv, r.d[0], r.d[1], r.d[3], r.d[5], I.kc1, I.kroot, I.eps, a.tiny);
-  r.dk = want_dk ? dgetcoef_dphi(r.h, r.hu, r.hv, r.d[0], r.d[1], r.d[3], r.d[5], I.kc1, I.kroot,
-                                 I.dkroot, I.eps, a.tiny)
+  const MatDev M = mat_of(I, a.mat, a.mtab, c);
+  r.k = getcoef(r.h, r.hu, r.hv, r.d[0], r.d[1], r.d[3], r.d[5], I.kc1, M.kroot, I.eps, a.tiny);
+  r.dk = want_dk ? dgetcoef_dphi(r.h, r.hu, r.hv, r.d[0], r.d[1], r.d[3], r.d[5], I.kc1, M.kroot,
+                                 M.dkroot, I.eps, a.tiny)
                  : 0.;
   r.si = src_switches(r.h, r.hu, r.hv, r.d[0], r.d[2], r.d[3], r.d[4], a.S.kx[c], a.S.ky[c],
-                      I.tan_bed, r.g2, a.ft, a.tiny);
+                      M.tan_bed, r.g2, a.ft, a.tiny);
   return r;
 }
 
@@ -825,7 +828,8 @@ __device__ double dphi_term(const AdjArgs& a, const InstDev& I, const Cell& q,
   if (C.h - dtdx * (fE[0] - fW[0]) - dtdy * (fN[0] - fS[0]) < 0.) out[0] = 0.;
   if (C.h > a.tiny) {
     const long long c = q.c;
-    const double sec2 = 1. + I.tan_bed * I.tan_bed;
+    const double tbc = mat_of(I, a.mat, a.mtab, c).tan_bed;
+    const double sec2 = 1. + tbc * tbc;
     const double vx = L.vx[rc], vy = L.vy[rc];  // hu / h, hv / h (staged)
     const double g2 = C.g2, dg0 = a.S.dgx[c], dg1 = a.S.dgy[c], k0 = a.S.kx[c], k1 = a.S.ky[c];
     double ds2x = C.si.sgn[0] * C.h * C.dk * (g2 * C.d[3] + dg1 * C.h) * I.sin_int;
@@ -981,7 +985,8 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
           const double alpha = I.sin_int * C.si.sgn[0] * C.k;
           const double betta = I.sin_int * C.si.sgn[1] * C.k;
           const double vx = L.vx[rself], vy = L.vy[rself];  // hu / h, hv / h (staged)
-          const double vx_sq = vx * vx, vy_sq = vy * vy, tb = I.tan_bed;
+          const double vx_sq = vx * vx, vy_sq = vy * vy;
+          const double tb = mat_of(I, a.mat, a.mtab, c).tan_bed;
           const double g0 = a.S.gx[c], g1 = a.S.gy[c], g2 = C.g2;
           const double dg0 = a.S.dgx[c], dg1 = a.S.dgy[c], k0 = a.S.kx[c], k1 = a.S.ky[c];
           const double dh0 = C.hsx[0], dh1 = C.hsy[0];

@@ -94,8 +94,8 @@ __global__ __launch_bounds__(BLOCK) void k32_coef(StepArgsM<S> a, int reduce) {
     if (h > tiny) {
       const W ih = inv_h(h);
       const W u = vel(hu, h, ih), v = vel(hv, h, ih);
-      k = kact<W>(h, u, v, ih, hx, hux, hy, hvy, (float)I.kc1, (float)I.kroot, (float)I.eps,
-                  tiny);
+      k = kact<W>(h, u, v, ih, hx, hux, hy, hvy, (float)I.kc1,
+                  (float)mat_of(I, a.mat, a.mtab, c).kroot, (float)I.eps, tiny);
       if (reduce) {
         const float c2 = fsqrt(k * a.S32.gz[c] * (float)h);
         const double ev = (double)fmax(fabs(u), fabs(v)) + (double)c2;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(BLOCK) void k32_update(StepArgsM<S> a) {
     const float sgn3 = (float)tiny_sgn<W>(v, ft);
     const float k = a.kact[c], g2 = a.S32.gz[c];
     const float hf = (float)h, uf = (float)u, vf = (float)v, huf = (float)hu, hvf = (float)hv;
-    const float sint = (float)I.sin_int, tanb = (float)I.tan_bed;
+    const float sint = (float)I.sin_int, tanb = (float)mat_of(I, a.mat, a.mtab, c).tan_bed;
     float s1 = a.S32.gx[c] * hf;
     float s2 = sgn0 * hf * k * (g2 * (float)hy + a.S32.dgy[c] * hf) * sint;
     float s3 = sgn2 * fmaxf(g2 * hf + uf * huf * a.S32.kx[c], 0.f) * tanb;
@@ -394,9 +394,10 @@ __device__ __forceinline__ DualInfoT<F> dual_info(const AdjArgsM<S>& a, const In
   r.nz = h != W(0);
   r.hx = (F)d[0];
   r.hy = (F)d[3];
-  r.k = kact<W, F>(h, u, v, ih, d[0], d[1], d[3], d[5], (F)I.kc1, (F)I.kroot, (F)I.eps, tiny);
-  r.dk = want_dk ? dkact_dphi<W, F>(h, u, v, ih, d[0], d[1], d[3], d[5], (F)I.kc1, (F)I.kroot,
-                                    (F)I.dkroot, (F)I.eps, tiny)
+  const MatDev M = mat_of(I, a.mat, a.mtab, c);
+  r.k = kact<W, F>(h, u, v, ih, d[0], d[1], d[3], d[5], (F)I.kc1, (F)M.kroot, (F)I.eps, tiny);
+  r.dk = want_dk ? dkact_dphi<W, F>(h, u, v, ih, d[0], d[1], d[3], d[5], (F)I.kc1, (F)M.kroot,
+                                    (F)M.dkroot, (F)I.eps, tiny)
                  : F(0);
   r.sgn[0] = r.sgn[1] = r.sgn[2] = r.sgn[3] = F(0);
   r.stop[0] = r.stop[1] = 0;
@@ -407,7 +408,7 @@ __device__ __forceinline__ DualInfoT<F> dual_info(const AdjArgsM<S>& a, const In
     r.sgn[1] = (F)tiny_sgn<W>(ih * (d[2] - v * d[0]), ft);
     r.sgn[3] = (F)tiny_sgn<W>(v, ft);
     // stop: the Coulomb term's max(.,0) clamps to zero (or tan(phi_bed) = 0)
-    const W g2 = (W)stat<F>(a, 2, c), tanb = (W)I.tan_bed;
+    const W g2 = (W)stat<F>(a, 2, c), tanb = (W)M.tan_bed;
     r.stop[0] = r.sgn[2] != F(0) && (!(g2 * h + u * hu * (W)stat<F>(a, 5, c) > W(0)) || tanb == W(0));
     r.stop[1] = r.sgn[3] != F(0) && (!(g2 * h + v * hv * (W)stat<F>(a, 6, c) > W(0)) || tanb == W(0));
   }
@@ -554,7 +555,8 @@ __device__ __forceinline__ F dphi_term(const AdjArgsM<S>& a, const InstDev& I, l
   for (int v = 0; v < 3; v++) out[v] = -dtdx * (dE[v] - dW[v]) - dtdy * (dN[v] - dS[v]);
   if (C.h - dtdx * (fE[0] - fW[0]) - dtdy * (fN[0] - fS[0]) < F(0)) out[0] = F(0);
   if (C.thick) {
-    const F sec2 = (F)(1. + I.tan_bed * I.tan_bed), sint = (F)I.sin_int;
+    const double tbc = mat_of(I, a.mat, a.mtab, c).tan_bed;
+    const F sec2 = (F)(1. + tbc * tbc), sint = (F)I.sin_int;
     const F g2 = C.g2;
     F ds2x = C.sgn[0] * C.h * C.dk * (g2 * C.hy + stat<F>(a, 4, c) * C.h) * sint;
     F ds3x = C.sgn[2] * fmax(g2 * C.h + C.u * C.hu * stat<F>(a, 5, c), F(0)) * sec2;
@@ -606,7 +608,7 @@ __device__ __forceinline__ double gather(const AdjArgsM<S>& a, const InstDev& I,
           J.a[i][j] = dtdx * (pE - nW) + dtdy * (pN - nS);
         }
       if (self.thick) {
-        const F sint = (F)I.sin_int, tb = (F)I.tan_bed;
+        const F sint = (F)I.sin_int, tb = (F)mat_of(I, a.mat, a.mtab, c).tan_bed;
         const F alpha = sint * self.sgn[0] * self.k, betta = sint * self.sgn[1] * self.k;
         const F u = self.u, v = self.v;
         const F g2 = self.g2, k0 = stat<F>(a, 5, c), k1 = stat<F>(a, 6, c);

@@ -209,7 +209,8 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_coef(StepArgs a, LeafDev L, int 
         hy = lslope_fwd(a.U.h, c, qy, a.gamma, a.tiny);
         hvy = lslope_fwd(a.U.hv, c, qy, a.gamma, a.tiny);
       }
-      const double k = getcoef(h, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.eps, a.tiny);
+      const double k = getcoef(h, hu, hv, hx, hux, hy, hvy, I.kc1,
+                               mat_of(I, a.mat, a.mtab, c).kroot, I.eps, a.tiny);
       if (reduce) {
         double ev = eigen(h, hu, hv, k, a.S.gz[c], a.tiny);
         if (ev > 1000000000.) atomicOr(a.err, 2);
@@ -293,7 +294,8 @@ __device__ __forceinline__ void leaf_update_full(const StepArgs& a, const LeafDe
   const double prev[3] = {h, hu, hv};
   SrcInfo si;
   update_states(prev, out, F[1], F[2], F[3], F[0], dtdx, dtdy, dt, hx, hvx, hy, huy,
-                a.S.kx[c], a.S.ky[c], I.sin_int, I.tan_bed, a.S.gx[c], a.S.gy[c], g2,
+                a.S.kx[c], a.S.ky[c], I.sin_int, mat_of(I, a.mat, a.mtab, c).tan_bed,
+                a.S.gx[c], a.S.gy[c], g2,
                 a.S.dgx[c], a.S.dgy[c], k, a.ft, a.tiny, &si);
 }
 
@@ -376,9 +378,10 @@ __device__ __forceinline__ void leaf_dualk_wet(const AdjArgs& a, const LeafDev& 
     hy = lslope_dual(a.U.h, c, qy, a.gamma, a.tiny, false).d;
     hvy = lslope_dual(a.U.hv, c, qy, a.gamma, a.tiny, false).d;
   }
-  B.kd[c] = getcoef(h, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.eps, a.tiny);
+  const MatDev M = mat_of(I, a.mat, a.mtab, c);
+  B.kd[c] = getcoef(h, hu, hv, hx, hux, hy, hvy, I.kc1, M.kroot, I.eps, a.tiny);
   if (want_dk)
-    B.dkd[c] = dgetcoef_dphi(h, hu, hv, hx, hux, hy, hvy, I.kc1, I.kroot, I.dkroot, I.eps,
+    B.dkd[c] = dgetcoef_dphi(h, hu, hv, hx, hux, hy, hvy, I.kc1, M.kroot, M.dkroot, I.eps,
                              a.tiny);
 }
 
@@ -507,7 +510,8 @@ __device__ double leaf_dphi(const AdjArgs& a, const LeafDev& L, const LeafBufs& 
     out[v] = -dtdx * (dF[1][v] - dF[3][v]) - dtdy * (dF[2][v] - dF[0][v]);
   if (h - dtdx * (F[1][0] - F[3][0]) - dtdy * (F[2][0] - F[0][0]) < 0.) out[0] = 0.;
   if (h > a.tiny) {
-    const double sec2 = 1. + I.tan_bed * I.tan_bed;
+    const double tbc = mat_of(I, a.mat, a.mtab, c).tan_bed;
+    const double sec2 = 1. + tbc * tbc;
     const double vx = hu / h, vy = hv / h, dk = B.dkd[c];
     const double g2 = a.S.gz[c], dg0 = a.S.dgx[c], dg1 = a.S.dgy[c], k0 = a.S.kx[c],
                  k1 = a.S.ky[c];
@@ -639,7 +643,8 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_jac(AdjArgs a, LeafDev L, LeafBu
   if (h > a.tiny) {
     dual_slopes(a, L, c, q.meta, d, hs);
     // orgSrcSgn and stop of update_states (they do not depend on the fluxes)
-    si = src_switches(h, hu, hv, d[0], d[2], d[3], d[4], cv0, cv1, I.tan_bed, g2, a.ft, a.tiny);
+    si = src_switches(h, hu, hv, d[0], d[2], d[3], d[4], cv0, cv1,
+                      mat_of(I, a.mat, a.mtab, c).tan_bed, g2, a.ft, a.tiny);
   }
   q.g2 = g2;
   q.dt = a.dt_rec[a.step];
@@ -678,7 +683,8 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_jac(AdjArgs a, LeafDev L, LeafBu
       for (int i = 0; i < 3; i++) J.a[i][i] -= 1.;
       if (q.wet) {
         const double vx = hu / h, vy = hv / h;
-        const double vx_sq = vx * vx, vy_sq = vy * vy, tb = I.tan_bed;
+        const double vx_sq = vx * vx, vy_sq = vy * vy;
+        const double tb = mat_of(I, a.mat, a.mtab, c).tan_bed;
         J.a[1][0] -= dt * (g0 - alpha * h * (2 * dg1 + g2 * hs[1][0]) - alpha * g2 * d[3]);
         if (si.sgn[2] != 0. && !si.stop[0]) {
           J.a[1][0] -= -dt * si.sgn[2] * tb * (g2 - vx_sq * cv0);
@@ -711,7 +717,8 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_dphi(AdjArgs a, LeafDev L, LeafB
     double d[6], hs[2][5];
     dual_slopes(a, L, c, meta, d, hs);
     const SrcInfo si = src_switches(a.U.h[c], a.U.hu[c], a.U.hv[c], d[0], d[2], d[3], d[4],
-                                    a.S.kx[c], a.S.ky[c], I.tan_bed, a.S.gz[c], a.ft, a.tiny);
+                                    a.S.kx[c], a.S.ky[c], mat_of(I, a.mat, a.mtab, c).tan_bed,
+                                    a.S.gz[c], a.ft, a.tiny);
     const double dt = a.dt_rec[a.step];
     const double dtdx = dt / L.cdx[c], dtdy = dt / L.cdx[L.n + c];
     dphi = leaf_dphi(a, L, B, I, c, meta, d, si, dt, dtdx, dtdy);

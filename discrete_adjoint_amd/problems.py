@@ -39,6 +39,10 @@ class Problem:
     epsilon: float = 1.0
     gravity_scale: float = 1.0
     meta: dict = field(default_factory=dict)
+    # per-cell material index (uint8, n) and the bed friction angle (deg) of
+    # each material; None: one material, phi_bed_deg
+    mat: np.ndarray = None
+    materials: tuple = None
 
     @property
     def n(self):
@@ -227,6 +231,8 @@ class LeafProblem:
     epsilon: float = 1.0
     gravity_scale: float = 1.0
     meta: dict = field(default_factory=dict)
+    mat: np.ndarray = None    # as Problem
+    materials: tuple = None
 
     @property
     def n(self):

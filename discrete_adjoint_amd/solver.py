@@ -77,11 +77,25 @@ class Simulation:
         cv = np.ascontiguousarray(problem.curv.reshape(-1))
         self._arrays += [g, dg, cv]
         m.grav, m.dgrav, m.curv = _lib.dptr(g), _lib.dptr(dg), _lib.dptr(cv)
+        _mesh_materials(m, problem, self._arrays)
         self._check(self.lib.tg_set_mesh(self.ctx, C.byref(m)))
+        self._set_materials(0, problem)
         self.set_params(problem.phi_int_deg, problem.phi_bed_deg)
         self.set_state(problem.h, problem.hu, problem.hv)
 
     # -- plumbing -------------------------------------------------------------
+    def _set_materials(self, member, problem):
+        """tg_set_materials from the problem's per-material bed friction angles."""
+        if problem.mat is None or not problem.materials:
+            return
+        mats = (_lib.tg_material * len(problem.materials))()
+        for i, phi_bed in enumerate(problem.materials):
+            rc = self.lib.tg_material_from_angles(math.radians(problem.phi_int_deg),
+                                                  math.radians(phi_bed), C.byref(mats[i]))
+            if rc:
+                raise TgError(rc, "tg_material_from_angles")
+        self._check(self.lib.tg_set_materials(self.ctx, member, len(problem.materials), mats))
+
     def _check(self, rc, nocontext=False):
         if rc != _lib.TG_OK:
             msg = "" if nocontext else self.lib.tg_last_error(self.ctx).decode()
@@ -316,7 +330,19 @@ def _mesh_of(problem, keep):
     cv = np.ascontiguousarray(problem.curv.reshape(-1))
     keep += [g, dg, cv]
     m.grav, m.dgrav, m.curv = _lib.dptr(g), _lib.dptr(dg), _lib.dptr(cv)
+    _mesh_materials(m, problem, keep)
     return m
+
+
+def _mesh_materials(m, problem, keep):
+    """tg_mesh.mat from the problem's per-cell material indices (or NULL)."""
+    if problem.mat is None:
+        return
+    mat = np.ascontiguousarray(problem.mat, dtype=np.uint8).reshape(-1)
+    if mat.size != problem.n:
+        raise ValueError("problem.mat must hold one index per cell")
+    keep.append(mat)
+    m.mat = mat.ctypes.data_as(C.POINTER(C.c_uint8))
 
 
 class Ensemble(Simulation):
@@ -340,6 +366,8 @@ class Ensemble(Simulation):
                                                           p.epsilon, p.gravity_scale)
                                        for p in self.problems])
         self._check(self.lib.tg_set_ensemble(self.ctx, nm, meshes, prms))
+        for b, p in enumerate(self.problems):
+            self._set_materials(b, p)
         self.params = None
         self.offsets = []
         for b in range(nm):

@@ -111,6 +111,12 @@ typedef struct tg_mesh {
   const int32_t* gen;   /* [n]   Element::generation                         */
   const double* cdx;    /* [2*n] Element::dx (cell size x, y)                */
   const double* xc;     /* [2*n] Element::coord (centre x, y)                */
+  /* Both kinds: material index of every cell / leaf (Element::material,
+   * read by get_coef_and_eigen.C:112-114 and corrector.C:108 as
+   * bedfrict[material]), 0 .. TG_MAX_MATERIALS-1; NULL = one material.
+   * The reference numbers its materials from 1 (GIS raster ids,
+   * element2.C:3690): pass id - 1. */
+  const uint8_t* mat;
 } tg_mesh;
 
 /* Material and time parameters of one run (MatProps, TimeProps). */
@@ -126,6 +132,17 @@ typedef struct tg_params {
   double nextoutput;     /* next output time (incrtime clamp); +inf = none   */
   double nextsave;       /* next save time (incrtime clamp); +inf = none     */
 } tg_params;
+
+/* Bed friction of one material (MatProps::bedfrict[material] with the
+ * internal friction of tg_params): tan(phi_bed) (residual.C:213),
+ * sqrt|1 - (1 + tan^2 phi_bed) cos^2 phi_int| (getcoef.f:41) and its
+ * phi_bed derivative (bed-friction gradient). */
+#define TG_MAX_MATERIALS 16
+typedef struct tg_material {
+  double tan_bed;
+  double kact_root;
+  double dkroot_dphi;
+} tg_material;
 
 /* Mass/height statistics of the last completed step (step.C:205 realvolume,
  * get_coef_and_eigen.C:102-104 hmax); FP64 sums on the device. */
@@ -159,6 +176,10 @@ void tg_config_default(tg_config* cfg);
 int tg_params_from_angles(double phi_int, double phi_bed, double epsilon,
                           double gravity_scale, tg_params* out);
 
+/* One material's host-side transcendentals (frict.data read, datread.C:379-403,
+ * with getcoef.f:39-43).  Angles in radians. */
+int tg_material_from_angles(double phi_int, double phi_bed, tg_material* out);
+
 /* Creates a context on cfg->device (owns device buffers and stream). */
 int tg_create(const tg_config* cfg, tg_ctx** out);
 void tg_destroy(tg_ctx* ctx);
@@ -167,6 +188,15 @@ const char* tg_last_error(const tg_ctx* ctx);
 /* Copies mesh tables and static coefficients to the device (replaces
  * setup_geoflow's per-element data, setup_geoflow.C:24; called per re-mesh). */
 int tg_set_mesh(tg_ctx* ctx, const tg_mesh* mesh);
+
+/* Materials 0 .. nmat-1 of a member whose mesh gave per-cell materials
+ * (tg_mesh.mat): the MatProps::bedfrict table (datread.C:379-403).  A
+ * material index no call gave takes the member's own tg_params values (the
+ * ones tg_set_ensemble or the run's prm hold).  dJ/dphi_bed of
+ * tg_adjoint_run is then the derivative with respect to one common change of
+ * every material's bed friction angle (the sum of the per-material
+ * derivatives).  TG_ERR_STATE if the mesh has no material indices. */
+int tg_set_materials(tg_ctx* ctx, int member, int nmat, const tg_material* mats);
 
 /* State U = (h, hu, hv), FP64 host arrays of n (Element::state_vars).
  * With several members, the arrays hold all cells: member b's cells at
@@ -280,7 +310,11 @@ int tg_adjoint_step_grad(tg_ctx* ctx, const tg_params* prm, int s, double dt,
 
 /* ---- timing (HIP events around the context's kernels) ---------------------- */
 
-/* Kernel-only time of the last forward/adjoint run in milliseconds. */
+/* Time of the last forward run and reverse sweep in milliseconds, from HIP
+ * events on the context's stream around their launches (graph replays or
+ * kernel by kernel): forward_ms the forward sweep, adjoint_ms the whole
+ * reverse sweep, recompute_ms its checkpoint-segment recomputation (restore
+ * and forward steps), so adjoint_ms - recompute_ms is the reverse steps. */
 int tg_last_timing(tg_ctx* ctx, double* forward_ms, double* adjoint_ms,
                    double* recompute_ms);
 /* The part of the last reverse sweep that read the stored trajectory (no

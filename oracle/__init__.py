@@ -15,11 +15,16 @@ OMP_LIB_PATH = os.path.join(HERE, "liboracle_omp.so")
 _dp = C.POINTER(C.c_double)
 
 
+class or_materials(C.Structure):
+    _fields_ = [("mat", C.POINTER(C.c_ubyte)), ("tan_bed", _dp), ("kact_root", _dp),
+                ("dkroot", _dp)]
+
+
 class or_mesh(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
                 ("x0", C.c_double), ("y0", C.c_double),
                 ("gx", _dp), ("gy", _dp), ("gz", _dp), ("dgx", _dp), ("dgy", _dp),
-                ("kx", _dp), ("ky", _dp)]
+                ("kx", _dp), ("ky", _dp), ("mats", or_materials)]
 
 
 class or_params(C.Structure):
@@ -70,6 +75,32 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(_dp)
 
 
+def bed_constants(phi_int_deg, phi_bed_deg):
+    """MatProps + getcoef.f:39-43 constants of one bed friction angle, with
+    libm: (tan_bed, kact_c1, kact_root, d kact_root / d phi_bed)."""
+    pi_, pb_ = math.radians(phi_int_deg), math.radians(phi_bed_deg)
+    cosphi, tandel = math.cos(pi_), math.tan(pb_)
+    w = 1.0 - (1.0 + tandel * tandel) * (cosphi * cosphi)
+    root = math.sqrt(abs(w))
+    droot = (-(-1.0 if w < 0 else 1.0) * (cosphi * cosphi) * tandel * (1.0 + tandel * tandel) /
+             root) if root > 0 else 0.0
+    return tandel, 2.0 / (cosphi * cosphi), root, droot
+
+
+def _materials(obj, mat, materials, phi_int_deg):
+    """or_materials of per-cell indices `mat` and bed angles `materials`
+    (kept alive on obj)."""
+    mm = or_materials()
+    if mat is None:
+        return mm
+    obj._mat = np.ascontiguousarray(mat, dtype=np.uint8).reshape(-1)
+    consts = [bed_constants(phi_int_deg, b) for b in materials]
+    obj._mtab = [np.array([c[i] for c in consts], dtype=np.float64) for i in (0, 2, 3)]
+    mm.mat = obj._mat.ctypes.data_as(C.POINTER(C.c_ubyte))
+    mm.tan_bed, mm.kact_root, mm.dkroot = [_p(a) for a in obj._mtab]
+    return mm
+
+
 def frict_tiny(s, switch=50, early=0.1, late=0.000000001):
     """hpfem.C:232-235 schedule for step index s (iter before the step)."""
     return early if s < switch else late
@@ -79,7 +110,8 @@ class Oracle:
     """FP64 CPU restatement bound to one uniform problem."""
 
     def __init__(self, nx, ny, dx, dy, x0, y0, grav, dgrav, curv, phi_int_deg, phi_bed_deg,
-                 epsilon=1.0, gscale=1.0, tiny=1e-4, gamma=1.0, omp=False):
+                 epsilon=1.0, gscale=1.0, tiny=1e-4, gamma=1.0, omp=False, mat=None,
+                 materials=None):
         self.lib = load(omp)
         self.n = nx * ny
         self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
@@ -87,6 +119,7 @@ class Oracle:
         m = or_mesh()
         m.nx, m.ny, m.dx, m.dy, m.x0, m.y0 = nx, ny, dx, dy, x0, y0
         m.gx, m.gy, m.gz, m.dgx, m.dgy, m.kx, m.ky = [_p(a) for a in self._keep]
+        m.mats = _materials(self, mat, materials, phi_int_deg)
         self.mesh = m
         self.tiny, self.gamma, self.eps, self.gscale = tiny, gamma, epsilon, gscale
         self.set_angles(phi_int_deg, phi_bed_deg)
@@ -95,20 +128,15 @@ class Oracle:
     @classmethod
     def from_problem(cls, pb, omp=False):
         return cls(pb.nx, pb.ny, pb.dx, pb.dy, pb.x0, pb.y0, pb.grav, pb.dgrav, pb.curv,
-                   pb.phi_int_deg, pb.phi_bed_deg, pb.epsilon, pb.gravity_scale, omp=omp)
+                   pb.phi_int_deg, pb.phi_bed_deg, pb.epsilon, pb.gravity_scale, omp=omp,
+                   mat=getattr(pb, "mat", None), materials=getattr(pb, "materials", None))
 
     def set_angles(self, phi_int_deg, phi_bed_deg):
         """MatProps + getcoef.f:39-43 constants, evaluated with libm."""
-        pi_, pb_ = math.radians(phi_int_deg), math.radians(phi_bed_deg)
         p = or_params()
-        cosphi, tandel = math.cos(pi_), math.tan(pb_)
-        p.sin_int = math.sin(pi_)
-        p.tan_bed = tandel
-        p.kact_c1 = 2.0 / (cosphi * cosphi)
-        w = 1.0 - (1.0 + tandel * tandel) * (cosphi * cosphi)
-        p.kact_root = math.sqrt(abs(w))
-        p.dkroot_dphi = (-(-1.0 if w < 0 else 1.0) * (cosphi * cosphi) * tandel *
-                         (1.0 + tandel * tandel) / p.kact_root) if p.kact_root > 0 else 0.0
+        p.sin_int = math.sin(math.radians(phi_int_deg))
+        p.tan_bed, p.kact_c1, p.kact_root, p.dkroot_dphi = bed_constants(phi_int_deg,
+                                                                         phi_bed_deg)
         p.epsilon, p.gscale, p.tiny, p.gamma = self.eps, self.gscale, self.tiny, self.gamma
         p.cfl_inner = p.cfl_outer = 0.5
         p.maxtime = p.nextoutput = p.nextsave = math.inf
@@ -195,7 +223,7 @@ _ip = C.POINTER(C.c_int)
 class or_leaf_mesh(C.Structure):
     _fields_ = [("n", C.c_int), ("nbr", _ip), ("nproc", _ip), ("gen", _ip), ("cdx", _dp),
                 ("xc", _dp), ("gx", _dp), ("gy", _dp), ("gz", _dp), ("dgx", _dp), ("dgy", _dp),
-                ("kx", _dp), ("ky", _dp)]
+                ("kx", _dp), ("ky", _dp), ("mats", or_materials)]
 
 
 def _leaf_sigs(lib):
@@ -218,7 +246,7 @@ class LeafOracle(Oracle):
     nbr / nproc (n, 8), gen (n,), cdx / xc (n, 2))."""
 
     def __init__(self, nbr, nproc, gen, cdx, xc, grav, dgrav, curv, phi_int_deg, phi_bed_deg,
-                 epsilon=1.0, gscale=1.0, tiny=1e-4, gamma=1.0):
+                 epsilon=1.0, gscale=1.0, tiny=1e-4, gamma=1.0, mat=None, materials=None):
         self.lib = load()
         _leaf_sigs(self.lib)
         self.n = len(gen)
@@ -230,6 +258,7 @@ class LeafOracle(Oracle):
         m.n = self.n
         m.nbr, m.nproc, m.gen = [a.ctypes.data_as(_ip) for a in ints]
         (m.cdx, m.xc, m.gx, m.gy, m.gz, m.dgx, m.dgy, m.kx, m.ky) = [_p(a) for a in dbl]
+        m.mats = _materials(self, mat, materials, phi_int_deg)
         self.mesh = m
         self.tiny, self.gamma, self.eps, self.gscale = tiny, gamma, epsilon, gscale
         self.set_angles(phi_int_deg, phi_bed_deg)
@@ -239,7 +268,8 @@ class LeafOracle(Oracle):
     def from_problem(cls, pb):
         t = pb.table
         return cls(t.nbr, t.nkind, t.gen, t.dx, t.xc, pb.grav, pb.dgrav, pb.curv,
-                   pb.phi_int_deg, pb.phi_bed_deg, pb.epsilon, pb.gravity_scale)
+                   pb.phi_int_deg, pb.phi_bed_deg, pb.epsilon, pb.gravity_scale,
+                   mat=getattr(pb, "mat", None), materials=getattr(pb, "materials", None))
 
     def step(self, U, s, dt=None):
         h, hu, hv = U

@@ -179,7 +179,10 @@ int orl_step(const or_leaf_mesh* m, const or_params* p, double frict_tiny, doubl
   double min_ev = 10000000, hmax = 0;
   for (int e = 0; e < n; e++) {
     slopes_fwd(m, p, U, e, d + 6 * e);
-    kact[e] = (h[e] > p->tiny) ? getcoef(p, h[e], hu[e], hv[e], d + 6 * e) : p->epsilon;
+    or_params t;
+    kact[e] = (h[e] > p->tiny)
+                  ? getcoef(cell_params(&m->mats, p, e, &t), h[e], hu[e], hv[e], d + 6 * e)
+                  : p->epsilon;
     if (h[e] > p->tiny) {
       if (hmax < h[e]) hmax = h[e];
       double ev = eigen(p, h[e], hu[e], hv[e], kact[e], m->gz[e]);
@@ -262,10 +265,11 @@ int orl_step(const or_leaf_mesh* m, const or_params* p, double frict_tiny, doubl
     double dtdx = dt / m->cdx[2 * e], dtdy = dt / m->cdx[2 * e + 1];
     double kc = (prev[0] > p->tiny) ? kact[e] : p->epsilon;
     src_info si;
+    or_params pt;
     update_states(p, prev, out, nf + NV * node[4 * e + 1], nf + NV * node[4 * e + 2],
                   nf + NV * node[4 * e + 3], nf + NV * node[4 * e + 0], dtdx, dtdy, dt,
-                  d + 6 * e, d + 6 * e + 3, cv, p->sin_int, p->tan_bed, g, dg, kc, frict_tiny,
-                  &si);
+                  d + 6 * e, d + 6 * e + 3, cv, p->sin_int,
+                  cell_params(&m->mats, p, e, &pt)->tan_bed, g, dg, kc, frict_tiny, &si);
     for (int v = 0; v < NV; v++) {
       if (!isfinite(out[v])) rc = -5;
       Un[(size_t)v * n + e] = out[v];
@@ -317,7 +321,8 @@ static leaf_dual* build_dual(const or_leaf_mesh* m, const or_params* p, double f
   lst[1] = (int*)malloc(sizeof(int) * 2 * (size_t)n);
   for (int e = 0; e < n; e++) {
     slopes_dual(m, p, U, e, dc[e].d, dc[e].hs);
-    dc[e].k = getcoef(p, U[0][e], U[1][e], U[2][e], dc[e].d);
+    or_params t;
+    dc[e].k = getcoef(cell_params(&m->mats, p, e, &t), U[0][e], U[1][e], U[2][e], dc[e].d);
   }
   for (int e = 0; e < n; e++) {
     for (int side = 0; side < 2; side++) {
@@ -389,9 +394,11 @@ static leaf_dual* build_dual(const or_leaf_mesh* m, const or_params* p, double f
     double zf[3] = {0, 0, 0};
     double dtdx = dt / m->cdx[2 * e], dtdy = dt / m->cdx[2 * e + 1];
     src_info si;
+    or_params pt;
+    const double tan_bed = cell_params(&m->mats, p, e, &pt)->tan_bed;
     /* orgSrcSgn and stop do not depend on the fluxes */
     update_states(p, prev, out, zf, zf, zf, zf, dtdx, dtdy, dt, dc[e].d, dc[e].d + 3, cv,
-                  p->sin_int, p->tan_bed, g, dg, dc[e].k, ft, &si);
+                  p->sin_int, tan_bed, g, dg, dc[e].k, ft, &si);
     for (int eff = 0; eff < 9; eff++) {
       M3* J = &dc[e].J[eff];
       int zero = (eff == 0 && prev[0] == 0.);
@@ -450,7 +457,7 @@ static leaf_dual* build_dual(const or_leaf_mesh* m, const or_params* p, double f
         } else {
           double vx = prev[1] / prev[0], vy = prev[2] / prev[0];
           double vx_sq = vx * vx, vy_sq = vy * vy;
-          double tb = p->tan_bed;
+          double tb = tan_bed;
           (*J)[1][0] -= dt * (g[0] - alpha * prev[0] * (2 * dg[1] + g[2] * dh[1])
                               - alpha * g[2] * dc[e].d[3]);
           if (si.sgn[2] != 0. && !si.stop[0]) {
@@ -531,8 +538,10 @@ void orl_dstep_dphi(const or_leaf_mesh* m, const or_params* p, double ft, const 
   double hs[2][5];
   for (int e = 0; e < n; e++) {
     slopes_dual(m, p, U, e, d + 6 * e, hs);
-    k[e] = getcoef(p, h[e], hu[e], hv[e], d + 6 * e);
-    dk[e] = dgetcoef_dphi(p, h[e], hu[e], hv[e], d + 6 * e);
+    or_params t;
+    const or_params* pc = cell_params(&m->mats, p, e, &t);
+    k[e] = getcoef(pc, h[e], hu[e], hv[e], d + 6 * e);
+    dk[e] = dgetcoef_dphi(pc, h[e], hu[e], hv[e], d + 6 * e);
   }
   for (int e = 0; e < n; e++) {
     double flux[4][3], dflux[4][3]; /* E, N, W, S */
@@ -549,9 +558,11 @@ void orl_dstep_dphi(const or_leaf_mesh* m, const or_params* p, double ft, const 
       double g[3] = {m->gx[e], m->gy[e], m->gz[e]}, dg[2] = {m->dgx[e], m->dgy[e]};
       double cv[2] = {m->kx[e], m->ky[e]};
       src_info si;
+      or_params pt;
+      const double tbc = cell_params(&m->mats, p, e, &pt)->tan_bed;
       update_states(p, prev, o2, zf, zf, zf, zf, dtdx, dtdy, dt, d + 6 * e, d + 6 * e + 3, cv,
-                    p->sin_int, p->tan_bed, g, dg, k[e], ft, &si);
-      double sec2 = 1. + p->tan_bed * p->tan_bed;
+                    p->sin_int, tbc, g, dg, k[e], ft, &si);
+      double sec2 = 1. + tbc * tbc;
       double vx = prev[1] / prev[0], vy = prev[2] / prev[0];
       const double* dUx = d + 6 * e;
       const double* dUy = d + 6 * e + 3;

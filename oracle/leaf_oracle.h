@@ -29,6 +29,7 @@ typedef struct or_leaf_mesh {
   const double* cdx;  /* [2n] Element::dx                                       */
   const double* xc;   /* [2n] Element::coord                                    */
   const double *gx, *gy, *gz, *dgx, *dgy, *kx, *ky;  /* static coefficients [n] */
+  or_materials mats;  /* per-leaf bed friction (mat NULL: one material)       */
 } or_leaf_mesh;
 
 /* One forward step (step.C:29-275, order 1), U updated in place.  dt_in > 0

@@ -17,6 +17,22 @@
 
 #define NV 3
 
+/* ---- per-cell material ------------------------------------------------ */
+
+/* The parameters of cell c: p with the bed friction of the cell's material
+ * (get_coef_and_eigen.C:112-114 and corrector.C:108 read
+ * bedfrict[material]); p itself when the mesh has one material. */
+static inline const or_params* cell_params(const or_materials* mm, const or_params* p, int c,
+                                           or_params* tmp) {
+  if (!mm->mat) return p;
+  const int k = mm->mat[c];
+  *tmp = *p;
+  tmp->tan_bed = mm->tan_bed[k];
+  tmp->kact_root = mm->kact_root[k];
+  tmp->dkroot_dphi = mm->dkroot[k];
+  return tmp;
+}
+
 /* ---- scalar helpers ------------------------------------------------- */
 
 /* c_sgn, src/header/geoflow.h:90-100 (threshold GEOFLOW_TINY) */

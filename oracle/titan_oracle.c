@@ -155,7 +155,10 @@ static void all_kact_fwd(const or_mesh* m, const or_params* p, const double* U[3
   for (int c = 0; c < n; c++) {
     slopes_fwd(m, p->gamma, p->tiny, U, c, dU + 6 * c);
     if (U[0][c] > p->tiny)
-      kact[c] = getcoef(p, U[0][c], U[1][c], U[2][c], dU + 6 * c);
+    {
+      or_params t;
+      kact[c] = getcoef(cell_params(&m->mats, p, c, &t), U[0][c], U[1][c], U[2][c], dU + 6 * c);
+    }
     else
       kact[c] = p->epsilon;
   }
@@ -257,9 +260,11 @@ int or_step(const or_mesh* m, const or_params* p, double frict_tiny, double* h,
     double cv[2] = {m->kx[c], m->ky[c]};
     double kc = (prev[0] > p->tiny) ? kact[c] : p->epsilon;
     src_info si;
+    or_params pt;
     update_states(p, prev, out, fx + NV * xface(m, c, 1), fy + NV * yface(m, c, 1),
                   fx + NV * xface(m, c, 0), fy + NV * yface(m, c, 0), dtdx, dtdy, dt,
-                  dU + 6 * c, dU + 6 * c + 3, cv, p->sin_int, p->tan_bed, g, dg, kc,
+                  dU + 6 * c, dU + 6 * c + 3, cv, p->sin_int,
+                  cell_params(&m->mats, p, c, &pt)->tan_bed, g, dg, kc,
                   frict_tiny, &si);
     for (int v = 0; v < NV; v++) {
       if (!isfinite(out[v])) rc = -5;
@@ -300,7 +305,8 @@ static dual_cell* build_dual(const or_mesh* m, const or_params* p, double ft,
 #pragma omp parallel for schedule(static)
   for (int c = 0; c < n; c++) {
     slopes_dual(m, p->gamma, p->tiny, U, c, dc[c].d, dc[c].hs);
-    dc[c].k = getcoef(p, U[0][c], U[1][c], U[2][c], dc[c].d);
+    or_params t;
+    dc[c].k = getcoef(cell_params(&m->mats, p, c, &t), U[0][c], U[1][c], U[2][c], dc[c].d);
   }
   /* calc_flux (edge_states.C:79-124) -> DualElem::calc_flux (dualmesh.C:1853-2239):
    * the face between c and its + neighbour e is written by c alone */
@@ -351,9 +357,11 @@ static dual_cell* build_dual(const or_mesh* m, const or_params* p, double ft,
     double cv[2] = {m->kx[c], m->ky[c]};
     double zf[3] = {0, 0, 0};
     src_info si;
+    or_params pt;
+    const double tan_bed = cell_params(&m->mats, p, c, &pt)->tan_bed;
     /* only orgSrcSgn and stop are used; they do not depend on the fluxes */
     update_states(p, prev, out, zf, zf, zf, zf, dtdx, dtdy, dt, dc[c].d, dc[c].d + 3, cv,
-                  p->sin_int, p->tan_bed, g, dg, dc[c].k, ft, &si);
+                  p->sin_int, tan_bed, g, dg, dc[c].k, ft, &si);
     for (int eff = 0; eff < 5; eff++) { /* slots 5..8: neigh_proc == -2, zero */
       M3* J = &dc[c].J[eff];
       int zero = 0;
@@ -405,7 +413,7 @@ static dual_cell* build_dual(const or_mesh* m, const or_params* p, double ft,
         } else {
           double vx = prev[1] / prev[0], vy = prev[2] / prev[0];
           double vx_sq = vx * vx, vy_sq = vy * vy;
-          double tb = p->tan_bed;
+          double tb = tan_bed;
           (*J)[1][0] -= dt * (g[0] - alpha * prev[0] * (2 * dg[1] + g[2] * dh[1])
                               - alpha * g[2] * dc[c].d[3]);
           if (si.sgn[2] != 0. && !si.stop[0]) {
@@ -451,8 +459,10 @@ void or_dstep_dphi(const or_mesh* m, const or_params* p, double ft, const double
   for (int c = 0; c < n; c++) {
     double hs[2][5];
     slopes_dual(m, p->gamma, p->tiny, U, c, d + 6 * c, hs);
-    k[c] = getcoef(p, h[c], hu[c], hv[c], d + 6 * c);
-    dk[c] = dgetcoef_dphi(p, h[c], hu[c], hv[c], d + 6 * c);
+    or_params t;
+    const or_params* pc = cell_params(&m->mats, p, c, &t);
+    k[c] = getcoef(pc, h[c], hu[c], hv[c], d + 6 * c);
+    dk[c] = dgetcoef_dphi(pc, h[c], hu[c], hv[c], d + 6 * c);
   }
   double dtdx = dt / m->dx, dtdy = dt / m->dy;
 #pragma omp parallel for schedule(static)
@@ -489,13 +499,15 @@ void or_dstep_dphi(const or_mesh* m, const or_params* p, double ft, const double
     if (h[c] - dtdx * (flux[0][0] - flux[2][0]) - dtdy * (flux[1][0] - flux[3][0]) < 0.)
       out[0] = 0.;
     if (h[c] > p->tiny) {
+      or_params pt;
+      const double tbc = cell_params(&m->mats, p, c, &pt)->tan_bed;
       double prev[3] = {h[c], hu[c], hv[c]}, o2[3], zf[3] = {0, 0, 0};
       double g[3] = {m->gx[c], m->gy[c], m->gz[c]}, dg[2] = {m->dgx[c], m->dgy[c]};
       double cv[2] = {m->kx[c], m->ky[c]};
       src_info si;
       update_states(p, prev, o2, zf, zf, zf, zf, dtdx, dtdy, dt, d + 6 * c, d + 6 * c + 3, cv,
-                    p->sin_int, p->tan_bed, g, dg, k[c], ft, &si);
-      double sec2 = 1. + p->tan_bed * p->tan_bed;
+                    p->sin_int, tbc, g, dg, k[c], ft, &si);
+      double sec2 = 1. + tbc * tbc;
       double vx = prev[1] / prev[0], vy = prev[2] / prev[0];
       const double* dUx = d + 6 * c;
       const double* dUy = d + 6 * c + 3;

@@ -17,6 +17,14 @@
 extern "C" {
 #endif
 
+/* Per-cell materials (Element::material, MatProps::bedfrict[material]):
+ * cell c uses entry mat[c] of tan_bed, kact_root and dkroot (d kact_root /
+ * d phi_bed); mat NULL: every cell uses or_params' values. */
+typedef struct or_materials {
+  const unsigned char* mat;
+  const double *tan_bed, *kact_root, *dkroot;
+} or_materials;
+
 typedef struct or_mesh {
   int nx, ny;
   double dx, dy;     /* cell sizes; centre coordinates x0+i*dx, y0+j*dy */
@@ -25,6 +33,7 @@ typedef struct or_mesh {
   const double *gx, *gy, *gz;   /* gravity[0..2]            */
   const double *dgx, *dgy;      /* d_gravity[0..1]          */
   const double *kx, *ky;        /* curvature[0..1]          */
+  or_materials mats;            /* per-cell bed friction (mat NULL: one)   */
 } or_mesh;
 
 typedef struct or_params {

@@ -104,3 +104,20 @@ def small_leaf_flow_problem(nx0=16, ny0=12, max_level=2, nsteps=30, slope=25.0, 
                                  h=paraboloid(0.35 * L, 0.5 * ny0 * dx0, 60.0, 45.0, 8.0),
                                  phi_int=32.0, phi_bed=phi_bed, nsteps=nsteps,
                                  curv=(1e-3, -5e-4))
+
+
+def with_materials(pb, phis, split=0.5):
+    """pb with two bed-friction materials: material 0 (phis[0] deg) where the
+    cell / leaf centre lies left of `split` of the x extent, material 1
+    (phis[1] deg) right of it."""
+    import copy
+    q = copy.copy(pb)
+    if hasattr(pb, "table"):
+        x = pb.table.xc[:, 0]
+        L = pb.nx0 * pb.dx0
+    else:
+        x = np.tile(pb.x0 + pb.dx * np.arange(pb.nx), pb.ny)
+        L = pb.nx * pb.dx
+    q.mat = (x >= split * L).astype(np.uint8)
+    q.materials = tuple(phis)
+    return q

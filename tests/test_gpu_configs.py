@@ -1,0 +1,157 @@
+"""GPU: the benchmark configurations at their real sizes.
+
+- 256-member ensemble (uniform blocks of side 64..512 on seeded fractal DEMs,
+  500 steps, checkpoints every 25): the batched run equals every member run
+  alone bit for bit (states, dt records, lambda^0; dJ/dphi_bed to the order of
+  its per-tile sums), and a sample of members equals the oracle over their
+  first steps (states and the reverse sweep of a functional placed there).
+- 2048^2 block (2.5 m) on the fractal DEM with three piles, 1500 steps: the
+  reduced-precision modes against the FP64 path, with the error bounds each
+  mode is held to written below.
+- the 0.88 M-leaf adaptive mesh (fractal DEM, two piles), 200 steps: graph
+  replay equals kernel-by-kernel launches, the stored trajectory equals
+  checkpoints only, mass is conserved while the flow stays inside, and the
+  first steps and their reverse sweep equal the leaf oracle bit for bit on
+  the whole mesh.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from discrete_adjoint_amd import _lib, problems
+from discrete_adjoint_amd.solver import Ensemble, Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def test_ensemble_256_members(oracle_lib):
+    K, k, s_or = 500, 25, 8
+    E = problems.ensemble_problems(256, seed=2024, nsteps=K, checkpoint_every=k)
+    ws = [p.box_weights().reshape(3, -1) for p in E]
+    _say("case ensemble 256 members 500 steps batched")
+    with Ensemble(E, checkpoint_every=k, max_steps=K, use_graph=1) as ens:
+        ens.run(K)
+        states = [ens.member_state(b) for b in range(len(E))]
+        dts = [ens.member_dt_record(b) for b in range(len(E))]
+        lams, dphis = ens.dual_solver(ws, want_dphi=True)
+        # the same run, a functional at step s_or for the oracle sample
+        early = [np.array(x, copy=True) for x in ens.trajectory_state(s_or)]
+        lam_early, _ = ens.dual_solver(ws, step=s_or)
+    sample = {}
+    for b, p in enumerate(E):
+        if p.nx not in sample:
+            sample[p.nx] = b
+    assert sorted(sample) == [64, 128, 256, 512]
+    _say("case ensemble 256 members one by one")
+    for b, pb in enumerate(E):
+        with Simulation(pb, checkpoint_every=k, max_steps=K, use_graph=0) as sim:
+            sim.run(K)
+            ref = sim.get_state()
+            assert np.array_equal(sim.dt_record(), dts[b]), b
+            lam, dphi = sim.dual_solver(ws[b], want_dphi=True)
+        for x, y in zip(states[b], ref):
+            assert np.array_equal(x, y), b
+        assert np.array_equal(lams[b], lam), b
+        assert dphis[b] == pytest.approx(dphi, rel=1e-12, abs=1e-300), b
+    assert max(s[0].max() for s in states) > 0
+    for side, b in sorted(sample.items()):
+        _say(f"case ensemble member {b} (side {side}) vs oracle, {s_or} steps")
+        pb = E[b]
+        o, n = sum(p.n for p in E[:b]), pb.n
+        orc = oracle_lib.Oracle.from_problem(pb)
+        UN, _, lam_ref, _ = orc.forward_adjoint([pb.h, pb.hu, pb.hv], s_or, ws[b])
+        for x, y in zip(early, UN):
+            assert np.array_equal(x[o:o + n], y), b
+        assert np.array_equal(lam_early[b], lam_ref), b
+
+
+# Error bounds of the reduced-precision modes against the FP64 path on the
+# 2048^2 fractal block, 1500 steps: TG_FWD64_ADJ32 keeps the forward sweep
+# exact and holds lambda^0 and dJ/dphi_bed to 1e-6; TG_MIXED runs FP32
+# increments through 1500 steps of wetting / drying and Coulomb switches
+# and is held to the bounds below (h relative L2 / Linf, functional, lambda^0
+# relative L2, dJ/dphi_bed).
+MIXED_BOUNDS = {"h_l2": 5e-2, "h_linf": 0.3, "hu_l2": 0.1, "hv_l2": 0.1, "J": 5e-3,
+                "lam_l2": 0.3, "dphi": 5e-2}
+
+
+def test_grid_2048_precision_modes():
+    n, K, k = 2048, 1500, 50
+    pb = problems.fractal_problem(n=n, dx=5120.0 / n, nsteps=K, npiles=3, seed=11, dem_seed=1,
+                                  checkpoint_every=k)
+    res = {}
+    w = None
+    for mode in (_lib.TG_FP64, _lib.TG_FWD64_ADJ32, _lib.TG_MIXED):
+        _say(f"case grid 2048 fractal 3 piles {K} steps precision {mode}")
+        with Simulation(pb, precision=mode, checkpoint_every=k, max_steps=K) as sim:
+            sim.run(K)
+            H = sim.get_state()
+            if w is None:
+                box = problems.target_box(pb, H[0], cells=20, seed=5)
+                w = pb.box_weights(box)
+            lam, dphi = sim.dual_solver(w, want_dphi=True)
+        res[mode] = (H, float(np.sum(w[:pb.n] * H[0])), lam, float(dphi))
+    H64, J64, lam64, g64 = res[_lib.TG_FP64]
+    assert J64 > 0 and np.isfinite(lam64).all()
+    H, J, lam, g = res[_lib.TG_FWD64_ADJ32]
+    for x, y in zip(H, H64):
+        assert np.array_equal(x, y)
+    assert J == J64
+    e_lam, e_g = _rel(lam, lam64), abs(g - g64) / abs(g64)
+    _say(f"adj32 errors: lambda0 L2 {e_lam:.3g}, dJ/dphi {e_g:.3g}")
+    assert e_lam <= 1e-6 and e_g <= 1e-6
+    H, J, lam, g = res[_lib.TG_MIXED]
+    err = {"h_l2": _rel(H[0], H64[0]),
+           "h_linf": float(np.abs(H[0] - H64[0]).max() / np.abs(H64[0]).max()),
+           "hu_l2": _rel(H[1], H64[1]), "hv_l2": _rel(H[2], H64[2]),
+           "J": abs(J - J64) / abs(J64), "lam_l2": _rel(lam, lam64),
+           "dphi": abs(g - g64) / abs(g64)}
+    _say("mixed errors: " + ", ".join(f"{a} {v:.3g}" for a, v in err.items()))
+    for a, v in err.items():
+        assert v <= MIXED_BOUNDS[a], (a, v)
+
+
+def test_adaptive_mesh_0p88m_leaves(oracle_lib):
+    K, k, s_or = 200, 25, 3
+    pb = problems.adaptive_fractal_problem(nsteps=K, checkpoint_every=k)
+    assert pb.n > 850000
+    w = pb.box_weights()
+    dxy = pb.table.dx[:, 0] * pb.table.dx[:, 1]
+    out = []
+    for graph, store in ((1, 0), (0, 0), (1, 1)):
+        _say(f"case adaptive {pb.n} leaves {K} steps graph={graph} store={store}")
+        extra = {"store_bytes": 32 * pb.n * (K + 1)} if store else {}
+        with Simulation(pb, checkpoint_every=k, max_steps=K, use_graph=graph, **extra) as sim:
+            st = sim.run(K)
+            U = sim.get_state()
+            lam, dphi = sim.dual_solver(w, want_dphi=True)
+            if graph and not store:
+                early = [np.array(x, copy=True) for x in sim.trajectory_state(s_or)]
+                lam_early, _ = sim.dual_solver(w, step=s_or)
+        out.append((st.mass, U, lam, dphi))
+    (m0, U0, l0, g0) = out[0]
+    for (m, U, lam, g) in out[1:]:
+        for x, y in zip(U, U0):
+            assert np.array_equal(x, y)
+        assert np.array_equal(lam, l0) and g == pytest.approx(g0, rel=1e-12)
+    # the flow stays off the domain boundary: mass is conserved
+    on_edge = (pb.table.nkind[:, :4] == -1).any(axis=1)
+    assert not U0[0][on_edge].any()
+    assert m0 == pytest.approx(float(np.sum(pb.h * dxy)), rel=1e-12)
+    assert U0[0].min() >= 0 and np.isfinite(l0).all() and np.abs(l0).max() > 0
+    _say(f"case adaptive {pb.n} leaves vs leaf oracle, {s_or} steps")
+    orc = oracle_lib.LeafOracle.from_problem(pb)
+    UN, _, lam_ref, _ = orc.forward_adjoint([pb.h, pb.hu, pb.hv], s_or, w)
+    for x, y in zip(early, UN):
+        assert np.array_equal(x, y)
+    assert np.array_equal(lam_early, lam_ref)
