@@ -140,6 +140,14 @@ class tg_leaf_table(C.Structure):
 
 
 
+class tg_adapt_params(C.Structure):
+    _fields_ = [("max_level", C.c_int), ("num_buffer_layer", C.c_int), ("tiny", C.c_double)]
+
+
+class tg_adapt_op(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("lev", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {
     "tg_config_default": (None, [C.POINTER(tg_config)]),
@@ -153,6 +161,7 @@ SIGNATURES = {
     "tg_set_mesh": (C.c_int, [C.c_void_p, C.POINTER(tg_mesh)]),
     "tg_set_state": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "tg_get_state": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "tg_set_clock": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "tg_set_ensemble": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(tg_mesh),
                                   C.POINTER(tg_params)]),
     "tg_num_members": (C.c_int, [C.c_void_p]),
@@ -206,6 +215,17 @@ SIGNATURES = {
     "tg_host_static_coefs": (C.c_int, [C.POINTER(tg_dem), C.c_int, C.c_int, C.c_double,
                                        C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_double, _dp, _dp, _dp]),
+    "tg_host_leaf_static_coefs": (C.c_int, [C.POINTER(tg_dem), C.c_int, _dp, C.c_double,
+                                            C.c_double, _dp, _dp]),
+    "tg_host_adapt": (C.c_int, [C.c_int, C.c_int, C.POINTER(tg_adapt_params), C.c_int, _i32p,
+                                _i32p, _i32p, _dp, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_int),
+                                _i32p, _i32p, _i32p, _dp, _dp, _dp, C.c_int,
+                                C.POINTER(C.c_int), C.POINTER(tg_adapt_op)]),
+    "tg_host_adapt_forward": (C.c_int, [C.c_int, C.POINTER(tg_adapt_op), C.c_int, C.c_int,
+                                        _i32p, _i32p, _i32p, _dp, C.c_int, _i32p, _i32p, _i32p,
+                                        _dp]),
+    "tg_host_adapt_adjoint": (C.c_int, [C.c_int, C.POINTER(tg_adapt_op), C.c_int, _i32p, _i32p,
+                                        _i32p, _dp, C.c_int, _i32p, _i32p, _i32p, _dp]),
     "tg_host_piles": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                 C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                 _dp, _dp]),

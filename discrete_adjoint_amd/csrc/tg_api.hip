@@ -112,6 +112,10 @@ struct tg_ctx {
   std::vector<MatDev> mtab_h;
   std::vector<int> mat_count;
 
+  // step index of the state tg_set_state gave (tg_set_clock): the run's
+  // step s is step step0 + s of the simulation (frict_tiny schedule)
+  int step0 = 0;
+
   // forward record
   int rec_steps = 0;  // steps in the dt record
   int run_steps = 0;  // steps of the last forward run/replay (0 = none)
@@ -254,7 +258,15 @@ void free_fields(tg_ctx* ctx) {
 }
 
 double frict_tiny(const tg_ctx* ctx, int s) {
-  return (s < ctx->cfg.frict_tiny_switch) ? ctx->cfg.frict_tiny_early : ctx->cfg.frict_tiny_late;
+  return (s + ctx->step0 < ctx->cfg.frict_tiny_switch) ? ctx->cfg.frict_tiny_early
+                                                       : ctx->cfg.frict_tiny_late;
+}
+
+// the first step index of a run at which frict_tiny takes its late value
+// (what captured sweeps depend on besides their length)
+long long frict_switch_local(const tg_ctx* ctx) {
+  const long long v = (long long)ctx->cfg.frict_tiny_switch - ctx->step0;
+  return v < 0 ? 0 : v;
 }
 
 void params_into(InstDev& I, const tg_params* p) {
@@ -792,8 +804,8 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
   };
   void** endU = nullptr;
   if (ctx->cfg.use_graph && !ctx->instrument) {
-    const long long key[8] = {nsteps, dt_seq != nullptr, (long long)(intptr_t)ctx->U[0],
-                              ctx->sel[0], ctx->sel[1], k, M,
+    const long long key[8] = {nsteps, (dt_seq != nullptr) + 2 * frict_switch_local(ctx),
+                              (long long)(intptr_t)ctx->U[0], ctx->sel[0], ctx->sel[1], k, M,
                               ctx->nbox_chunks > 0 ? ctx->box_version : 0};
     if (!same_key(ctx->gf, key)) {
       rc = capture(ctx, ctx->gf, key, [&]() { endU = body(); });
@@ -974,7 +986,8 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
       v.push_back((long long)(intptr_t)src);
       stage = fnv(v);
     }
-    const long long key[8] = {N, k, dphi, ctx->sel[0], ctx->sel[1], ctx->sel[2], s0, stage};
+    const long long key[8] = {N, k, dphi + 2 * frict_switch_local(ctx), ctx->sel[0], ctx->sel[1],
+                              ctx->sel[2], s0, stage};
     if (!same_key(ctx->ga, key)) {
       drop_segment_graphs(ctx);
       rc = capture(ctx, ctx->ga, key, [&]() {
@@ -1603,8 +1616,20 @@ int tg_set_state(tg_ctx* ctx, const double* h, const double* hu, const double* h
   clear_marks(ctx, ctx->U[0]);
   int rc = reset_time(ctx);
   if (rc) return rc;
+  ctx->step0 = 0;
   ctx->rec_steps = ctx->run_steps = 0;
   ctx->have_state = true;
+  return TG_OK;
+}
+
+int tg_set_clock(tg_ctx* ctx, int step0, double time) {
+  if (!ctx || step0 < 0 || !(time >= 0) || !std::isfinite(time)) return TG_ERR_ARG;
+  if (!ctx->have_state) return set_err(ctx, TG_ERR_STATE, "no state");
+  ctx->step0 = step0;
+  std::vector<double> t(ctx->ninst, time);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->time_d, t.data(), sizeof(double) * ctx->ninst,
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return TG_OK;
 }
 

@@ -107,6 +107,21 @@ void get_curvature(const Raster& xs, const Raster& ys, double x, double y, doubl
   *cy = bilinear(x1, y1, yy);
 }
 
+// calc_gravity_vector, element2.C:3545-3564: gravity of the surface with
+// slope zeta = (dz/dx, dz/dy)
+void gravity_of(const double zeta[2], double g[3]) {
+  const double max_slope = std::sqrt(zeta[0] * zeta[0] + zeta[1] * zeta[1]);
+  const double max_angle = std::atan(max_slope);
+  const double dsg = 9.8 * std::sin(max_angle);
+  if (std::fabs(dsg) > kTiny) {
+    g[0] = -dsg * zeta[0] / max_slope;
+    g[1] = -dsg * zeta[1] / max_slope;
+    g[2] = 9.8 * std::cos(max_angle);
+  } else {
+    g[0] = 0; g[1] = 0; g[2] = 9.8;
+  }
+}
+
 // splitmix64: deterministic, seedable, platform independent
 struct Rng {
   uint64_t s;
@@ -206,18 +221,8 @@ extern "C" int tg_host_static_coefs(const tg_dem* dem, int nx, int ny, double dx
       get_curvature(xs, ys, X, Y, &cv[0], &cv[1]);
       curv[c] = cv[0] * length_scale;
       curv[n + c] = cv[1] * length_scale;
-      // calc_gravity_vector, element2.C:3545-3564
-      double max_slope = std::sqrt(zeta[0] * zeta[0] + zeta[1] * zeta[1]);
-      double max_angle = std::atan(max_slope);
-      double dsg = 9.8 * std::sin(max_angle);
       double g[3];
-      if (std::fabs(dsg) > kTiny) {
-        g[0] = -dsg * zeta[0] / max_slope;
-        g[1] = -dsg * zeta[1] / max_slope;
-        g[2] = 9.8 * std::cos(max_angle);
-      } else {
-        g[0] = 0; g[1] = 0; g[2] = 9.8;
-      }
+      gravity_of(zeta, g);
       for (int k = 0; k < 3; k++) grav[k * n + c] = g[k] / gravity_scale;
     }
   // calc_d_gravity, element2.C:3574-3653 (one-sided where a neighbour is missing)
@@ -248,6 +253,33 @@ extern "C" int tg_host_static_coefs(const tg_dem* dem, int nx, int ny, double dx
         dgrav[dir * n + c] = out;
       }
     }
+  return 0;
+}
+
+extern "C" int tg_host_leaf_static_coefs(const tg_dem* dem, int n, const double* xc,
+                                         double length_scale, double gravity_scale,
+                                         double* grav, double* curv) {
+  if (!dem || !dem->elev || n < 0 || (n > 0 && (!xc || !grav || !curv))) return -1;
+  Raster e{dem->nrows, dem->ncols, dem->xmin, dem->ymax, dem->res,
+           std::vector<float>(dem->elev, dem->elev + (size_t)dem->nrows * dem->ncols)};
+  const double xmax = dem->xmin + (dem->ncols - 1) * dem->res;
+  const double ymin = dem->ymax - (dem->nrows - 1) * dem->res;
+  for (int c = 0; c < n; c++) {
+    const double X = xc[2 * c] * length_scale, Y = xc[2 * c + 1] * length_scale;
+    if (X < dem->xmin || X > xmax || Y < ymin || Y > dem->ymax) return -2;
+  }
+  Raster xs, ys;
+  horn_slopes(e, xs, ys);
+  for (int c = 0; c < n; c++) {
+    const double X = xc[2 * c] * length_scale, Y = xc[2 * c + 1] * length_scale;
+    double zeta[2], cv[2], g[3];
+    get_slope(xs, ys, X, Y, &zeta[0], &zeta[1]);
+    get_curvature(xs, ys, X, Y, &cv[0], &cv[1]);
+    curv[c] = cv[0] * length_scale;
+    curv[(size_t)n + c] = cv[1] * length_scale;
+    gravity_of(zeta, g);
+    for (int k = 0; k < 3; k++) grav[(size_t)k * n + c] = g[k] / gravity_scale;
+  }
   return 0;
 }
 

@@ -217,3 +217,87 @@ def refine_tree(nx0, ny0, max_level, want):
         ix.append(ii.astype(np.int32))
         iy.append(jj.astype(np.int32))
     return np.concatenate(lev), np.concatenate(ix), np.concatenate(iy)
+
+
+# ---- mesh adaptation (tg_host_adapt, tg_host_adapt_forward / _adjoint) ----------------
+
+@dataclass
+class AdaptPass:
+    """One adaptation pass: the leaves before and after (tree order: lev, ix,
+    iy arrays) and the split / merge operations in the order applied
+    (structured array kind, lev, ix, iy)."""
+    old: tuple
+    new: tuple
+    ops: np.ndarray
+
+    def forward(self, fields):
+        """The state transfer applied to fields on the old leaves (nf, n_old)
+        -> (nf, n_new)."""
+        f = np.ascontiguousarray(fields, dtype=np.float64)
+        nf = f.shape[0]
+        out = np.empty((nf, len(self.new[0])))
+        o, nw = [np.ascontiguousarray(a, dtype=np.int32) for a in self.old], \
+            [np.ascontiguousarray(a, dtype=np.int32) for a in self.new]
+        rc = _lib.load().tg_host_adapt_forward(
+            len(self.ops), _ops_ptr(self.ops), nf, len(o[0]), *[_lib.iptr(a) for a in o],
+            _lib.dptr(f.reshape(-1)), len(nw[0]), *[_lib.iptr(a) for a in nw],
+            _lib.dptr(out.reshape(-1)))
+        if rc:
+            raise ValueError(f"tg_host_adapt_forward returned {rc}")
+        return out
+
+    def adjoint(self, lam_new):
+        """The transposed transfer: adjoint (3, n_new) on the new leaves ->
+        (3, n_old) on the old ones."""
+        lam = np.ascontiguousarray(lam_new, dtype=np.float64).reshape(3, -1)
+        out = np.empty((3, len(self.old[0])))
+        o, nw = [np.ascontiguousarray(a, dtype=np.int32) for a in self.old], \
+            [np.ascontiguousarray(a, dtype=np.int32) for a in self.new]
+        rc = _lib.load().tg_host_adapt_adjoint(
+            len(self.ops), _ops_ptr(self.ops), len(nw[0]), *[_lib.iptr(a) for a in nw],
+            _lib.dptr(lam.reshape(-1)), len(o[0]), *[_lib.iptr(a) for a in o],
+            _lib.dptr(out.reshape(-1)))
+        if rc:
+            raise ValueError(f"tg_host_adapt_adjoint returned {rc}")
+        return out
+
+
+OP_DTYPE = np.dtype([("kind", np.int32), ("lev", np.int32), ("ix", np.int32), ("iy", np.int32)])
+
+
+def _ops_ptr(ops):
+    if len(ops) == 0:
+        return None
+    return ops.ctypes.data_as(C.POINTER(_lib.tg_adapt_op))
+
+
+def adapt(nx0, ny0, lev, ix, iy, U, max_level, num_buffer_layer=5, tiny=1e-4, el_error=None):
+    """One adaptation pass of the tree (lev, ix, iy) with state U = (h, hu,
+    hv) on its leaves: returns (lev, ix, iy, U) of the new tree and the
+    AdaptPass."""
+    lib = _lib.load()
+    lv, x, y = [np.ascontiguousarray(a, dtype=np.int32) for a in (lev, ix, iy)]
+    h, hu, hv = [np.ascontiguousarray(a, dtype=np.float64) for a in U]
+    err = None if el_error is None else np.ascontiguousarray(el_error, dtype=np.float64)
+    prm = _lib.tg_adapt_params(max_level, num_buffer_layer, tiny)
+    n = len(lv)
+    cap, op_cap = 4 * n + 64, 4 * n + 64
+    while True:
+        out = [np.empty(cap, dtype=np.int32) for _ in range(3)]
+        Uo = [np.empty(cap) for _ in range(3)]
+        ops = np.zeros(op_cap, dtype=OP_DTYPE)
+        n_out, nops = C.c_int(), C.c_int()
+        rc = lib.tg_host_adapt(nx0, ny0, C.byref(prm), n, _lib.iptr(lv), _lib.iptr(x),
+                               _lib.iptr(y), _lib.dptr(h), _lib.dptr(hu), _lib.dptr(hv),
+                               None if err is None else _lib.dptr(err), cap, C.byref(n_out),
+                               *[_lib.iptr(a) for a in out], *[_lib.dptr(a) for a in Uo],
+                               op_cap, C.byref(nops), _ops_ptr(ops))
+        if rc == -2:
+            cap, op_cap = max(cap, n_out.value), max(op_cap, nops.value)
+            continue
+        if rc:
+            raise ValueError(f"tg_host_adapt returned {rc}")
+        m = n_out.value
+        new = tuple(a[:m].copy() for a in out)
+        return (*new, tuple(a[:m].copy() for a in Uo),
+                AdaptPass((lv, x, y), new, ops[:nops.value].copy()))

@@ -56,6 +56,13 @@ class Simulation:
                                   device=device, **cfg_overrides)
         self.ctx = C.c_void_p()
         self._check(self.lib.tg_create(C.byref(self.cfg), C.byref(self.ctx)), nocontext=True)
+        self.set_problem(problem)
+
+    def set_problem(self, problem):
+        """Hands a (new) mesh with its static coefficients, materials and
+        initial state to the context (tg_set_mesh: a re-mesh keeps the
+        context's configuration)."""
+        self.problem = problem
         self._arrays = []
         m = _lib.tg_mesh()
         if hasattr(problem, "table"):  # adaptive mesh: the flattened leaf tables
@@ -82,6 +89,10 @@ class Simulation:
         self._set_materials(0, problem)
         self.set_params(problem.phi_int_deg, problem.phi_bed_deg)
         self.set_state(problem.h, problem.hu, problem.hv)
+
+    def set_clock(self, step0, time):
+        """The state is U^{step0} at simulated time `time` (tg_set_clock)."""
+        self._check(self.lib.tg_set_clock(self.ctx, int(step0), float(time)))
 
     # -- plumbing -------------------------------------------------------------
     def _set_materials(self, member, problem):

@@ -204,6 +204,14 @@ int tg_set_materials(tg_ctx* ctx, int member, int nmat, const tg_material* mats)
 int tg_set_state(tg_ctx* ctx, const double* h, const double* hu, const double* hv);
 int tg_get_state(tg_ctx* ctx, double* h, double* hu, double* hv);
 
+/* The simulation clock of the state just set (tg_set_state resets it to step
+ * 0, time 0): the state is U^{step0} at simulated time `time`, so a run's
+ * step s is the simulation's step step0 + s (frict_tiny schedule,
+ * hpfem.C:232-235) and incrtime clamps against time + elapsed
+ * (properties.h:397-410).  A time loop that re-meshes (hpfem.C:244-258)
+ * hands each mesh's segment over with its clock. */
+int tg_set_clock(tg_ctx* ctx, int step0, double time);
+
 /* ---- ensembles ----------------------------------------------------------
  * Several independent uniform blocks (members, e.g. samples of a stochastic
  * study, lhsbed.C / lhsvol.C) advanced together: every kernel launch covers

@@ -41,6 +41,17 @@ int tg_host_static_coefs(const tg_dem* dem, int nx, int ny, double dx, double dy
                          double x0, double y0, double length_scale, double gravity_scale,
                          double* grav, double* dgrav, double* curv);
 
+/* Gravity vector and curvature of the leaves of an adaptive mesh (or any
+ * list of points) at their centres xc [2n] (x, y interleaved): the
+ * per-element set-up of tg_host_static_coefs (calc_topo_data,
+ * element2.C:3655-3703; calc_gravity_vector :3545-3564) at arbitrary centres,
+ * as the reference evaluates it for every new son and father
+ * (element2.C:238-240, 524-526).  grav [3n], curv [2n] FP64 SoA; d gravity_z
+ * comes from tg_host_leaf_d_gravity on the flattened table.
+ * Returns 0; -1 bad argument; -2 a centre outside the raster. */
+int tg_host_leaf_static_coefs(const tg_dem* dem, int n, const double* xc, double length_scale,
+                              double gravity_scale, double* grav, double* curv);
+
 /* Elliptical paraboloid piles (elliptical_pile_height, init_piles.C:167-277,
  * PARABALOID branch): each cell takes the 9-node weighted average
  * (corners + 2*edges + 4*centre)/16 of max-over-piles heights. Rotation in
@@ -123,6 +134,68 @@ int tg_host_flatten(int nx0, int ny0, double xmin, double ymin, double dx0, doub
  * gravity_z; dgrav FP64 SoA [2n].  Returns 0; -1 bad argument. */
 int tg_host_leaf_d_gravity(int n, const int32_t* nbr, const double* xc, const double* gz,
                            double* dgrav);
+
+/* ---- mesh adaptation ------------------------------------------------------ */
+
+/* One adaptation pass (H_adapt then unrefine, as hpfem.C:244-258 runs them
+ * every few steps) of a 2:1-balanced quadtree over an nx0 x ny0 base grid,
+ * given by its leaves (leaf e = cell (ix[e], iy[e]) of level lev[e]) and
+ * their state:
+ *  - refinement (hadpt.C:50-323): every leaf below max_level on one of the
+ *    pile contours h = tiny, 5 tiny, 15 tiny, 40 tiny (if_pile_boundary,
+ *    element2.C:4020-4065) -- or, with el_error, whose error exceeds the mean
+ *    error of the leaves with hu > tiny (element_weight.C:36-79) -- is split,
+ *    coarser leaves across its sides first (2:1 cascade); then
+ *    num_buffer_layer buffer layers around the contours (if_first_ and
+ *    if_next_buffer_boundary, element2.C:4136-4220), refining where a
+ *    buffer leaf is not yet at max_level;
+ *  - unrefinement (unrefine.C:54-352): four brothers that were neither split
+ *    nor made buffer in this pass and none of which has a finer neighbour
+ *    become their father.
+ * A son takes its father's state (element2.C:246-250), a father the sum of
+ * 0.25 x its sons' states in son order (element2.C:527-532).  Sons of a leaf
+ * are numbered from its corners: 0 SW, 1 SE, 2 NE, 3 NW.  No mass sources.
+ * Output: the new leaves (the input order, a split leaf replaced by its sons,
+ * a father at its son 0's place) and state, and the operations in the order
+ * applied (for tg_host_adapt_forward / tg_host_adapt_adjoint).
+ * Returns 0; -1 bad argument; -2 cap / op_cap too small (*n_out, *nops give
+ * the sizes needed); -3 the input leaves do not form a 2:1-balanced tree. */
+typedef struct tg_adapt_params {
+  int max_level;         /* REFINE_LEVEL (datread.C:693-696), at most 28    */
+  int num_buffer_layer;  /* 5 in the reference's time loop (hpfem.C:247)   */
+  double tiny;           /* GEOFLOW_TINY                                   */
+} tg_adapt_params;
+/* kind 0: leaf (lev, ix, iy) split into its four sons; kind 1: the four
+ * sons of (lev, ix, iy) merged into it. */
+typedef struct tg_adapt_op {
+  int32_t kind, lev, ix, iy;
+} tg_adapt_op;
+int tg_host_adapt(int nx0, int ny0, const tg_adapt_params* prm, int n, const int32_t* lev,
+                  const int32_t* ix, const int32_t* iy, const double* h, const double* hu,
+                  const double* hv, const double* el_error, int cap, int* n_out,
+                  int32_t* lev_out, int32_t* ix_out, int32_t* iy_out, double* h_out,
+                  double* hu_out, double* hv_out, int op_cap, int* nops, tg_adapt_op* ops);
+
+/* The state transfer of a pass applied to any fields: f_old [nf * n_old]
+ * (nf SoA fields on the old leaves) -> f_new [nf * n_new] on the new leaves,
+ * with the rules of tg_host_adapt.  Returns 0; -1 bad argument; -3 the
+ * leaves and the operations do not match. */
+int tg_host_adapt_forward(int nops, const tg_adapt_op* ops, int nf, int n_old,
+                          const int32_t* lev_old, const int32_t* ix_old, const int32_t* iy_old,
+                          const double* f_old, int n_new, const int32_t* lev_new,
+                          const int32_t* ix_new, const int32_t* iy_new, double* f_new);
+
+/* Its transpose, the adjoint transfer of the reverse sweep across the pass
+ * (dual_refine_unrefine.C:20-450 with the DualElem constructors,
+ * dualmesh.C:708-709, 986-993): from lambda on the new leaves (SoA [3 n_new])
+ * to lambda on the old ones -- a split leaf takes the sum of its sons'
+ * adjoints in son order, each son of a merge 0.25 of its father's -- with
+ * the operations undone in reverse order.  Returns 0; -1 bad argument; -3
+ * mismatch. */
+int tg_host_adapt_adjoint(int nops, const tg_adapt_op* ops, int n_new, const int32_t* lev_new,
+                          const int32_t* ix_new, const int32_t* iy_new, const double* lam_new,
+                          int n_old, const int32_t* lev_old, const int32_t* ix_old,
+                          const int32_t* iy_old, double* lam_old);
 
 /* Trajectory spill file of one step (SolRec::wrtie_sol_to_disk,
  * dualmesh.C:87-142; the reference names it solution_<rank %04d>_<step %08d>):

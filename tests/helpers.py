@@ -121,3 +121,39 @@ def with_materials(pb, phis, split=0.5):
     q.mat = (x >= split * L).astype(np.uint8)
     q.materials = tuple(phis)
     return q
+
+
+def small_adaptive_run(nx0=12, ny0=10, max_level=2, adapt_every=4, num_buffer_layer=2,
+                       slope=25.0, phi_bed=15.0):
+    """An adaptive-run set-up (discrete_adjoint_amd.adaptive.AdaptiveRun) on a
+    plane: the small_leaf_flow_problem pile on a tree refined around it,
+    constant gravity and curvature at every leaf centre."""
+    from discrete_adjoint_amd import adaptive, mesh
+    dx0 = 20.0
+    L = nx0 * dx0
+    lev, ix, iy = mesh.refine_tree(nx0, ny0, max_level, disc(0.35 * nx0, 0.5 * ny0, 2.0))
+    s = dx0 / (1 << lev)
+    X, Y = (ix + 0.5) * s, (iy + 0.5) * s
+    h = paraboloid(0.35 * L, 0.5 * ny0 * dx0, 50.0, 40.0, 6.0)(X, Y)
+    U = np.stack([h, np.zeros_like(h), np.zeros_like(h)])
+    base = analytic_problem(1, 1, dx0, slope_x_deg=slope, slope_y_deg=3.0, curv=(1e-3, -5e-4))
+
+    def statics(xc):
+        n = len(xc)
+        return np.repeat(base.grav, n, axis=1), np.repeat(base.curv, n, axis=1)
+
+    return adaptive.AdaptiveRun((nx0, ny0, 0.0, 0.0, dx0, dx0), (lev, ix, iy), U, statics, 32.0,
+                                phi_bed, max_level, adapt_every=adapt_every,
+                                num_buffer_layer=num_buffer_layer, checkpoint_every=3)
+
+
+def box_weights_of(run, tree, box):
+    """Weights (3, n) of 'h integrated over the box' on a tree (tree order)."""
+    nx0, ny0, xmin, ymin, dx0, dy0 = run.geometry
+    lev, ix, iy = tree
+    s = 1.0 / (1 << np.asarray(lev))
+    X, Y = xmin + (ix + 0.5) * dx0 * s, ymin + (iy + 0.5) * dy0 * s
+    inside = (X > box[0]) & (X < box[1]) & (Y > box[2]) & (Y < box[3])
+    w = np.zeros((3, len(lev)))
+    w[0, inside] = (dx0 * s * dy0 * s)[inside]
+    return w
