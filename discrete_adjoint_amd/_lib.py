@@ -215,6 +215,8 @@ SIGNATURES = {
     "tg_host_static_coefs": (C.c_int, [C.POINTER(tg_dem), C.c_int, C.c_int, C.c_double,
                                        C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_double, _dp, _dp, _dp]),
+    "tg_host_read_grass": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                     C.c_longlong, C.POINTER(tg_dem), _fp]),
     "tg_host_leaf_static_coefs": (C.c_int, [C.POINTER(tg_dem), C.c_int, _dp, C.c_double,
                                             C.c_double, _dp, _dp]),
     "tg_host_adapt": (C.c_int, [C.c_int, C.c_int, C.POINTER(tg_adapt_params), C.c_int, _i32p,

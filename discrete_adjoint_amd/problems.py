@@ -65,6 +65,22 @@ def _dem(nrows, ncols, xmin, ymax, res, elev):
     return d
 
 
+def read_grass(gisdbase, location, mapset, name):
+    """A GRASS elevation raster (tg_host_read_grass) as the DEM arguments
+    (nrows, ncols, xmin, ymax, res, elev float32) of static_coefs."""
+    lib = _lib.load()
+    d = _lib.tg_dem()
+    args = [str(a).encode() for a in (gisdbase, location, mapset, name)]
+    rc = lib.tg_host_read_grass(*args, 0, C.byref(d), None)
+    if rc not in (0, -2):
+        raise ValueError(f"tg_host_read_grass returned {rc}")
+    elev = np.empty(d.nrows * d.ncols, dtype=np.float32)
+    rc = lib.tg_host_read_grass(*args, len(elev), C.byref(d), _lib.fptr(elev))
+    if rc:
+        raise ValueError(f"tg_host_read_grass returned {rc}")
+    return (d.nrows, d.ncols, d.xmin, d.ymax, d.res, elev)
+
+
 def static_coefs(dem_args, nx, ny, dx, dy, x0, y0):
     lib = _lib.load()
     n = nx * ny

@@ -21,6 +21,22 @@ typedef struct tg_dem {
   const float* elev;    /* [nrows*ncols], row-major */
 } tg_dem;
 
+/* Reads a GRASS elevation raster (the reference's GIS input, GisApi.C:153-285):
+ * <gisdbase>/<location>/<mapset>/cellhd/<name> (proj, zone, north, south,
+ * east, west, cols, rows, e-w resol, n-s resol, format, compressed -- read by
+ * position, GisRasterHdr.C:26-60) and .../fcell/<name>: big-endian float32
+ * rows, raw or compressed (byte 0 = 4 or 8, the size of the rows+1 big-endian
+ * row offsets that follow; a row starting with '1' is zlib-deflated, any
+ * other flag byte stores it raw; GisBinFile.C:131-228).  Fills dem (nrows,
+ * ncols, res = e-w resolution, xmin = west, ymax = south + rows * n-s
+ * resolution as set_from_header does, GisApi.C:1455-1480; dem->elev = elev)
+ * and, when capacity >= rows * cols, elev [rows * cols] (row 0 north).
+ * Returns 0; -1 bad argument; -2 capacity too small (dem holds the size);
+ * -4 missing or malformed header (also: e-w and n-s resolutions differ by
+ * a tenth or more); -5 missing, truncated or undecodable data. */
+int tg_host_read_grass(const char* gisdbase, const char* location, const char* mapset,
+                       const char* name, long long capacity, tg_dem* dem, float* elev);
+
 /* Analytic DEM: a plane of slope angle slope_deg falling towards +x that
  * ends at x_toe and blends into a flat runout with a C1 cosine transition of
  * total width blend (no reference counterpart; synthetic input). */
