@@ -148,6 +148,28 @@ class tg_adapt_op(C.Structure):
     _fields_ = [("kind", C.c_int32), ("lev", C.c_int32), ("ix", C.c_int32), ("iy", C.c_int32)]
 
 
+class tg_pile(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("height", "xc", "yc", "major", "minor", "rot_deg",
+                                          "cosrot", "sinrot", "vx", "vy")]
+
+
+class tg_run_inputs(C.Structure):
+    _fields_ = [("srctype", C.c_int), ("npiles", C.c_int), ("nsources", C.c_int),
+                ("length_scale", C.c_double), ("height_scale", C.c_double),
+                ("gravity_scale", C.c_double), ("epsilon", C.c_double),
+                ("time_scale", C.c_double), ("smallest_axis", C.c_double),
+                ("cells_across", C.c_int), ("maxiter", C.c_int), ("maxtime", C.c_double),
+                ("timeoutput", C.c_double), ("timesave", C.c_double),
+                ("nd_maxtime", C.c_double), ("nd_nextoutput", C.c_double),
+                ("nd_nextsave", C.c_double), ("adaptflag", C.c_int), ("vizflag", C.c_int),
+                ("order", C.c_int), ("gis_main", C.c_char * 300), ("gis_sub", C.c_char * 100),
+                ("gis_mapset", C.c_char * 100), ("gis_map", C.c_char * 100),
+                ("extramaps", C.c_int), ("edge_height", C.c_double),
+                ("test_height", C.c_double), ("test_x", C.c_double), ("test_y", C.c_double),
+                ("nplanes", C.c_int), ("nmat", C.c_int), ("matnames", (C.c_char * 64) * 16),
+                ("intfrict", C.c_double), ("bedfrict", C.c_double * 16)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {
     "tg_config_default": (None, [C.POINTER(tg_config)]),
@@ -217,6 +239,8 @@ SIGNATURES = {
                                        C.c_double, _dp, _dp, _dp]),
     "tg_host_read_grass": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                      C.c_longlong, C.POINTER(tg_dem), _fp]),
+    "tg_host_read_run_inputs": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(tg_pile),
+                                          C.POINTER(tg_run_inputs)]),
     "tg_host_leaf_static_coefs": (C.c_int, [C.POINTER(tg_dem), C.c_int, _dp, C.c_double,
                                             C.c_double, _dp, _dp]),
     "tg_host_adapt": (C.c_int, [C.c_int, C.c_int, C.POINTER(tg_adapt_params), C.c_int, _i32p,

@@ -65,6 +65,36 @@ def _dem(nrows, ncols, xmin, ymax, res, elev):
     return d
 
 
+def read_run_inputs(directory):
+    """The reference's run inputs (tg_host_read_run_inputs: simulation.data,
+    scale.data, frict.data in `directory`), non-dimensionalised: returns
+    (fields dict, list of pile dicts)."""
+    lib = _lib.load()
+    out = _lib.tg_run_inputs()
+    cap = 64
+    while True:
+        piles = (_lib.tg_pile * cap)()
+        rc = lib.tg_host_read_run_inputs(str(directory).encode(), cap, piles, C.byref(out))
+        if rc == -2:
+            cap = out.npiles
+            continue
+        if rc:
+            raise ValueError(f"tg_host_read_run_inputs returned {rc}")
+        break
+    f = {}
+    for name, _ in _lib.tg_run_inputs._fields_:
+        v = getattr(out, name)
+        if name == "matnames":
+            v = [bytes(v[i].value).decode() for i in range(out.nmat)]
+        elif name == "bedfrict":
+            v = list(v[: out.nmat])
+        elif isinstance(v, bytes):
+            v = v.decode()
+        f[name] = v
+    pl = [{n: getattr(piles[i], n) for n, _ in _lib.tg_pile._fields_} for i in range(out.npiles)]
+    return f, pl
+
+
 def read_grass(gisdbase, location, mapset, name):
     """A GRASS elevation raster (tg_host_read_grass) as the DEM arguments
     (nrows, ncols, xmin, ymax, res, elev float32) of static_coefs."""

@@ -37,6 +37,47 @@ typedef struct tg_dem {
 int tg_host_read_grass(const char* gisdbase, const char* location, const char* mapset,
                        const char* name, long long capacity, tg_dem* dem, float* elev);
 
+/* ---- run inputs -------------------------------------------------------------
+ * The reference's input files (Read_data, datread.C:36-451, written by its
+ * GUI): <dir>/simulation.data, <dir>/scale.data (absent: no scaling) and
+ * <dir>/frict.data, read token by token in the reference's order and
+ * non-dimensionalised as Read_data does (lengths by the length scale, heights
+ * by the height scale, speeds by sqrt(L g), times by sqrt(L / g)).  Flux
+ * sources and discharge planes are counted and skipped (this path has no
+ * mass sources).  piles [pile_cap]: the piles (tg_host_piles takes their
+ * centres, radii, rotation and height).
+ * Returns 0; -1 bad argument; -2 more piles than pile_cap (out->npiles holds
+ * the count); -4 simulation.data or frict.data missing; -5 malformed. */
+#define TG_HOST_MAX_MATERIALS 16
+typedef struct tg_pile {
+  double height, xc, yc, major, minor;  /* non-dimensional                     */
+  double rot_deg, cosrot, sinrot;       /* counter-clockwise rotation          */
+  double vx, vy;                        /* initial velocity, non-dimensional   */
+} tg_pile;
+typedef struct tg_run_inputs {
+  int srctype, npiles, nsources;
+  double length_scale, height_scale, gravity_scale;
+  double epsilon;          /* height / length scale                           */
+  double time_scale;       /* sqrt(length / gravity scale)                    */
+  double smallest_axis;    /* 2 x the smallest pile radius (non-dimensional)  */
+  int cells_across;        /* cells across the smallest pile axis             */
+  int maxiter;
+  double maxtime, timeoutput, timesave;          /* as read (seconds)          */
+  double nd_maxtime, nd_nextoutput, nd_nextsave;  /* non-dimensional (tg_params) */
+  int adaptflag, vizflag, order;
+  char gis_main[300], gis_sub[100], gis_mapset[100], gis_map[100];
+  int extramaps;
+  double edge_height, test_height, test_x, test_y;  /* non-dimensional; no test
+                                                      point: HUGE_VAL          */
+  int nplanes;
+  int nmat;                /* materials in frict.data                         */
+  char matnames[TG_HOST_MAX_MATERIALS][64];
+  double intfrict;         /* internal friction angle (radians), material 1's */
+  double bedfrict[TG_HOST_MAX_MATERIALS];  /* bed friction angles (radians) of
+                                              materials 1..nmat at [0..nmat-1] */
+} tg_run_inputs;
+int tg_host_read_run_inputs(const char* dir, int pile_cap, tg_pile* piles, tg_run_inputs* out);
+
 /* Analytic DEM: a plane of slope angle slope_deg falling towards +x that
  * ends at x_toe and blends into a flat runout with a C1 cosine transition of
  * total width blend (no reference counterpart; synthetic input). */
