@@ -80,13 +80,23 @@ def test_ensemble_256_members(oracle_lib):
 # exact and holds lambda^0 and dJ/dphi_bed to 1e-6; TG_MIXED runs FP32
 # increments through 1500 steps of wetting / drying and Coulomb switches
 # and is held to the bounds below (h relative L2 / Linf, functional, lambda^0
-# relative L2, dJ/dphi_bed).
-MIXED_BOUNDS = {"h_l2": 5e-2, "h_linf": 0.3, "hu_l2": 0.1, "hv_l2": 0.1, "J": 5e-3,
-                "lam_l2": 0.3, "dphi": 5e-2}
+# relative L2, dJ/dphi_bed; the state at step 100).  The momenta are the
+# sensitive part: the piles start at rest, and with frict_tiny = 0.1 over the
+# first 50 steps the friction signs of slow cells switch on thresholds that
+# FP32 increments cross differently; by step 1500 most of the mass has stopped
+# and the momenta left are a small residue (L2 437 against 1469 at step 100)
+# decided cell by cell by the stopping test.  So the momenta are held to
+# 0.2 / 0.1 at step 100 and at step 1500 only to stay of the same size (1.5).
+# Measured on MI355X: h L2 1.2e-2, h Linf 8.6e-2, hu / hv L2 0.89 / 0.56, J
+# 7.1e-5, lambda^0 2.9e-2, dJ/dphi 1.1e-3; at step 100 h 2.7e-3, hu 9.3e-2,
+# hv 3.8e-2.
+MIXED_BOUNDS = {"h_l2": 3e-2, "h_linf": 0.2, "hu_l2": 1.5, "hv_l2": 1.5, "J": 1e-3,
+                "lam_l2": 0.1, "dphi": 1e-2, "h_l2@100": 1e-2, "hu_l2@100": 0.2,
+                "hv_l2@100": 0.1}
 
 
 def test_grid_2048_precision_modes():
-    n, K, k = 2048, 1500, 50
+    n, K, k, s_early = 2048, 1500, 50, 100
     pb = problems.fractal_problem(n=n, dx=5120.0 / n, nsteps=K, npiles=3, seed=11, dem_seed=1,
                                   checkpoint_every=k)
     res = {}
@@ -100,22 +110,26 @@ def test_grid_2048_precision_modes():
                 box = problems.target_box(pb, H[0], cells=20, seed=5)
                 w = pb.box_weights(box)
             lam, dphi = sim.dual_solver(w, want_dphi=True)
-        res[mode] = (H, float(np.sum(w[:pb.n] * H[0])), lam, float(dphi))
-    H64, J64, lam64, g64 = res[_lib.TG_FP64]
+            early = [np.array(x, copy=True) for x in sim.trajectory_state(s_early)]
+        res[mode] = (H, float(np.sum(w[:pb.n] * H[0])), lam, float(dphi), early)
+    H64, J64, lam64, g64, E64 = res[_lib.TG_FP64]
     assert J64 > 0 and np.isfinite(lam64).all()
-    H, J, lam, g = res[_lib.TG_FWD64_ADJ32]
+    _say("fp64 momentum L2 at step %d / %d: %.4g / %.4g" % (
+        s_early, K, np.linalg.norm(E64[1]), np.linalg.norm(H64[1])))
+    H, J, lam, g, E = res[_lib.TG_FWD64_ADJ32]
     for x, y in zip(H, H64):
         assert np.array_equal(x, y)
     assert J == J64
     e_lam, e_g = _rel(lam, lam64), abs(g - g64) / abs(g64)
     _say(f"adj32 errors: lambda0 L2 {e_lam:.3g}, dJ/dphi {e_g:.3g}")
     assert e_lam <= 1e-6 and e_g <= 1e-6
-    H, J, lam, g = res[_lib.TG_MIXED]
+    H, J, lam, g, E = res[_lib.TG_MIXED]
     err = {"h_l2": _rel(H[0], H64[0]),
            "h_linf": float(np.abs(H[0] - H64[0]).max() / np.abs(H64[0]).max()),
            "hu_l2": _rel(H[1], H64[1]), "hv_l2": _rel(H[2], H64[2]),
            "J": abs(J - J64) / abs(J64), "lam_l2": _rel(lam, lam64),
-           "dphi": abs(g - g64) / abs(g64)}
+           "dphi": abs(g - g64) / abs(g64), "h_l2@100": _rel(E[0], E64[0]),
+           "hu_l2@100": _rel(E[1], E64[1]), "hv_l2@100": _rel(E[2], E64[2])}
     _say("mixed errors: " + ", ".join(f"{a} {v:.3g}" for a, v in err.items()))
     for a, v in err.items():
         assert v <= MIXED_BOUNDS[a], (a, v)

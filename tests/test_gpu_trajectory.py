@@ -106,12 +106,12 @@ def test_ensemble_states(oracle_lib):
 
 
 def test_store_that_does_not_fit_falls_back_to_checkpoints():
-    # a stored trajectory larger than the device (3000 states of a 2048^2
-    # block, ~300 GB) cannot be allocated: the run keeps checkpoints only,
-    # the failed allocation is not reported as the run's error, and results
-    # equal a run that asked for no store
+    # a stored trajectory larger than the device (4001 states of a 2048^2
+    # block, 403 GB against 288 GiB = 309 GB of HBM) cannot be allocated:
+    # the run keeps checkpoints only, the failed allocation is not reported
+    # as the run's error, and results equal a run that asked for no store
     from helpers import analytic_problem, paraboloid
-    n, N = 2048, 3000
+    n, N = 2048, 4000
     pb = analytic_problem(n, n, 5.0, slope_x_deg=20.0, h=paraboloid(2000.0, 5120.0, 60.0, 50.0,
                                                                    6.0), nsteps=N)
     w = np.zeros((3, pb.n))
