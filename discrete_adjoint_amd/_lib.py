@@ -21,6 +21,8 @@ TG_ERR_NONFINITE = -5
 TG_ERR_EIGEN = -6
 TG_ERR_UNSUPPORTED = -7
 TG_ERR_NODEVICE = -8
+TG_ERR_TOO_LARGE = -9
+TG_NOALLOC_MAX_CELLS = 1 << 20
 
 TG_FP64 = 0
 TG_MIXED = 1
@@ -140,6 +142,11 @@ class tg_leaf_table(C.Structure):
 
 
 
+class tg_noalloc_block(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("statics", C.c_void_p)]
+
+
 class tg_adapt_params(C.Structure):
     _fields_ = [("max_level", C.c_int), ("num_buffer_layer", C.c_int), ("tiny", C.c_double)]
 
@@ -226,6 +233,11 @@ SIGNATURES = {
                                   C.POINTER(C.c_int)]),
     "tg_autotune": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.POINTER(C.c_int),
                               _dp]),
+    "tg_step_noalloc": (C.c_int, [C.POINTER(tg_config), C.POINTER(tg_noalloc_block),
+                                  C.POINTER(tg_params), C.c_int, C.c_double, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, _dp]),
+    "tg_noalloc_scratch_bytes": (C.c_longlong, []),
     "tg_get_tile_selection": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "tg_set_tile_selection": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     # tg_host.h

@@ -30,7 +30,9 @@ typedef enum tg_status {
   TG_ERR_NONFINITE = -5,    /* NaN/Inf in the state (residual.C:137-147)     */
   TG_ERR_EIGEN = -6,        /* eigenvalue > 1e9 (get_coef_and_eigen.C:144)   */
   TG_ERR_UNSUPPORTED = -7,  /* mesh kind / mode not built                    */
-  TG_ERR_NODEVICE = -8      /* no HIP device                                 */
+  TG_ERR_NODEVICE = -8,     /* no HIP device                                 */
+  TG_ERR_TOO_LARGE = -9     /* block larger than tg_step_noalloc's fixed
+                               scratch (TG_NOALLOC_MAX_CELLS)                */
 } tg_status;
 
 typedef enum tg_precision {
@@ -315,6 +317,43 @@ int tg_adjoint_step(tg_ctx* ctx, const tg_params* prm, int s, double dt,
  * (dJ_dphibed [number of members]); the terms tg_adjoint_run sums. */
 int tg_adjoint_step_grad(tg_ctx* ctx, const tg_params* prm, int s, double dt,
                          const double* lambda_next, double* lambda_out, double* dJ_dphibed);
+
+/* ---- one step without allocation ----------------------------------------------
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  tg_step_noalloc is
+ * tg_forward_step (one order-1 step, step.C:29-275) on caller-owned device
+ * arrays with every intermediate -- kact, the CFL reduction, dt, the tile
+ * tables and the update's tile list -- in a fixed device array of the
+ * library (TG_NOALLOC_MAX_CELLS cells at most, about 9 MiB per device, held
+ * until the process ends).  It allocates and frees nothing.  Results equal
+ * tg_forward_step's on the same state bit for bit.  Calls on one device share
+ * the array: calls that overlap on one device race (the call synchronises the
+ * stream before it returns).
+ *   blk:     nx x ny uniform block, cell spacing dx, dy; statics = device
+ *            array [7 * nx * ny] FP64: gravity x | y | z, d gravity_z dx | dy,
+ *            curvature x | y (tg_mesh grav, dgrav, curv in that order);
+ *   cfg:     thresholds, CFL factors, frict_tiny schedule and device
+ *            (tg_config; other fields ignored);
+ *   step:    step index s (selects frict_tiny); time: simulated time before
+ *            the step (incrtime clamp against prm's maxtime / output / save);
+ *   d_h ...: device arrays [nx * ny] of U^s; d_h_out ...: U^{s+1} (distinct
+ *            arrays); stream: a hipStream_t or NULL (the null stream);
+ *   dt_out:  the dt taken (host).
+ * Returns TG_OK, TG_ERR_ARG, TG_ERR_TOO_LARGE (more than
+ * TG_NOALLOC_MAX_CELLS cells), TG_ERR_NONFINITE, TG_ERR_EIGEN or TG_ERR_HIP. */
+#define TG_NOALLOC_MAX_CELLS (1 << 20)
+typedef struct tg_noalloc_block {
+  int nx, ny;
+  double dx, dy;
+  const double* statics;
+} tg_noalloc_block;
+int tg_step_noalloc(const tg_config* cfg, const tg_noalloc_block* blk, const tg_params* prm,
+                    int step, double time, const double* d_h, const double* d_hu,
+                    const double* d_hv, double* d_h_out, double* d_hu_out, double* d_hv_out,
+                    void* stream, double* dt_out);
+/* Bytes of the fixed device array tg_step_noalloc uses. */
+long long tg_noalloc_scratch_bytes(void);
 
 /* ---- timing (HIP events around the context's kernels) ---------------------- */
 
