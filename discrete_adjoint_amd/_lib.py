@@ -142,6 +142,14 @@ class tg_leaf_table(C.Structure):
 
 
 
+class tg_error_grid(C.Structure):
+    _fields_ = [("n", C.c_int), ("nbr", C.POINTER(C.c_int32)), ("nproc", C.POINTER(C.c_int32)),
+                ("gen", C.POINTER(C.c_int32)), ("cdx", _dp), ("xc", _dp), ("grav", _dp),
+                ("dgrav", _dp), ("curv", _dp), ("mat", C.POINTER(C.c_uint8)),
+                ("father", C.POINTER(C.c_int32)), ("kind", C.POINTER(C.c_int32)),
+                ("pts", C.POINTER(C.c_int32)), ("pxy", _dp)]
+
+
 class tg_noalloc_block(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
                 ("statics", C.c_void_p)]
@@ -184,6 +192,10 @@ SIGNATURES = {
                                         C.POINTER(tg_params)]),
     "tg_material_from_angles": (C.c_int, [C.c_double, C.c_double, C.POINTER(tg_material)]),
     "tg_set_materials": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(tg_material)]),
+    "tg_set_error_grid": (C.c_int, [C.c_void_p, C.POINTER(tg_error_grid)]),
+    "tg_error_indicator": (C.c_int, [C.c_void_p, C.POINTER(tg_params), C.c_int, C.c_double, _dp,
+                                     _dp, _dp, _dp, _dp, _dp, _dp]),
+    "tg_error_last_ms": (C.c_int, [C.c_void_p, _dp]),
     "tg_create": (C.c_int, [C.POINTER(tg_config), C.POINTER(C.c_void_p)]),
     "tg_destroy": (None, [C.c_void_p]),
     "tg_last_error": (C.c_char_p, [C.c_void_p]),
@@ -279,6 +291,8 @@ SIGNATURES = {
                                       _i32p]),
     "tg_host_traversal": (C.c_int, [C.c_int, _u32p, C.c_double, C.c_double, C.c_int,
                                     _i32p]),
+    "tg_host_error_stencil": (C.c_int, [C.c_int, _i32p, _i32p, _dp, _dp, _i32p, _i32p, C.c_int,
+                                        _i32p, _i32p, _dp]),
     "tg_host_flatten": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                   C.c_double, C.c_int, _i32p, _i32p, _i32p, C.c_int,
                                   C.POINTER(tg_leaf_table)]),

@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -102,6 +103,25 @@ struct tg_ctx {
   double* leaf_cdx = nullptr;
   double* leaf_xc = nullptr;
   LeafBufs lb{};
+
+  // error grid (tg_set_error_grid): its tables, static coefficients and
+  // interpolation stencils, and the buffers of one tg_error_indicator call
+  int err_n = 0;
+  LeafDev err_leaf{};
+  int* err_nbr = nullptr;
+  uint32_t* err_meta = nullptr;
+  double* err_cdx = nullptr;
+  double* err_xc = nullptr;
+  double* err_statics = nullptr;  // 7 * err_n: gx, gy, gz, dgx, dgy, kx, ky
+  uint8_t* err_mat = nullptr;
+  int* err_father = nullptr;
+  int* err_kind = nullptr;
+  int* err_pts = nullptr;
+  double* err_pxy = nullptr;
+  double* err_fwd = nullptr;  // 9 * ncell: previous state, state, adjoint
+  double* err_buf = nullptr;  // 15 * err_n: reconstruction 9, kact 1, residual 3, error, correction
+  hipEvent_t err_ev[4] = {};
+  double err_ms[3] = {-1., -1., -1.};
 
   // per-cell materials (tg_mesh.mat, tg_set_materials): the cells' material
   // index (null: one material per member) and the material table, MAX_MAT
@@ -201,6 +221,23 @@ void drop_segment_graphs(tg_ctx* ctx) {
   ctx->gs.clear();
 }
 
+void free_error_grid(tg_ctx* ctx) {
+  dfree(ctx->err_nbr);
+  dfree(ctx->err_meta);
+  dfree(ctx->err_cdx);
+  dfree(ctx->err_xc);
+  dfree(ctx->err_statics);
+  dfree(ctx->err_mat);
+  dfree(ctx->err_father);
+  dfree(ctx->err_kind);
+  dfree(ctx->err_pts);
+  dfree(ctx->err_pxy);
+  dfree(ctx->err_fwd);
+  dfree(ctx->err_buf);
+  ctx->err_n = 0;
+  ctx->err_leaf = LeafDev{};
+}
+
 void free_fields(tg_ctx* ctx) {
   drop_graph(ctx->gf);
   drop_graph(ctx->ga);
@@ -231,6 +268,7 @@ void free_fields(tg_ctx* ctx) {
   dfree(ctx->thin_count);
   dfree(ctx->act_list);
   dfree(ctx->act_count);
+  free_error_grid(ctx);
   dfree(ctx->leaf_nbr);
   dfree(ctx->leaf_meta);
   dfree(ctx->leaf_cnb_off);
@@ -1503,6 +1541,165 @@ int tg_set_materials(tg_ctx* ctx, int member, int nmat, const tg_material* mats)
   return upload_params(ctx, nullptr);
 }
 
+int tg_set_error_grid(tg_ctx* ctx, const tg_error_grid* g) {
+  if (!ctx || !g) return TG_ERR_ARG;
+  if (!ctx->have_mesh) return set_err(ctx, TG_ERR_STATE, "no mesh");
+  if (ctx->ninst != 1) return set_err(ctx, TG_ERR_UNSUPPORTED, "error grid: one member");
+  if (!g->father || !g->kind || !g->pts || !g->pxy)
+    return set_err(ctx, TG_ERR_ARG, "error grid: no father / stencil tables");
+  if (g->mat && !ctx->mtab_d)
+    return set_err(ctx, TG_ERR_STATE,
+                   "error grid materials need a mesh with per-cell materials (tg_mesh.mat)");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  free_error_grid(ctx);
+  tg_mesh m{};
+  m.kind = TG_MESH_LEAVES;
+  m.n = g->n;
+  m.nbr = g->nbr;
+  m.nproc = g->nproc;
+  m.gen = g->gen;
+  m.cdx = g->cdx;
+  m.xc = g->xc;
+  m.grav = g->grav;
+  m.dgrav = g->dgrav;
+  m.curv = g->curv;
+  std::vector<int> nbrT;
+  std::vector<uint32_t> meta;
+  int rc = leaf_tables(ctx, &m, nbrT, meta);
+  if (rc) return rc;
+  const int n = g->n;
+  const long long nf = ctx->ncell;
+  for (int e = 0; e < n; e++) {
+    const int k = g->kind[e];
+    if (g->father[e] < 0 || g->father[e] >= nf)
+      return set_err(ctx, TG_ERR_ARG, "error element " + std::to_string(e) + ": bad father");
+    if (k < ERR_COPY || k > ERR_BILINEAR)
+      return set_err(ctx, TG_ERR_ARG, "error element " + std::to_string(e) + ": bad rule");
+    const int np = k == ERR_COPY ? 1 : (k == ERR_BARYCENTRIC ? 3 : (k == ERR_BILINEAR ? 4 : 2));
+    for (int q = 0; q < np; q++)
+      if (g->pts[4 * (size_t)e + q] < 0 || g->pts[4 * (size_t)e + q] >= nf)
+        return set_err(ctx, TG_ERR_ARG,
+                       "error element " + std::to_string(e) + ": stencil point out of range");
+    if (g->mat && g->mat[e] >= MAX_MAT)
+      return set_err(ctx, TG_ERR_ARG, "error element " + std::to_string(e) + ": bad material");
+  }
+  std::vector<double> cdx(2 * (size_t)n), xc(2 * (size_t)n), st(7 * (size_t)n);
+  for (int e = 0; e < n; e++)
+    for (int d = 0; d < 2; d++) {
+      cdx[(size_t)d * n + e] = g->cdx[2 * (size_t)e + d];
+      xc[(size_t)d * n + e] = g->xc[2 * (size_t)e + d];
+    }
+  for (int k = 0; k < 3; k++) std::copy(g->grav + (size_t)k * n, g->grav + (size_t)(k + 1) * n, st.begin() + (size_t)k * n);
+  for (int k = 0; k < 2; k++) std::copy(g->dgrav + (size_t)k * n, g->dgrav + (size_t)(k + 1) * n, st.begin() + (size_t)(3 + k) * n);
+  for (int k = 0; k < 2; k++) std::copy(g->curv + (size_t)k * n, g->curv + (size_t)(k + 1) * n, st.begin() + (size_t)(5 + k) * n);
+  struct Up {
+    void** d;
+    const void* h;
+    size_t bytes;
+  } ups[] = {
+      {(void**)&ctx->err_nbr, nbrT.data(), sizeof(int) * nbrT.size()},
+      {(void**)&ctx->err_meta, meta.data(), sizeof(uint32_t) * meta.size()},
+      {(void**)&ctx->err_cdx, cdx.data(), sizeof(double) * cdx.size()},
+      {(void**)&ctx->err_xc, xc.data(), sizeof(double) * xc.size()},
+      {(void**)&ctx->err_statics, st.data(), sizeof(double) * st.size()},
+      {(void**)&ctx->err_mat, g->mat, g->mat ? (size_t)n : 0},
+      {(void**)&ctx->err_father, g->father, sizeof(int) * (size_t)n},
+      {(void**)&ctx->err_kind, g->kind, sizeof(int) * (size_t)n},
+      {(void**)&ctx->err_pts, g->pts, sizeof(int) * 4 * (size_t)n},
+      {(void**)&ctx->err_pxy, g->pxy, sizeof(double) * 8 * (size_t)n},
+      {(void**)&ctx->err_fwd, nullptr, sizeof(double) * 9 * (size_t)nf},
+      {(void**)&ctx->err_buf, nullptr, sizeof(double) * 15 * (size_t)n},
+  };
+  for (auto& u : ups) {
+    if (!u.bytes && !u.h && u.d == (void**)&ctx->err_mat) continue;  // one material
+    rc = dalloc(ctx, u.d, u.bytes);
+    if (!rc && u.h) {
+      const hipError_t e = hipMemcpy(*u.d, u.h, u.bytes, hipMemcpyHostToDevice);
+      if (e != hipSuccess) rc = set_err(ctx, TG_ERR_HIP, hipGetErrorString(e));
+    }
+    if (rc) {
+      free_error_grid(ctx);
+      return rc;
+    }
+  }
+  for (auto& e : ctx->err_ev)
+    if (!e) HIPCHK(ctx, hipEventCreate(&e));
+  ctx->err_leaf = LeafDev{n, ctx->err_nbr, ctx->err_meta, ctx->err_cdx, ctx->err_xc, nullptr,
+                          nullptr};
+  ctx->err_n = n;
+  return TG_OK;
+}
+
+int tg_error_indicator(tg_ctx* ctx, const tg_params* prm, int step, double dt,
+                       const double* prev, const double* state, const double* lambda,
+                       double* residual, double* error, double* correction, double* recon) {
+  if (!ctx || !prm || !prev || !state || !lambda || step < 0) return TG_ERR_ARG;
+  if (!ctx->err_n) return set_err(ctx, TG_ERR_STATE, "no error grid (tg_set_error_grid)");
+  if (!(dt > 0) || !std::isfinite(dt)) return set_err(ctx, TG_ERR_ARG, "dt must be > 0");
+  HIPCHK(ctx, hipSetDevice(ctx->cfg.device));
+  int rc = upload_params(ctx, prm);
+  if (rc) return rc;
+  const long long nf = ctx->ncell;
+  const size_t n = (size_t)ctx->err_n;
+  const double* src[3] = {prev, state, lambda};
+  for (int k = 0; k < 3; k++)
+    HIPCHK(ctx, hipMemcpyAsync(ctx->err_fwd + 3 * (size_t)k * nf, src[k],
+                               sizeof(double) * 3 * (size_t)nf, hipMemcpyHostToDevice,
+                               ctx->stream));
+  const long long ne = (long long)n;
+  ErrArgs a{};
+  a.n = ctx->err_n;
+  a.nfwd = (int)nf;
+  a.L = ctx->err_leaf;
+  a.S = Statics{at<double>(ctx->err_statics, 0, ne), at<double>(ctx->err_statics, 1, ne),
+                at<double>(ctx->err_statics, 2, ne), at<double>(ctx->err_statics, 3, ne),
+                at<double>(ctx->err_statics, 4, ne), at<double>(ctx->err_statics, 5, ne),
+                at<double>(ctx->err_statics, 6, ne)};
+  a.I = ctx->inst_h[0];
+  a.mat = ctx->err_mat;
+  a.mtab = ctx->mtab_d;
+  a.father = ctx->err_father;
+  a.kind = ctx->err_kind;
+  a.pts = ctx->err_pts;
+  a.pxy = ctx->err_pxy;
+  a.fwd = ctx->err_fwd;
+  a.rec = ctx->err_buf;
+  a.kact = ctx->err_buf + 9 * n;
+  a.res = ctx->err_buf + 10 * n;
+  a.err = ctx->err_buf + 13 * n;
+  a.corr = ctx->err_buf + 14 * n;
+  a.dt = dt;
+  a.ft = frict_tiny(ctx, step);
+  a.tiny = ctx->cfg.tiny;
+  a.gamma = ctx->cfg.gamma;
+  launch_error(a, ctx->stream, ctx->err_ev);
+  HIPCHK(ctx, hipGetLastError());
+  struct Down {
+    double* h;
+    const double* d;
+    size_t count;
+  } downs[] = {{residual, a.res, 3 * n}, {error, a.err, n}, {correction, a.corr, n},
+               {recon, a.rec, 9 * n}};
+  for (auto& d : downs)
+    if (d.h)
+      HIPCHK(ctx, hipMemcpyAsync(d.h, d.d, sizeof(double) * d.count, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < 3; k++) {
+    float ms = -1.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->err_ev[k], ctx->err_ev[k + 1]));
+    ctx->err_ms[k] = ms;
+  }
+  return TG_OK;
+}
+
+int tg_error_last_ms(const tg_ctx* ctx, double ms[3]) {
+  if (!ctx || !ms) return TG_ERR_ARG;
+  for (int k = 0; k < 3; k++) ms[k] = ctx->err_ms[k];
+  return TG_OK;
+}
+
 int tg_create(const tg_config* cfg, tg_ctx** out) {
   if (!out) return TG_ERR_ARG;
   *out = nullptr;
@@ -1573,6 +1770,8 @@ void tg_destroy(tg_ctx* ctx) {
   dfree(ctx->active_d);
   for (hipEvent_t e : ctx->iev) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->seg_ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->err_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -321,6 +321,32 @@ struct LeafBufs {
   uint32_t* kdmk;
 };
 
+// Error indicator on the error grid (tg_error.hip): interpolation rules of
+// tg_host_error_stencil, and the buffers of one evaluation.
+constexpr int ERR_COPY = 0, ERR_LINEAR_X = 1, ERR_LINEAR_Y = 2, ERR_BARYCENTRIC = 3,
+              ERR_BILINEAR = 4;
+struct ErrArgs {
+  int n, nfwd;           // error elements, leaves of the forward mesh
+  LeafDev L;             // error grid tables
+  Statics S;             // error grid static coefficients
+  InstDev I;             // kact / friction constants (member 0)
+  const uint8_t* mat;    // error elements' materials (null: one material)
+  const MatDev* mtab;
+  const int* father;     // [n] forward leaf of each element
+  const int* kind;       // [n] interpolation rule
+  const int* pts;        // [4n] forward leaves of the stencil points
+  const double* pxy;     // [8n] their coordinates
+  const double* fwd;     // [9 nfwd] previous state, state, adjoint of the forward leaves
+  double* rec;           // [9n] reconstructed previous state, state, adjoint
+  double* kact;          // [n]
+  double* res;           // [3n] residual
+  double* err;           // [n] error (el_error[1])
+  double* corr;          // [n] correction
+  double dt, ft, tiny, gamma;
+};
+// the three kernels of one evaluation; ev (4 events, or null) bracket them
+void launch_error(const ErrArgs& a, hipStream_t s, hipEvent_t* ev);
+
 void launch_leaf_coef(const StepArgs& a, const LeafDev& L, bool reduce, hipStream_t s);
 void launch_leaf_update(const StepArgs& a, const LeafDev& L, hipStream_t s);
 void launch_leaf_adjoint(const AdjArgs& a, const LeafDev& L, const LeafBufs& B, hipStream_t s);

@@ -415,4 +415,13 @@ def adaptive_fractal_problem(nx0=128, dx0=40.0, max_level=3, radius=2600.0, npil
                      h, hu, hv, phi_int, phi_bed, nsteps, checkpoint_every, box=box)
     pb.meta["levels"] = np.bincount(lev, minlength=max_level + 1).tolist()
     pb.meta["piles"] = specs
+    pb.meta["dem"] = dem
     return pb
+
+
+def leaf_dem_statics(pb):
+    """statics(xc) -> (grav, curv) at arbitrary centres on the raster a leaf
+    problem was built on (tg_host_leaf_static_coefs): the set-up of new
+    sons, e.g. of an error grid."""
+    from .adaptive import dem_statics
+    return dem_statics(pb.meta["dem"])

@@ -269,6 +269,40 @@ class Simulation:
                                                   C.cast(C.byref(g), _lib._dp)))
         return out.reshape(3, -1), g.value
 
+    def set_error_grid(self, grid):
+        """Hands an errorgrid.ErrorGrid of this context's mesh to the device
+        (tg_set_error_grid)."""
+        self._err_grid = grid
+        self._err_struct = grid.c_struct()
+        self._check(self.lib.tg_set_error_grid(self.ctx, C.byref(self._err_struct)))
+
+    def error_indicator(self, prev, state, lam, s, dt, want_recon=False):
+        """The error indicator of step s (U^s = prev -> U^{s+1} = state, dt)
+        weighted with the adjoint lam, each (3, cells) in this context's
+        order (tg_error_indicator): dict of residual (3, n), error (n,),
+        correction (n,) per error element and, with want_recon, recon
+        (9, n): reconstructed previous state, state and adjoint."""
+        n = self._err_grid.n
+        src = [np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1))
+               for x in (prev, state, lam)]
+        res, err, corr = np.empty(3 * n), np.empty(n), np.empty(n)
+        rec = np.empty(9 * n) if want_recon else None
+        self._check(self.lib.tg_error_indicator(self.ctx, C.byref(self.params), int(s), float(dt),
+                                                *[_lib.dptr(x) for x in src], _lib.dptr(res),
+                                                _lib.dptr(err), _lib.dptr(corr),
+                                                _lib.dptr(rec)))
+        out = dict(residual=res.reshape(3, n), error=err, correction=corr)
+        if want_recon:
+            out["recon"] = rec.reshape(9, n)
+        return out
+
+    def error_kernel_ms(self):
+        """Kernel times (ms) of the last error_indicator: reconstruction,
+        kact, residual."""
+        ms = np.empty(3)
+        self._check(self.lib.tg_error_last_ms(self.ctx, _lib.dptr(ms)))
+        return ms
+
     def instrument(self, on=True):
         """Per-launch kernel timing and full-path cell counts (no graph
         replay while on); turning it on resets the totals."""

@@ -318,6 +318,61 @@ int tg_adjoint_step(tg_ctx* ctx, const tg_params* prm, int s, double dt,
 int tg_adjoint_step_grad(tg_ctx* ctx, const tg_params* prm, int s, double dt,
                          const double* lambda_next, double* lambda_out, double* dJ_dphibed);
 
+/* ---- error indicator on the error grid ----------------------------------------
+ * The reference's Error build (dual_solver.C:143-175, 208-240) estimates
+ * the error of the forward solution on the error grid -- the mesh with
+ * every leaf refined once (uinform_refine, uniform_refine.C:24-130) -- from
+ * bilinear reconstructions of the state, the previous state and the
+ * adjoint (bilinear_interp, bilinear_interp.C:126-556): the residual of one
+ * step of the reconstructed previous state against the reconstructed state
+ * (update_bilinear_error_grid, error_grid_routines.C:13-53, and
+ * error_compute, error_compute.C:24-100), weighted by the adjoint.
+ *
+ * tg_set_error_grid hands over the error grid of the context's mesh (one
+ * member): the error elements' leaf tables as for tg_mesh (nbr, nproc, gen,
+ * cdx, xc; host layout), their static coefficients (grav, dgrav, curv; mat
+ * with the materials of the context's material table, NULL = one
+ * material), the mesh cell each one refines (father) and the interpolation
+ * stencils of tg_host_error_stencil (tg_host.h: kind, pts, pxy).  The grid
+ * lives until the next tg_set_mesh / tg_set_error_grid. */
+typedef struct tg_error_grid {
+  int n;                  /* error elements                                    */
+  const int32_t* nbr;     /* [8*n] as tg_mesh                                  */
+  const int32_t* nproc;   /* [8*n]                                             */
+  const int32_t* gen;     /* [n]                                               */
+  const double* cdx;      /* [2*n]                                             */
+  const double* xc;       /* [2*n]                                             */
+  const double* grav;     /* [3*n]                                             */
+  const double* dgrav;    /* [2*n]                                             */
+  const double* curv;     /* [2*n]                                             */
+  const uint8_t* mat;     /* [n] or NULL                                       */
+  const int32_t* father;  /* [n] cell of the context's mesh                    */
+  const int32_t* kind;    /* [n] TG_ERR_INTERP_* rule                          */
+  const int32_t* pts;     /* [4*n] mesh cells of the stencil points            */
+  const double* pxy;      /* [8*n] their coordinates                           */
+} tg_error_grid;
+int tg_set_error_grid(tg_ctx* ctx, const tg_error_grid* g);
+
+/* One evaluation for the step U^s -> U^{s+1} of the context's mesh: prev =
+ * U^s, state = U^{s+1}, lambda = the adjoint the estimate is weighted with,
+ * each FP64 SoA [3 * cells] (h | hu | hv) on the host; step s selects
+ * frict_tiny as tg_forward_step does (error_compute.C:63-66), dt the step's
+ * dt; prm the material constants.  Outputs (host, any may be NULL), per
+ * error element: residual [3*n] (predicted - reconstructed state,
+ * ErrorElem::residual), error [n] (sum_j |res_j (lambda_j - lambda~_j)|,
+ * el_error[1]), correction [n] (sum_j res_j lambda_j), recon [9*n] (the
+ * reconstructed previous state, state and adjoint, bilin_prev_state,
+ * bilin_state, bilin_adj).  lambda_j is the element's father's adjoint
+ * (send_from_dual_to_error, error_grid_routines.C:130-170), lambda~ its
+ * reconstruction.  Synchronous.  Returns TG_OK, TG_ERR_ARG, TG_ERR_STATE
+ * (no error grid) or TG_ERR_HIP. */
+int tg_error_indicator(tg_ctx* ctx, const tg_params* prm, int step, double dt,
+                       const double* prev, const double* state, const double* lambda,
+                       double* residual, double* error, double* correction, double* recon);
+/* Kernel times of the last tg_error_indicator in milliseconds (HIP events):
+ * reconstruction, kact, residual. */
+int tg_error_last_ms(const tg_ctx* ctx, double ms[3]);
+
 /* ---- one step without allocation ----------------------------------------------
  * No counterpart in the reference: an entry point for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving

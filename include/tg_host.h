@@ -275,6 +275,35 @@ int tg_host_write_solution(const char* path, int64_t n, const uint32_t* keys,
 int tg_host_read_solution(const char* path, int64_t n, const uint32_t* keys, double* h,
                           double* hu, double* hv, int64_t* found, int64_t* unknown);
 
+/* ---- error grid ------------------------------------------------------------
+ * The reference's error grid is the mesh with every leaf refined once
+ * (uinform_refine, uniform_refine.C:24-130); its elements carry their
+ * father's state, previous state and adjoint, and bilinear_interp
+ * (bilinear_interp.C:126-556) reconstructs each from a stencil of four
+ * neighbouring elements.  tg_host_error_stencil picks every error element's
+ * stencil as bilinear_interp does and returns the rule and the points:
+ *   kind[e]:  TG_ERR_INTERP_COPY (one distinct point: the element keeps its
+ *             father's values), _LINEAR_X / _LINEAR_Y (two points, sorted by
+ *             x then y: linear_interp along that axis), _BARYCENTRIC (three),
+ *             _BILINEAR (four: the 4x4 fit of a0 + a1 x + a2 y + a3 x y);
+ *   pts[4e + k]: father leaf of point k (-1 unused); pxy[8e + 2k], [8e + 2k + 1]:
+ *             its x, y (the father's centre as the reference computes it:
+ *             the element's centre +- half its size by which_son).
+ * Inputs: the error grid's tables as tg_host_flatten returns them (nbr,
+ * nproc [8n], dx, xc [2n]), which_son[e] (0 SW, 1 SE, 2 NE, 3 NW) and the
+ * father leaf father[e] in 0 .. nfather-1 (every father has four sons).
+ * Returns 0; -1 bad argument; -2 sons and fathers do not match. */
+enum {
+  TG_ERR_INTERP_COPY = 0,
+  TG_ERR_INTERP_LINEAR_X = 1,
+  TG_ERR_INTERP_LINEAR_Y = 2,
+  TG_ERR_INTERP_BARYCENTRIC = 3,
+  TG_ERR_INTERP_BILINEAR = 4
+};
+int tg_host_error_stencil(int n, const int32_t* nbr, const int32_t* nproc, const double* cdx,
+                          const double* xc, const int32_t* which_son, const int32_t* father,
+                          int nfather, int32_t* kind, int32_t* pts, double* pxy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -193,6 +193,16 @@ class Oracle:
                                  C.cast(C.byref(acc), _dp) if want_dphi else None)
         return out.reshape(3, -1), acc.value
 
+    def error_update(self, bp, s, dt):
+        """The state update_states predicts on an error grid (this oracle's
+        mesh) from the reconstructed previous state bp (3, n), step s, dt."""
+        b = [np.ascontiguousarray(x, dtype=np.float64) for x in bp]
+        out = np.empty((3, self.n))
+        self.lib.orl_error_update(C.byref(self.mesh), C.byref(self.params), frict_tiny(s),
+                                  _p(b[0]), _p(b[1]), _p(b[2]), dt, _p(out[0]), _p(out[1]),
+                                  _p(out[2]))
+        return out
+
     def dstep_dphi(self, U, s, dt):
         out = np.empty(3 * self.n)
         self.lib.or_dstep_dphi(C.byref(self.mesh), C.byref(self.params), frict_tiny(s),
@@ -239,6 +249,9 @@ def _leaf_sigs(lib):
     lib.orl_dstep_dphi.argtypes = [M, P, C.c_double, _dp, _dp, _dp, C.c_double, _dp]
     lib.orl_slopes.restype = None
     lib.orl_slopes.argtypes = [M, P, _dp, _dp, _dp, _dp]
+    lib.orl_error_update.restype = None
+    lib.orl_error_update.argtypes = [M, P, C.c_double, _dp, _dp, _dp, C.c_double, _dp, _dp,
+                                     _dp]
 
 
 class LeafOracle(Oracle):
@@ -307,6 +320,16 @@ class LeafOracle(Oracle):
                                   _p(U[0]), _p(U[1]), _p(U[2]), dt, _p(ln), _p(gg), _p(out),
                                   C.cast(C.byref(acc), _dp) if want_dphi else None)
         return out.reshape(3, -1), acc.value
+
+    def error_update(self, bp, s, dt):
+        """The state update_states predicts on an error grid (this oracle's
+        mesh) from the reconstructed previous state bp (3, n), step s, dt."""
+        b = [np.ascontiguousarray(x, dtype=np.float64) for x in bp]
+        out = np.empty((3, self.n))
+        self.lib.orl_error_update(C.byref(self.mesh), C.byref(self.params), frict_tiny(s),
+                                  _p(b[0]), _p(b[1]), _p(b[2]), dt, _p(out[0]), _p(out[1]),
+                                  _p(out[2]))
+        return out
 
     def dstep_dphi(self, U, s, dt):
         out = np.empty(3 * self.n)

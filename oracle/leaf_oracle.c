@@ -158,51 +158,16 @@ static void hfv_of(const or_leaf_mesh* m, const or_params* p, const double* U[3]
   hfv_fwd(p, dir, U[0][e], U[1][e], U[2][e], kact[e], m->gz[e], f);
 }
 
-int orl_step(const or_leaf_mesh* m, const or_params* p, double frict_tiny, double* h,
-             double* hu, double* hv, double dt_in, double* time, double* dt_out,
-             double* mass_out) {
+/* calc_edge_states (element2.C:2744-3072) for one process, then the
+ * outflow copies of Element::boundary_flux (element2.C:3374-3411): the flux
+ * of every side node, nf[NV * node[4e + s]]. */
+static void edge_states(const or_leaf_mesh* m, const or_params* p, const double* U[3],
+                        const double* kact, const int* node, double* nf) {
   const int n = m->n;
-  const double* U[3] = {h, hu, hv};
-  double* d = (double*)malloc(sizeof(double) * 6 * (size_t)n);
-  double* kact = (double*)malloc(sizeof(double) * (size_t)n);
-  double* Un = (double*)malloc(sizeof(double) * 3 * (size_t)n);
   int* lst[2];
   int nl[2] = {0, 0};
   lst[0] = (int*)malloc(sizeof(int) * 2 * (size_t)n);
   lst[1] = (int*)malloc(sizeof(int) * 2 * (size_t)n);
-  int nnode;
-  int* node = build_nodes(m, &nnode);
-  double* nf = (double*)calloc((size_t)nnode * NV, sizeof(double));
-  int rc = 0;
-
-  /* slopes() + get_coef_and_eigen() (get_coef_and_eigen.C:29-180) */
-  double min_ev = 10000000, hmax = 0;
-  for (int e = 0; e < n; e++) {
-    slopes_fwd(m, p, U, e, d + 6 * e);
-    or_params t;
-    kact[e] = (h[e] > p->tiny)
-                  ? getcoef(cell_params(&m->mats, p, e, &t), h[e], hu[e], hv[e], d + 6 * e)
-                  : p->epsilon;
-    if (h[e] > p->tiny) {
-      if (hmax < h[e]) hmax = h[e];
-      double ev = eigen(p, h[e], hu[e], hv[e], kact[e], m->gz[e]);
-      min_ev = dmin2(dmin2(m->cdx[2 * e], m->cdx[2 * e + 1]) / ev, min_ev);
-    }
-  }
-  double dt;
-  if (dt_in > 0) {
-    dt = dt_in;
-  } else {
-    double dt0 = p->cfl_inner * min_ev;
-    double dt1 = -0.9 * sqrt(hmax * p->epsilon * p->gscale / 9.8);
-    dt = p->cfl_outer * dmin2(dt0, -dt1);
-    if (dt == 0.0) dt = 0.5 * HUGE_VAL;
-    dt = incrtime(p, *time, dt);
-  }
-  *time += dt;
-  if (dt_out) *dt_out = dt;
-
-  /* calc_edge_states (element2.C:2744-3072), one process */
   for (int e = 0; e < n; e++) {
     for (int side = 0; side < 2; side++) {
       int zp = PLUS[side], zm = MINUS[side];
@@ -255,6 +220,52 @@ int orl_step(const or_leaf_mesh* m, const or_params* p, double frict_tiny, doubl
         for (int v = 0; v < NV; v++) nm[v] = np[v];
     }
 
+  free(lst[0]);
+  free(lst[1]);
+}
+
+int orl_step(const or_leaf_mesh* m, const or_params* p, double frict_tiny, double* h,
+             double* hu, double* hv, double dt_in, double* time, double* dt_out,
+             double* mass_out) {
+  const int n = m->n;
+  const double* U[3] = {h, hu, hv};
+  double* d = (double*)malloc(sizeof(double) * 6 * (size_t)n);
+  double* kact = (double*)malloc(sizeof(double) * (size_t)n);
+  double* Un = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+  int nnode;
+  int* node = build_nodes(m, &nnode);
+  double* nf = (double*)calloc((size_t)nnode * NV, sizeof(double));
+  int rc = 0;
+
+  /* slopes() + get_coef_and_eigen() (get_coef_and_eigen.C:29-180) */
+  double min_ev = 10000000, hmax = 0;
+  for (int e = 0; e < n; e++) {
+    slopes_fwd(m, p, U, e, d + 6 * e);
+    or_params t;
+    kact[e] = (h[e] > p->tiny)
+                  ? getcoef(cell_params(&m->mats, p, e, &t), h[e], hu[e], hv[e], d + 6 * e)
+                  : p->epsilon;
+    if (h[e] > p->tiny) {
+      if (hmax < h[e]) hmax = h[e];
+      double ev = eigen(p, h[e], hu[e], hv[e], kact[e], m->gz[e]);
+      min_ev = dmin2(dmin2(m->cdx[2 * e], m->cdx[2 * e + 1]) / ev, min_ev);
+    }
+  }
+  double dt;
+  if (dt_in > 0) {
+    dt = dt_in;
+  } else {
+    double dt0 = p->cfl_inner * min_ev;
+    double dt1 = -0.9 * sqrt(hmax * p->epsilon * p->gscale / 9.8);
+    dt = p->cfl_outer * dmin2(dt0, -dt1);
+    if (dt == 0.0) dt = 0.5 * HUGE_VAL;
+    dt = incrtime(p, *time, dt);
+  }
+  *time += dt;
+  if (dt_out) *dt_out = dt;
+
+  edge_states(m, p, U, kact, node, nf);
+
   /* correct() (corrector.C:36-140) -> update_states */
   double mass = 0.0;
   for (int e = 0; e < n; e++) {
@@ -280,8 +291,54 @@ int orl_step(const or_leaf_mesh* m, const or_params* p, double frict_tiny, doubl
   memcpy(hu, Un + n, sizeof(double) * n);
   memcpy(hv, Un + 2 * (size_t)n, sizeof(double) * n);
   if (mass_out) *mass_out = mass;
-  free(d); free(kact); free(Un); free(lst[0]); free(lst[1]); free(node); free(nf);
+  free(d); free(kact); free(Un); free(node); free(nf);
   return rc;
+}
+
+/* ---- error grid (error_grid_routines.C:13-53, error_compute.C:24-100) ---- */
+
+/* The error grid's predicted state from the reconstructed previous state bp
+ * (update_bilinear_error_grid + the update_states call of error_compute):
+ * ErrorElem::get_slopes_prev (dualmesh.C:3145-3279: the dual slopes'
+ * arithmetic on bp), gmfggetcoef for every element (its dry branch gives
+ * epsilon), the edge states of ErrorElem::xdirflux / ydirflux
+ * (dualmesh.C:3308-3387: the forward flux of bp), update_states with the
+ * given dt and kact (effect_bedfrict = bedfrict[material],
+ * element2.C:4008-4018). */
+void orl_error_update(const or_leaf_mesh* m, const or_params* p, double frict_tiny,
+                      const double* h, const double* hu, const double* hv, double dt,
+                      double* out_h, double* out_hu, double* out_hv) {
+  const int n = m->n;
+  const double* U[3] = {h, hu, hv};
+  double* d = (double*)malloc(sizeof(double) * 6 * (size_t)n);
+  double* kact = (double*)calloc((size_t)n + 1, sizeof(double));
+  int nnode;
+  int* node = build_nodes(m, &nnode);
+  double* nf = (double*)calloc((size_t)nnode * NV, sizeof(double));
+  for (int e = 0; e < n; e++) {
+    double hs[2][5];
+    or_params t;
+    slopes_dual(m, p, U, e, d + 6 * e, hs);
+    kact[e] = getcoef(cell_params(&m->mats, p, e, &t), h[e], hu[e], hv[e], d + 6 * e);
+  }
+  edge_states(m, p, U, kact, node, nf);
+  for (int e = 0; e < n; e++) {
+    double prev[3] = {h[e], hu[e], hv[e]}, o[3];
+    double g[3] = {m->gx[e], m->gy[e], m->gz[e]};
+    double dg[2] = {m->dgx[e], m->dgy[e]};
+    double cv[2] = {m->kx[e], m->ky[e]};
+    double dtdx = dt / m->cdx[2 * e], dtdy = dt / m->cdx[2 * e + 1];
+    src_info si;
+    or_params pt;
+    update_states(p, prev, o, nf + NV * node[4 * e + 1], nf + NV * node[4 * e + 2],
+                  nf + NV * node[4 * e + 3], nf + NV * node[4 * e + 0], dtdx, dtdy, dt,
+                  d + 6 * e, d + 6 * e + 3, cv, p->sin_int,
+                  cell_params(&m->mats, p, e, &pt)->tan_bed, g, dg, kact[e], frict_tiny, &si);
+    out_h[e] = o[0];
+    out_hu[e] = o[1];
+    out_hv[e] = o[2];
+  }
+  free(d); free(kact); free(node); free(nf);
 }
 
 /* ---- dual grid: FluxJac and Jacobian blocks ------------------------------ */

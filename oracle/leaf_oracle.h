@@ -57,6 +57,14 @@ void orl_dstep_dphi(const or_leaf_mesh* m, const or_params* p, double frict_tiny
                     const double* h, const double* hu, const double* hv, double dt,
                     double* dU);
 
+/* Error grid: the state update_states predicts from the reconstructed
+ * previous state (h, hu, hv) with the error elements' slopes
+ * (ErrorElem::get_slopes_prev), kact and fluxes, for the given dt
+ * (error_grid_routines.C:13-53, error_compute.C:40-84). */
+void orl_error_update(const or_leaf_mesh* m, const or_params* p, double frict_tiny,
+                      const double* h, const double* hu, const double* hv, double dt,
+                      double* out_h, double* out_hu, double* out_hv);
+
 /* Forward limited slopes d[6n] (Element::get_slopes, element2.C:830-939). */
 void orl_slopes(const or_leaf_mesh* m, const or_params* p, const double* h, const double* hu,
                 const double* hv, double* d);
