@@ -533,21 +533,22 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
       inst_mark(ctx, 1, false);
       return;
     }
-    use_tiles(ctx, 0, a);
+    // the fused coefficient + classifying pass walks the update variant's
+    // blocks (it lists that variant's tiles)
+    use_tiles(ctx, 1, a);
+    a.dt_known = compute_dt ? 0 : 1;
+    a.dt_abn = ctx->act_count + 1;
     a.active = inst_counter(ctx, 0, true);
     a.short_cells = inst_short(ctx, 0, true);
+    a.short_upd = inst_short(ctx, 1, true);
     inst_mark(ctx, 0, true);
-    launch_coef(a, compute_dt, ctx->stream);
+    launch_coef_classify(a, compute_dt, ctx->stream);
     inst_mark(ctx, 0, false);
-    if (compute_dt) {
-      a.dt_clears_count = 1;
-      launch_dt(a, ctx->ninst, ctx->stream);
-    }
-    use_tiles(ctx, 1, a);
+    if (compute_dt) launch_dt(a, ctx->ninst, ctx->stream);
     a.active = inst_counter(ctx, 1, true);
     a.short_cells = inst_short(ctx, 1, true);
     inst_mark(ctx, 1, true);
-    launch_update(a, ctx->stream);
+    launch_update_listed(a, ctx->stream);
     inst_mark(ctx, 1, false);
   } else if (ctx->es == 8) {
     enqueue_step_m<double>(ctx, in, out, s, compute_dt);
@@ -1363,7 +1364,8 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     rc = dalloc(ctx, (void**)&ctx->thin_list, sizeof(int2) * (size_t)n);
   if (!rc && ctx->mode != TG_FP64) rc = dalloc(ctx, (void**)&ctx->thin_count, sizeof(int));
   if (!rc) rc = dalloc(ctx, (void**)&ctx->act_list, sizeof(int) * (size_t)ctx->ntiles);
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->act_count, sizeof(int));
+  // [0] the full-path list length, [1] the step's dt flag (k_dt)
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->act_count, 2 * sizeof(int));
   if (!rc) rc = dalloc(ctx, (void**)&ctx->inst_d, sizeof(InstDev) * nm);
   for (int v = 0; v < NSHAPE && !rc; v++)
     rc = dalloc(ctx, (void**)&ctx->tiles_d[v], sizeof(TileDev) * ctx->nt[v]);
@@ -2204,8 +2206,11 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
         else launch_leaf_update(a, ctx->leaf, ctx->stream);
       } else if (w == 0) {
         launch_coef(a, false, ctx->stream);
-      } else {
-        launch_update(a, ctx->stream);
+      } else {  // the step's own pair: fused pass (dt of step 0 known) + listed tiles
+        a.dt_known = 1;
+        a.dt_abn = ctx->act_count + 1;
+        launch_coef_classify(a, false, ctx->stream);
+        launch_update_listed(a, ctx->stream);
       }
       inst_mark(ctx, w, false);
     } else if (ctx->es == 8) {

@@ -149,7 +149,7 @@ class Oracle:
         t = C.c_double(self.time)
         dto, mass = C.c_double(), C.c_double()
         rc = self.lib.or_step(C.byref(self.mesh), C.byref(self.params), frict_tiny(s), _p(h),
-                              _p(hu), _p(hv), -1.0 if dt is None else float(dt), C.byref(t),
+                              _p(hu), _p(hv), math.nan if dt is None else float(dt), C.byref(t),
                               C.byref(dto), C.byref(mass))
         if rc:
             raise FloatingPointError("oracle: non-finite state")
@@ -289,7 +289,7 @@ class LeafOracle(Oracle):
         t = C.c_double(self.time)
         dto, mass = C.c_double(), C.c_double()
         rc = self.lib.orl_step(C.byref(self.mesh), C.byref(self.params), frict_tiny(s), _p(h),
-                               _p(hu), _p(hv), -1.0 if dt is None else float(dt), C.byref(t),
+                               _p(hu), _p(hv), math.nan if dt is None else float(dt), C.byref(t),
                                C.byref(dto), C.byref(mass))
         if rc:
             raise FloatingPointError("oracle: non-finite state")

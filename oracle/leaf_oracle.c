@@ -252,7 +252,7 @@ int orl_step(const or_leaf_mesh* m, const or_params* p, double frict_tiny, doubl
     }
   }
   double dt;
-  if (dt_in > 0) {
+  if (!isnan(dt_in)) { /* a replayed dt, any value */
     dt = dt_in;
   } else {
     double dt0 = p->cfl_inner * min_ev;

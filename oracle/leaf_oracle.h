@@ -32,8 +32,8 @@ typedef struct or_leaf_mesh {
   or_materials mats;  /* per-leaf bed friction (mat NULL: one material)       */
 } or_leaf_mesh;
 
-/* One forward step (step.C:29-275, order 1), U updated in place.  dt_in > 0
- * replays that dt; otherwise the CFL dt is computed and *time advanced.
+/* One forward step (step.C:29-275, order 1), U updated in place.  dt_in not
+ * NaN replays that dt (any value); otherwise the CFL dt is computed and *time advanced.
  * Returns 0, -5 on a non-finite state. */
 int orl_step(const or_leaf_mesh* m, const or_params* p, double frict_tiny, double* h,
              double* hu, double* hv, double dt_in, double* time, double* dt_out,

@@ -202,7 +202,7 @@ int or_step(const or_mesh* m, const or_params* p, double frict_tiny, double* h,
   /* slopes() + get_coef_and_eigen() */
   all_kact_fwd(m, p, U, kact, dU);
   double dt;
-  if (dt_in > 0) {
+  if (!isnan(dt_in)) { /* a replayed dt, any value */
     dt = dt_in;
   } else {
     dt = incrtime(p, *time, cfl_dt(m, p, U, kact));

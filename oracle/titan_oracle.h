@@ -53,8 +53,8 @@ typedef struct or_params {
   double nextsave;
 } or_params;
 
-/* One forward step (step.C:29-275, order 1).  If dt_in > 0 the step uses
- * it (replay with a recorded dt); otherwise the CFL dt is computed
+/* One forward step (step.C:29-275, order 1).  If dt_in is not NaN the step
+ * uses it (replay with a recorded dt, any value); otherwise the CFL dt is computed
  * (get_coef_and_eigen.C + incrtime) and *time is advanced.  U is updated in
  * place.  Returns 0, or -5 if a non-finite state appears. */
 int or_step(const or_mesh* m, const or_params* p, double frict_tiny,
