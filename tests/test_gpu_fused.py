@@ -54,8 +54,8 @@ def test_steps_with_dt_not_positive(oracle_lib, time0, maxtime):
             st = sim.step()
             assert dt_ref <= 0 and st.dt == dt_ref, s
             assert _same(sim.get_state(), U), s
-    if time0 > maxtime:  # the formula made the dry cells' -0.0 into +0.0
-        assert not np.signbit(U[1][pb.h == 0]).any()
+    if time0 > maxtime:  # the formula turned -0.0 of dry cells into +0.0
+        assert (~np.signbit(U[1][pb.h == 0])).any()
 
 
 def test_all_dry_domain_is_reported(oracle_lib):
