@@ -60,6 +60,13 @@ struct tg_ctx {
   size_t state_bytes = 0, mk_off = 0;
   long long nsuper = 0;
   bool marks = false;
+  // tile shape (SHAPE_TX) whose tiles the mark bytes of the state buffers
+  // [0] and of the adjoint buffers [1] describe: the update's and the
+  // adjoint's variant when they were last written (0: none yet)
+  int mk_tx[2] = {0, 0};
+  // the autotuner's timing launches: single-tile skips off, so every
+  // variant is timed on the same footing (marks are kept in one shape only)
+  bool bytes_off = false;
   void* U[3] = {nullptr, nullptr, nullptr};
   void* Ut[3] = {nullptr, nullptr, nullptr};
   void* U_home = nullptr;  // U[0] as allocated
@@ -320,8 +327,11 @@ void params_into(InstDev& I, const tg_params* p) {
   I.nextsave = p->nextsave;
 }
 
+void marks_follow_selection(tg_ctx* ctx);
+
 // prm == nullptr keeps each member's own parameters (tg_set_ensemble)
 int upload_params(tg_ctx* ctx, const tg_params* p) {
+  if (ctx->marks) marks_follow_selection(ctx);
   if (p)
     for (auto& I : ctx->inst_h) params_into(I, p);
   HIPCHK(ctx, hipMemcpyAsync(ctx->inst_d, ctx->inst_h.data(), sizeof(InstDev) * ctx->ninst,
@@ -378,6 +388,31 @@ uint32_t* marks_of(const tg_ctx* ctx, void* const f[3]) {
 void clear_marks(tg_ctx* ctx, void* base) {
   if (ctx->nsuper > 0)
     launch_zero((char*)base + ctx->mk_off, sizeof(uint32_t) * (size_t)ctx->nsuper, ctx->stream);
+}
+
+// Every state and adjoint buffer's marks: nothing known.
+void clear_all_marks(tg_ctx* ctx) {
+  void* bufs[] = {ctx->U[0], ctx->Ut[0], ctx->lam[0][0], ctx->lam[1][0], ctx->fw, ctx->wsrc};
+  for (void* b : bufs)
+    if (b) clear_marks(ctx, b);
+  for (void* b : ctx->ckpt)
+    if (b) clear_marks(ctx, b);
+  for (void* b : ctx->seg)
+    if (b) clear_marks(ctx, b);
+  for (size_t i = 0; i < ctx->traj_cap; i++)
+    clear_marks(ctx, (char*)ctx->traj_block + i * ctx->state_bytes);
+}
+
+// A mark byte describes one tile of the writer's shape.  State buffers are
+// written by the update pass and adjoint buffers by the adjoint pass; when
+// the selected shapes change, bytes written in the old shapes are dropped
+// (whole-word marks mean the same in every shape, single bytes do not).
+void marks_follow_selection(tg_ctx* ctx) {
+  const int tx[2] = {SHAPE_TX[ctx->sel[1]], SHAPE_TX[ctx->sel[2]]};
+  if (tx[0] == ctx->mk_tx[0] && tx[1] == ctx->mk_tx[1]) return;
+  clear_all_marks(ctx);
+  ctx->mk_tx[0] = tx[0];
+  ctx->mk_tx[1] = tx[1];
 }
 
 // Instrumentation: resolves the recorded event pairs into the per-kernel
@@ -517,6 +552,7 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
     a.tile_count = ctx->act_count;
     a.mk_in = marks_of(ctx, in);
     a.mk_out = marks_of(ctx, out);
+    a.zbytes = !ctx->bytes_off && SHAPE_TX[ctx->sel[1]] == ctx->mk_tx[0];
     if (ctx->leaves) {
       a.mat = ctx->mat_d;
       a.mtab = ctx->mtab_d;
@@ -533,22 +569,23 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
       inst_mark(ctx, 1, false);
       return;
     }
-    // the fused coefficient + classifying pass walks the update variant's
-    // blocks (it lists that variant's tiles)
-    use_tiles(ctx, 1, a);
-    a.dt_known = compute_dt ? 0 : 1;
-    a.dt_abn = ctx->act_count + 1;
+    use_tiles(ctx, 0, a);
+    a.zbytes = !ctx->bytes_off && SHAPE_TX[ctx->sel[0]] == ctx->mk_tx[0];
     a.active = inst_counter(ctx, 0, true);
     a.short_cells = inst_short(ctx, 0, true);
-    a.short_upd = inst_short(ctx, 1, true);
     inst_mark(ctx, 0, true);
-    launch_coef_classify(a, compute_dt, ctx->stream);
+    launch_coef(a, compute_dt, ctx->stream);
     inst_mark(ctx, 0, false);
-    if (compute_dt) launch_dt(a, ctx->ninst, ctx->stream);
+    if (compute_dt) {
+      a.dt_clears_count = 1;
+      launch_dt(a, ctx->ninst, ctx->stream);
+    }
+    use_tiles(ctx, 1, a);
+    a.zbytes = !ctx->bytes_off && SHAPE_TX[ctx->sel[1]] == ctx->mk_tx[0];
     a.active = inst_counter(ctx, 1, true);
     a.short_cells = inst_short(ctx, 1, true);
     inst_mark(ctx, 1, true);
-    launch_update_listed(a, ctx->stream);
+    launch_update(a, ctx->stream);
     inst_mark(ctx, 1, false);
   } else if (ctx->es == 8) {
     enqueue_step_m<double>(ctx, in, out, s, compute_dt);
@@ -579,6 +616,8 @@ void enqueue_adjoint(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* 
     a.mk_U = marks_of(ctx, U);
     a.mk_lnext = marks_of(ctx, lnext);
     a.mk_out = marks_of(ctx, lout);
+    a.u_bytes = !ctx->bytes_off && SHAPE_TX[ctx->sel[2]] == ctx->mk_tx[0];
+    a.l_bytes = !ctx->bytes_off && SHAPE_TX[ctx->sel[2]] == ctx->mk_tx[1];
     a.active = inst_counter(ctx, 2, true);
     a.short_cells = inst_short(ctx, 2, true);
     inst_mark(ctx, 2, true);
@@ -1364,8 +1403,7 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
     rc = dalloc(ctx, (void**)&ctx->thin_list, sizeof(int2) * (size_t)n);
   if (!rc && ctx->mode != TG_FP64) rc = dalloc(ctx, (void**)&ctx->thin_count, sizeof(int));
   if (!rc) rc = dalloc(ctx, (void**)&ctx->act_list, sizeof(int) * (size_t)ctx->ntiles);
-  // [0] the full-path list length, [1] the step's dt flag (k_dt)
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->act_count, 2 * sizeof(int));
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->act_count, sizeof(int));
   if (!rc) rc = dalloc(ctx, (void**)&ctx->inst_d, sizeof(InstDev) * nm);
   for (int v = 0; v < NSHAPE && !rc; v++)
     rc = dalloc(ctx, (void**)&ctx->tiles_d[v], sizeof(TileDev) * ctx->nt[v]);
@@ -2172,6 +2210,7 @@ int tg_last_stored_sweep(tg_ctx* ctx, double* stored_ms, int* stored_steps) {
 // selected one), over reps back-to-back launches on the current state.
 int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms) {
   const int keep[3] = {ctx->sel[0], ctx->sel[1], ctx->sel[2]};
+  ctx->bytes_off = true;
   if (v >= 0) ctx->sel[which] = v;
   // every variant reads U and writes scratch (Ut or lam[1]) with step 0's
   // dt; the adjoint runs with its bed-friction gradient term, as in
@@ -2198,6 +2237,7 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
       a.tile_count = ctx->act_count;
       a.mk_in = marks_of(ctx, ctx->U);
       a.mk_out = marks_of(ctx, ctx->Ut);
+      a.zbytes = 0;  // (timing: see bytes_off)
       a.active = inst_counter(ctx, w, true);
       a.short_cells = inst_short(ctx, w, true);
       inst_mark(ctx, w, true);
@@ -2206,11 +2246,8 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
         else launch_leaf_update(a, ctx->leaf, ctx->stream);
       } else if (w == 0) {
         launch_coef(a, false, ctx->stream);
-      } else {  // the step's own pair: fused pass (dt of step 0 known) + listed tiles
-        a.dt_known = 1;
-        a.dt_abn = ctx->act_count + 1;
-        launch_coef_classify(a, false, ctx->stream);
-        launch_update_listed(a, ctx->stream);
+      } else {
+        launch_update(a, ctx->stream);
       }
       inst_mark(ctx, w, false);
     } else if (ctx->es == 8) {
@@ -2236,6 +2273,12 @@ int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms)
               hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) != hipSuccess))
     rc = set_err(ctx, TG_ERR_HIP, "kernel timing failed");
   for (int k = 0; k < 3; k++) ctx->sel[k] = keep[k];
+  ctx->bytes_off = false;
+  // the scratch buffers carry marks in the timed variant's shape
+  if (ctx->marks) {
+    clear_marks(ctx, ctx->Ut[0]);
+    clear_marks(ctx, ctx->lam[1][0]);
+  }
   if (rc) return rc;
   *avg_ms = ms / reps;
   return check_err(ctx);

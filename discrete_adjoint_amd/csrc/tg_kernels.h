@@ -162,14 +162,10 @@ struct StepArgs {
   // per member, the InstDev fields)
   const uint8_t* mat;
   const MatDev* mtab;
-  // fused coefficient + classifying pass (k_coef_classify): dt_known 1 when
-  // the step's dt is in dt_rec before the pass (replayed steps); dt_abn, a
-  // flag k_dt raises when a member's dt is not a positive finite number (the
-  // update then rewrites the dry tiles with update_states' formula);
-  // short_upd, the update role's short-path counter (null: not counting)
-  int dt_known;
-  int* dt_abn;
-  unsigned long long* short_upd;
+  // 1 when the mark bytes of U and of the output follow this launch's tile
+  // shape (byte b = tile b of the block), so single tiles may be skipped;
+  // 0: only whole mark words are meaningful
+  int zbytes;
 };
 
 struct AdjArgs {
@@ -202,6 +198,9 @@ struct AdjArgs {
   unsigned long long* short_cells;
   const uint8_t* mat;     // as in StepArgs
   const MatDev* mtab;
+  // as StepArgs::zbytes, for the marks of U^s (u_bytes) and of the adjoint
+  // buffers lambda^{s+1}, lambda^s (l_bytes)
+  int u_bytes, l_bytes;
 };
 
 // Mixed-precision counterparts: FP32 arithmetic, FP32 static coefficients
@@ -364,11 +363,6 @@ void launch_leaf_stats(const LeafDev& L, Fields U, double* part_mass, double* pa
 void launch_coef(const StepArgs& a, bool reduce, hipStream_t s);
 void launch_dt(const StepArgs& a, int ninst, hipStream_t s);
 void launch_update(const StepArgs& a, hipStream_t s);
-// the FP64 uniform step as launched by the context: the list length and the
-// dt flag (tile_count[0..1]) cleared, the fused coefficient + classifying
-// pass on the update variant's blocks; then (after k_dt) the listed tiles
-void launch_coef_classify(const StepArgs& a, bool reduce, hipStream_t s);
-void launch_update_listed(const StepArgs& a, hipStream_t s);
 void launch_adjoint(const AdjArgs& a, hipStream_t s);
 void launch_reset_red(RedDev* red, int ninst, hipStream_t s);
 // rec[member * max_steps + step] = max(itself, largest h > 0 over the listed

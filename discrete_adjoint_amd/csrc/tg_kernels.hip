@@ -157,6 +157,12 @@ __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
   const InstDev I = a.inst[B.inst];
   Cell q[4];
   block_cells<TX>(B, q);
+  // a tile the marks show to hold +0.0 has h < tiny everywhere: nothing to
+  // read or write
+  const uint32_t zin = (a.mk_in && a.zbytes) ? a.mk_in[blockIdx.x] : 0u;
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+    if ((zin >> (8 * b)) & 1u) q[b].act = false;
   double h[4];
 #pragma unroll
   for (int b = 0; b < 4; b++) h[b] = q[b].act ? a.U.h[q[b].c] : 0.;
@@ -306,7 +312,6 @@ __global__ void k_dt(StepArgs a, int ninst) {
   if (time + dt > I.maxtime) dt = I.maxtime - time;
   a.time[b] = time + dt;
   a.dt_rec[(long long)b * a.max_steps + a.step] = dt;
-  if (a.dt_abn && !(dt > 0. && dt < HUGE_VAL)) atomicOr(a.dt_abn, 1);
   a.red[b].min_ev = dbits(10000000.);
   a.red[b].hmax = dbits(0.);
 }
@@ -360,14 +365,20 @@ __device__ __forceinline__ Hfv rec_hfv(const FaceLds<TX>& L, int dir, int r, dou
 // and riemannflux of two such sides is zero) and no source terms
 // (update_states adds them only above tiny); its cells get update_states'
 // own arithmetic with zero fluxes, bit for bit, here.  Every other tile is
-// appended to the list pass 2 walks.
+// appended to the list pass 2 walks.  A tile the input marks show to hold
+// +0.0 is not read (its cells are +0.0); when it is dry, its output tile
+// already holds +0.0 and dt is finite, the formula gives +0.0 again and
+// nothing is written.
 template <int TX>
-__device__ __forceinline__ void classify_block(const StepArgs& a, int blk, bool list) {
+__global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
   constexpr int TY = BLOCK / TX;
-  const BlockDev B = a.blocks[blk];
+  const BlockDev B = a.blocks[blockIdx.x];
   const double dt = a.dt_rec[(long long)B.inst * a.max_steps + a.step];
   if (a.mk_in && update_skip(a.mk_in, a.mk_out, B, dt)) return;
   const InstDev I = a.inst[B.inst];
+  // tiles of the block holding +0.0 in the input / output state (mark bytes)
+  const uint32_t zin = (a.mk_in && a.zbytes) ? a.mk_in[B.mw] : 0u;
+  const uint32_t zout = (a.mk_out && a.zbytes && isfinite(dt)) ? a.mk_out[B.mw] : 0u;
   __shared__ unsigned s_wet, s_nz;
   if (threadIdx.x == 0) { s_wet = 0u; s_nz = 0u; }
   Cell q[4];
@@ -382,9 +393,13 @@ __device__ __forceinline__ void classify_block(const StepArgs& a, int blk, bool 
   for (int b = 0; b < 4; b++) {
     if (!q[b].act) continue;
     const long long c = q[b].c;
-    out[b][0] = a.U.h[c];
-    out[b][1] = a.U.hu[c];
-    out[b][2] = a.U.hv[c];
+    if ((zin >> (8 * b)) & 1u) {
+      out[b][0] = out[b][1] = out[b][2] = 0.;
+    } else {
+      out[b][0] = a.U.h[c];
+      out[b][1] = a.U.hu[c];
+      out[b][2] = a.U.hv[c];
+    }
     bool w = !(out[b][0] < a.tiny);
     if (tx == 0 && q[b].hasW) w = w || !(a.U.h[c - 1] < a.tiny);
     if (tx == TX - 1 && q[b].hasE) w = w || !(a.U.h[c + 1] < a.tiny);
@@ -396,7 +411,7 @@ __device__ __forceinline__ void classify_block(const StepArgs& a, int blk, bool 
   if (wet) atomicOr(&s_wet, wet);
   __syncthreads();
   const unsigned W = s_wet;
-  if (list && threadIdx.x == 0 && W) {
+  if (threadIdx.x == 0 && W) {
     int k = atomicAdd(a.tile_count, __popc(W));
 #pragma unroll
     for (int b = 0; b < 4; b++)
@@ -416,7 +431,7 @@ __device__ __forceinline__ void classify_block(const StepArgs& a, int blk, bool 
     if (!isfinite(out[b][0]) || !isfinite(out[b][1]) || !isfinite(out[b][2])) atomicOr(a.err, 1);
     if (!bitzero3(out[b][0], out[b][1], out[b][2])) nz |= 1u << b;
   }
-  if (list && a.short_cells) {
+  if (a.short_cells) {
     const int cnt = (int)wave_sum((double)nshort);
     if (threadIdx.x % 64 == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
   }
@@ -425,9 +440,15 @@ __device__ __forceinline__ void classify_block(const StepArgs& a, int blk, bool 
     if (nz) atomicOr(&s_nz, nz);
     __syncthreads();
   }
+  // a dry tile of +0.0 whose output already holds +0.0 (finite dt): the
+  // formula gives +0.0 again, nothing to write
+  unsigned keep = 0u;
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+    if (((zin & zout) >> (8 * b)) & 1u) keep |= 1u << b;
 #pragma unroll
   for (int b = 0; b < 4; b++) {
-    if (!q[b].act || ((W >> b) & 1u)) continue;
+    if (!q[b].act || (((W | keep) >> b) & 1u)) continue;
     a.h_out[q[b].c] = out[b][0];
     a.hu_out[q[b].c] = out[b][1];
     a.hv_out[q[b].c] = out[b][2];
@@ -436,156 +457,8 @@ __device__ __forceinline__ void classify_block(const StepArgs& a, int blk, bool 
     const unsigned NZ = s_nz;
 #pragma unroll
     for (int b = 0; b < 4; b++)
-      if (B.tile[b] >= 0 && !((W >> b) & 1u)) mark_byte_store(a.mk_out, B.mw, b, !((NZ >> b) & 1u));
-  }
-}
-
-template <int TX>
-__global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
-  classify_block<TX>(a, (int)blockIdx.x, true);
-}
-
-// The coefficient pass fused with the update's classification (one
-// workgroup per mark block of the update variant, one cell of each of its
-// four tiles per thread): every cell's state is read once for both.  The
-// coefficients are k_coef's; the classification is classify_block's.  A
-// dry tile's update is update_states with zero fluxes, U - dt/dx (0 - 0) -
-// dt/dy (0 - 0) with h clamped at 0: with the step's dt known (a replayed
-// step) that formula is evaluated; otherwise (the dt is reduced from this
-// very pass) the tile is written as the formula evaluates for every positive
-// finite dt -- the state itself, h clamped -- and k_dt flags a step whose dt
-// is not positive and finite, in which case k_update rewrites every dry tile
-// with the formula (classify_block, skips and marks included).
-template <int TX>
-__global__ __launch_bounds__(BLOCK) void k_coef_classify(StepArgs a, int reduce) {
-  constexpr int TY = BLOCK / TX;
-  const BlockDev B = a.blocks[blockIdx.x];
-  double dt = 1.;
-  if (a.dt_known) {
-    dt = a.dt_rec[(long long)B.inst * a.max_steps + a.step];
-    if (a.mk_in && update_skip(a.mk_in, a.mk_out, B, dt)) return;
-  } else if (a.mk_in && update_skip(a.mk_in, a.mk_out, B, 1.)) {
-    return;  // (a dt that is not positive and finite revisits the block)
-  }
-  const InstDev I = a.inst[B.inst];
-  __shared__ unsigned s_wet, s_nz;
-  if (threadIdx.x == 0) { s_wet = 0u; s_nz = 0u; }
-  Cell q[4];
-  block_cells<TX>(B, q);
-  const long long nx = I.nx;
-  const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
-  unsigned wet = 0u;
-  double out[4][3];
-  double minev = HUGE_VAL, hmax = 0.;
-  int nwet = 0, nshort = 0;
-#pragma unroll
-  for (int b = 0; b < 4; b++) {
-    if (!q[b].act) continue;
-    const long long c = q[b].c;
-    const double hc = a.U.h[c], hu = a.U.hu[c], hv = a.U.hv[c];
-    out[b][0] = hc;
-    out[b][1] = hu;
-    out[b][2] = hv;
-    bool w = !(hc < a.tiny);
-    if (tx == 0 && q[b].hasW) w = w || !(a.U.h[c - 1] < a.tiny);
-    if (tx == TX - 1 && q[b].hasE) w = w || !(a.U.h[c + 1] < a.tiny);
-    if (ty == 0 && q[b].hasS) w = w || !(a.U.h[c - nx] < a.tiny);
-    if (ty == TY - 1 && q[b].hasN) w = w || !(a.U.h[c + nx] < a.tiny);
-    if (w) wet |= 1u << b;
-    if (hc > a.tiny) {  // k_coef
-      nwet++;
-      double hx = 0., hux = 0., hy = 0., hvy = 0.;
-      if (q[b].hasE && q[b].hasW && q[b].hasN && q[b].hasS) {  // boundary cells: zero slopes
-        hx = slope_fwd(a.U.h[c + 1], hc, a.U.h[c - 1], I.dx, a.gamma, a.tiny);
-        hux = slope_fwd(a.U.hu[c + 1], hu, a.U.hu[c - 1], I.dx, a.gamma, a.tiny);
-        hy = slope_fwd(a.U.h[c + nx], hc, a.U.h[c - nx], I.dy, a.gamma, a.tiny);
-        hvy = slope_fwd(a.U.hv[c + nx], hv, a.U.hv[c - nx], I.dy, a.gamma, a.tiny);
-      }
-      const double kroot = mat_of(I, a.mat, a.mtab, c).kroot;
-      const double k = getcoef(hc, hu, hv, hx, hux, hy, hvy, I.kc1, kroot, I.eps, a.tiny);
-      if (reduce) {
-        double ev = eigen(hc, hu, hv, k, a.S.gz[c], a.tiny);
-        if (ev > 1000000000.) atomicOr(a.err, 2);
-        double mdx = (I.dx > I.dy) ? I.dy : I.dx;
-        const double m = mdx / ev;
-        minev = (m < minev) ? m : minev;
-        hmax = (hc > hmax) ? hc : hmax;
-      }
-      a.kact[c] = k;
-    } else {
-      nshort++;
-      if (!(hc < a.tiny)) a.kact[c] = I.eps;
-    }
-  }
-  if (a.active) {
-    const int cnt = (int)wave_sum((double)nwet);
-    if (threadIdx.x % 64 == 0 && cnt) atomicAdd(a.active, (unsigned long long)cnt);
-  }
-  if (a.short_cells) {
-    const int cnt = (int)wave_sum((double)nshort);
-    if (threadIdx.x % 64 == 0 && cnt) atomicAdd(a.short_cells, (unsigned long long)cnt);
-  }
-  __syncthreads();  // s_wet, s_nz set
-  if (wet) atomicOr(&s_wet, wet);
-  __syncthreads();
-  const unsigned W = s_wet;
-  if (threadIdx.x == 0 && W) {
-    int k = atomicAdd(a.tile_count, __popc(W));
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-      if ((W >> b) & 1u) a.tile_list[k++] = B.tile[b];
-  }
-  const double dtdx = dt / I.dx, dtdy = dt / I.dy, zero = 0.;
-  unsigned nz = 0u;
-  int ndry = 0;
-#pragma unroll
-  for (int b = 0; b < 4; b++) {
-    if (!q[b].act || ((W >> b) & 1u)) continue;
-    ndry++;
-    if (a.dt_known)
-#pragma unroll
-      for (int v = 0; v < 3; v++)
-        out[b][v] = out[b][v] - dtdx * (zero - zero) - dtdy * (zero - zero);
-    if (out[b][0] < 0.) out[b][0] = 0;
-    if (!isfinite(out[b][0]) || !isfinite(out[b][1]) || !isfinite(out[b][2])) atomicOr(a.err, 1);
-    if (!bitzero3(out[b][0], out[b][1], out[b][2])) nz |= 1u << b;
-  }
-  if (a.short_upd) {
-    const int cnt = (int)wave_sum((double)ndry);
-    if (threadIdx.x % 64 == 0 && cnt) atomicAdd(a.short_upd, (unsigned long long)cnt);
-  }
-  if (a.mk_out) {
-    if (nz) atomicOr(&s_nz, nz);
-    __syncthreads();
-  }
-#pragma unroll
-  for (int b = 0; b < 4; b++) {
-    if (!q[b].act || ((W >> b) & 1u)) continue;
-    a.h_out[q[b].c] = out[b][0];
-    a.hu_out[q[b].c] = out[b][1];
-    a.hv_out[q[b].c] = out[b][2];
-  }
-  if (a.mk_out && threadIdx.x == 0) {
-    const unsigned NZ = s_nz;
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-      if (B.tile[b] >= 0 && !((W >> b) & 1u)) mark_byte_store(a.mk_out, B.mw, b, !((NZ >> b) & 1u));
-  }
-  if (reduce && __syncthreads_or(nwet > 0)) {  // k_coef's CFL reduction
-    __shared__ double smin[BLOCK / 64], smax[BLOCK / 64];
-    minev = wave_min(minev);
-    hmax = wave_max(hmax);
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    if (l == 0) { smin[w] = minev; smax[w] = hmax; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int k = 1; k < BLOCK / 64; k++) {
-        minev = (smin[k] < minev) ? smin[k] : minev;
-        hmax = (smax[k] > hmax) ? smax[k] : hmax;
-      }
-      if (minev < HUGE_VAL) atomicMin(&a.red[B.inst].min_ev, dbits(minev));
-      if (hmax > 0.) atomicMax(&a.red[B.inst].hmax, dbits(hmax));
-    }
+      if (B.tile[b] >= 0 && !(((W | keep) >> b) & 1u))
+        mark_byte_store(a.mk_out, B.mw, b, !((NZ >> b) & 1u));
   }
 }
 
@@ -699,12 +572,6 @@ __device__ __forceinline__ void update_tile(const StepArgs& a, int tile, FaceLds
 template <int TX>
 __global__ __launch_bounds__(BLOCK) void k_update(StepArgs a) {
   __shared__ FaceLds<TX> L;
-  if (a.dt_abn && *a.dt_abn) {  // dry tiles written before dt was known: rewrite
-    for (int b = (int)blockIdx.x; b < a.nblocks; b += (int)gridDim.x) {
-      classify_block<TX>(a, b, false);
-      __syncthreads();  // its shared words are reused by the next block
-    }
-  }
   const int n = *a.tile_count;
   for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x) {
     update_tile<TX>(a, a.tile_list[i], L);
@@ -1014,12 +881,21 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_classify(AdjArgs a) {
     return;
   __shared__ unsigned s_busy;
   if (threadIdx.x == 0) s_busy = 0u;
+  // tiles the marks show to be void (U +0.0 and, with the gradient,
+  // lambda^{s+1} +0.0): nothing to read; output tiles already +0.0: nothing
+  // to write for a void tile
+  uint32_t zvoid = 0u, zout = 0u;
+  if (a.mk_out && a.l_bytes) {
+    if (a.u_bytes) zvoid = a.mk_U[B.mw];
+    if (a.dphi_part) zvoid &= a.mk_lnext[B.mw];
+    zout = a.mk_out[B.mw];
+  }
   Cell q[4];
   block_cells<TX>(B, q);
   unsigned busy = 0u;
 #pragma unroll
   for (int b = 0; b < 4; b++) {
-    if (!q[b].act) continue;
+    if (!q[b].act || ((zvoid >> (8 * b)) & 1u)) continue;
     const long long c = q[b].c;
     bool u = a.U.h[c] != 0.;
     if (a.dphi_part)
@@ -1041,6 +917,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_classify(AdjArgs a) {
   for (int b = 0; b < 4; b++) {
     if (!q[b].act || ((W >> b) & 1u)) continue;
     nshort++;
+    if ((zout >> (8 * b)) & 1u) continue;  // already +0.0
     a.lam_h[q[b].c] = 0.;
     a.lam_hu[q[b].c] = 0.;
     a.lam_hv[q[b].c] = 0.;
@@ -1258,13 +1135,6 @@ void launch_dt(const StepArgs& a, int ninst, hipStream_t s) {
 void launch_update(const StepArgs& a, hipStream_t s) {
   if (!a.dt_clears_count) launch_zero(a.tile_count, sizeof(int), s);
   TG_LAUNCH_TX(a.shape, k_update_classify, dim3(a.nblocks), s, a);
-  TG_LAUNCH_TX(a.shape, k_update, dim3(a.ntiles < LIST_GRID ? a.ntiles : LIST_GRID), s, a);
-}
-void launch_coef_classify(const StepArgs& a, bool reduce, hipStream_t s) {
-  launch_zero(a.tile_count, 2 * sizeof(int), s);
-  TG_LAUNCH_TX(a.shape, k_coef_classify, dim3(a.nblocks), s, a, reduce ? 1 : 0);
-}
-void launch_update_listed(const StepArgs& a, hipStream_t s) {
   TG_LAUNCH_TX(a.shape, k_update, dim3(a.ntiles < LIST_GRID ? a.ntiles : LIST_GRID), s, a);
 }
 void launch_adjoint(const AdjArgs& a, hipStream_t s) {

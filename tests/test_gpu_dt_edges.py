@@ -1,12 +1,11 @@
-"""GPU: the fused coefficient + classifying pass of the FP64 uniform step and
-its fallback, against the oracle bit for bit.  A dry tile is written before
-the step's dt is known, as update_states' zero-flux formula evaluates for
-every positive finite dt (the state itself, h clamped at 0); a step whose dt
-is not positive and finite rewrites every dry tile with the formula.  Cases:
+"""GPU: forward steps whose dt is not a positive finite number, against the
+oracle bit for bit.  The update's short path writes a dry tile as
+update_states' zero-flux formula U - dt/dx (0 - 0) - dt/dy (0 - 0) with h
+clamped at 0, and skips tiles whose input and output hold +0.0 only while dt
+is finite; these cases check that the formula, not a copy, is what lands:
 the clock past maxtime (negative dt) and at maxtime (dt = 0), on states with
--0.0 in dry cells (the formula with a negative dt turns them into +0.0, a
-copy would not); an all-dry domain (dt = inf: non-finite, reported); steps
-replayed with a negative dt (dt known before the pass)."""
+-0.0 in dry cells (a negative dt turns them into +0.0); an all-dry domain
+(dt = inf: non-finite, reported); steps replayed with a negative dt."""
 import sys
 
 import numpy as np
@@ -41,7 +40,7 @@ def _same(a, b):
 @pytest.mark.parametrize("time0,maxtime", [(0.25, 0.0), (0.5, 0.5)])
 def test_steps_with_dt_not_positive(oracle_lib, time0, maxtime):
     pb = _negative_zeros(small_flow_problem(70, 45, 10))
-    _say(f"case fused step, clock {time0}, maxtime {maxtime}")
+    _say(f"case dt edge: clock {time0}, maxtime {maxtime}")
     orc = oracle_lib.Oracle.from_problem(pb)
     orc.params.maxtime = maxtime
     orc.time = time0
@@ -62,7 +61,7 @@ def test_all_dry_domain_is_reported(oracle_lib):
     pb = small_flow_problem(40, 30, 4)
     for f in (pb.h, pb.hu, pb.hv):
         f[:] = 0.0
-    _say("case fused step, all dry")
+    _say("case dt edge: all dry")
     orc = oracle_lib.Oracle.from_problem(pb)
     with pytest.raises(FloatingPointError):
         orc.step([pb.h.copy(), pb.hu.copy(), pb.hv.copy()], 0)
@@ -74,7 +73,7 @@ def test_all_dry_domain_is_reported(oracle_lib):
 
 def test_replayed_negative_dt(oracle_lib):
     pb = _negative_zeros(small_flow_problem(70, 45, 10))
-    _say("case fused replay with a negative dt")
+    _say("case dt edge: replay with a negative dt")
     dts = [0.05, -0.03, 0.04, 0.0]
     orc = oracle_lib.Oracle.from_problem(pb)
     U = [pb.h.copy(), pb.hu.copy(), pb.hv.copy()]

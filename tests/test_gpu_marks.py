@@ -130,6 +130,36 @@ def test_marks_with_mixed_tile_shapes(oracle_lib, sel):
             assert dphi[b] == pytest.approx(dphi_ref, rel=1e-12), (sel, b)
 
 
+def test_marks_across_selection_changes(oracle_lib):
+    """Sweeps on one context with the tile selection changed between them,
+    and the autotuner's timing launches (which write the scratch buffers in
+    other shapes) in between: the single-tile skips read a mark byte only in
+    the shape that wrote it."""
+    mem = _members(24)
+    _say("case marks across tile selection changes")
+    weights = [_box(p, 0.2, 0.8, 0.2, 0.8).reshape(3, -1) for p in mem]
+    refs = []
+    for b, pb in enumerate(mem):
+        orc = oracle_lib.Oracle.from_problem(pb)
+        refs.append(orc.forward_adjoint(_U(pb), pb.nsteps, weights[b], want_dphi=True))
+    h0 = [np.concatenate([getattr(p, f) for p in mem]) for f in ("h", "hu", "hv")]
+    with Ensemble(mem, precision=_lib.TG_FP64, checkpoint_every=6, use_graph=1) as ens:
+        for sel in [(3, 3, 3), (1, 2, 3), None, (0, 3, 2), (2, 2, 2), (3, 3, 3)]:
+            if sel is None:
+                ens.autotune(reps=2)
+            else:
+                ens.select_tiles(sel)
+            ens.set_state(*h0)
+            ens.run(mem[0].nsteps)
+            lam, dphi = ens.dual_solver(weights, want_dphi=True)
+            for b, (UN, dts, lam_ref, dphi_ref) in enumerate(refs):
+                assert np.array_equal(ens.member_dt_record(b), dts), (sel, b)
+                for x, y in zip(ens.member_state(b), UN):
+                    assert np.array_equal(x, y), (sel, b)
+                assert np.array_equal(lam[b], lam_ref), (sel, b)
+                assert dphi[b] == pytest.approx(dphi_ref, rel=1e-12), (sel, b)
+
+
 def test_single_steps_after_a_marked_run(oracle_lib):
     """tg_forward_step / tg_adjoint_step_grad after a marked sweep: the
     single-step paths see the same marks discipline (buffers swapped, uploads
