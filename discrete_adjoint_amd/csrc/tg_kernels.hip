@@ -944,7 +944,8 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
   double dphi = 0.;
   bool zero_out = true;
   const bool want_dphi = a.dphi_part != nullptr;
-  constexpr int TY = BLOCK / TX, RX = TX + 2, RY = TY + 2, NHALO = RX * RY - BLOCK;
+  // the halo is the four side strips: the 5-point stencil reads no corner
+  constexpr int TY = BLOCK / TX, RX = TX + 2, RY = TY + 2, NHALO = 2 * TX + 2 * TY;
   static_assert(RX * RY <= RN && NHALO <= BLOCK, "tile does not fit the LDS records");
   if (a.active) {
     const int cnt = __syncthreads_count(q.act);
@@ -961,10 +962,10 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
   if ((int)threadIdx.x < NHALO) {
     const int h = (int)threadIdx.x;
     int rx, ry;
-    if (h < RX) { rx = h; ry = 0; }
-    else if (h < 2 * RX) { rx = h - RX; ry = RY - 1; }
-    else if (h < 2 * RX + TY) { rx = 0; ry = h - 2 * RX + 1; }
-    else { rx = RX - 1; ry = h - 2 * RX - TY + 1; }
+    if (h < TX) { rx = h + 1; ry = 0; }
+    else if (h < 2 * TX) { rx = h - TX + 1; ry = RY - 1; }
+    else if (h < 2 * TX + TY) { rx = 0; ry = h - 2 * TX + 1; }
+    else { rx = RX - 1; ry = h - 2 * TX - TY + 1; }
     const int gi = t.x0 - 1 + rx, gj = t.y0 - 1 + ry;
     if (gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny)
       lds_put(L, ry * RX + rx, dual_info(a, I, gi, gj, want_dphi), a.tiny);
@@ -1084,8 +1085,12 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs& a, int tile, DualLds
       const double lam[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
       asm volatile("" ::: "memory");
       // every rn[] slot lies inside the LDS tile; records of absent
-      // neighbours are passed but not read (dphi_term checks q.has*)
-      dphi = dphi_term(a, I, q, self, L, rself, rn[1], rn[2], rn[3], rn[4], dt, lam);
+      // neighbours are passed but not read (dphi_term checks q.has*).  A
+      // cell with lambda^{s+1} = 0 adds lambda . dU/dphi = 0 (dU/dphi is
+      // finite): not evaluated (the tile's sum may then hold +0.0 where the
+      // full evaluation would give -0.0)
+      if (!(lam[0] == 0. && lam[1] == 0. && lam[2] == 0.))
+        dphi = dphi_term(a, I, q, self, L, rself, rn[1], rn[2], rn[3], rn[4], dt, lam);
     }
   }
   if (want_dphi) {

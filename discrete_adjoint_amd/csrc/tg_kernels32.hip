@@ -699,7 +699,8 @@ __device__ __forceinline__ void adjoint32_tile(const AdjArgsM<S>& a, int tile, D
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
   const bool want_dphi = a.dphi_part != nullptr;
-  constexpr int TY = BLOCK / TX, RX = TX + 2, RY = TY + 2, NHALO = RX * RY - BLOCK;
+  // the halo is the four side strips: the 5-point stencil reads no corner
+  constexpr int TY = BLOCK / TX, RX = TX + 2, RY = TY + 2, NHALO = 2 * TX + 2 * TY;
   static_assert(RX * RY <= RN && NHALO <= BLOCK, "tile does not fit the LDS records");
   if (a.active) {
     const int cnt = __syncthreads_count(q.act);
@@ -715,10 +716,10 @@ __device__ __forceinline__ void adjoint32_tile(const AdjArgsM<S>& a, int tile, D
   if ((int)threadIdx.x < NHALO) {
     const int h = (int)threadIdx.x;
     int rx, ry;
-    if (h < RX) { rx = h; ry = 0; }
-    else if (h < 2 * RX) { rx = h - RX; ry = RY - 1; }
-    else if (h < 2 * RX + TY) { rx = 0; ry = h - 2 * RX + 1; }
-    else { rx = RX - 1; ry = h - 2 * RX - TY + 1; }
+    if (h < TX) { rx = h + 1; ry = 0; }
+    else if (h < 2 * TX) { rx = h - TX + 1; ry = RY - 1; }
+    else if (h < 2 * TX + TY) { rx = 0; ry = h - 2 * TX + 1; }
+    else { rx = RX - 1; ry = h - 2 * TX - TY + 1; }
     const int gi = t.x0 - 1 + rx, gj = t.y0 - 1 + ry;
     if (gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny)
       lds_put(L, ry * RX + rx, dual_info<S, float>(a, I, gi, gj, want_dphi));
