@@ -584,6 +584,54 @@ LEAF_TABLE_BYTES = 68
 LEAF_SHORT_EXTRA = {"coef": 0, "update": 52, "adjoint": 52}
 
 
+# Algorithmic bytes per error element of one error-indicator evaluation:
+# reconstruction reads the rule (4), the stencil's cells (16) and points
+# (64) and a quarter of the father leaves' nine fields (18), writes nine
+# fields (72); kact reads the reconstructed previous state (24) and the
+# tables (68: nbr 32, packed word 4, size 16, centre 16), writes kact (8);
+# the residual reads the three reconstructions (72), kact (8), seven statics
+# (56), the tables (68) and a quarter of the father's adjoint (6), writes the
+# residual, error and correction (40).
+ERROR_BYTES = {"recon": 4 + 16 + 64 + 18 + 72, "kact": 24 + 68 + 8,
+               "residual": 72 + 8 + 56 + 68 + 6 + 40}
+
+
+def error_indicator_leg(sim, pb, K, w, reps=5):
+    """The error indicator on the mesh's error grid (every leaf refined once),
+    evaluated as the reference's first error_compute call: the last step's
+    previous state U^{K-1} and state U^K with the functional's sensitivity
+    as the adjoint.  Kernel times from HIP events, median of reps calls."""
+    import numpy as np
+
+    from discrete_adjoint_amd import errorgrid, problems
+    t0 = time.perf_counter()
+    geom = (pb.nx0, pb.ny0, pb.xmin, pb.ymin, pb.dx0, pb.dy0)
+    eg = errorgrid.build(geom, errorgrid.cells_of_table(pb.table, pb.xmin, pb.ymin),
+                         problems.leaf_dem_statics(pb))
+    setup = time.perf_counter() - t0
+    prev = np.array(sim.trajectory_state(K - 1))
+    state = np.array(sim.trajectory_state(K))
+    dt = float(sim.dt_record()[K - 1])
+    lam = np.asarray(w, dtype=np.float64).reshape(3, -1)
+    sim.set_error_grid(eg)
+    ms = []
+    for _ in range(reps):
+        out = sim.error_indicator(prev, state, lam, K - 1, dt)
+        ms.append(sim.error_kernel_ms())
+    ms = np.median(np.array(ms), axis=0)
+    names = ("recon", "kact", "residual")
+    per = {nm: {"ms": float(m), "bytes_per_element": ERROR_BYTES[nm],
+                "gbs": ERROR_BYTES[nm] * eg.n / (m * 1e-3) / 1e9} for nm, m in zip(names, ms)}
+    tot = float(ms.sum())
+    res = {"elements": eg.n, "setup_s": setup, "kernels": per, "kernel_ms": tot,
+           "gbs": sum(ERROR_BYTES.values()) * eg.n / (tot * 1e-3) / 1e9,
+           "rules": np.bincount(eg.kind, minlength=5).tolist(),
+           "error_sum": float(out["error"].sum()), "correction_sum": float(out["correction"].sum())}
+    log(f"error indicator: {eg.n} elements, kernels {tot:.3f} ms ({res['gbs']:.0f} GB/s), "
+        f"host set-up {setup:.1f} s")
+    return res
+
+
 def adaptive_leg(K=200, k=25, graph=True, cpu_steps=0):
     """The adaptive-mesh (leaf table) FP64 path on the C1-size mesh: about
     0.88 M leaves (5 m around two piles, 40 m base cells, three levels of
@@ -612,6 +660,7 @@ def adaptive_leg(K=200, k=25, graph=True, cpu_steps=0):
         ab = {n: v + LEAF_TABLE_BYTES for n, v in algo_bytes(_lib.TG_FP64).items()}
         sb = {n: v + LEAF_SHORT_EXTRA[n] for n, v in short_bytes(_lib.TG_FP64).items()}
         _, kin, _ = instrumented_roofline(sim, _lib.TG_FP64, pb.n, K, w, h0, ab=ab, sb=sb)
+        err_ind = error_indicator_leg(sim, pb, K, w)
     dom, rl = replay_roofline(r, kin, K, k)
     cs = pb.n * K
     out = {
@@ -624,6 +673,7 @@ def adaptive_leg(K=200, k=25, graph=True, cpu_steps=0):
         "algorithmic_bytes_per_leaf": ab, "short_path_bytes_per_leaf": sb,
         "forward_step_hbm": forward_step_json(rl, _lib.TG_FP64, ab, sb),
         "dJ_dphibed": float(r["dphi"][0]),
+        "error_indicator": err_ind,
     }
     log(f"adaptive: {out['cell_steps_per_s']:.3g} cell-steps/s fwd+adj, timed run "
         + ", ".join(f"{a} {v['avg_launch_ms']:.4f} ms/launch ({v['gbs']:.0f} GB/s)"
