@@ -525,6 +525,11 @@ void enqueue_adjoint_m(tg_ctx* ctx, void* const U[3], void* const lnext[3], void
   a.ncell = ctx->ncell;
   a.tile_list = ctx->act_list;
   a.tile_count = ctx->act_count;
+  a.mk_U = marks_of(ctx, U);
+  a.mk_lnext = marks_of(ctx, lnext);
+  a.mk_out = marks_of(ctx, lout);
+  a.u_bytes = !ctx->bytes_off && SHAPE_TX[ctx->sel[2]] == ctx->mk_tx[0];
+  a.l_bytes = !ctx->bytes_off && SHAPE_TX[ctx->sel[2]] == ctx->mk_tx[1];
   a.active = inst_counter(ctx, 2, true);
   inst_short(ctx, 2, false);
   inst_mark(ctx, 2, true);
@@ -1377,9 +1382,10 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   auto align256 = [](size_t b) { return (b + 255) / 256 * 256; };
   ctx->mk_off = align256(3 * fb);
   ctx->state_bytes = ctx->mk_off + align256(sizeof(uint32_t) * (size_t)ctx->nsuper);
-  // the FP64 uniform kernels keep zero marks; the leaf and the
-  // mixed-precision kernels do not
-  ctx->marks = ctx->mode == TG_FP64 && ctx->nsuper > 0;
+  // the FP64 uniform kernels keep zero marks, and so does the FP32 adjoint
+  // of TG_FWD64_ADJ32 (its forward runs the FP64 kernels); the leaf kernels
+  // and the FP32 forward do not
+  ctx->marks = fwd64(ctx) && ctx->nsuper > 0;
   int rc = 0;
   void** bufs[4] = {ctx->U, ctx->Ut, ctx->lam[0], ctx->lam[1]};
   for (int i = 0; i < 4 && !rc; i++) {

@@ -104,6 +104,12 @@ constexpr int MARK_W = 64, MARK_H = 16;
 constexpr uint32_t MARK_FULL = 0x01010101u;
 static_assert(MARK_W % 64 == 0 && MARK_H % (BLOCK / 16) == 0 && MARK_W * MARK_H == 4 * BLOCK,
               "every tile shape must cut a mark block into four tiles");
+// the byte of its block's word that a tile of width TX owns
+template <int TX>
+__device__ __forceinline__ int mark_byte(const TileDev& t) {
+  constexpr int TY = BLOCK / TX;
+  return ((t.y0 % MARK_H) / TY) * (MARK_W / TX) + (t.x0 % MARK_W) / TX;
+}
 
 struct Fields {
   const double* h;
@@ -276,6 +282,12 @@ struct AdjArgsM {
   int* tile_count;
   const uint8_t* mat;    // as in StepArgs
   const MatDev* mtab;
+  // zero marks and their byte flags, as in AdjArgs (kept when the forward
+  // runs the FP64 kernels, which keep the marks of U; else all null)
+  const uint32_t* mk_U;
+  const uint32_t* mk_lnext;
+  uint32_t* mk_out;
+  int u_bytes, l_bytes;
 };
 // workgroups of the FP64 thin-layer adjoint pass (strides over its list)
 constexpr int THIN_GRID = 1024;

@@ -69,12 +69,6 @@ __device__ __forceinline__ Cell cell_of(const TileDev& t, const InstDev& I) {
 
 // ------------------------------------------------------------- zero marks --
 
-// the tile's byte within its block's word
-template <int TX>
-__device__ __forceinline__ int mark_byte(const TileDev& t) {
-  constexpr int TY = BLOCK / TX;
-  return ((t.y0 % MARK_H) / TY) * (MARK_W / TX) + (t.x0 % MARK_W) / TX;
-}
 __device__ __forceinline__ bool bitzero3(double a, double b, double c) {
   return (dbits(a) | dbits(b) | dbits(c)) == 0ull;
 }
@@ -386,20 +380,15 @@ __global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
   const long long nx = I.nx;
   const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
   unsigned wet = 0u;
-  // the state of every cell is read here, before the barriers: a dry tile's
-  // update needs it, and one round trip to memory serves both uses
+  // h of every cell not known to be +0.0 (the wet test); the momenta only
+  // once the tile is known to be dry
   double out[4][3];
 #pragma unroll
   for (int b = 0; b < 4; b++) {
     if (!q[b].act) continue;
     const long long c = q[b].c;
-    if ((zin >> (8 * b)) & 1u) {
-      out[b][0] = out[b][1] = out[b][2] = 0.;
-    } else {
-      out[b][0] = a.U.h[c];
-      out[b][1] = a.U.hu[c];
-      out[b][2] = a.U.hv[c];
-    }
+    out[b][0] = out[b][1] = out[b][2] = 0.;
+    if (!((zin >> (8 * b)) & 1u)) out[b][0] = a.U.h[c];
     bool w = !(out[b][0] < a.tiny);
     if (tx == 0 && q[b].hasW) w = w || !(a.U.h[c - 1] < a.tiny);
     if (tx == TX - 1 && q[b].hasE) w = w || !(a.U.h[c + 1] < a.tiny);
@@ -416,6 +405,14 @@ __global__ __launch_bounds__(BLOCK) void k_update_classify(StepArgs a) {
 #pragma unroll
     for (int b = 0; b < 4; b++)
       if ((W >> b) & 1u) a.tile_list[k++] = B.tile[b];
+  }
+  // the momenta of the dry tiles that hold something (a wet tile's cells are
+  // read again by the full path)
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    if (!q[b].act || ((W >> b) & 1u) || ((zin >> (8 * b)) & 1u)) continue;
+    out[b][1] = a.U.hu[q[b].c];
+    out[b][2] = a.U.hv[q[b].c];
   }
   const double dtdx = dt / I.dx, dtdy = dt / I.dy, zero = 0.;
   unsigned nz = 0u;

@@ -83,15 +83,15 @@ __global__ __launch_bounds__(BLOCK) void k32_coef(StepArgsM<S> a, int reduce) {
     const long long c = q.c, nx = I.nx;
     const W h = (W)a.U.h[c], hu = (W)a.U.hu[c], hv = (W)a.U.hv[c];
     const W ix = (W)I.inv_dx, iy = (W)I.inv_dy;
-    W hx = 0, hux = 0, hy = 0, hvy = 0;
-    if (q.hasE && q.hasW && q.hasN && q.hasS) {
-      hx = slope<W>((W)a.U.h[c + 1], h, (W)a.U.h[c - 1], ix, gamma, tiny);
-      hux = slope<W>((W)a.U.hu[c + 1], hu, (W)a.U.hu[c - 1], ix, gamma, tiny);
-      hy = slope<W>((W)a.U.h[c + nx], h, (W)a.U.h[c - nx], iy, gamma, tiny);
-      hvy = slope<W>((W)a.U.hv[c + nx], hv, (W)a.U.hv[c - nx], iy, gamma, tiny);
-    }
     float k = (float)I.eps;
-    if (h > tiny) {
+    if (h > tiny) {  // (kact reads the slopes only above tiny)
+      W hx = 0, hux = 0, hy = 0, hvy = 0;
+      if (q.hasE && q.hasW && q.hasN && q.hasS) {
+        hx = slope<W>((W)a.U.h[c + 1], h, (W)a.U.h[c - 1], ix, gamma, tiny);
+        hux = slope<W>((W)a.U.hu[c + 1], hu, (W)a.U.hu[c - 1], ix, gamma, tiny);
+        hy = slope<W>((W)a.U.h[c + nx], h, (W)a.U.h[c - nx], iy, gamma, tiny);
+        hvy = slope<W>((W)a.U.hv[c + nx], hv, (W)a.U.hv[c - nx], iy, gamma, tiny);
+      }
       const W ih = inv_h(h);
       const W u = vel(hu, h, ih), v = vel(hv, h, ih);
       k = kact<W>(h, u, v, ih, hx, hux, hy, hvy, (float)I.kc1,
@@ -123,51 +123,118 @@ __global__ __launch_bounds__(BLOCK) void k32_coef(StepArgsM<S> a, int reduce) {
   }
 }
 
-template <class S>
-__device__ __forceinline__ Hfv hfv_at(const StepArgsM<S>& a, int dir, long long c) {
-  using W = Wt<S>;
-  const W h = (W)a.U.h[c], hu = (W)a.U.hu[c], hv = (W)a.U.hv[c];
-  const bool wet = !(h < (W)a.tiny);
-  const W un = wet ? vel(dir ? hv : hu, h, inv_h(h)) : W(0);
-  return hfv(dir, (float)h, (float)hu, (float)hv, (float)un, a.kact[c], a.S32.gz[c], wet);
+// Records of a tile and its one-cell frame, staged once per workgroup: the
+// state in the switch type (slopes, the wet test, sources) and the inputs of
+// the physical flux (hfv_at's normal velocities, kact, g_z); then every face
+// flux of the tile once (a face is shared by two cells).
+template <class W, int TX>
+struct Lds32 {
+  static constexpr int TY = BLOCK / TX, RX = TX + 2, RXY = (TX + 2) * (TY + 2);
+  W h[RXY], hu[RXY], hv[RXY];
+  float ux[RXY], uy[RXY], k[RXY], g[RXY];
+  int wet[RXY];
+  float fx[3][TY][TX + 1], fy[3][TY + 1][TX];
+};
+
+template <class W, int TX>
+__device__ __forceinline__ Hfv rec_hfv32(const Lds32<W, TX>& L, int dir, int r) {
+  return hfv(dir, (float)L.h[r], (float)L.hu[r], (float)L.hv[r], dir ? L.uy[r] : L.ux[r], L.k[r],
+             L.g[r], L.wet[r] != 0);
 }
 
+// One workgroup per tile.  A tile whose cells and frame are all below tiny
+// has every face flux zero and no sources: its cells take the update's own
+// arithmetic with zero fluxes, without the flux and slope work.
 template <class S, int TX>
 __global__ __launch_bounds__(BLOCK) void k32_update(StepArgsM<S> a) {
   using W = Wt<S>;
+  using L32 = Lds32<W, TX>;
+  constexpr int TY = L32::TY, RX = L32::RX, RXY = L32::RXY;
+  __shared__ L32 L;
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
-  if (!q.act) return;
-  const long long c = q.c, nx = I.nx;
+  const long long nx = I.nx;
   const W tiny = (W)a.tiny, gamma = (W)a.gamma;
+  // phase 0: records of the tile and its frame (cells of the member only)
+  bool any_wet = false;
+  for (int r = (int)threadIdx.x; r < RXY; r += BLOCK) {
+    const int gi = t.x0 - 1 + r % RX, gj = t.y0 - 1 + r / RX;
+    if (gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny) {
+      const long long c = I.off + (long long)gj * nx + gi;
+      const W h = (W)a.U.h[c], hu = (W)a.U.hu[c], hv = (W)a.U.hv[c];
+      const bool wet = !(h < tiny);
+      L.h[r] = h; L.hu[r] = hu; L.hv[r] = hv;
+      L.wet[r] = wet ? 1 : 0;
+      float ux = 0.f, uy = 0.f, k = 0.f, g = 0.f;
+      if (wet) {
+        ux = (float)vel(hu, h, inv_h(h));
+        uy = (float)vel(hv, h, inv_h(h));
+        k = a.kact[c];
+        g = a.S32.gz[c];
+        any_wet = true;
+      }
+      L.ux[r] = ux; L.uy[r] = uy; L.k[r] = k; L.g[r] = g;
+    }
+  }
+  const bool tile_wet = __syncthreads_or(any_wet);
+  if (tile_wet) {
+    // phase 1: faces with a cell of the member on both sides
+    for (int f = (int)threadIdx.x; f < (TX + 1) * TY; f += BLOCK) {
+      const int fi = f % (TX + 1), fj = f / (TX + 1);
+      const int gi = t.x0 + fi, gj = t.y0 + fj;
+      if (gi > 0 && gi < I.nx && gj < I.ny) {
+        const int r = (fj + 1) * RX + fi + 1;
+        float fl[3];
+        riemann(rec_hfv32<W, TX>(L, 0, r - 1), rec_hfv32<W, TX>(L, 0, r), fl);
+        for (int v = 0; v < 3; v++) L.fx[v][fj][fi] = fl[v];
+      }
+    }
+    for (int f = (int)threadIdx.x; f < TX * (TY + 1); f += BLOCK) {
+      const int fi = f % TX, fj = f / TX;
+      const int gi = t.x0 + fi, gj = t.y0 + fj;
+      if (gj > 0 && gj < I.ny && gi < I.nx) {
+        const int r = (fj + 1) * RX + fi + 1;
+        float fl[3];
+        riemann(rec_hfv32<W, TX>(L, 1, r - RX), rec_hfv32<W, TX>(L, 1, r), fl);
+        for (int v = 0; v < 3; v++) L.fy[v][fj][fi] = fl[v];
+      }
+    }
+    __syncthreads();
+  }
+  if (!q.act) return;
+  const long long c = q.c;
+  const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
+  const int rs = (ty + 1) * RX + tx + 1;
   const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
   const float dtf = (float)dt, dtdx = (float)(dt * I.inv_dx), dtdy = (float)(dt * I.inv_dy);
   const S hs = a.U.h[c], hus = a.U.hu[c], hvs = a.U.hv[c];
-  const W h = (W)hs, hu = (W)hus, hv = (W)hvs;
-  const W ix = (W)I.inv_dx, iy = (W)I.inv_dy;
-  W hx = 0, hvx = 0, hy = 0, huy = 0;
-  if (q.hasE && q.hasW && q.hasN && q.hasS) {
-    hx = slope<W>((W)a.U.h[c + 1], h, (W)a.U.h[c - 1], ix, gamma, tiny);
-    hvx = slope<W>((W)a.U.hv[c + 1], hv, (W)a.U.hv[c - 1], ix, gamma, tiny);
-    hy = slope<W>((W)a.U.h[c + nx], h, (W)a.U.h[c - nx], iy, gamma, tiny);
-    huy = slope<W>((W)a.U.hu[c + nx], hu, (W)a.U.hu[c - nx], iy, gamma, tiny);
-  }
+  const W h = L.h[rs], hu = L.hu[rs], hv = L.hv[rs];
   float fE[3] = {0.f, 0.f, 0.f}, fW[3] = {0.f, 0.f, 0.f}, fN[3] = {0.f, 0.f, 0.f},
         fS[3] = {0.f, 0.f, 0.f};
-  {
-    const Hfv self = hfv_at(a, 0, c);
-    if (q.hasE) riemann(self, hfv_at(a, 0, c + 1), fE);
-    if (q.hasW) riemann(hfv_at(a, 0, c - 1), self, fW);
-    if (!q.hasE) { fE[0] = fW[0]; fE[1] = fW[1]; fE[2] = fW[2]; }
-    if (!q.hasW) { fW[0] = fE[0]; fW[1] = fE[1]; fW[2] = fE[2]; }
-  }
-  {
-    const Hfv self = hfv_at(a, 1, c);
-    if (q.hasN) riemann(self, hfv_at(a, 1, c + nx), fN);
-    if (q.hasS) riemann(hfv_at(a, 1, c - nx), self, fS);
-    if (!q.hasN) { fN[0] = fS[0]; fN[1] = fS[1]; fN[2] = fS[2]; }
-    if (!q.hasS) { fS[0] = fN[0]; fS[1] = fN[1]; fS[2] = fN[2]; }
+  W hx = 0, hvx = 0, hy = 0, huy = 0;
+  if (tile_wet) {
+    const W ix = (W)I.inv_dx, iy = (W)I.inv_dy;
+    if (q.hasE && q.hasW && q.hasN && q.hasS) {
+      hx = slope<W>(L.h[rs + 1], h, L.h[rs - 1], ix, gamma, tiny);
+      hvx = slope<W>(L.hv[rs + 1], hv, L.hv[rs - 1], ix, gamma, tiny);
+      hy = slope<W>(L.h[rs + RX], h, L.h[rs - RX], iy, gamma, tiny);
+      huy = slope<W>(L.hu[rs + RX], hu, L.hu[rs - RX], iy, gamma, tiny);
+    }
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+      if (q.hasE) fE[v] = L.fx[v][ty][tx + 1];
+      if (q.hasW) fW[v] = L.fx[v][ty][tx];
+      if (q.hasN) fN[v] = L.fy[v][ty + 1][tx];
+      if (q.hasS) fS[v] = L.fy[v][ty][tx];
+    }
+#pragma unroll
+    for (int v = 0; v < 3; v++) {  // outflow copies
+      if (!q.hasE) fE[v] = fW[v];
+      if (!q.hasW) fW[v] = fE[v];
+      if (!q.hasN) fN[v] = fS[v];
+      if (!q.hasS) fS[v] = fN[v];
+    }
   }
   // increments in FP32, applied to the stored state type S
   float inc[3];
@@ -663,16 +730,50 @@ __device__ __forceinline__ double gather(const AdjArgsM<S>& a, const InstDev& I,
                                  self.lam);
 }
 
+// Zero marks (kept when the forward runs the FP64 kernels).  Tile t is known
+// to hold +0.0 in all three fields of a marked buffer: its block's word is
+// MARK_FULL, or its own byte is 1 and the bytes follow this tile shape.
+template <int TX>
+__device__ __forceinline__ bool tile_marked_zero(const uint32_t* mk, const TileDev& t,
+                                                 int bytes) {
+  const uint32_t w = mk[t.mw];
+  return w == MARK_FULL || (bytes && ((w >> (8 * mark_byte<TX>(t))) & 0xffu) == 1u);
+}
+template <class S>
+__device__ __forceinline__ bool bit_zero(S v) {
+  if constexpr (std::is_same<S, double>::value) return __double_as_longlong(v) == 0ll;
+  else return __float_as_uint(v) == 0u;
+}
+// the tile's byte of the output marks (thread 0 stores it)
+template <int TX>
+__device__ __forceinline__ void tile_mark_store(uint32_t* mk, const TileDev& t, bool zero) {
+  if (threadIdx.x == 0) ((unsigned char*)(mk + t.mw))[mark_byte<TX>(t)] = zero ? 1 : 0;
+}
+
 // Adjoint pass 1 (one workgroup per tile): a tile of void cells (h == 0)
 // gets lambda^s = 0 -- every block that multiplies a void cell's state is
 // zero (calc_jacobain.C:117-122), as in pass 2 -- and, with lambda^{s+1} == 0
 // on the tile, zero bed-friction terms; it is written here.  Every other
-// tile is appended to the list pass 2 walks.
+// tile is appended to the list pass 2 walks.  With marks, a tile whose U^s
+// (and lambda^{s+1}, when the gradient is wanted) is marked +0.0 is void
+// without reading the fields, and its lambda^s is not rewritten when that
+// is marked +0.0 already.
 template <class S, int TX>
 __global__ __launch_bounds__(BLOCK) void k32_adjoint_classify(AdjArgsM<S> a) {
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
   const Cell q = cell_of<TX>(t, I);
+  if (a.mk_out && tile_marked_zero<TX>(a.mk_U, t, a.u_bytes) &&
+      (!a.dphi_part || tile_marked_zero<TX>(a.mk_lnext, t, a.l_bytes))) {
+    if (tile_marked_zero<TX>(a.mk_out, t, a.l_bytes)) return;
+    if (q.act) {
+      a.lam_h[q.c] = (S)0;
+      a.lam_hu[q.c] = (S)0;
+      a.lam_hv[q.c] = (S)0;
+    }
+    tile_mark_store<TX>(a.mk_out, t, true);
+    return;
+  }
   bool busy = false;
   if (q.act) {
     busy = a.U.h[q.c] != (S)0;
@@ -689,6 +790,7 @@ __global__ __launch_bounds__(BLOCK) void k32_adjoint_classify(AdjArgsM<S> a) {
     a.lam_hu[q.c] = (S)0;
     a.lam_hv[q.c] = (S)0;
   }
+  if (a.mk_out) tile_mark_store<TX>(a.mk_out, t, true);
 }
 
 // Adjoint pass 2, one listed tile.
@@ -726,6 +828,7 @@ __device__ __forceinline__ void adjoint32_tile(const AdjArgsM<S>& a, int tile, D
   }
   __syncthreads();
   double dphi = 0.;
+  bool zero_out = true;  // this cell's lambda^s is +0.0 (or no cell)
   if (q.act) {
     const long long c = q.c;
     const double dtd = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
@@ -744,6 +847,7 @@ __device__ __forceinline__ void adjoint32_tile(const AdjArgsM<S>& a, int tile, D
       // handed to k32_adjoint_thin (FP64), which writes this cell's lambda
       const int slot = atomicAdd(a.thin_count, 1);
       a.thin_list[slot] = make_int2(tile, (int)threadIdx.x);
+      zero_out = false;
     } else {
       const S lc[3] = {a.lam_next.h[c], a.lam_next.hu[c], a.lam_next.hv[c]};
       float adj[3];
@@ -759,8 +863,10 @@ __device__ __forceinline__ void adjoint32_tile(const AdjArgsM<S>& a, int tile, D
       a.lam_h[c] = out[0];
       a.lam_hu[c] = out[1];
       a.lam_hv[c] = out[2];
+      zero_out = bit_zero(out[0]) && bit_zero(out[1]) && bit_zero(out[2]);
     }
   }
+  if (a.mk_out) tile_mark_store<TX>(a.mk_out, t, __syncthreads_and(zero_out));
   if (want_dphi) {
     dphi = wave_sum(dphi);
     if (threadIdx.x % 64 == 0) sp[threadIdx.x / 64] = dphi;

@@ -73,15 +73,16 @@ def test_mixed_forward_and_adjoint_track_fp64(mode):
                 dphi=abs(a["dphi"] - b["dphi"]) / abs(b["dphi"]),
                 mass=abs(a["mass"] - b["mass"]) / b["mass"])
     print("mixed vs fp64 relative errors:", errs, flush=True)
-    assert errs["mass"] < 1e-6
-    assert errs["dt"] < 1e-4
-    # h differs locally where the reference's sign switches (tiny_sgn of the
-    # velocity gradient with frict_tiny = 1e-9, the 1e-4 limiter dead zone)
-    # flip under FP32 rounding; the functionals stay far tighter (below)
-    assert errs["h"] < 1e-2
-    assert errs["J"] < 1e-3
-    assert errs["lam"] < 1e-2
-    assert errs["dphi"] < 1e-2
+    # Bounds about ten times the errors measured on this case (MIXED: dt
+    # 3.3e-8, h 7.4e-8, J 5.4e-8, lambda 1.1e-7, dJ/dphi 8.3e-8, mass
+    # 1.9e-11; FP32: 1.5e-7, 2.0e-6, 5.0e-7, 5.7e-7, 4.5e-7, 1.2e-8).  Over
+    # longer runs the reference's sign switches (tiny_sgn with frict_tiny =
+    # 1e-9, the limiter's 1e-4 dead zone) can flip under FP32 rounding and
+    # the errors grow past these; this case stays clear of such flips.
+    bound = {_lib.TG_MIXED: dict(dt=5e-7, h=1e-6, J=1e-6, lam=2e-6, dphi=1e-6, mass=1e-9),
+             _lib.TG_FP32: dict(dt=2e-6, h=2e-5, J=5e-6, lam=1e-5, dphi=5e-6, mass=2e-7)}[mode]
+    for q, b in bound.items():
+        assert errs[q] < b, (q, errs[q], b)
 
 
 def test_mixed_ensemble_equals_individual_mixed_runs():
@@ -121,6 +122,41 @@ def test_fwd64_adj32_forward_is_fp64_and_adjoint_tracks():
         # lambda^0 within FP32 rounding of the Jacobian arithmetic
         assert _rel(a[2][m], b[2][m]) < 1e-5
     assert np.allclose(a[3], b[3], rtol=1e-4, atol=0)
+
+
+@pytest.mark.parametrize("sel", [(3, 3, 3), (1, 2, 3)])
+def test_fwd64_adj32_marks_change_work_not_result(sel):
+    """TG_FWD64_ADJ32 keeps the FP64 forward's zero marks, and its FP32
+    adjoint skips the tiles they show void and leaves outputs already +0.0
+    unwritten.  The marked sweeps (two in a row: the second starts from the
+    marks the first left) equal the same reverse steps taken one at a time
+    from uploaded states -- an upload clears the marks, so there nothing is
+    skipped by a mark -- bit for bit; the gradient at rtol 1e-12 (partial
+    sums ordered differently)."""
+    from helpers import analytic_problem, paraboloid
+    nx, ny, n = 200, 150, 40
+    L, W = nx * 5.0, ny * 5.0
+    pb = analytic_problem(nx, ny, 5.0, slope_x_deg=20.0, slope_y_deg=4.0,
+                          h=paraboloid(0.2 * L, 0.3 * W, 45.0, 35.0, 6.0), phi_int=33.0,
+                          phi_bed=16.0, nsteps=n, curv=(5e-4, -2e-4))
+    w = pb.box_weights((0.25 * L, 0.6 * L, 0.1 * W, 0.6 * W))
+    print(f"case fwd64-adj32 marks, tile variants {sel}", flush=True)
+    with Simulation(pb, precision=_lib.TG_FWD64_ADJ32, checkpoint_every=7, use_graph=1) as sim:
+        sim.select_tiles(sel)
+        sim.run()
+        dts = sim.dt_record()
+        traj = [[np.array(x, copy=True) for x in sim.trajectory_state(s)] for s in range(n)]
+        sweeps = [sim.dual_solver(w, want_dphi=True) for _ in range(2)]
+        lam = np.asarray(w, dtype=float).reshape(3, -1)
+        g = 0.0
+        for s in range(n - 1, -1, -1):
+            sim.set_state(*traj[s])
+            lam, gs = sim.adjoint_step_grad(s, dts[s], lam)
+            g += gs
+    assert np.abs(lam).max() > 0
+    for lam_s, g_s in sweeps:
+        assert np.array_equal(lam_s, lam)
+        assert g_s == pytest.approx(g, rel=1e-12)
 
 
 def _edge_members():
