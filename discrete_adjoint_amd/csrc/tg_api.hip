@@ -478,6 +478,8 @@ StepArgsM<S> step_args_m(tg_ctx* ctx, void* const in[3], void* const out[3], int
   a.tiny = ctx->cfg.tiny; a.gamma = ctx->cfg.gamma;
   a.h_out = (S*)out[0]; a.hu_out = (S*)out[1]; a.hv_out = (S*)out[2];
   a.err = ctx->err_d;
+  a.mk_in = marks_of(ctx, in);
+  a.mk_out = marks_of(ctx, out);
   return a;
 }
 
@@ -490,6 +492,7 @@ void enqueue_step_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s, b
   inst_short(ctx, 0, false);
   inst_short(ctx, 1, false);
   inst_mark(ctx, 0, true);
+  a.zbytes = !ctx->bytes_off && SHAPE_TX[ctx->sel[0]] == ctx->mk_tx[0];
   launch_coef_m<S>(a, compute_dt, ctx->stream);
   inst_mark(ctx, 0, false);
   if (compute_dt) {
@@ -501,6 +504,7 @@ void enqueue_step_m(tg_ctx* ctx, void* const in[3], void* const out[3], int s, b
   }
   use_tiles(ctx, 1, a);
   inst_mark(ctx, 1, true);
+  a.zbytes = !ctx->bytes_off && SHAPE_TX[ctx->sel[1]] == ctx->mk_tx[0];
   launch_update_m<S>(a, ctx->stream);
   inst_mark(ctx, 1, false);
 }
@@ -1382,10 +1386,10 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   auto align256 = [](size_t b) { return (b + 255) / 256 * 256; };
   ctx->mk_off = align256(3 * fb);
   ctx->state_bytes = ctx->mk_off + align256(sizeof(uint32_t) * (size_t)ctx->nsuper);
-  // the FP64 uniform kernels keep zero marks, and so does the FP32 adjoint
-  // of TG_FWD64_ADJ32 (its forward runs the FP64 kernels); the leaf kernels
-  // and the FP32 forward do not
-  ctx->marks = fwd64(ctx) && ctx->nsuper > 0;
+  // every kernel of the uniform path keeps zero marks (FP64 and the FP32
+  // kernels of the other precisions), and so do the leaf kernels (FP64
+  // only), whose marks are per chunk of leaves
+  ctx->marks = ctx->nsuper > 0;
   int rc = 0;
   void** bufs[4] = {ctx->U, ctx->Ut, ctx->lam[0], ctx->lam[1]};
   for (int i = 0; i < 4 && !rc; i++) {

@@ -252,6 +252,11 @@ struct StepArgsM {
   unsigned long long* active;  // as in StepArgs (not counted by these kernels)
   const uint8_t* mat;          // as in StepArgs
   const MatDev* mtab;
+  // zero marks of U and of the output state and their byte flag, as in
+  // StepArgs (null: no marks); the tiles read and store their own bytes
+  const uint32_t* mk_in;
+  uint32_t* mk_out;
+  int zbytes;
 };
 template <class S>
 struct AdjArgsM {
@@ -282,8 +287,7 @@ struct AdjArgsM {
   int* tile_count;
   const uint8_t* mat;    // as in StepArgs
   const MatDev* mtab;
-  // zero marks and their byte flags, as in AdjArgs (kept when the forward
-  // runs the FP64 kernels, which keep the marks of U; else all null)
+  // zero marks and their byte flags, as in AdjArgs (all null: no marks)
   const uint32_t* mk_U;
   const uint32_t* mk_lnext;
   uint32_t* mk_out;

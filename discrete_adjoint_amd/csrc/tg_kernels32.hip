@@ -69,6 +69,58 @@ __device__ __forceinline__ Cell cell_of(const TileDev& t, const InstDev& I) {
   return q;
 }
 
+// ------------------------------------------------------------- zero marks --
+
+// Tile t is known to hold +0.0 in all three fields of a marked buffer: its
+// block's word is MARK_FULL, or its own byte is 1 and the bytes follow this
+// tile shape (bytes != 0).
+template <int TX>
+__device__ __forceinline__ bool tile_marked_zero(const uint32_t* mk, const TileDev& t,
+                                                 int bytes) {
+  const uint32_t w = mk[t.mw];
+  return w == MARK_FULL || (bytes && ((w >> (8 * mark_byte<TX>(t))) & 0xffu) == 1u);
+}
+template <class S>
+__device__ __forceinline__ bool bit_zero(S v) {
+  if constexpr (std::is_same<S, double>::value) return __double_as_longlong(v) == 0ll;
+  else return __float_as_uint(v) == 0u;
+}
+// the tile's byte of the output marks (thread 0 stores it)
+template <int TX>
+__device__ __forceinline__ void tile_mark_store(uint32_t* mk, const TileDev& t, bool zero) {
+  if (threadIdx.x == 0) ((unsigned char*)(mk + t.mw))[mark_byte<TX>(t)] = zero ? 1 : 0;
+}
+
+// The tile next to t on side s (0 W, 1 E, 2 S, 3 N) holds +0.0 per the
+// marks, or there is none (t lies on the member's edge).
+template <int TX>
+__device__ __forceinline__ bool side_tile_zero(const uint32_t* mk, const TileDev& t,
+                                               const InstDev& I, int s, int bytes) {
+  constexpr int TY = BLOCK / TX;
+  TileDev n = t;
+  int w;
+  if (s == 0) {
+    if (t.x0 == 0) return true;
+    n.x0 = t.x0 - TX;
+    w = (t.x0 % MARK_W == 0) ? t.mwW : t.mw;
+  } else if (s == 1) {
+    if (t.x0 + TX >= I.nx) return true;
+    n.x0 = t.x0 + TX;
+    w = (n.x0 % MARK_W == 0) ? t.mwE : t.mw;
+  } else if (s == 2) {
+    if (t.y0 == 0) return true;
+    n.y0 = t.y0 - TY;
+    w = (t.y0 % MARK_H == 0) ? t.mwS : t.mw;
+  } else {
+    if (t.y0 + TY >= I.ny) return true;
+    n.y0 = t.y0 + TY;
+    w = (n.y0 % MARK_H == 0) ? t.mwN : t.mw;
+  }
+  if (w < 0) return false;  // (a neighbour inside the member has a block)
+  n.mw = w;
+  return tile_marked_zero<TX>(mk, n, bytes);
+}
+
 // ---------------------------------------------------------------- forward --
 
 template <class S, int TX>
@@ -76,6 +128,10 @@ __global__ __launch_bounds__(BLOCK) void k32_coef(StepArgsM<S> a, int reduce) {
   using W = Wt<S>;
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
+  // a tile of U marked +0.0: h = +0.0 < tiny everywhere -- no kact read
+  // (the update and the adjoint take kact only above tiny), nothing for the
+  // CFL reduction
+  if (a.mk_in && tile_marked_zero<TX>(a.mk_in, t, a.zbytes)) return;
   const Cell q = cell_of<TX>(t, I);
   const W tiny = (W)a.tiny, gamma = (W)a.gamma;
   double minev = HUGE_VAL, hmax = 0.;
@@ -153,6 +209,16 @@ __global__ __launch_bounds__(BLOCK) void k32_update(StepArgsM<S> a) {
   __shared__ L32 L;
   const TileDev t = a.tiles[blockIdx.x];
   const InstDev I = a.inst[t.inst];
+  const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
+  // the tile and its four side tiles of U marked +0.0: zero fluxes, no
+  // sources (h < tiny), so U^{s+1} = +0.0 + (-dtdx * 0 - dtdy * 0) = +0.0 for
+  // a finite dt; nothing to do when the output tile is marked +0.0 already
+  if (a.mk_in && a.mk_out && isfinite(dt) && tile_marked_zero<TX>(a.mk_out, t, a.zbytes) &&
+      tile_marked_zero<TX>(a.mk_in, t, a.zbytes) && side_tile_zero<TX>(a.mk_in, t, I, 0, a.zbytes) &&
+      side_tile_zero<TX>(a.mk_in, t, I, 1, a.zbytes) &&
+      side_tile_zero<TX>(a.mk_in, t, I, 2, a.zbytes) &&
+      side_tile_zero<TX>(a.mk_in, t, I, 3, a.zbytes))
+    return;
   const Cell q = cell_of<TX>(t, I);
   const long long nx = I.nx;
   const W tiny = (W)a.tiny, gamma = (W)a.gamma;
@@ -202,72 +268,75 @@ __global__ __launch_bounds__(BLOCK) void k32_update(StepArgsM<S> a) {
     }
     __syncthreads();
   }
-  if (!q.act) return;
-  const long long c = q.c;
-  const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
-  const int rs = (ty + 1) * RX + tx + 1;
-  const double dt = a.dt_rec[(long long)t.inst * a.max_steps + a.step];
-  const float dtf = (float)dt, dtdx = (float)(dt * I.inv_dx), dtdy = (float)(dt * I.inv_dy);
-  const S hs = a.U.h[c], hus = a.U.hu[c], hvs = a.U.hv[c];
-  const W h = L.h[rs], hu = L.hu[rs], hv = L.hv[rs];
-  float fE[3] = {0.f, 0.f, 0.f}, fW[3] = {0.f, 0.f, 0.f}, fN[3] = {0.f, 0.f, 0.f},
-        fS[3] = {0.f, 0.f, 0.f};
-  W hx = 0, hvx = 0, hy = 0, huy = 0;
-  if (tile_wet) {
-    const W ix = (W)I.inv_dx, iy = (W)I.inv_dy;
-    if (q.hasE && q.hasW && q.hasN && q.hasS) {
-      hx = slope<W>(L.h[rs + 1], h, L.h[rs - 1], ix, gamma, tiny);
-      hvx = slope<W>(L.hv[rs + 1], hv, L.hv[rs - 1], ix, gamma, tiny);
-      hy = slope<W>(L.h[rs + RX], h, L.h[rs - RX], iy, gamma, tiny);
-      huy = slope<W>(L.hu[rs + RX], hu, L.hu[rs - RX], iy, gamma, tiny);
-    }
+  bool zero_out = true;  // this cell's U^{s+1} is +0.0 (or no cell)
+  if (q.act) {
+    const long long c = q.c;
+    const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
+    const int rs = (ty + 1) * RX + tx + 1;
+    const float dtf = (float)dt, dtdx = (float)(dt * I.inv_dx), dtdy = (float)(dt * I.inv_dy);
+    const S hs = a.U.h[c], hus = a.U.hu[c], hvs = a.U.hv[c];
+    const W h = L.h[rs], hu = L.hu[rs], hv = L.hv[rs];
+    float fE[3] = {0.f, 0.f, 0.f}, fW[3] = {0.f, 0.f, 0.f}, fN[3] = {0.f, 0.f, 0.f},
+          fS[3] = {0.f, 0.f, 0.f};
+    W hx = 0, hvx = 0, hy = 0, huy = 0;
+    if (tile_wet) {
+      const W ix = (W)I.inv_dx, iy = (W)I.inv_dy;
+      if (q.hasE && q.hasW && q.hasN && q.hasS) {
+        hx = slope<W>(L.h[rs + 1], h, L.h[rs - 1], ix, gamma, tiny);
+        hvx = slope<W>(L.hv[rs + 1], hv, L.hv[rs - 1], ix, gamma, tiny);
+        hy = slope<W>(L.h[rs + RX], h, L.h[rs - RX], iy, gamma, tiny);
+        huy = slope<W>(L.hu[rs + RX], hu, L.hu[rs - RX], iy, gamma, tiny);
+      }
 #pragma unroll
-    for (int v = 0; v < 3; v++) {
-      if (q.hasE) fE[v] = L.fx[v][ty][tx + 1];
-      if (q.hasW) fW[v] = L.fx[v][ty][tx];
-      if (q.hasN) fN[v] = L.fy[v][ty + 1][tx];
-      if (q.hasS) fS[v] = L.fy[v][ty][tx];
-    }
+      for (int v = 0; v < 3; v++) {
+        if (q.hasE) fE[v] = L.fx[v][ty][tx + 1];
+        if (q.hasW) fW[v] = L.fx[v][ty][tx];
+        if (q.hasN) fN[v] = L.fy[v][ty + 1][tx];
+        if (q.hasS) fS[v] = L.fy[v][ty][tx];
+      }
 #pragma unroll
-    for (int v = 0; v < 3; v++) {  // outflow copies
-      if (!q.hasE) fE[v] = fW[v];
-      if (!q.hasW) fW[v] = fE[v];
-      if (!q.hasN) fN[v] = fS[v];
-      if (!q.hasS) fS[v] = fN[v];
+      for (int v = 0; v < 3; v++) {  // outflow copies
+        if (!q.hasE) fE[v] = fW[v];
+        if (!q.hasW) fW[v] = fE[v];
+        if (!q.hasN) fN[v] = fS[v];
+        if (!q.hasS) fS[v] = fN[v];
+      }
     }
+    // increments in FP32, applied to the stored state type S
+    float inc[3];
+#pragma unroll
+    for (int v = 0; v < 3; v++) inc[v] = -dtdx * (fE[v] - fW[v]) - dtdy * (fN[v] - fS[v]);
+    if (h > tiny) {
+      const W ih = inv_h(h);
+      const W u = vel(hu, h, ih), v = vel(hv, h, ih);
+      const W ft = (W)a.ft;
+      const float sgn0 = (float)tiny_sgn<W>(ih * (huy - u * hy), ft);
+      const float sgn2 = (float)tiny_sgn<W>(u, ft);
+      const float sgn1 = (float)tiny_sgn<W>(ih * (hvx - v * hx), ft);
+      const float sgn3 = (float)tiny_sgn<W>(v, ft);
+      const float k = a.kact[c], g2 = a.S32.gz[c];
+      const float hf = (float)h, uf = (float)u, vf = (float)v, huf = (float)hu, hvf = (float)hv;
+      const float sint = (float)I.sin_int, tanb = (float)mat_of(I, a.mat, a.mtab, c).tan_bed;
+      float s1 = a.S32.gx[c] * hf;
+      float s2 = sgn0 * hf * k * (g2 * (float)hy + a.S32.dgy[c] * hf) * sint;
+      float s3 = sgn2 * fmaxf(g2 * hf + uf * huf * a.S32.kx[c], 0.f) * tanb;
+      inc[1] += dtf * (s1 - s2 - s3);
+      s1 = a.S32.gy[c] * hf;
+      s2 = sgn1 * hf * k * (g2 * (float)hx + a.S32.dgx[c] * hf) * sint;
+      s3 = sgn3 * fmaxf(g2 * hf + vf * hvf * a.S32.ky[c], 0.f) * tanb;
+      inc[2] += dtf * (s1 - s2 - s3);
+    }
+    S out0 = hs + (S)inc[0];
+    if (out0 < (S)0) out0 = (S)0;
+    const S out1 = hus + (S)inc[1], out2 = hvs + (S)inc[2];
+    if (!isfinite((double)out0) || !isfinite((double)out1) || !isfinite((double)out2))
+      atomicOr(a.err, 1);
+    a.h_out[c] = out0;
+    a.hu_out[c] = out1;
+    a.hv_out[c] = out2;
+    zero_out = bit_zero(out0) && bit_zero(out1) && bit_zero(out2);
   }
-  // increments in FP32, applied to the stored state type S
-  float inc[3];
-#pragma unroll
-  for (int v = 0; v < 3; v++) inc[v] = -dtdx * (fE[v] - fW[v]) - dtdy * (fN[v] - fS[v]);
-  if (h > tiny) {
-    const W ih = inv_h(h);
-    const W u = vel(hu, h, ih), v = vel(hv, h, ih);
-    const W ft = (W)a.ft;
-    const float sgn0 = (float)tiny_sgn<W>(ih * (huy - u * hy), ft);
-    const float sgn2 = (float)tiny_sgn<W>(u, ft);
-    const float sgn1 = (float)tiny_sgn<W>(ih * (hvx - v * hx), ft);
-    const float sgn3 = (float)tiny_sgn<W>(v, ft);
-    const float k = a.kact[c], g2 = a.S32.gz[c];
-    const float hf = (float)h, uf = (float)u, vf = (float)v, huf = (float)hu, hvf = (float)hv;
-    const float sint = (float)I.sin_int, tanb = (float)mat_of(I, a.mat, a.mtab, c).tan_bed;
-    float s1 = a.S32.gx[c] * hf;
-    float s2 = sgn0 * hf * k * (g2 * (float)hy + a.S32.dgy[c] * hf) * sint;
-    float s3 = sgn2 * fmaxf(g2 * hf + uf * huf * a.S32.kx[c], 0.f) * tanb;
-    inc[1] += dtf * (s1 - s2 - s3);
-    s1 = a.S32.gy[c] * hf;
-    s2 = sgn1 * hf * k * (g2 * (float)hx + a.S32.dgx[c] * hf) * sint;
-    s3 = sgn3 * fmaxf(g2 * hf + vf * hvf * a.S32.ky[c], 0.f) * tanb;
-    inc[2] += dtf * (s1 - s2 - s3);
-  }
-  S out0 = hs + (S)inc[0];
-  if (out0 < (S)0) out0 = (S)0;
-  const S out1 = hus + (S)inc[1], out2 = hvs + (S)inc[2];
-  if (!isfinite((double)out0) || !isfinite((double)out1) || !isfinite((double)out2))
-    atomicOr(a.err, 1);
-  a.h_out[c] = out0;
-  a.hu_out[c] = out1;
-  a.hv_out[c] = out2;
+  if (a.mk_out) tile_mark_store<TX>(a.mk_out, t, __syncthreads_and(zero_out));
 }
 
 __global__ __launch_bounds__(BLOCK) void k32_stats(const InstDev* inst, const TileDev* tiles,
@@ -728,26 +797,6 @@ __device__ __forceinline__ double gather(const AdjArgsM<S>& a, const InstDev& I,
   asm volatile("" ::: "memory");
   return (double)dphi_term<S, F>(a, I, c, self, nbr(1), nbr(2), nbr(3), nbr(4), has, dt,
                                  self.lam);
-}
-
-// Zero marks (kept when the forward runs the FP64 kernels).  Tile t is known
-// to hold +0.0 in all three fields of a marked buffer: its block's word is
-// MARK_FULL, or its own byte is 1 and the bytes follow this tile shape.
-template <int TX>
-__device__ __forceinline__ bool tile_marked_zero(const uint32_t* mk, const TileDev& t,
-                                                 int bytes) {
-  const uint32_t w = mk[t.mw];
-  return w == MARK_FULL || (bytes && ((w >> (8 * mark_byte<TX>(t))) & 0xffu) == 1u);
-}
-template <class S>
-__device__ __forceinline__ bool bit_zero(S v) {
-  if constexpr (std::is_same<S, double>::value) return __double_as_longlong(v) == 0ll;
-  else return __float_as_uint(v) == 0u;
-}
-// the tile's byte of the output marks (thread 0 stores it)
-template <int TX>
-__device__ __forceinline__ void tile_mark_store(uint32_t* mk, const TileDev& t, bool zero) {
-  if (threadIdx.x == 0) ((unsigned char*)(mk + t.mw))[mark_byte<TX>(t)] = zero ? 1 : 0;
 }
 
 // Adjoint pass 1 (one workgroup per tile): a tile of void cells (h == 0)

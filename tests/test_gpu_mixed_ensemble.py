@@ -124,22 +124,39 @@ def test_fwd64_adj32_forward_is_fp64_and_adjoint_tracks():
     assert np.allclose(a[3], b[3], rtol=1e-4, atol=0)
 
 
+def _marked_pile(n):
+    """A small pile on a 200 x 150 block: most tiles stay +0.0 (zero marks)."""
+    from helpers import analytic_problem, paraboloid
+    nx, ny = 200, 150
+    L, W = nx * 5.0, ny * 5.0
+    pb = analytic_problem(nx, ny, 5.0, slope_x_deg=20.0, slope_y_deg=4.0,
+                          h=paraboloid(0.2 * L, 0.3 * W, 45.0, 35.0, 6.0), phi_int=33.0,
+                          phi_bed=16.0, nsteps=n, curv=(5e-4, -2e-4))
+    return pb, pb.box_weights((0.25 * L, 0.6 * L, 0.1 * W, 0.6 * W))
+
+
+def _unmarked_reverse(sim, traj, dts, w):
+    """The reverse steps one at a time from uploaded states: an upload
+    clears the marks, so no tile is skipped by a mark."""
+    lam = np.asarray(w, dtype=float).reshape(3, -1)
+    g = 0.0
+    for s in range(len(traj) - 1, -1, -1):
+        sim.set_state(*traj[s])
+        lam, gs = sim.adjoint_step_grad(s, dts[s], lam)
+        g += gs
+    return lam, g
+
+
 @pytest.mark.parametrize("sel", [(3, 3, 3), (1, 2, 3)])
 def test_fwd64_adj32_marks_change_work_not_result(sel):
     """TG_FWD64_ADJ32 keeps the FP64 forward's zero marks, and its FP32
     adjoint skips the tiles they show void and leaves outputs already +0.0
     unwritten.  The marked sweeps (two in a row: the second starts from the
     marks the first left) equal the same reverse steps taken one at a time
-    from uploaded states -- an upload clears the marks, so there nothing is
-    skipped by a mark -- bit for bit; the gradient at rtol 1e-12 (partial
+    from uploaded states bit for bit; the gradient at rtol 1e-12 (partial
     sums ordered differently)."""
-    from helpers import analytic_problem, paraboloid
-    nx, ny, n = 200, 150, 40
-    L, W = nx * 5.0, ny * 5.0
-    pb = analytic_problem(nx, ny, 5.0, slope_x_deg=20.0, slope_y_deg=4.0,
-                          h=paraboloid(0.2 * L, 0.3 * W, 45.0, 35.0, 6.0), phi_int=33.0,
-                          phi_bed=16.0, nsteps=n, curv=(5e-4, -2e-4))
-    w = pb.box_weights((0.25 * L, 0.6 * L, 0.1 * W, 0.6 * W))
+    n = 40
+    pb, w = _marked_pile(n)
     print(f"case fwd64-adj32 marks, tile variants {sel}", flush=True)
     with Simulation(pb, precision=_lib.TG_FWD64_ADJ32, checkpoint_every=7, use_graph=1) as sim:
         sim.select_tiles(sel)
@@ -147,12 +164,43 @@ def test_fwd64_adj32_marks_change_work_not_result(sel):
         dts = sim.dt_record()
         traj = [[np.array(x, copy=True) for x in sim.trajectory_state(s)] for s in range(n)]
         sweeps = [sim.dual_solver(w, want_dphi=True) for _ in range(2)]
-        lam = np.asarray(w, dtype=float).reshape(3, -1)
-        g = 0.0
-        for s in range(n - 1, -1, -1):
+        lam, g = _unmarked_reverse(sim, traj, dts, w)
+    assert np.abs(lam).max() > 0
+    for lam_s, g_s in sweeps:
+        assert np.array_equal(lam_s, lam)
+        assert g_s == pytest.approx(g, rel=1e-12)
+
+
+@pytest.mark.parametrize("mode,sel", [(_lib.TG_MIXED, (3, 3, 3)), (_lib.TG_MIXED, (0, 3, 2)),
+                                      (_lib.TG_FP32, (1, 2, 3))])
+def test_fp32_forward_marks_change_work_not_result(mode, sel):
+    """The FP32 forward kernels keep zero marks too (the coefficient pass
+    skips tiles of U marked +0.0, the update skips a tile whose U, side
+    tiles and output are all marked +0.0).  The marked run (checkpoints,
+    graph replay) equals steps taken one at a time from uploaded states (no
+    marks), states and dt bit for bit; its adjoint sweeps equal the single
+    reverse steps."""
+    n = 40
+    pb, w = _marked_pile(n)
+    print(f"case fp32 forward marks, precision {mode}, tile variants {sel}", flush=True)
+    with Simulation(pb, precision=mode, checkpoint_every=7, use_graph=1) as sim:
+        sim.select_tiles(sel)
+        sim.run()
+        dts = sim.dt_record()
+        final = [np.array(x, copy=True) for x in sim.get_state()]
+        traj = [[np.array(x, copy=True) for x in sim.trajectory_state(s)] for s in range(n)]
+        sweeps = [sim.dual_solver(w, want_dphi=True) for _ in range(2)]
+        t = 0.0
+        for s in range(n):
             sim.set_state(*traj[s])
-            lam, gs = sim.adjoint_step_grad(s, dts[s], lam)
-            g += gs
+            sim.set_clock(s, t)
+            sim.step()
+            assert sim.dt_record()[0] == dts[s], s
+            for x, y in zip(sim.get_state(), traj[s + 1] if s + 1 < n else final):
+                assert np.array_equal(x, y), s
+            t += dts[s]
+        lam, g = _unmarked_reverse(sim, traj, dts, w)
+    assert (final[0] > 0).sum() < final[0].size // 4  # most of the block stays dry
     assert np.abs(lam).max() > 0
     for lam_s, g_s in sweeps:
         assert np.array_equal(lam_s, lam)
