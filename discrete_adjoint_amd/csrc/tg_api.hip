@@ -92,6 +92,7 @@ struct tg_ctx {
   int* err_d = nullptr;
   double* part_mass = nullptr;
   double* part_hmax = nullptr;
+  double* dt_last = nullptr;  // [ninst] each member's last recorded dt (stats)
   double* dphi_part = nullptr;
   int2* thin_list = nullptr;  // FP32 adjoint modes: cells gathered in FP64
   int* thin_count = nullptr;
@@ -270,6 +271,7 @@ void free_fields(tg_ctx* ctx) {
   dfree(ctx->mat_d);
   dfree(ctx->mtab_d);
   dfree(ctx->part_hmax);
+  dfree(ctx->dt_last);
   dfree(ctx->dphi_part);
   dfree(ctx->thin_list);
   dfree(ctx->thin_count);
@@ -704,11 +706,12 @@ int fill_stats(tg_ctx* ctx, tg_stats* st, int nst) {
   HIPCHK(ctx, hipMemcpyAsync(t.data(), ctx->time_d, sizeof(double) * ctx->ninst,
                              hipMemcpyDeviceToHost, ctx->stream));
   std::vector<double> dtl(ctx->ninst, 0.);
-  if (ctx->rec_steps > 0)
-    for (int b = 0; b < ctx->ninst; b++)
-      HIPCHK(ctx, hipMemcpyAsync(&dtl[b], ctx->dt_rec + (long long)b * ctx->cfg.max_steps +
-                                              ctx->rec_steps - 1,
-                                 sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (ctx->rec_steps > 0) {  // one copy for all members
+    launch_gather_stride(ctx->dt_rec, ctx->cfg.max_steps, ctx->rec_steps - 1, ctx->ninst,
+                         ctx->dt_last, ctx->stream);
+    HIPCHK(ctx, hipMemcpyAsync(dtl.data(), ctx->dt_last, sizeof(double) * ctx->ninst,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<double> m(ctx->ninst, 0.), hm(ctx->ninst, 0.);
   for (int k = 0; k < nt0; k++) {
@@ -1408,6 +1411,7 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   if (!rc) rc = dalloc(ctx, &ctx->kact, sb);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->part_mass, sizeof(double) * ctx->ntiles);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->part_hmax, sizeof(double) * ctx->ntiles);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->dt_last, sizeof(double) * ctx->ninst);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->dphi_part, sizeof(double) * ctx->ntiles);
   if (!rc && ctx->mode != TG_FP64)
     rc = dalloc(ctx, (void**)&ctx->thin_list, sizeof(int2) * (size_t)n);

@@ -381,6 +381,9 @@ void launch_dt(const StepArgs& a, int ninst, hipStream_t s);
 void launch_update(const StepArgs& a, hipStream_t s);
 void launch_adjoint(const AdjArgs& a, hipStream_t s);
 void launch_reset_red(RedDev* red, int ninst, hipStream_t s);
+// out[b] = src[b * stride + idx], b < n
+void launch_gather_stride(const double* src, long long stride, long long idx, int n, double* out,
+                          hipStream_t s);
 // rec[member * max_steps + step] = max(itself, largest h > 0 over the listed
 // box cells); rec holds FP64 values >= +0.0
 template <class S>

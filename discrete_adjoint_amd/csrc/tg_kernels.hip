@@ -221,6 +221,13 @@ __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
   }
 }
 
+// out[b] = src[b * stride + idx] for b < n (one value per member, e.g. the
+// last recorded dt, fetched with one copy instead of one per member)
+__global__ void k_gather_stride(const double* src, long long stride, long long idx, int n,
+                                double* out) {
+  const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b < n) out[b] = src[(long long)b * stride + idx];
+}
 __global__ void k_reset_red(RedDev* red, int ninst) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < ninst) {
@@ -1175,6 +1182,11 @@ void launch_member_fill(const void* src, void* dst, int es, long long ncell, lon
   const long long n = count * w > nmw ? count * w : nmw;
   hipLaunchKernelGGL(k_member_fill, dim3(grid_for(n)), dim3(256), 0, s, (const uint32_t*)src,
                      (uint32_t*)dst, ncell * w, off * w, count * w, mk, mw0, nmw);
+}
+void launch_gather_stride(const double* src, long long stride, long long idx, int n, double* out,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_stride, dim3((n + 255) / 256), dim3(256), 0, s, src, stride, idx, n,
+                     out);
 }
 void launch_reset_red(RedDev* red, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_red, dim3((ninst + 63) / 64), dim3(64), 0, s, red, ninst);
