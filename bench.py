@@ -1083,7 +1083,8 @@ def main():
         trace_csv, out_path = args.trace_summary.split(",")
         trace_summary(trace_csv, out_path,
                       "cd /tmp && TMPDIR=/tmp rocprofv3 --kernel-trace -d /tmp/prof_sw -o sw "
-                      "--output-format csv -- python3 $REPO/bench.py --sweeps 2 --no-graph")
+                      "--output-format csv -- python3 $REPO/bench.py --sweeps 2 --no-graph"
+                      + (f" --steps {args.steps}" if args.steps else ""))
         return
     if args.sweeps:
         args.steps = args.steps or 500
