@@ -64,6 +64,9 @@ def load(omp=False):
         lib.or_dstep_dphi.restype = None
         lib.or_dstep_dphi.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), C.c_double, _dp,
                                       _dp, _dp, C.c_double, _dp]
+        lib.or_switches.restype = None
+        lib.or_switches.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), C.c_double, _dp,
+                                    _dp, _dp, C.c_double, _dp, C.POINTER(C.c_int)]
         lib.or_kact_slopes.restype = None
         lib.or_kact_slopes.argtypes = [C.POINTER(or_mesh), C.POINTER(or_params), _dp, _dp, _dp,
                                        _dp, _dp]
@@ -203,6 +206,16 @@ class Oracle:
                                   _p(out[2]))
         return out
 
+    def switches(self, U, s, dt):
+        """(sg (2, n), stop (2, n)): the h-slope limiter's sign factor in x
+        and y and the Coulomb stop flags of state U at step s (or_switches)."""
+        sg = np.empty(2 * self.n)
+        stop = np.empty(2 * self.n, dtype=np.int32)
+        self.lib.or_switches(C.byref(self.mesh), C.byref(self.params), frict_tiny(s), _p(U[0]),
+                             _p(U[1]), _p(U[2]), dt, _p(sg),
+                             stop.ctypes.data_as(C.POINTER(C.c_int)))
+        return sg.reshape(2, -1), stop.reshape(2, -1)
+
     def dstep_dphi(self, U, s, dt):
         out = np.empty(3 * self.n)
         self.lib.or_dstep_dphi(C.byref(self.mesh), C.byref(self.params), frict_tiny(s),
@@ -330,6 +343,16 @@ class LeafOracle(Oracle):
                                   _p(b[0]), _p(b[1]), _p(b[2]), dt, _p(out[0]), _p(out[1]),
                                   _p(out[2]))
         return out
+
+    def switches(self, U, s, dt):
+        """(sg (2, n), stop (2, n)): the h-slope limiter's sign factor in x
+        and y and the Coulomb stop flags of state U at step s (or_switches)."""
+        sg = np.empty(2 * self.n)
+        stop = np.empty(2 * self.n, dtype=np.int32)
+        self.lib.or_switches(C.byref(self.mesh), C.byref(self.params), frict_tiny(s), _p(U[0]),
+                             _p(U[1]), _p(U[2]), dt, _p(sg),
+                             stop.ctypes.data_as(C.POINTER(C.c_int)))
+        return sg.reshape(2, -1), stop.reshape(2, -1)
 
     def dstep_dphi(self, U, s, dt):
         out = np.empty(3 * self.n)

@@ -448,6 +448,44 @@ void or_jacobian(const or_mesh* m, const or_params* p, double frict_tiny,
           Jout[(size_t)c * 81 + s * 9 + i * 3 + j] = (s < 5) ? dc[c].J[s][i][j] : 0.;
 }
 
+/* Where the reference's adjoint approximations act, per cell (test
+ * infrastructure: classifies analytic-vs-FD Jacobian mismatches):
+ * sg[2n]   the h-slope limiter's sign factor .5 (c_sgn(dp) + c_sgn(dm)) in x
+ *          and y as slopes_dual forms it (0 on the boundary ring); its
+ *          hslope_sens ignores the factor, so it is exact only for |sg| = 1
+ *          or 0 (element2.C:886-897, dualmesh.C:1393-1626);
+ * stop[2n] update_states' Coulomb stop flags: the neighbour blocks drop the
+ *          internal-friction slope term under stop (calc_jacobain.C). */
+void or_switches(const or_mesh* m, const or_params* p, double ft, const double* h,
+                 const double* hu, const double* hv, double dt, double* sg, int* stop) {
+  int n = m->nx * m->ny;
+  const double* U[3] = {h, hu, hv};
+  dual_cell* dc = build_dual(m, p, ft, U, dt);
+  for (int c = 0; c < n; c++) {
+    sg[c] = sg[n + c] = 0.;
+    if (!on_boundary(m, c))
+      for (int dir = 0; dir < 2; dir++) {
+        int ep = nbr_of(m, c, dir), em = nbr_of(m, c, dir + 2);
+        double dxp = dir ? yc(m, ep) - yc(m, c) : xc(m, ep) - xc(m, c);
+        double dxm = dir ? yc(m, c) - yc(m, em) : xc(m, c) - xc(m, em);
+        double dp = (h[ep] - h[c]) * (1 / dxp), dm = (h[c] - h[em]) * (1 / dxm);
+        sg[dir * n + c] = .5 * (c_sgn(dp, p->tiny) + c_sgn(dm, p->tiny));
+      }
+    double prev[3] = {h[c], hu[c], hv[c]}, out[3];
+    double g[3] = {m->gx[c], m->gy[c], m->gz[c]};
+    double dg[2] = {m->dgx[c], m->dgy[c]};
+    double cv[2] = {m->kx[c], m->ky[c]};
+    double zf[3] = {0, 0, 0};
+    src_info si;
+    or_params pt;
+    const double tan_bed = cell_params(&m->mats, p, c, &pt)->tan_bed;
+    update_states(p, prev, out, zf, zf, zf, zf, 0., 0., dt, dc[c].d, dc[c].d + 3, cv,
+                  p->sin_int, tan_bed, g, dg, dc[c].k, ft, &si);
+    stop[c] = si.stop[0];
+    stop[n + c] = si.stop[1];
+  }
+}
+
 void or_dstep_dphi(const or_mesh* m, const or_params* p, double ft, const double* h,
                    const double* hu, const double* hv, double dt, double* dU) {
   int n = m->nx * m->ny;

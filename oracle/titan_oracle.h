@@ -89,6 +89,13 @@ void or_adjoint_step(const or_mesh* m, const or_params* p, double frict_tiny,
                      double dt, const double* lam_next, const double* g,
                      double* lam_out, double* dJdphi);
 
+/* Per cell: the h-slope limiter's sign factor in x and y (sg[2n]) and the
+ * Coulomb stop flags (stop[2n]) of U at step s -- where the reference's
+ * adjoint approximations act (test classification only). */
+void or_switches(const or_mesh* m, const or_params* p, double frict_tiny,
+                 const double* h, const double* hu, const double* hv, double dt,
+                 double* sg, int* stop);
+
 /* d U^{s+1} / d phi_bed for every cell at frozen dt (3 SoA arrays). */
 void or_dstep_dphi(const or_mesh* m, const or_params* p, double frict_tiny,
                    const double* h, const double* hu, const double* hv,

@@ -174,6 +174,74 @@ def test_jacobian_blocks_match_finite_differences(oracle_lib, s_extra):
     assert agree >= 0.97 * checked, (agree, checked)
 
 
+def _liftoff_problem():
+    """A fast sheet over a convex bed (curvature -0.02 / m along x): the
+    Coulomb term's max(g_z h + u hu k_x, 0) clamps (stop) on most wet cells."""
+    return analytic_problem(30, 20, 10.0, slope_x_deg=30.0, slope_y_deg=2.0,
+                            h=paraboloid(70.0, 100.0, 60.0, 50.0, 8.0), phi_int=32.0,
+                            phi_bed=15.0, nsteps=40, curv=(-0.02, 0.0))
+
+
+@pytest.mark.parametrize("case", ["flow", "liftoff"])
+def test_jacobian_fd_mismatches_are_the_reference_approximations(oracle_lib, case):
+    """Where the analytic blocks differ from central FD of the forward step
+    (frozen dt, no switch flipping inside the FD stencil), the block belongs
+    to a cell where one of the reference's own adjoint approximations acts:
+    the h-slope limiter with one difference in the c_sgn dead zone (sign
+    factor 1/2, which hslope_sens does not carry), or the Coulomb stop (the
+    neighbour blocks drop the internal-friction slope term,
+    calc_jacobain.C), or the positivity clamp of h^{s+1} (not
+    differentiated).  Every other block of every interior wet cell equals
+    FD, and both approximations do show up.  So the gap between the adjoint
+    and FD derivatives (the C2 comparison) is the reference's, not a slip of
+    the restatement."""
+    if case == "flow":
+        pb, nadv, s = small_flow_problem(nsteps=20), 20, 75  # frict_tiny 1e-9
+    else:
+        pb, nadv, s = _liftoff_problem(), 40, 40
+    orc = _orc(oracle_lib, pb)
+    U = _U(pb)
+    _advance(orc, U, nadv)
+    dt = orc.compute_dt(U)
+    J = orc.jacobian(U, s, dt)
+    sg, stop = orc.switches(U, s, dt)
+    nxt = [a.copy() for a in U]
+    t = orc.time
+    orc.step(nxt, s, dt=dt)
+    orc.time = t
+    nx, ny = pb.nx, pb.ny
+    cells = [c for c in range(pb.n)
+             if 1 < c % nx < nx - 2 and 1 < c // nx < ny - 2 and U[0][c] > 1e-3]
+    checked = 0
+    why = {"half-sign": 0, "stop": 0, "clamp": 0}
+    unexplained = []
+    for c in cells:
+        for comp in range(3):
+            e1 = 1e-6 * max(abs(U[comp][c]), U[0][c], 1e-2)
+            fd1 = _fd_step(orc, U, s, dt, c, comp, e1)
+            fd2 = _fd_step(orc, U, s, dt, c, comp, 2 * e1)
+            for j, slot in ((c, 0), (c - 1, 1), (c - nx, 2), (c + 1, 3), (c + nx, 4)):
+                an = -J[j, slot][:, comp]
+                f1, f2 = fd1[:, j], fd2[:, j]
+                if not np.allclose(f1, f2, rtol=1e-4, atol=1e-9):
+                    continue  # a switch flipped inside the stencil
+                checked += 1
+                if np.allclose(an, f1, rtol=1e-5, atol=1e-7 * max(1.0, np.abs(f1).max())):
+                    continue
+                if abs(sg[0, j]) == 0.5 or abs(sg[1, j]) == 0.5:
+                    why["half-sign"] += 1
+                elif stop[:, j].any():
+                    why["stop"] += 1
+                elif nxt[0][j] == 0.0:
+                    why["clamp"] += 1
+                else:
+                    unexplained.append((j, slot, comp))
+    print(case, "checked", checked, "mismatches by cause", why)
+    assert checked > 3000
+    assert not unexplained, unexplained[:10]
+    assert why["half-sign" if case == "flow" else "stop"] > 0
+
+
 def _tangent(J, dU, nx, ny):
     """delta U^{s+1}_j = -sum_slots J_j[slot] delta U^s_{nbr(j,slot)}."""
     n = nx * ny
