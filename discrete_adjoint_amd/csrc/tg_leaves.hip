@@ -371,12 +371,13 @@ __device__ __forceinline__ void side_flux_dk(const AdjArgs& a, const LeafDev& L,
   }
 }
 
+// The gradient term of leaf c from the face fluxes F[s] and their phi_bed
+// derivatives dF[s] of its four sides s (side_flux_dk; a boundary side's
+// entries are replaced by the opposite side's here).
 __device__ double leaf_dphi(const AdjArgs& a, const LeafDev& L, const LeafBufs& B,
                             const InstDev& I, int c, uint32_t meta, const double d[6],
-                            const SrcInfo& si, double dt, double dtdx, double dtdy) {
-  double F[4][3], dF[4][3];
-  for (int s = 0; s < 4; s++)
-    if (kind_of(meta, s) != LEAF_BOUNDARY) side_flux_dk(a, L, B, c, meta, s, F[s], dF[s]);
+                            const SrcInfo& si, double dt, double dtdx, double dtdy,
+                            double F[4][3], double dF[4][3]) {
   for (int s = 0; s < 4; s++)
     if (kind_of(meta, s) == LEAF_BOUNDARY)
       for (int v = 0; v < 3; v++) {
@@ -583,17 +584,40 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_jac(AdjArgs a, LeafDev L, LeafBu
   }
 }
 
-// lambda^{s+1}_c . d U^{s+1}_c / d phi_bed at frozen dt, per workgroup
-// partial sums accumulated over the sweep (bed-friction gradient)
+// lambda^{s+1}_c . d U^{s+1}_c / d phi_bed at frozen dt, per chunk partial
+// sums accumulated over the sweep (bed-friction gradient).  Four lanes per
+// leaf (a workgroup takes a quarter of a chunk): lane k evaluates the face
+// flux(es) of side k and their phi_bed derivatives -- the sides are
+// independent -- and lane 0 gathers the four and adds the source terms, so
+// each lane carries a quarter of the face work of the leaf's chain.
+constexpr int DPHI_LANES = 4;
 __global__ __launch_bounds__(BLOCK) void k_leaf_dphi(AdjArgs a, LeafDev L, LeafBufs B) {
+  constexpr int LPW = BLOCK / DPHI_LANES;  // leaves per workgroup
+  const int chunk = (int)blockIdx.x / DPHI_LANES;
   // lambda^{s+1} = +0.0 on the whole chunk: zero terms
-  if (a.mk_lnext && a.mk_lnext[blockIdx.x] == MARK_FULL) return;
-  const int c = blockIdx.x * BLOCK + threadIdx.x;
-  double dphi = 0.;
+  if (a.mk_lnext && a.mk_lnext[chunk] == MARK_FULL) return;
+  const int c = (int)blockIdx.x * LPW + (int)threadIdx.x / DPHI_LANES;
+  const int side = (int)threadIdx.x % DPHI_LANES;
   // lambda^{s+1}_c == 0 contributes a zero term
-  if (c < L.n && (a.lam_next.h[c] != 0. || a.lam_next.hu[c] != 0. || a.lam_next.hv[c] != 0.)) {
+  const bool act = c < L.n && (a.lam_next.h[c] != 0. || a.lam_next.hu[c] != 0. ||
+                               a.lam_next.hv[c] != 0.);
+  const uint32_t meta = act ? L.meta[c] : 0u;
+  double F1[3] = {0., 0., 0.}, dF1[3] = {0., 0., 0.};
+  if (act && kind_of(meta, side) != LEAF_BOUNDARY) side_flux_dk(a, L, B, c, meta, side, F1, dF1);
+  // the leaf's four lanes are adjacent in the wave: lane 0 of the quad reads
+  // the other three
+  double F[4][3], dF[4][3];
+  const int q0 = (int)(threadIdx.x % 64) & ~(DPHI_LANES - 1);
+#pragma unroll
+  for (int k = 0; k < DPHI_LANES; k++)
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+      F[k][v] = __shfl(F1[v], q0 + k, 64);
+      dF[k][v] = __shfl(dF1[v], q0 + k, 64);
+    }
+  double dphi = 0.;
+  if (act && side == 0) {
     const InstDev I = a.inst[0];
-    const uint32_t meta = L.meta[c];
     double d[6], hs[2][5];
     dual_slopes(a, L, c, meta, d, hs);
     const SrcInfo si = src_switches(a.U.h[c], a.U.hu[c], a.U.hv[c], d[0], d[2], d[3], d[4],
@@ -601,7 +625,7 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_dphi(AdjArgs a, LeafDev L, LeafB
                                     a.S.gz[c], a.ft, a.tiny);
     const double dt = a.dt_rec[a.step];
     const double dtdx = dt / L.cdx[c], dtdy = dt / L.cdx[L.n + c];
-    dphi = leaf_dphi(a, L, B, I, c, meta, d, si, dt, dtdx, dtdy);
+    dphi = leaf_dphi(a, L, B, I, c, meta, d, si, dt, dtdx, dtdy, F, dF);
   }
   __shared__ double sp[BLOCK / 64];
   dphi = lwave_sum(dphi);
@@ -609,7 +633,7 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_dphi(AdjArgs a, LeafDev L, LeafB
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int k = 1; k < BLOCK / 64; k++) dphi += sp[k];
-    a.dphi_part[blockIdx.x] += dphi;
+    atomicAdd(&a.dphi_part[chunk], dphi);  // the chunk's DPHI_LANES workgroups
   }
 }
 
@@ -681,7 +705,8 @@ void launch_leaf_adjoint(const AdjArgs& a, const LeafDev& L, const LeafBufs& B, 
   const dim3 g(leaf_grid(L.n)), b(BLOCK);
   hipLaunchKernelGGL(k_leaf_dualk, g, b, 0, s, a, L, B, a.dphi_part ? 1 : 0);
   hipLaunchKernelGGL(k_leaf_jac, g, b, 0, s, a, L, B);
-  if (a.dphi_part) hipLaunchKernelGGL(k_leaf_dphi, g, b, 0, s, a, L, B);
+  if (a.dphi_part)
+    hipLaunchKernelGGL(k_leaf_dphi, dim3(leaf_grid(L.n) * DPHI_LANES), b, 0, s, a, L, B);
   hipLaunchKernelGGL(k_leaf_gather, g, b, 0, s, a, L, B);
 }
 
