@@ -168,6 +168,19 @@ class tg_pile(C.Structure):
                                           "cosrot", "sinrot", "vx", "vy")]
 
 
+class tg_funky_header(C.Structure):
+    _fields_ = [("version", C.c_int), ("nnodes", C.c_int), ("nelems", C.c_int),
+                ("keyrange", C.c_double * 2), ("xrange", C.c_double * 2),
+                ("yrange", C.c_double * 2)]
+
+
+class tg_funky_elem(C.Structure):
+    _fields_ = [("node_key", (C.c_uint32 * 2) * 9), ("neigh_proc", C.c_int32 * 4),
+                ("neigh_key", (C.c_uint32 * 2) * 4), ("bc_type", C.c_int32 * 4),
+                ("bc_value", ((C.c_float * 2) * 2) * 4), ("elm_loc", C.c_int32 * 2),
+                ("opposite_brother", C.c_uint32 * 2), ("material", C.c_int32)]
+
+
 class tg_run_inputs(C.Structure):
     _fields_ = [("srctype", C.c_int), ("npiles", C.c_int), ("nsources", C.c_int),
                 ("length_scale", C.c_double), ("height_scale", C.c_double),
@@ -265,6 +278,9 @@ SIGNATURES = {
                                      C.c_longlong, C.POINTER(tg_dem), _fp]),
     "tg_host_read_run_inputs": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(tg_pile),
                                           C.POINTER(tg_run_inputs)]),
+    "tg_host_read_funky": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(tg_funky_header),
+                                     C.c_int, C.POINTER(C.c_uint32), _dp, C.c_int,
+                                     C.POINTER(tg_funky_elem)]),
     "tg_host_leaf_static_coefs": (C.c_int, [C.POINTER(tg_dem), C.c_int, _dp, C.c_double,
                                             C.c_double, _dp, _dp]),
     "tg_host_adapt": (C.c_int, [C.c_int, C.c_int, C.POINTER(tg_adapt_params), C.c_int, _i32p,

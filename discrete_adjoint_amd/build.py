@@ -22,7 +22,7 @@ ARCH = os.environ.get("TG_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["tg_api.hip", "tg_kernels.hip", "tg_kernels32.hip", "tg_leaves.hip", "tg_noalloc.hip",
            "tg_error.hip", "tg_host.cpp",
            "tg_flatten.cpp", "tg_solrec.cpp", "tg_adapt.cpp",
-           "tg_gis.cpp", "tg_io.cpp", "tg_error_host.cpp"]
+           "tg_gis.cpp", "tg_io.cpp", "tg_error_host.cpp", "tg_funky.cpp"]
 HEADERS = ["tg_kernels.h", "tg_leafops.h", "tg_math.h", "tg_math32.h"]
 
 FLAGS = [

@@ -95,6 +95,48 @@ def read_run_inputs(directory):
     return f, pl
 
 
+def read_funky(path, length_scale=1.0):
+    """The reference's grid file (tg_host_read_funky): returns (header dict,
+    node keys (n, 2) uint32, node coordinates (n, 2), elements: dict of
+    arrays node_key (m, 9, 2), neigh_proc (m, 4), neigh_key (m, 4, 2),
+    bc_type (m, 4), bc_value (m, 4, 2, 2), elm_loc (m, 2), opposite_brother
+    (m, 2), material (m,)); lengths divided by length_scale."""
+    lib = _lib.load()
+    hdr = _lib.tg_funky_header()
+    rc = lib.tg_host_read_funky(str(path).encode(), float(length_scale), C.byref(hdr), 0, None,
+                                None, 0, None)
+    if rc not in (0, -2):
+        raise ValueError(f"tg_host_read_funky returned {rc}")
+    nn, ne = hdr.nnodes, hdr.nelems
+    keys = np.zeros((max(nn, 1), 2), dtype=np.uint32)
+    xy = np.zeros((max(nn, 1), 2))
+    elems = (_lib.tg_funky_elem * max(ne, 1))()
+    rc = lib.tg_host_read_funky(str(path).encode(), float(length_scale), C.byref(hdr), nn,
+                                keys.ctypes.data_as(C.POINTER(C.c_uint32)), _lib.dptr(xy), ne,
+                                elems)
+    if rc:
+        raise ValueError(f"tg_host_read_funky returned {rc}")
+    raw = np.ctypeslib.as_array(elems)[:ne]
+    el = {name: np.array(raw[name]) for name, _ in _lib.tg_funky_elem._fields_}
+    h = {"version": hdr.version, "nnodes": nn, "nelems": ne,
+         "keyrange": list(hdr.keyrange), "xrange": list(hdr.xrange), "yrange": list(hdr.yrange)}
+    return h, keys[:nn], xy[:nn], el
+
+
+def funky_base_grid(header, elems):
+    """The base grid a grid file's elements tile (the preprocessor's
+    uniform nx x ny grid over the x / y ranges, createfunky.C:250-345):
+    (nx0, ny0, xmin, ymin, dx0, dy0, material (ny0, nx0) with -1 where no
+    element of this file lies)."""
+    loc = elems["elm_loc"]
+    nx0, ny0 = int(loc[:, 0].max()) + 1, int(loc[:, 1].max()) + 1
+    xmin, xmax = header["xrange"]
+    ymin, ymax = header["yrange"]
+    mat = np.full((ny0, nx0), -1, dtype=np.int32)
+    mat[loc[:, 1], loc[:, 0]] = elems["material"]
+    return nx0, ny0, xmin, ymin, (xmax - xmin) / nx0, (ymax - ymin) / ny0, mat
+
+
 def read_grass(gisdbase, location, mapset, name):
     """A GRASS elevation raster (tg_host_read_grass) as the DEM arguments
     (nrows, ncols, xmin, ymax, res, elev float32) of static_coefs."""

@@ -78,6 +78,40 @@ typedef struct tg_run_inputs {
 } tg_run_inputs;
 int tg_host_read_run_inputs(const char* dir, int pile_cap, tg_pile* piles, tg_run_inputs* out);
 
+/* ---- the grid file -----------------------------------------------------------
+ * funky%04d.inp (replaces Read_grid, datread.C:460-660; the format its
+ * preprocessor writes, preprocess.C:560-660, node.C:82-93,
+ * element2.C:233-283): binary in the writer's byte order; version 20061109
+ * (ranges and coordinates as float) or 20061110 (as double).  Node
+ * coordinates and the x / y ranges are divided by length_scale, as Read_grid
+ * does.  Per element: the keys of its 9 nodes (0-3 corners SW SE NE NW, 4-7
+ * side nodes, 8 the bubble node = the element's key), per side 0..3 the
+ * neighbour's process (-1: domain boundary) and key, the boundary
+ * conditions (Read_grid reads and discards them: its BC object is not
+ * attached to the element), elm_loc (the element's i, j in the base grid),
+ * the opposite brother's key and the material index.  The material names
+ * that follow are not read (Read_grid does not either).
+ * node_keys [2*node_cap], node_xy [2*node_cap] (key / x, y interleaved).
+ * Returns 0; -1 bad argument; -2 a capacity too small (hdr holds the counts);
+ * -4 file missing; -5 unknown version, truncated or malformed. */
+typedef struct tg_funky_header {
+  int version, nnodes, nelems;
+  double keyrange[2], xrange[2], yrange[2];
+} tg_funky_header;
+typedef struct tg_funky_elem {
+  uint32_t node_key[9][2];
+  int32_t neigh_proc[4];
+  uint32_t neigh_key[4][2];
+  int32_t bc_type[4];       /* 0 none, 1 essential, 2 natural, 3 both          */
+  float bc_value[4][2][2];  /* [side][essential, natural][x, y]                */
+  int32_t elm_loc[2];
+  uint32_t opposite_brother[2];
+  int32_t material;
+} tg_funky_elem;
+int tg_host_read_funky(const char* path, double length_scale, tg_funky_header* hdr,
+                       int node_cap, uint32_t* node_keys, double* node_xy, int elem_cap,
+                       tg_funky_elem* elems);
+
 /* Analytic DEM: a plane of slope angle slope_deg falling towards +x that
  * ends at x_toe and blends into a flat runout with a C1 cosine transition of
  * total width blend (no reference counterpart; synthetic input). */
