@@ -93,6 +93,11 @@ struct tg_ctx {
   double* part_mass = nullptr;
   double* part_hmax = nullptr;
   double* dt_last = nullptr;  // [ninst] each member's last recorded dt (stats)
+  // leaf path: the step and state buffer whose dual kact the last reverse
+  // launch made ahead (-1: none) and whether with its phi_bed derivative
+  int kd_step = -1;
+  void* kd_U = nullptr;
+  bool kd_dk = false;
   double* dphi_part = nullptr;
   int2* thin_list = nullptr;  // FP32 adjoint modes: cells gathered in FP64
   int* thin_count = nullptr;
@@ -605,8 +610,11 @@ void enqueue_step(tg_ctx* ctx, void* const in[3], void* const out[3], int s, boo
   }
 }
 
+// Unext: U^{s-1} when the sweep's next reverse step reads it from a buffer
+// that already holds it (the leaf path then makes its dual kact in this
+// step's gather launch), else null.
 void enqueue_adjoint(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* const lout[3],
-                     int s, bool dphi) {
+                     int s, bool dphi, void* const Unext[3] = nullptr) {
   const long long n = ctx->ncell;
   if (ctx->mode == TG_FP64) {
     AdjArgs a{};
@@ -632,8 +640,23 @@ void enqueue_adjoint(tg_ctx* ctx, void* const U[3], void* const lnext[3], void* 
     a.active = inst_counter(ctx, 2, true);
     a.short_cells = inst_short(ctx, 2, true);
     inst_mark(ctx, 2, true);
-    if (ctx->leaves) launch_leaf_adjoint(a, ctx->leaf, ctx->lb, ctx->stream);
-    else launch_adjoint(a, ctx->stream);
+    if (ctx->leaves) {
+      // the dual kact of U^s was made by the previous step's gather launch
+      // when that launch was given this step's buffer
+      const bool have_kd = ctx->kd_step == s && ctx->kd_U == U[0] && ctx->kd_dk == dphi;
+      AdjArgs an = a;
+      if (Unext) {
+        an.U = Fields{(double*)Unext[0], (double*)Unext[1], (double*)Unext[2]};
+        an.mk_U = marks_of(ctx, Unext);
+        an.step = s - 1;
+      }
+      launch_leaf_adjoint(a, Unext ? &an : nullptr, !have_kd, ctx->leaf, ctx->lb, ctx->stream);
+      ctx->kd_step = Unext ? s - 1 : -1;
+      ctx->kd_U = Unext ? Unext[0] : nullptr;
+      ctx->kd_dk = dphi;
+    } else {
+      launch_adjoint(a, ctx->stream);
+    }
     inst_mark(ctx, 2, false);
   } else if (ctx->es == 8) {
     enqueue_adjoint_m<double>(ctx, U, lnext, lout, s, dphi);
@@ -951,6 +974,7 @@ long long fnv(const std::vector<long long>& v) {
 int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* steps,
                  double* lambda0, double* dJ) {
   if (ctx->run_steps <= 0) return set_err(ctx, TG_ERR_STATE, "no forward run recorded");
+  ctx->kd_step = -1;  // nothing made ahead for this sweep
   const int N = ctx->run_steps, k = ctx->cfg.checkpoint_every, s0 = ctx->traj_s0;
   std::vector<int> st(ctx->ninst, N);
   int S = 0;
@@ -1026,8 +1050,11 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
   auto stored = [&](int& cur) {
     for (int s = S - 1; s >= s0; s--) {
       void* Us[3];
+      void* Un[3];
       split(ctx, traj_state(ctx, s - s0), Us);
-      enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi);
+      const bool next = s - 1 >= s0;  // U^{s-1} stored too
+      if (next) split(ctx, traj_state(ctx, s - 1 - s0), Un);
+      enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi, next ? Un : nullptr);
       inject(s, ctx->lam[1 - cur][0]);
       cur = 1 - cur;
     }
@@ -1049,8 +1076,11 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
     const int a = sg * k, b = seg_end(a);
     for (int s = b - 1; s >= a; s--) {
       void* Us[3];
+      void* Un[3];
       split(ctx, ctx->seg[s - a], Us);
-      enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi);
+      const bool next = s - 1 >= a;  // U^{s-1} recomputed in this segment
+      if (next) split(ctx, ctx->seg[s - 1 - a], Un);
+      enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi, next ? Un : nullptr);
       inject(s, ctx->lam[1 - cur][0]);
       cur = 1 - cur;
     }
@@ -2164,6 +2194,7 @@ int adjoint_step_impl(tg_ctx* ctx, const tg_params* prm, int s, double dt,
   if (!ctx->have_state) return set_err(ctx, TG_ERR_STATE, "no state");
   int rc = upload_params(ctx, prm);
   if (rc) return rc;
+  ctx->kd_step = -1;
   const long long n = ctx->ncell;
   for (int b = 0; b < ctx->ninst; b++)  // the step's dt goes where the kernel reads it
     HIPCHK(ctx, hipMemcpyAsync(ctx->dt_rec + (long long)b * ctx->cfg.max_steps + s, &dt,
@@ -2224,6 +2255,7 @@ int tg_last_stored_sweep(tg_ctx* ctx, double* stored_ms, int* stored_steps) {
 // selected one), over reps back-to-back launches on the current state.
 int time_kernel_variant(tg_ctx* ctx, int which, int v, int reps, double* avg_ms) {
   const int keep[3] = {ctx->sel[0], ctx->sel[1], ctx->sel[2]};
+  ctx->kd_step = -1;
   ctx->bytes_off = true;
   if (v >= 0) ctx->sel[which] = v;
   // every variant reads U and writes scratch (Ut or lam[1]) with step 0's

@@ -372,7 +372,12 @@ void launch_error(const ErrArgs& a, hipStream_t s, hipEvent_t* ev);
 
 void launch_leaf_coef(const StepArgs& a, const LeafDev& L, bool reduce, hipStream_t s);
 void launch_leaf_update(const StepArgs& a, const LeafDev& L, hipStream_t s);
-void launch_leaf_adjoint(const AdjArgs& a, const LeafDev& L, const LeafBufs& B, hipStream_t s);
+// one reverse step s: the dual kact of U^s (unless a previous launch made
+// it: dualk false), the Jacobian products, the gradient term, the gather;
+// with next (the arguments of step s - 1), the gather launch also makes the
+// dual kact of U^{s-1}
+void launch_leaf_adjoint(const AdjArgs& a, const AdjArgs* next, bool dualk, const LeafDev& L,
+                         const LeafBufs& B, hipStream_t s);
 void launch_leaf_stats(const LeafDev& L, Fields U, double* part_mass, double* part_hmax,
                        double tiny, hipStream_t s);
 

@@ -132,6 +132,7 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_coef(StepArgs a, LeafDev L, int 
   }
 }
 
+
 // A leaf whose own h and its neighbours' are all below tiny has zero face
 // fluxes and no sources: update_states' own arithmetic with zero fluxes.
 __device__ __forceinline__ bool leaf_dry(const StepArgs& a, const LeafDev& L, int c,
@@ -267,10 +268,12 @@ __device__ __forceinline__ void leaf_dualk_wet(const AdjArgs& a, const LeafDev& 
 
 // A chunk of +0.0 leaves whose kd (and dkd, when wanted) already hold the
 // dry values is skipped.
-__global__ __launch_bounds__(BLOCK) void k_leaf_dualk(AdjArgs a, LeafDev L, LeafBufs B,
-                                                      int want_dk) {
+__device__ __forceinline__ bool dualk_skip(const AdjArgs& a, const LeafBufs& B, int want_dk) {
   const uint32_t need = want_dk ? 3u : 1u;
-  if (a.mk_U && a.mk_U[blockIdx.x] == MARK_FULL && (B.kdmk[blockIdx.x] & need) == need) return;
+  return a.mk_U && a.mk_U[blockIdx.x] == MARK_FULL && (B.kdmk[blockIdx.x] & need) == need;
+}
+__device__ __forceinline__ void leaf_dualk_chunk(const AdjArgs& a, const LeafDev& L,
+                                                 const LeafBufs& B, int want_dk) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   bool dry = true;
   if (c < L.n) {
@@ -290,6 +293,11 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_dualk(AdjArgs a, LeafDev L, Leaf
       B.kdmk[blockIdx.x] = all ? (want_dk ? 3u : (1u | (old & 2u))) : 0u;
     }
   }
+}
+__global__ __launch_bounds__(BLOCK) void k_leaf_dualk(AdjArgs a, LeafDev L, LeafBufs B,
+                                                      int want_dk) {
+  if (dualk_skip(a, B, want_dk)) return;
+  leaf_dualk_chunk(a, L, B, want_dk);
 }
 
 __device__ __forceinline__ DualHfv leaf_dual_hfv(const AdjArgs& a, const double* kd, int dir,
@@ -667,8 +675,11 @@ __device__ __forceinline__ void leaf_gather_full(const AdjArgs& a, const LeafDev
 // has every block zero (void_neigh_elem) and lambda = 0, none of its record
 // slots written; a chunk of +0.0 leaves whose output already holds +0.0 is
 // skipped.
-__global__ __launch_bounds__(BLOCK) void k_leaf_gather(AdjArgs a, LeafDev L, LeafBufs B) {
-  if (a.mk_U && a.mk_U[blockIdx.x] == MARK_FULL && a.mk_out[blockIdx.x] == MARK_FULL) return;
+__device__ __forceinline__ bool gather_skip(const AdjArgs& a) {
+  return a.mk_U && a.mk_U[blockIdx.x] == MARK_FULL && a.mk_out[blockIdx.x] == MARK_FULL;
+}
+__device__ __forceinline__ void leaf_gather_chunk(const AdjArgs& a, const LeafDev& L,
+                                                  const LeafBufs& B) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
   const bool full = c < L.n && a.U.h[c] != 0.;
   if (a.active) {  // leaves on the full path: h != 0
@@ -688,6 +699,19 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_gather(AdjArgs a, LeafDev L, Lea
   }
   if (a.mk_out) chunk_mark(a.mk_out, lzero3(out[0], out[1], out[2]));
 }
+__global__ __launch_bounds__(BLOCK) void k_leaf_gather(AdjArgs a, LeafDev L, LeafBufs B) {
+  if (gather_skip(a)) return;
+  leaf_gather_chunk(a, L, B);
+}
+// The gather of step s (a) and the dual kact of step s - 1 (an: U^{s-1})
+// in one launch: the gather reads the step's product records, the dual
+// kact writes kd / dkd, which step s's passes have finished reading.
+__global__ __launch_bounds__(BLOCK) void k_leaf_gather_dualk(AdjArgs a, AdjArgs an, LeafDev L,
+                                                             LeafBufs B, int want_dk) {
+  const bool g = !gather_skip(a), d = !dualk_skip(an, B, want_dk);
+  if (g) leaf_gather_chunk(a, L, B);
+  if (d) leaf_dualk_chunk(an, L, B, want_dk);
+}
 
 unsigned leaf_grid(int n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
@@ -701,13 +725,16 @@ void launch_leaf_update(const StepArgs& a, const LeafDev& L, hipStream_t s) {
   hipLaunchKernelGGL(k_leaf_update, dim3(leaf_grid(L.n)), dim3(BLOCK), 0, s, a, L);
 }
 
-void launch_leaf_adjoint(const AdjArgs& a, const LeafDev& L, const LeafBufs& B, hipStream_t s) {
+void launch_leaf_adjoint(const AdjArgs& a, const AdjArgs* next, bool dualk,
+                         const LeafDev& L, const LeafBufs& B, hipStream_t s) {
   const dim3 g(leaf_grid(L.n)), b(BLOCK);
-  hipLaunchKernelGGL(k_leaf_dualk, g, b, 0, s, a, L, B, a.dphi_part ? 1 : 0);
+  const int want_dk = a.dphi_part ? 1 : 0;
+  if (dualk) hipLaunchKernelGGL(k_leaf_dualk, g, b, 0, s, a, L, B, want_dk);
   hipLaunchKernelGGL(k_leaf_jac, g, b, 0, s, a, L, B);
   if (a.dphi_part)
     hipLaunchKernelGGL(k_leaf_dphi, dim3(leaf_grid(L.n) * DPHI_LANES), b, 0, s, a, L, B);
-  hipLaunchKernelGGL(k_leaf_gather, g, b, 0, s, a, L, B);
+  if (next) hipLaunchKernelGGL(k_leaf_gather_dualk, g, b, 0, s, a, *next, L, B, want_dk);
+  else hipLaunchKernelGGL(k_leaf_gather, g, b, 0, s, a, L, B);
 }
 
 void launch_leaf_stats(const LeafDev& L, Fields U, double* part_mass, double* part_hmax,
