@@ -92,7 +92,13 @@ struct tg_ctx {
   int* err_d = nullptr;
   double* part_mass = nullptr;
   double* part_hmax = nullptr;
-  double* dt_last = nullptr;  // [ninst] each member's last recorded dt (stats)
+  // per-member reductions (k_member_reduce): each variant's tiles grouped by
+  // member (member b: red_idx[v][red_off[v][b] .. red_off[v][b+1]), row-major
+  // within the member), the results on the device and in pinned host memory
+  int* red_off[NSHAPE] = {};
+  int* red_idx[NSHAPE] = {};
+  double* red_out = nullptr;   // [4 * ninst]
+  double* red_host = nullptr;  // [4 * ninst], pinned
   // leaf path: the step and state buffer whose dual kact the last reverse
   // launch made ahead (-1: none) and whether with its phi_bed derivative
   int kd_step = -1;
@@ -276,7 +282,11 @@ void free_fields(tg_ctx* ctx) {
   dfree(ctx->mat_d);
   dfree(ctx->mtab_d);
   dfree(ctx->part_hmax);
-  dfree(ctx->dt_last);
+  dfree(ctx->red_out);
+  for (auto& t : ctx->red_off) dfree(t);
+  for (auto& t : ctx->red_idx) dfree(t);
+  if (ctx->red_host) (void)hipHostFree(ctx->red_host);
+  ctx->red_host = nullptr;
   dfree(ctx->dphi_part);
   dfree(ctx->thin_list);
   dfree(ctx->thin_count);
@@ -720,35 +730,36 @@ int fill_stats(tg_ctx* ctx, tg_stats* st, int nst) {
     launch_stats32(ctx->inst_d, ctx->tiles_d[0], ctx->nt[0],
                    FieldsT<float>{(float*)ctx->U[0], (float*)ctx->U[1], (float*)ctx->U[2]},
                    ctx->part_mass, ctx->part_hmax, ctx->cfg.tiny, ctx->stream);
-  const int nt0 = ctx->nt[0];
-  std::vector<double> pm(nt0), ph(nt0), t(ctx->ninst);
-  HIPCHK(ctx, hipMemcpyAsync(pm.data(), ctx->part_mass, sizeof(double) * nt0,
+  // mass, hmax, time and last dt of every member in one launch and one copy
+  const int nm = ctx->ninst;
+  const bool dt = ctx->rec_steps > 0;
+  launch_member_reduce(ctx->part_mass, ctx->part_hmax, ctx->red_off[0], ctx->red_idx[0], nm,
+                       ctx->time_d, dt ? ctx->dt_rec : nullptr, ctx->cfg.max_steps,
+                       ctx->rec_steps - 1, ctx->red_out, ctx->stream);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->red_host, ctx->red_out, sizeof(double) * 4 * nm,
                              hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ph.data(), ctx->part_hmax, sizeof(double) * nt0,
-                             hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(t.data(), ctx->time_d, sizeof(double) * ctx->ninst,
-                             hipMemcpyDeviceToHost, ctx->stream));
-  std::vector<double> dtl(ctx->ninst, 0.);
-  if (ctx->rec_steps > 0) {  // one copy for all members
-    launch_gather_stride(ctx->dt_rec, ctx->cfg.max_steps, ctx->rec_steps - 1, ctx->ninst,
-                         ctx->dt_last, ctx->stream);
-    HIPCHK(ctx, hipMemcpyAsync(dtl.data(), ctx->dt_last, sizeof(double) * ctx->ninst,
-                               hipMemcpyDeviceToHost, ctx->stream));
-  }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  std::vector<double> m(ctx->ninst, 0.), hm(ctx->ninst, 0.);
-  for (int k = 0; k < nt0; k++) {
-    const int b = ctx->tiles_h[0][k].inst;
-    m[b] += pm[k];
-    hm[b] = (ph[k] > hm[b]) ? ph[k] : hm[b];
-  }
-  for (int b = 0; b < nst && b < ctx->ninst; b++) {
-    st[b].dt = dtl[b];
-    st[b].time = t[b];
-    st[b].mass = m[b];
-    st[b].hmax = hm[b];
+  const double* r = ctx->red_host;
+  for (int b = 0; b < nst && b < nm; b++) {
+    st[b].dt = dt ? r[3 * nm + b] : 0.;
+    st[b].time = r[2 * nm + b];
+    st[b].mass = r[b];
+    st[b].hmax = r[nm + b];
     st[b].step = ctx->rec_steps;
   }
+  return TG_OK;
+}
+
+// Each member's dJ/dphi_bed from the adjoint's per-tile partials (tiles of
+// the adjoint's variant), reduced on the device.
+int member_dphi(tg_ctx* ctx, double* out) {
+  const int nm = ctx->ninst, v = ctx->sel[2];
+  launch_member_reduce(ctx->dphi_part, nullptr, ctx->red_off[v], ctx->red_idx[v], nm, nullptr,
+                       nullptr, 0, 0, ctx->red_out, ctx->stream);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->red_host, ctx->red_out, sizeof(double) * nm,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int b = 0; b < nm; b++) out[b] = ctx->red_host[b];
   return TG_OK;
 }
 
@@ -1168,16 +1179,10 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
     }
   ctx->member_dphi.assign(ctx->ninst, 0.);
   if (dphi) {
-    std::vector<double> part(ctx->ntiles);
-    HIPCHK(ctx, hipMemcpyAsync(part.data(), ctx->dphi_part, sizeof(double) * ctx->ntiles,
-                               hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    rc = member_dphi(ctx, ctx->member_dphi.data());
+    if (rc) return rc;
     double acc = 0;
-    const std::vector<TileDev>& tl = ctx->tiles_h[ctx->sel[2]];
-    for (int t = 0; t < (int)tl.size(); t++) {
-      acc += part[t];
-      ctx->member_dphi[tl[t].inst] += part[t];
-    }
+    for (double v : ctx->member_dphi) acc += v;
     *dJ = acc;
   }
   return TG_OK;
@@ -1441,7 +1446,14 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   if (!rc) rc = dalloc(ctx, &ctx->kact, sb);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->part_mass, sizeof(double) * ctx->ntiles);
   if (!rc) rc = dalloc(ctx, (void**)&ctx->part_hmax, sizeof(double) * ctx->ntiles);
-  if (!rc) rc = dalloc(ctx, (void**)&ctx->dt_last, sizeof(double) * ctx->ninst);
+  if (!rc) rc = dalloc(ctx, (void**)&ctx->red_out, sizeof(double) * 4 * nm);
+  for (int v = 0; v < NSHAPE && !rc; v++) {
+    rc = dalloc(ctx, (void**)&ctx->red_off[v], sizeof(int) * (nm + 1));
+    if (!rc) rc = dalloc(ctx, (void**)&ctx->red_idx[v], sizeof(int) * (ctx->nt[v] + 1));
+  }
+  if (!rc && hipHostMalloc((void**)&ctx->red_host, sizeof(double) * 4 * nm,
+                           hipHostMallocDefault) != hipSuccess)
+    rc = set_err(ctx, TG_ERR_NOMEM, "pinned host buffer of the per-member results");
   if (!rc) rc = dalloc(ctx, (void**)&ctx->dphi_part, sizeof(double) * ctx->ntiles);
   if (!rc && ctx->mode != TG_FP64)
     rc = dalloc(ctx, (void**)&ctx->thin_list, sizeof(int2) * (size_t)n);
@@ -1531,6 +1543,25 @@ int set_members(tg_ctx* ctx, int nm, const tg_mesh* meshes, const tg_params* par
   for (int v = 0; v < NSHAPE; v++)
     HIPCHK(ctx, hipMemcpyAsync(ctx->tiles_d[v], ctx->tiles_h[v].data(),
                                sizeof(TileDev) * ctx->nt[v], hipMemcpyHostToDevice, ctx->stream));
+  std::vector<int> roff[NSHAPE], ridx[NSHAPE];
+  for (int v = 0; v < NSHAPE; v++) {  // tiles grouped by member, row-major within it
+    const std::vector<TileDev>& tl = ctx->tiles_h[v];
+    ridx[v].resize(tl.size());
+    for (int i = 0; i < (int)tl.size(); i++) ridx[v][i] = i;
+    std::stable_sort(ridx[v].begin(), ridx[v].end(), [&](int a, int b) {
+      const TileDev &p = tl[a], &q = tl[b];
+      if (p.inst != q.inst) return p.inst < q.inst;
+      return p.y0 != q.y0 ? p.y0 < q.y0 : p.x0 < q.x0;
+    });
+    roff[v].assign(nm + 1, 0);
+    for (const TileDev& t : tl) roff[v][t.inst + 1]++;
+    for (int b = 0; b < nm; b++) roff[v][b + 1] += roff[v][b];
+    HIPCHK(ctx, hipMemcpyAsync(ctx->red_off[v], roff[v].data(), sizeof(int) * (nm + 1),
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (!tl.empty())
+      HIPCHK(ctx, hipMemcpyAsync(ctx->red_idx[v], ridx[v].data(), sizeof(int) * tl.size(),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  }
   for (int v = 0; v < NSHAPE; v++)
     if (ctx->nb[v] > 0)
       HIPCHK(ctx, hipMemcpyAsync(ctx->blocks_d[v], ctx->blocks_h[v].data(),
@@ -2213,13 +2244,8 @@ int adjoint_step_impl(tg_ctx* ctx, const tg_params* prm, int s, double dt,
     if (rc) return rc;
   }
   if (dphi) {
-    const std::vector<TileDev>& tl = ctx->tiles_h[ctx->sel[2]];
-    std::vector<double> part(tl.size());
-    HIPCHK(ctx, hipMemcpyAsync(part.data(), ctx->dphi_part, sizeof(double) * tl.size(),
-                               hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    for (int b = 0; b < ctx->ninst; b++) dphi_member[b] = 0.;
-    for (size_t t = 0; t < tl.size(); t++) dphi_member[tl[t].inst] += part[t];
+    rc = member_dphi(ctx, dphi_member);
+    if (rc) return rc;
   }
   ctx->run_steps = 0;  // the dt record slot was overwritten
   return TG_OK;

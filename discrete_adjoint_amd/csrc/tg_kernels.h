@@ -386,9 +386,6 @@ void launch_dt(const StepArgs& a, int ninst, hipStream_t s);
 void launch_update(const StepArgs& a, hipStream_t s);
 void launch_adjoint(const AdjArgs& a, hipStream_t s);
 void launch_reset_red(RedDev* red, int ninst, hipStream_t s);
-// out[b] = src[b * stride + idx], b < n
-void launch_gather_stride(const double* src, long long stride, long long idx, int n, double* out,
-                          hipStream_t s);
 // rec[member * max_steps + step] = max(itself, largest h > 0 over the listed
 // box cells); rec holds FP64 values >= +0.0
 template <class S>
@@ -405,5 +402,11 @@ void launch_zero(void* p, size_t bytes, hipStream_t s);
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 void launch_stats(const InstDev* inst, const TileDev* tiles, int ntiles, Fields U,
                   double* part_mass, double* part_hmax, double tiny, hipStream_t s);
+// per-member reduction of per-tile partials (member b's tiles: idx[off[b] ..
+// off[b+1])): out[b] sums, out[n + b] maxima (max_part given), out[2n + b] =
+// time[b] (given), out[3n + b] = dt_rec[b * stride + col] (given)
+void launch_member_reduce(const double* sum_part, const double* max_part, const int* off,
+                          const int* idx, int n, const double* time, const double* dt_rec,
+                          long long stride, int col, double* out, hipStream_t s);
 
 }  // namespace tg

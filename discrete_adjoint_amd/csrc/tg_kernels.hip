@@ -221,13 +221,6 @@ __global__ __launch_bounds__(BLOCK) void k_coef(StepArgs a, int reduce) {
   }
 }
 
-// out[b] = src[b * stride + idx] for b < n (one value per member, e.g. the
-// last recorded dt, fetched with one copy instead of one per member)
-__global__ void k_gather_stride(const double* src, long long stride, long long idx, int n,
-                                double* out) {
-  const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (b < n) out[b] = src[(long long)b * stride + idx];
-}
 __global__ void k_reset_red(RedDev* red, int ninst) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < ninst) {
@@ -604,6 +597,39 @@ __global__ __launch_bounds__(BLOCK) void k_stats(const InstDev* inst, const Tile
     for (int k = 1; k < BLOCK / 64; k++) { m += sm[k]; hm = (sh[k] > hm) ? sh[k] : hm; }
     part_mass[blockIdx.x] = m;
     part_hmax[blockIdx.x] = hm;
+  }
+}
+
+// One workgroup per member b: out[b] = sum of sum_part over member b's
+// tiles idx[off[b] .. off[b+1]) (each lane adds its strided share in list
+// order, the lanes combine by a fixed tree, so the result does not depend on
+// the launch), out[n + b] = the max of max_part over them (when given),
+// out[2n + b] = time[b] and out[3n + b] = dt_rec[b * stride + col] (when
+// given).  Replaces a copy of every tile's partial and a host loop over them.
+__global__ __launch_bounds__(BLOCK) void k_member_reduce(const double* sum_part,
+                                                         const double* max_part,
+                                                         const int* off, const int* idx, int n,
+                                                         const double* time,
+                                                         const double* dt_rec, long long stride,
+                                                         int col, double* out) {
+  const int b = blockIdx.x;
+  double s = 0., m = 0.;
+  for (int j = off[b] + (int)threadIdx.x; j < off[b + 1]; j += BLOCK) {
+    const int t = idx[j];
+    s += sum_part[t];
+    if (max_part) m = (max_part[t] > m) ? max_part[t] : m;
+  }
+  __shared__ double ss[BLOCK / 64], sm[BLOCK / 64];
+  s = wave_sum(s);
+  m = wave_max(m);
+  if (threadIdx.x % 64 == 0) { ss[threadIdx.x / 64] = s; sm[threadIdx.x / 64] = m; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < BLOCK / 64; k++) { s += ss[k]; m = (sm[k] > m) ? sm[k] : m; }
+    out[b] = s;
+    if (max_part) out[n + b] = m;
+    if (time) out[2 * n + b] = time[b];
+    if (dt_rec) out[3 * n + b] = dt_rec[(long long)b * stride + col];
   }
 }
 
@@ -1183,11 +1209,6 @@ void launch_member_fill(const void* src, void* dst, int es, long long ncell, lon
   hipLaunchKernelGGL(k_member_fill, dim3(grid_for(n)), dim3(256), 0, s, (const uint32_t*)src,
                      (uint32_t*)dst, ncell * w, off * w, count * w, mk, mw0, nmw);
 }
-void launch_gather_stride(const double* src, long long stride, long long idx, int n, double* out,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_stride, dim3((n + 255) / 256), dim3(256), 0, s, src, stride, idx, n,
-                     out);
-}
 void launch_reset_red(RedDev* red, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_red, dim3((ninst + 63) / 64), dim3(64), 0, s, red, ninst);
 }
@@ -1195,6 +1216,14 @@ void launch_stats(const InstDev* inst, const TileDev* tiles, int ntiles, Fields 
                   double* part_mass, double* part_hmax, double tiny, hipStream_t s) {
   hipLaunchKernelGGL(k_stats, dim3(ntiles), dim3(BLOCK), 0, s, inst, tiles, U, part_mass,
                      part_hmax, tiny);
+}
+
+void launch_member_reduce(const double* sum_part, const double* max_part, const int* off,
+                          const int* idx, int n, const double* time, const double* dt_rec,
+                          long long stride, int col, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_member_reduce, dim3(n), dim3(BLOCK), 0, s, sum_part, max_part, off, idx,
+                     n, time, dt_rec, stride, col, out);
 }
 
 }  // namespace tg
