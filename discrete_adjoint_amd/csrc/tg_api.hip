@@ -1032,11 +1032,21 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
     }
     clear_marks(ctx, ctx->lam[0][0]);
   } else {
+    src = ctx->fw;
+  }
+  // the weights kept on the device are read in place by the first reverse
+  // step (lambda^S) unless members enter later (lambda^S then differs from
+  // them) or there is no step (lambda^0 = lambda^S is returned)
+  void* top[3] = {nullptr, nullptr, nullptr};
+  if (!w && !staged && S > 0) split(ctx, ctx->fw, top);
+  else if (!w) {
     void* f[3];
     split(ctx, ctx->fw, f);
     copy_state(ctx, ctx->lam[0][0], f);
-    src = ctx->fw;
   }
+  auto lam_next = [&](int s, int cur) -> void* const* {
+    return (top[0] && s == S - 1) ? top : ctx->lam[cur];
+  };
   std::vector<std::vector<int>> enter(S);  // members whose weights enter at lambda^s
   for (int b = 0; b < ctx->ninst; b++) {
     if (st[b] < S) {
@@ -1065,21 +1075,25 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
       split(ctx, traj_state(ctx, s - s0), Us);
       const bool next = s - 1 >= s0;  // U^{s-1} stored too
       if (next) split(ctx, traj_state(ctx, s - 1 - s0), Un);
-      enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi, next ? Un : nullptr);
+      enqueue_adjoint(ctx, Us, lam_next(s, cur), ctx->lam[1 - cur], s, dphi,
+                      next ? Un : nullptr);
       inject(s, ctx->lam[1 - cur][0]);
       cur = 1 - cur;
     }
   };
+  // segment sg's states: U^a read in place from its checkpoint (no kernel
+  // writes a step's input), U^{a+1} .. U^{b-1} recomputed into seg[1 ..]
+  auto seg_state = [&](int sg, int s, void* out[3]) {
+    const int a = sg * k;
+    split(ctx, s == a ? ctx->ckpt[sg] : ctx->seg[s - a], out);
+  };
   auto recompute = [&](int sg) {
     const int a = sg * k, b = seg_end(a);
-    void* base[3];
-    split(ctx, ctx->ckpt[sg], base);
-    copy_state(ctx, ctx->seg[0], base);
     for (int s = a; s + 1 < b; s++) {
       void* in[3];
       void* out[3];
-      split(ctx, ctx->seg[s - a], in);
-      split(ctx, ctx->seg[s - a + 1], out);
+      seg_state(sg, s, in);
+      seg_state(sg, s + 1, out);
       enqueue_step(ctx, in, out, s, false);
     }
   };
@@ -1088,10 +1102,11 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
     for (int s = b - 1; s >= a; s--) {
       void* Us[3];
       void* Un[3];
-      split(ctx, ctx->seg[s - a], Us);
+      seg_state(sg, s, Us);
       const bool next = s - 1 >= a;  // U^{s-1} recomputed in this segment
-      if (next) split(ctx, ctx->seg[s - 1 - a], Un);
-      enqueue_adjoint(ctx, Us, ctx->lam[cur], ctx->lam[1 - cur], s, dphi, next ? Un : nullptr);
+      if (next) seg_state(sg, s - 1, Un);
+      enqueue_adjoint(ctx, Us, lam_next(s, cur), ctx->lam[1 - cur], s, dphi,
+                      next ? Un : nullptr);
       inject(s, ctx->lam[1 - cur][0]);
       cur = 1 - cur;
     }
@@ -1117,7 +1132,8 @@ int adjoint_loop(tg_ctx* ctx, const tg_params* prm, const double* w, const int* 
       v.push_back((long long)(intptr_t)src);
       stage = fnv(v);
     }
-    const long long key[8] = {N, k, dphi + 2 * frict_switch_local(ctx), ctx->sel[0], ctx->sel[1],
+    const long long key[8] = {N, k + (top[0] ? (1LL << 32) : 0),
+                              dphi + 2 * frict_switch_local(ctx), ctx->sel[0], ctx->sel[1],
                               ctx->sel[2], s0, stage};
     if (!same_key(ctx->ga, key)) {
       drop_segment_graphs(ctx);

@@ -188,8 +188,9 @@ def test_single_steps_after_a_marked_run(oracle_lib):
 
 def test_resident_functional_equals_host_weights(oracle_lib):
     """dual_solver with the weights kept on the device (tg_set_functional,
-    fn NULL) gives the sweep with host weights bit for bit, on a single
-    member and on an ensemble."""
+    fn NULL; the first reverse step reads them in place) gives the sweep with
+    host weights bit for bit, on a single member and on an ensemble, and a
+    later sweep with host weights again the same."""
     pb = _pile_problem(128, 64, 0.3, 0.5, 25)
     _say("case resident functional")
     w = _box(pb, 0.2, 0.7, 0.2, 0.8)
@@ -199,8 +200,10 @@ def test_resident_functional_equals_host_weights(oracle_lib):
         sim.set_functional(w)
         lam_d, g_d = sim.dual_solver(None, want_dphi=True)
         lam_d2, g_d2 = sim.dual_solver(None, want_dphi=True)
+        lam_h2, g_h2 = sim.dual_solver(w, want_dphi=True)  # and back to host weights
     assert np.array_equal(lam_h, lam_d) and np.array_equal(lam_h, lam_d2)
-    assert g_h == g_d == g_d2
+    assert np.array_equal(lam_h, lam_h2)
+    assert g_h == g_d == g_d2 == g_h2
     mem = _members(12)
     weights = [_box(p, 0.2, 0.8, 0.2, 0.8).reshape(3, -1) for p in mem]
     with Ensemble(mem, precision=_lib.TG_FP64, checkpoint_every=5) as ens:
@@ -208,6 +211,7 @@ def test_resident_functional_equals_host_weights(oracle_lib):
         lam_h, g_h = ens.dual_solver(weights, want_dphi=True)
         ens.set_functional(weights)
         lam_d, g_d = ens.dual_solver(None, want_dphi=True)
-    for a, b in zip(lam_h, lam_d):
-        assert np.array_equal(a, b)
-    assert np.array_equal(g_h, g_d)
+        lam_h2, g_h2 = ens.dual_solver(weights, want_dphi=True)
+    for a, b, c in zip(lam_h, lam_d, lam_h2):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    assert np.array_equal(g_h, g_d) and np.array_equal(g_h, g_h2)
