@@ -725,7 +725,8 @@ int fill_stats(tg_ctx* ctx, tg_stats* st, int nst) {
   else if (ctx->es == 8)
     launch_stats(ctx->inst_d, ctx->tiles_d[0], ctx->nt[0],
                  Fields{(double*)ctx->U[0], (double*)ctx->U[1], (double*)ctx->U[2]},
-                 ctx->part_mass, ctx->part_hmax, ctx->cfg.tiny, ctx->stream);
+                 marks_of(ctx, ctx->U), ctx->part_mass, ctx->part_hmax, ctx->cfg.tiny,
+                 ctx->stream);
   else
     launch_stats32(ctx->inst_d, ctx->tiles_d[0], ctx->nt[0],
                    FieldsT<float>{(float*)ctx->U[0], (float*)ctx->U[1], (float*)ctx->U[2]},

@@ -400,8 +400,11 @@ void launch_member_fill(const void* src, void* dst, int es, long long ncell, lon
 // bytes must be a multiple of 4 (fields are FP32 or FP64)
 void launch_zero(void* p, size_t bytes, hipStream_t s);
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// per-tile mass and largest h of variant-0 tiles; blocks whose mark word in
+// mk (null: none) is MARK_FULL are not read
 void launch_stats(const InstDev* inst, const TileDev* tiles, int ntiles, Fields U,
-                  double* part_mass, double* part_hmax, double tiny, hipStream_t s);
+                  const uint32_t* mk, double* part_mass, double* part_hmax, double tiny,
+                  hipStream_t s);
 // per-member reduction of per-tile partials (member b's tiles: idx[off[b] ..
 // off[b+1])): out[b] sums, out[n + b] maxima (max_part given), out[2n + b] =
 // time[b] (given), out[3n + b] = dt_rec[b * stride + col] (given)

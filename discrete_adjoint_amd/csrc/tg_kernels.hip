@@ -577,9 +577,14 @@ __global__ __launch_bounds__(BLOCK) void k_update(StepArgs a) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_stats(const InstDev* inst, const TileDev* tiles,
-                                                 Fields U, double* part_mass,
-                                                 double* part_hmax, double tiny) {
+                                                 Fields U, const uint32_t* mk,
+                                                 double* part_mass, double* part_hmax,
+                                                 double tiny) {
   const TileDev t = tiles[blockIdx.x];
+  if (mk && mk[t.mw] == MARK_FULL) {  // the block holds +0.0: no mass, no depth
+    if (threadIdx.x == 0) { part_mass[blockIdx.x] = 0.; part_hmax[blockIdx.x] = 0.; }
+    return;
+  }
   const InstDev I = inst[t.inst];
   const Cell q = cell_of<64>(t, I);
   double m = 0., hm = 0.;
@@ -1213,8 +1218,9 @@ void launch_reset_red(RedDev* red, int ninst, hipStream_t s) {
   hipLaunchKernelGGL(k_reset_red, dim3((ninst + 63) / 64), dim3(64), 0, s, red, ninst);
 }
 void launch_stats(const InstDev* inst, const TileDev* tiles, int ntiles, Fields U,
-                  double* part_mass, double* part_hmax, double tiny, hipStream_t s) {
-  hipLaunchKernelGGL(k_stats, dim3(ntiles), dim3(BLOCK), 0, s, inst, tiles, U, part_mass,
+                  const uint32_t* mk, double* part_mass, double* part_hmax, double tiny,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_stats, dim3(ntiles), dim3(BLOCK), 0, s, inst, tiles, U, mk, part_mass,
                      part_hmax, tiny);
 }
 
