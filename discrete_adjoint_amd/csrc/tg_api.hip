@@ -810,6 +810,21 @@ void copy_state(tg_ctx* ctx, void* dst, void* const src[3]) {
                 sizeof(uint32_t) * (size_t)ctx->nsuper, ctx->stream);
 }
 
+// copy_state for a forward state that also gives the source and the copy
+// the zero marks of their values (the update's tile shape), so that a state
+// set from the host, whose marks say nothing, and its checkpoint are
+// skipped where empty from the first step on; a plain copy where the
+// uniform blocks are not there (leaf meshes, no marks)
+void copy_state_marking(tg_ctx* ctx, void* dst, void* const src[3]) {
+  const int v = ctx->sel[1];
+  if (ctx->leaves || !ctx->marks || ctx->nb[v] != ctx->nsuper) {
+    copy_state(ctx, dst, src);
+    return;
+  }
+  launch_copy_marking(ctx->blocks_d[v], ctx->nb[v], v, ctx->es, src, dst, ctx->ncell,
+                      marks_of(ctx, src), (uint32_t*)((char*)dst + ctx->mk_off), ctx->stream);
+}
+
 // The MaxH record of the state U just produced by step s.
 void enqueue_box_max(tg_ctx* ctx, void* const U[3], int s) {
   if (ctx->nbox_chunks <= 0) return;
@@ -906,8 +921,8 @@ int forward_loop(tg_ctx* ctx, const tg_params* prm, int nsteps, const double* dt
     void** cur = A;
     void* slot[3];
     for (int s = 0; s < nsteps; s++) {
-      if (s < s0 && s % k == 0) copy_state(ctx, ctx->ckpt[s / k], cur);
-      if (s == s0 && s0 == 0 && M > 0) copy_state(ctx, traj_state(ctx, 0), cur);
+      if (s < s0 && s % k == 0) copy_state_marking(ctx, ctx->ckpt[s / k], cur);
+      if (s == s0 && s0 == 0 && M > 0) copy_state_marking(ctx, traj_state(ctx, 0), cur);
       void* dst[3];
       void** dstp;
       if (s + 1 >= s0 && s + 1 < nsteps) {  // U^{s+1} is kept: write it in place

@@ -405,6 +405,13 @@ void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 void launch_stats(const InstDev* inst, const TileDev* tiles, int ntiles, Fields U,
                   const uint32_t* mk, double* part_mass, double* part_hmax, double tiny,
                   hipStream_t s);
+// copy of a state buffer's three fields (src field pointers, dst the copy's
+// base, fields ncell elements of es bytes apart) by the mark blocks of tile
+// variant `shape` (one per mark word), writing both buffers' zero marks from
+// the copied values
+void launch_copy_marking(const BlockDev* blocks, int nblocks, int shape, int es,
+                         const void* const src[3], void* dst, long long ncell, uint32_t* mk_src,
+                         uint32_t* mk_dst, hipStream_t s);
 // per-member reduction of per-tile partials (member b's tiles: idx[off[b] ..
 // off[b+1])): out[b] sums, out[n + b] maxima (max_part given), out[2n + b] =
 // time[b] (given), out[3n + b] = dt_rec[b * stride + col] (given)

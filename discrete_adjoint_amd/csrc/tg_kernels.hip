@@ -605,6 +605,46 @@ __global__ __launch_bounds__(BLOCK) void k_stats(const InstDev* inst, const Tile
   }
 }
 
+// Copies a state buffer's three fields (raw bits, W the element's width)
+// one mark block per workgroup and gives the source and the copy the zero
+// marks of what it read: byte b of the block's word is 1 when tile b
+// (block_cells' order, TX the update's tile width) holds +0.0 in all three
+// fields.  A source whose marks were cleared (a state set from the host)
+// and its checkpoint copy then let the next steps skip what is empty.
+template <int TX, class W>
+__global__ __launch_bounds__(BLOCK) void k_copy_marking(const BlockDev* blocks, const W* s0,
+                                                        const W* s1, const W* s2, W* dst,
+                                                        long long ncell, uint32_t* mk_src,
+                                                        uint32_t* mk_dst) {
+  const BlockDev B = blocks[blockIdx.x];
+  __shared__ unsigned s_nz;
+  if (threadIdx.x == 0) s_nz = 0u;
+  Cell q[4];
+  block_cells<TX>(B, q);
+  unsigned nz = 0u;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    if (!q[b].act) continue;
+    const long long c = q[b].c;
+    const W v0 = s0[c], v1 = s1[c], v2 = s2[c];
+    dst[c] = v0;
+    dst[ncell + c] = v1;
+    dst[2 * ncell + c] = v2;
+    if ((v0 | v1 | v2) != (W)0) nz |= 1u << b;
+  }
+  __syncthreads();  // s_nz cleared
+  if (nz) atomicOr(&s_nz, nz);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t w = 0u;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      if (B.tile[b] >= 0 && !((s_nz >> b) & 1u)) w |= 1u << (8 * b);
+    mk_src[B.mw] = w;
+    mk_dst[B.mw] = w;
+  }
+}
+
 // One workgroup per member b: out[b] = sum of sum_part over member b's
 // tiles idx[off[b] .. off[b+1]) (each lane adds its strided share in list
 // order, the lanes combine by a fixed tree, so the result does not depend on
@@ -1222,6 +1262,41 @@ void launch_stats(const InstDev* inst, const TileDev* tiles, int ntiles, Fields 
                   hipStream_t s) {
   hipLaunchKernelGGL(k_stats, dim3(ntiles), dim3(BLOCK), 0, s, inst, tiles, U, mk, part_mass,
                      part_hmax, tiny);
+}
+
+void launch_copy_marking(const BlockDev* blocks, int nblocks, int shape, int es,
+                         const void* const src[3], void* dst, long long ncell, uint32_t* mk_src,
+                         uint32_t* mk_dst, hipStream_t s) {
+  if (nblocks <= 0) return;
+  if (es == 8) {
+    const uint64_t* f[3] = {(const uint64_t*)src[0], (const uint64_t*)src[1],
+                            (const uint64_t*)src[2]};
+    switch (SHAPE_TX[shape]) {
+      case 64: hipLaunchKernelGGL((k_copy_marking<64, uint64_t>), dim3(nblocks), dim3(BLOCK), 0,
+                                  s, blocks, f[0], f[1], f[2], (uint64_t*)dst, ncell, mk_src,
+                                  mk_dst); break;
+      case 32: hipLaunchKernelGGL((k_copy_marking<32, uint64_t>), dim3(nblocks), dim3(BLOCK), 0,
+                                  s, blocks, f[0], f[1], f[2], (uint64_t*)dst, ncell, mk_src,
+                                  mk_dst); break;
+      default: hipLaunchKernelGGL((k_copy_marking<16, uint64_t>), dim3(nblocks), dim3(BLOCK), 0,
+                                  s, blocks, f[0], f[1], f[2], (uint64_t*)dst, ncell, mk_src,
+                                  mk_dst); break;
+    }
+  } else {
+    const uint32_t* f[3] = {(const uint32_t*)src[0], (const uint32_t*)src[1],
+                            (const uint32_t*)src[2]};
+    switch (SHAPE_TX[shape]) {
+      case 64: hipLaunchKernelGGL((k_copy_marking<64, uint32_t>), dim3(nblocks), dim3(BLOCK), 0,
+                                  s, blocks, f[0], f[1], f[2], (uint32_t*)dst, ncell, mk_src,
+                                  mk_dst); break;
+      case 32: hipLaunchKernelGGL((k_copy_marking<32, uint32_t>), dim3(nblocks), dim3(BLOCK), 0,
+                                  s, blocks, f[0], f[1], f[2], (uint32_t*)dst, ncell, mk_src,
+                                  mk_dst); break;
+      default: hipLaunchKernelGGL((k_copy_marking<16, uint32_t>), dim3(nblocks), dim3(BLOCK), 0,
+                                  s, blocks, f[0], f[1], f[2], (uint32_t*)dst, ncell, mk_src,
+                                  mk_dst); break;
+    }
+  }
 }
 
 void launch_member_reduce(const double* sum_part, const double* max_part, const int* off,
