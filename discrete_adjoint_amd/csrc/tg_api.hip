@@ -2149,7 +2149,14 @@ int tg_set_functional(tg_ctx* ctx, const tg_functional* fn) {
     int rc = h2d_field(ctx, f[c], 0, ctx->ncell, fn->terminal_weight + c * ctx->ncell);
     if (rc) return rc;
   }
-  clear_marks(ctx, ctx->fw);
+  // the weights' zero marks in the adjoint's tile shape (the first reverse
+  // step reads them as lambda^{S}); nothing known on leaf meshes
+  const int v = ctx->sel[2];
+  if (!ctx->leaves && ctx->marks && ctx->nb[v] == ctx->nsuper)
+    launch_copy_marking(ctx->blocks_d[v], ctx->nb[v], v, ctx->es, f, nullptr, ctx->ncell,
+                        marks_of(ctx, f), nullptr, ctx->stream);
+  else
+    clear_marks(ctx, ctx->fw);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->have_fw = true;
   if (fn->step) ctx->fw_steps.assign(fn->step, fn->step + ctx->ninst);

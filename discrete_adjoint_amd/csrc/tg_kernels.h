@@ -408,7 +408,7 @@ void launch_stats(const InstDev* inst, const TileDev* tiles, int ntiles, Fields 
 // copy of a state buffer's three fields (src field pointers, dst the copy's
 // base, fields ncell elements of es bytes apart) by the mark blocks of tile
 // variant `shape` (one per mark word), writing both buffers' zero marks from
-// the copied values
+// the copied values (dst and mk_dst null: the source's marks only)
 void launch_copy_marking(const BlockDev* blocks, int nblocks, int shape, int es,
                          const void* const src[3], void* dst, long long ncell, uint32_t* mk_src,
                          uint32_t* mk_dst, hipStream_t s);

@@ -610,7 +610,8 @@ __global__ __launch_bounds__(BLOCK) void k_stats(const InstDev* inst, const Tile
 // marks of what it read: byte b of the block's word is 1 when tile b
 // (block_cells' order, TX the update's tile width) holds +0.0 in all three
 // fields.  A source whose marks were cleared (a state set from the host)
-// and its checkpoint copy then let the next steps skip what is empty.
+// and its checkpoint copy then let the next steps skip what is empty.  With
+// dst and mk_dst null, only the source's marks are written.
 template <int TX, class W>
 __global__ __launch_bounds__(BLOCK) void k_copy_marking(const BlockDev* blocks, const W* s0,
                                                         const W* s1, const W* s2, W* dst,
@@ -627,9 +628,11 @@ __global__ __launch_bounds__(BLOCK) void k_copy_marking(const BlockDev* blocks, 
     if (!q[b].act) continue;
     const long long c = q[b].c;
     const W v0 = s0[c], v1 = s1[c], v2 = s2[c];
-    dst[c] = v0;
-    dst[ncell + c] = v1;
-    dst[2 * ncell + c] = v2;
+    if (dst) {
+      dst[c] = v0;
+      dst[ncell + c] = v1;
+      dst[2 * ncell + c] = v2;
+    }
     if ((v0 | v1 | v2) != (W)0) nz |= 1u << b;
   }
   __syncthreads();  // s_nz cleared
@@ -641,7 +644,7 @@ __global__ __launch_bounds__(BLOCK) void k_copy_marking(const BlockDev* blocks, 
     for (int b = 0; b < 4; b++)
       if (B.tile[b] >= 0 && !((s_nz >> b) & 1u)) w |= 1u << (8 * b);
     mk_src[B.mw] = w;
-    mk_dst[B.mw] = w;
+    if (mk_dst) mk_dst[B.mw] = w;
   }
 }
 
