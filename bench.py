@@ -903,11 +903,15 @@ def trace_summary(trace_csv, out_path, command):
         f.write("\n".join(lines))
 
 
+PROBE_WARM_SWEEPS = 2
+
+
 def probe(args):
     """Run for hardware-counter profiles (rocprofv3 --pmc): the measured
     ensemble run -- same members, K, checkpoints, stored trajectory and
-    (--tiles) tile variants as the bench's timed run -- once untimed (so the
-    zero marks are warm, as in the timed run), then once launched kernel by
+    (--tiles) tile variants as the bench's timed run -- twice untimed, as the
+    bench's capture and timed sweeps precede its instrumented one (so every
+    buffer's zero marks hold what that sweep finds), then once launched kernel by
     kernel with the kernels counting their full-path and short-path cells; per role (forward_step: coefficients + dt + update, forward and
     recompute launches; adjoint: the reverse steps) it logs the launches and
     the algorithmic bytes per launch of exactly these launches.  Then a 1 GiB
@@ -931,13 +935,16 @@ def probe(args):
                   store_bytes=store) as ens:
         if tiles:  # reproduce a tile selection (the bench's autotuned one)
             ens.select_tiles(tiles)
+        # sweeps first, as the bench's graph-capture and timed sweeps precede
+        # its instrumented one: every buffer's zero marks (segment slots and
+        # adjoint buffers included) hold what the bench's sweep finds there
+        for _ in range(PROBE_WARM_SWEEPS):
+            ens.set_functional(weights)
+            ens.set_state(*h0)
+            ens.run(K)
+            ens.dual_solver(None, want_dphi=True, want_lambda=False)
+        log(f"probe warm sweeps done: {PROBE_WARM_SWEEPS}")
         ens.set_functional(weights)
-        # one sweep first, as the bench's warm-up: every buffer's zero marks
-        # (segment slots included) hold what the timed run finds there
-        ens.set_state(*h0)
-        ens.run(K)
-        ens.dual_solver(None, want_dphi=True, want_lambda=False)
-        log("probe warm sweep done")
         ens.set_state(*h0)
         ens.instrument(True)
         ens.run(K)
@@ -993,16 +1000,20 @@ def traffic_summary(fetch_csv, write_csv, probe_log, kernel, out_path, commands)
     fk, fc = per_dispatch(fetch_csv, "FETCH_SIZE")
     wk, wc = per_dispatch(write_csv, "WRITE_SIZE")
 
-    def last_half(ker):
-        # the probe ran the sweep twice (untimed, then instrumented) with the
-        # same dispatch sequence: keep each kernel's second half
+    def last_share(ker, nsweeps):
+        # the probe ran the sweep nsweeps times (untimed, then instrumented)
+        # with the same dispatch sequence: keep each kernel's last 1/nsweeps
         by = {}
         for nm, v in ker:
             by.setdefault(nm, []).append((nm, v))
-        return [x for lst in by.values() for x in lst[len(lst) - len(lst) // 2:]]
+        return [x for lst in by.values() for x in lst[len(lst) - len(lst) // nsweeps:]]
 
-    if any(line.startswith("probe warm sweep") for line in open(probe_log)):
-        fk, wk = last_half(fk), last_half(wk)
+    for line in open(probe_log):
+        if line.startswith("probe warm sweeps done: "):
+            n = int(line.split()[-1]) + 1
+            fk, wk = last_share(fk, n), last_share(wk, n)
+        elif line.startswith("probe warm sweep done"):  # one warm sweep
+            fk, wk = last_share(fk, 2), last_share(wk, 2)
     launches = algo = cfg = None
     for line in open(probe_log):
         if line.startswith("probe config "):
