@@ -611,7 +611,9 @@ __global__ __launch_bounds__(BLOCK) void k_stats(const InstDev* inst, const Tile
 // (block_cells' order, TX the update's tile width) holds +0.0 in all three
 // fields.  A source whose marks were cleared (a state set from the host)
 // and its checkpoint copy then let the next steps skip what is empty.  With
-// dst and mk_dst null, only the source's marks are written.
+// dst and mk_dst null, only the source's marks are written.  A block of the
+// copy marked whole +0.0 (MARK_FULL, meaningful in every tile shape) whose
+// source block is +0.0 is not rewritten.
 template <int TX, class W>
 __global__ __launch_bounds__(BLOCK) void k_copy_marking(const BlockDev* blocks, const W* s0,
                                                         const W* s1, const W* s2, W* dst,
@@ -620,24 +622,34 @@ __global__ __launch_bounds__(BLOCK) void k_copy_marking(const BlockDev* blocks, 
   const BlockDev B = blocks[blockIdx.x];
   __shared__ unsigned s_nz;
   if (threadIdx.x == 0) s_nz = 0u;
+  // a copy whose block already holds +0.0 (word MARK_FULL) is not rewritten
+  // where the source block is +0.0 too
+  const bool dst_zero = mk_dst && mk_dst[B.mw] == MARK_FULL;
   Cell q[4];
   block_cells<TX>(B, q);
+  W v[4][3];
   unsigned nz = 0u;
 #pragma unroll
   for (int b = 0; b < 4; b++) {
+    v[b][0] = v[b][1] = v[b][2] = (W)0;
     if (!q[b].act) continue;
     const long long c = q[b].c;
-    const W v0 = s0[c], v1 = s1[c], v2 = s2[c];
-    if (dst) {
-      dst[c] = v0;
-      dst[ncell + c] = v1;
-      dst[2 * ncell + c] = v2;
-    }
-    if ((v0 | v1 | v2) != (W)0) nz |= 1u << b;
+    v[b][0] = s0[c]; v[b][1] = s1[c]; v[b][2] = s2[c];
+    if ((v[b][0] | v[b][1] | v[b][2]) != (W)0) nz |= 1u << b;
   }
   __syncthreads();  // s_nz cleared
   if (nz) atomicOr(&s_nz, nz);
   __syncthreads();
+  if (dst && !(dst_zero && s_nz == 0u)) {
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      if (!q[b].act) continue;
+      const long long c = q[b].c;
+      dst[c] = v[b][0];
+      dst[ncell + c] = v[b][1];
+      dst[2 * ncell + c] = v[b][2];
+    }
+  }
   if (threadIdx.x == 0) {
     uint32_t w = 0u;
 #pragma unroll
