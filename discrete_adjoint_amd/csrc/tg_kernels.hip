@@ -108,6 +108,16 @@ __device__ __forceinline__ bool adjoint_skip(const uint32_t* mu, const uint32_t*
 __device__ __forceinline__ void mark_byte_store(uint32_t* mk, int mw, int b, bool zero) {
   ((unsigned char*)(mk + mw))[b] = zero ? 1 : 0;
 }
+// Position p of n, walked by workgroup p % gridDim.x (on XCD p % NXCD when
+// gridDim.x is a multiple of NXCD), reads list entry start(x) + p / NXCD of
+// band x = p % NXCD, band x a contiguous run of q or q + 1 entries (q =
+// n / NXCD, the first n % NXCD bands one longer): a bijection on [0, n)
+// that keeps neighbouring list entries -- a block's tiles, appended
+// together -- on one XCD, whose L2 then serves their shared halos.
+__device__ __forceinline__ int list_band(int p, int n) {
+  const int q = n / NXCD, r = n % NXCD, x = p % NXCD;
+  return x * q + (x < r ? x : r) + p / NXCD;
+}
 // The cells of a block's four tiles this thread handles (one per tile).
 template <int TX>
 __device__ __forceinline__ void block_cells(const BlockDev& B, Cell q[4]) {
@@ -571,7 +581,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(StepArgs a) {
   __shared__ FaceLds<TX> L;
   const int n = *a.tile_count;
   for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x) {
-    update_tile<TX>(a, a.tile_list[i], L);
+    update_tile<TX>(a, a.tile_list[list_band(i, n)], L);
     __syncthreads();  // LDS reused by the next tile
   }
 }
@@ -1202,7 +1212,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a) {
   __shared__ double sp[BLOCK / 64];
   const int n = *a.tile_count;
   for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x) {
-    adjoint_tile<TX>(a, a.tile_list[i], L, sp);
+    adjoint_tile<TX>(a, a.tile_list[list_band(i, n)], L, sp);
     __syncthreads();  // LDS reused by the next tile
   }
 }
