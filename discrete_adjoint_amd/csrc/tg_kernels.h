@@ -104,6 +104,16 @@ constexpr int MARK_W = 64, MARK_H = 16;
 constexpr uint32_t MARK_FULL = 0x01010101u;
 static_assert(MARK_W % 64 == 0 && MARK_H % (BLOCK / 16) == 0 && MARK_W * MARK_H == 4 * BLOCK,
               "every tile shape must cut a mark block into four tiles");
+// Position p of n, walked by workgroup p % gridDim.x (on XCD p % NXCD when
+// gridDim.x is a multiple of NXCD), reads list entry start(x) + p / NXCD of
+// band x = p % NXCD, band x a contiguous run of q or q + 1 entries (q =
+// n / NXCD, the first n % NXCD bands one longer): a bijection on [0, n)
+// that keeps neighbouring list entries -- a block's tiles, appended
+// together -- on one XCD, whose L2 then serves their shared halos.
+__device__ __forceinline__ int list_band(int p, int n) {
+  const int q = n / NXCD, r = n % NXCD, x = p % NXCD;
+  return x * q + (x < r ? x : r) + p / NXCD;
+}
 // the byte of its block's word that a tile of width TX owns
 template <int TX>
 __device__ __forceinline__ int mark_byte(const TileDev& t) {

@@ -108,16 +108,6 @@ __device__ __forceinline__ bool adjoint_skip(const uint32_t* mu, const uint32_t*
 __device__ __forceinline__ void mark_byte_store(uint32_t* mk, int mw, int b, bool zero) {
   ((unsigned char*)(mk + mw))[b] = zero ? 1 : 0;
 }
-// Position p of n, walked by workgroup p % gridDim.x (on XCD p % NXCD when
-// gridDim.x is a multiple of NXCD), reads list entry start(x) + p / NXCD of
-// band x = p % NXCD, band x a contiguous run of q or q + 1 entries (q =
-// n / NXCD, the first n % NXCD bands one longer): a bijection on [0, n)
-// that keeps neighbouring list entries -- a block's tiles, appended
-// together -- on one XCD, whose L2 then serves their shared halos.
-__device__ __forceinline__ int list_band(int p, int n) {
-  const int q = n / NXCD, r = n % NXCD, x = p % NXCD;
-  return x * q + (x < r ? x : r) + p / NXCD;
-}
 // The cells of a block's four tiles this thread handles (one per tile).
 template <int TX>
 __device__ __forceinline__ void block_cells(const BlockDev& B, Cell q[4]) {

@@ -938,7 +938,7 @@ void k32_adjoint(AdjArgsM<S> a) {
   __shared__ double sp[BLOCK / 64];
   const int n = *a.tile_count;
   for (int i = (int)blockIdx.x; i < n; i += (int)gridDim.x) {
-    adjoint32_tile<S, TX>(a, a.tile_list[i], L, sp);
+    adjoint32_tile<S, TX>(a, a.tile_list[list_band(i, n)], L, sp);
     __syncthreads();  // LDS reused by the next tile
   }
 }
